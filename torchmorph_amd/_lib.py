@@ -1,0 +1,106 @@
+"""ctypes binding of libtorchmorph_hip.so (the C ABI in include/torchmorph_hip.h).
+
+torch is imported before the library is opened so that the library's
+``libamdhip64.so.7`` dependency resolves to the HIP runtime torch already
+loaded: device pointers and ``hipStream_t`` handles from torch are then valid
+in the library.  There is no fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtorchmorph_hip.so")
+
+TM_U8, TM_F32, TM_I32, TM_I64, TM_F64 = 0, 1, 2, 3, 4
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_SZ = ctypes.c_size_t
+_D = ctypes.c_double
+_F = ctypes.c_float
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); every C-ABI entry point is declared here
+SIGNATURES: dict[str, tuple] = {
+    "tm_last_error": (_I, [ctypes.c_char_p, _SZ]),
+    "tm_version": (ctypes.c_char_p, []),
+    "tm_edt_workspace_bytes": (_SZ, [_I64, _I, _I64P, _DP]),
+    "tm_edt": (_I, [_P, _I, _I64, _I, _I64P, _DP, _P, _P, _I, _P, _SZ, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises RuntimeError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"torchmorph_amd: {LIB_PATH} is missing; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or "
+            "`python -m torchmorph_amd._build`")
+    handle = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().tm_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise RuntimeError(f"{what}: {last_error()} (status {status})")
+
+
+def i64_array(values) -> ctypes.Array:
+    vals = [int(v) for v in values]
+    return (ctypes.c_int64 * max(1, len(vals)))(*vals)
+
+
+def f64_array(values) -> ctypes.Array:
+    vals = [float(v) for v in values]
+    return (ctypes.c_double * max(1, len(vals)))(*vals)
+
+
+def ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_of(device: torch.device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def workspace(nbytes: int, device: torch.device) -> torch.Tensor | None:
+    if nbytes <= 0:
+        return None
+    return torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+
+
+def mask_operand(x: torch.Tensor) -> tuple[torch.Tensor, int]:
+    """Contiguous device view of a mask for the kernels: bool/uint8 are read
+    as bytes, float32 directly; any other dtype is reduced to ``x != 0``."""
+    if x.dtype == torch.bool:
+        x = x.contiguous().view(torch.uint8)
+        return x, TM_U8
+    if x.dtype == torch.uint8:
+        return x.contiguous(), TM_U8
+    if x.dtype == torch.float32:
+        return x.contiguous(), TM_F32
+    return (x != 0).contiguous().view(torch.uint8), TM_U8

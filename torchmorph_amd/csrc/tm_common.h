@@ -1,0 +1,80 @@
+// Shared host/device helpers for the torchmorph HIP library (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/torchmorph_hip.h"
+
+namespace tmh {
+
+// Last error message, per host thread.
+void set_error(const char* fmt, ...);
+const char* get_error();
+
+constexpr int kMaxNdim = 8;
+
+// Small by-value shape descriptor passed to kernels (kernel arguments live in
+// the constant/kernarg segment, so no H2D copy is needed per call).
+struct Shape {
+  int ndim;
+  int64_t size[kMaxNdim];
+  int64_t stride[kMaxNdim];  // element stride of each spatial axis
+  int64_t numel;             // prod(size)
+};
+
+inline Shape make_shape(int ndim, const int64_t* s) {
+  Shape sh{};
+  sh.ndim = ndim;
+  int64_t st = 1;
+  for (int d = ndim - 1; d >= 0; --d) {
+    sh.size[d] = s[d];
+    sh.stride[d] = st;
+    st *= s[d];
+  }
+  sh.numel = st;
+  return sh;
+}
+
+#define TM_CHECK_ARG(cond, ...)        \
+  do {                                 \
+    if (!(cond)) {                     \
+      ::tmh::set_error(__VA_ARGS__);    \
+      return TM_ERR_ARG;               \
+    }                                  \
+  } while (0)
+
+#define TM_CHECK_HIP(expr)                                                   \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      ::tmh::set_error("%s failed: %s", #expr, hipGetErrorString(_e));        \
+      return TM_ERR_HIP;                                                     \
+    }                                                                        \
+  } while (0)
+
+#define TM_LAUNCH_CHECK()                                                    \
+  do {                                                                       \
+    hipError_t _e = hipGetLastError();                                       \
+    if (_e != hipSuccess) {                                                  \
+      ::tmh::set_error("kernel launch failed: %s", hipGetErrorString(_e));    \
+      return TM_ERR_HIP;                                                     \
+    }                                                                        \
+  } while (0)
+
+inline unsigned grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// Reads a mask element: nonzero means foreground (reference feeds
+// input.float(); any nonzero value is foreground).
+template <typename T>
+__device__ __forceinline__ bool is_fg(const T* p, int64_t i) {
+  return p[i] != T(0);
+}
+
+}  // namespace tmh

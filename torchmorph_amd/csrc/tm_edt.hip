@@ -1,0 +1,482 @@
+// Exact Euclidean distance transform for gfx950.
+//
+// Semantics are SciPy's distance_transform_edt, which the reference tests pin
+// (reference test/test_distance_transform.py:32-69, :289-320, :397-408): the
+// feature transform is the separable lower-envelope algorithm of Maurer, Qi
+// and Raghavan (one pass per axis, axis 0 first), with double arithmetic,
+// removal of the middle site when c*vR - b*uR - a*wR - a*b*c > 0, and a query
+// that moves to the next envelope site only when it is strictly closer (the
+// lower index wins ties).  Foreground voxels with no background anywhere get
+// the feature (-1, 0, ..., 0), the reference's virtual background
+// (edt_kernel.cu:415-437).  Distances are computed in double from the integer
+// feature deltas in axis order and rounded once to float32.
+//
+// Kernels:
+//  * edt2d_band_kernel  -- 2-D, unit sampling.  One workgroup per band of R
+//    rows of one image.  Axis-0 pass: nearest background per column found by
+//    chunked scans of the mask (coalesced along the contiguous axis).
+//    Axis-1 pass: the band's squared column distances sit in LDS and every
+//    pixel runs an exact expanding-window argmin with the lower-index tie
+//    rule.  With unit sampling every quantity is an exact integer, so this
+//    argmin equals the envelope result bit for bit.
+//  * edt_pass0_kernel   -- N-D, axis 0 from the mask: two sweeps per line.
+//  * edt_stack_kernel   -- N-D, axis d >= 1: one lane per line runs the
+//    double-precision envelope (stack of sites with their partial squared
+//    distances) exactly in SciPy's operation order; the stack is in LDS when
+//    the line fits the LDS depth, otherwise in an interleaved global scratch.
+//
+// Build with -ffp-contract=off: every double multiply and add must round
+// separately, as SciPy's C code does.
+#include "tm_common.h"
+
+namespace tmh {
+
+struct EdtOut {
+  float* dist;
+  void* idx;
+  int idx64;
+  int64_t plane;  // elements per index plane = batch * numel
+};
+
+struct Samp {
+  double s[kMaxNdim];
+};
+
+__device__ __forceinline__ void edt_store(const EdtOut& o, int ndim, int64_t g, const int64_t* c,
+                                          const int32_t* f, const Samp& sp) {
+  if (o.dist) {
+    double acc = 0.0;
+    for (int j = 0; j < ndim; ++j) {
+      int32_t delta = f[j] - (int32_t)c[j];
+      double t = (double)delta * sp.s[j];
+      t = t * t;
+      acc = (j == 0) ? t : acc + t;
+    }
+    o.dist[g] = (float)sqrt(acc);
+  }
+  if (o.idx) {
+    if (o.idx64) {
+      int64_t* p = (int64_t*)o.idx;
+      for (int j = 0; j < ndim; ++j) p[j * o.plane + g] = f[j];
+    } else {
+      int32_t* p = (int32_t*)o.idx;
+      for (int j = 0; j < ndim; ++j) p[j * o.plane + g] = f[j];
+    }
+  }
+}
+
+// ------------------------------------------------------------ 2-D band --
+// LDS per workgroup: R*W int32 squared column distances + R*W int32 rows.
+template <typename TIn, int R>
+__global__ __launch_bounds__(256) void edt2d_band_kernel(const TIn* __restrict__ in, int H, int W,
+                                                         EdtOut o) {
+  extern __shared__ int32_t lds[];
+  int32_t* hs = lds;          // [R][W] squared distance to column feature, -1 = none
+  int32_t* fr = lds + R * W;  // [R][W] feature row
+  const int img = blockIdx.y;
+  const int y0 = blockIdx.x * R;
+  const int rows = min(R, H - y0);
+  const TIn* im = in + (int64_t)img * H * W;
+
+  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+    // nearest background strictly above the band
+    int prev = -1;
+    for (int yb = y0 - 1; yb >= 0 && prev < 0; yb -= 16) {
+      uint32_t bits = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        int y = yb - k;
+        if (y >= 0 && !is_fg(im, (int64_t)y * W + x)) bits |= 1u << k;
+      }
+      if (bits) prev = yb - __builtin_ctz(bits);
+    }
+    // nearest background strictly below the band
+    int next = -1;
+    for (int yb = y0 + rows; yb < H && next < 0; yb += 16) {
+      uint32_t bits = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        int y = yb + k;
+        if (y < H && !is_fg(im, (int64_t)y * W + x)) bits |= 1u << k;
+      }
+      if (bits) next = yb + __builtin_ctz(bits);
+    }
+    bool bg[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) bg[r] = (r < rows) && !is_fg(im, (int64_t)(y0 + r) * W + x);
+    int nxt[R];
+    int nx = next;
+#pragma unroll
+    for (int r = R - 1; r >= 0; --r) {
+      if (bg[r]) nx = y0 + r;
+      nxt[r] = nx;
+    }
+    int pv = prev;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r < rows) {
+        const int y = y0 + r;
+        if (bg[r]) pv = y;
+        int f = -1;
+        if (pv >= 0 && (nxt[r] < 0 || (y - pv) <= (nxt[r] - y))) f = pv;
+        else if (nxt[r] >= 0) f = nxt[r];
+        fr[r * W + x] = f;
+        hs[r * W + x] = f >= 0 ? (y - f) * (y - f) : -1;
+      }
+    }
+  }
+  __syncthreads();
+
+  const int npx = rows * W;
+  for (int p = threadIdx.x; p < npx; p += blockDim.x) {
+    const int r = p / W;
+    const int x = p - r * W;
+    const int32_t* hr = hs + r * W;
+    int best = hr[x];
+    int bp = x;
+    const int reach = max(x, W - 1 - x);
+    for (int k = 1; k <= reach; ++k) {
+      const int kk = k * k;
+      if (best >= 0 && kk > best) break;
+      if (x - k >= 0) {
+        int hv = hr[x - k];
+        if (hv >= 0 && (best < 0 || hv + kk <= best)) {
+          best = hv + kk;
+          bp = x - k;
+        }
+      }
+      if (x + k < W) {
+        int hv = hr[x + k];
+        if (hv >= 0 && (best < 0 || hv + kk < best)) {
+          best = hv + kk;
+          bp = x + k;
+        }
+      }
+    }
+    const int y = y0 + r;
+    const int64_t g = ((int64_t)img * H + y) * W + x;
+    int32_t f[2];
+    if (best >= 0) {
+      f[0] = fr[r * W + bp];
+      f[1] = bp;
+    } else {
+      f[0] = -1;  // no background in the whole image
+      f[1] = 0;
+    }
+    if (o.dist) {
+      // exact integers: identical to the double delta/square/sum in edt_store
+      double acc;
+      if (best >= 0) {
+        acc = (double)best;
+      } else {
+        double a = (double)(-1 - y), b = (double)(-x);
+        acc = a * a + b * b;
+      }
+      o.dist[g] = (float)sqrt(acc);
+    }
+    if (o.idx) {
+      if (o.idx64) {
+        int64_t* ip = (int64_t*)o.idx;
+        ip[g] = f[0];
+        ip[o.plane + g] = f[1];
+      } else {
+        int32_t* ip = (int32_t*)o.idx;
+        ip[g] = f[0];
+        ip[o.plane + g] = f[1];
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------- N-D --
+__device__ __forceinline__ void decode_line(const Shape& sh, int d, int64_t t, int64_t* c,
+                                            int64_t* base, int64_t* b) {
+  // t enumerates (batch, c_0..c_{d-1}, c_{d+1}..c_{n-1}) with the inner axes fastest.
+  const int64_t inner = sh.stride[d];
+  int64_t ob = t / inner;
+  int64_t in = t - ob * inner;
+  int64_t rem = in;
+  for (int j = sh.ndim - 1; j > d; --j) {
+    c[j] = rem % sh.size[j];
+    rem /= sh.size[j];
+  }
+  rem = ob;
+  for (int j = d - 1; j >= 0; --j) {
+    c[j] = rem % sh.size[j];
+    rem /= sh.size[j];
+  }
+  *b = rem;
+  c[d] = 0;
+  *base = ob * sh.size[d] * inner + in;
+}
+
+// Axis-0 pass from the mask: nearest background along the line, lower index
+// on ties (with all partial distances zero every site stays on the envelope,
+// so SciPy's query reduces to this).  Writes the feature row into f0, or the
+// final outputs when ndim == 1.
+template <typename TIn, bool FINAL>
+__global__ __launch_bounds__(256) void edt_pass0_kernel(const TIn* __restrict__ in,
+                                                        int32_t* __restrict__ f0, Shape sh,
+                                                        int64_t nlines, Samp sp, EdtOut o) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlines) return;
+  int64_t c[kMaxNdim], base, b;
+  decode_line(sh, 0, t, c, &base, &b);
+  const int64_t step = sh.stride[0];
+  const int64_t L = sh.size[0];
+  int32_t last = -1;
+  for (int64_t i = 0; i < L; ++i) {
+    const int64_t v = base + i * step;
+    if (!is_fg(in, v)) last = (int32_t)i;
+    f0[v] = last;
+  }
+  int32_t nx = -1;
+  for (int64_t i = L - 1; i >= 0; --i) {
+    const int64_t v = base + i * step;
+    if (!is_fg(in, v)) nx = (int32_t)i;
+    const int32_t pv = f0[v];
+    int32_t r;
+    if (pv < 0) r = nx;
+    else if (nx < 0) r = pv;
+    else r = ((int32_t)i - pv <= nx - (int32_t)i) ? pv : nx;
+    if (FINAL) {
+      c[0] = i;
+      int32_t f[kMaxNdim];
+      f[0] = r;
+      for (int j = 1; j < sh.ndim; ++j) f[j] = r >= 0 ? (int32_t)c[j] : 0;
+      edt_store(o, sh.ndim, v, c, f, sp);
+    } else {
+      f0[v] = r;
+    }
+  }
+}
+
+// Axis-d pass (d >= 1): envelope of the sites on each line.  fin holds the d
+// feature planes of the previous pass, fout receives d+1 planes (or the final
+// outputs when FINAL).  LDS_DEPTH > 0: stack in LDS (requires L <= LDS_DEPTH);
+// LDS_DEPTH == 0: stack in the interleaved global scratch gpos/gh.
+template <bool FINAL, int LDS_DEPTH>
+__global__ __launch_bounds__(64) void edt_stack_kernel(const int32_t* __restrict__ fin,
+                                                       int32_t* __restrict__ fout, int64_t plane,
+                                                       Shape sh, int d, int64_t nlines, Samp sp,
+                                                       int32_t* __restrict__ gpos,
+                                                       double* __restrict__ gh, EdtOut o) {
+  __shared__ uint16_t lpos[LDS_DEPTH > 0 ? LDS_DEPTH * 64 : 1];
+  __shared__ double lh[LDS_DEPTH > 0 ? LDS_DEPTH * 64 : 1];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlines) return;
+  int64_t c[kMaxNdim], base, b;
+  decode_line(sh, d, t, c, &base, &b);
+  const int64_t step = sh.stride[d];
+  const int64_t L = sh.size[d];
+  const double sd = sp.s[d];
+  const int lane = threadIdx.x;
+
+  auto POS = [&](int64_t k) -> int32_t {
+    if (LDS_DEPTH > 0) return (int32_t)lpos[k * 64 + lane];
+    return gpos[k * nlines + t];
+  };
+  auto H = [&](int64_t k) -> double {
+    if (LDS_DEPTH > 0) return lh[k * 64 + lane];
+    return gh[k * nlines + t];
+  };
+
+  int64_t top = -1;
+  for (int64_t i = 0; i < L; ++i) {
+    const int64_t v = base + i * step;
+    const int32_t f0 = fin[v];
+    if (f0 < 0) continue;
+    double wR = 0.0;
+    {
+      double tt = (double)(f0 - (int32_t)c[0]) * sp.s[0];
+      wR += tt * tt;
+      for (int j = 1; j < d; ++j) {
+        double u = (double)(fin[j * plane + v] - (int32_t)c[j]) * sp.s[j];
+        wR += u * u;
+      }
+    }
+    while (top >= 1) {
+      const int32_t pv = POS(top), pu = POS(top - 1);
+      const double a = (double)(pv - pu) * sd;
+      const double bb = (double)((int32_t)i - pv) * sd;
+      const double cc = a + bb;
+      const double uR = H(top - 1), vR = H(top);
+      if (cc * vR - bb * uR - a * wR - a * bb * cc <= 0.0) break;
+      --top;
+    }
+    ++top;
+    if (LDS_DEPTH > 0) {
+      lpos[top * 64 + lane] = (uint16_t)i;
+      lh[top * 64 + lane] = wR;
+    } else {
+      gpos[top * nlines + t] = (int32_t)i;
+      gh[top * nlines + t] = wR;
+    }
+  }
+
+  int32_t f[kMaxNdim];
+  if (top < 0) {
+    // no site on this line: every voxel keeps the feature (-1, 0, ..., 0)
+    for (int j = 0; j < sh.ndim; ++j) f[j] = 0;
+    f[0] = -1;
+    for (int64_t i = 0; i < L; ++i) {
+      const int64_t v = base + i * step;
+      if (FINAL) {
+        c[d] = i;
+        edt_store(o, sh.ndim, v, c, f, sp);
+      } else {
+        for (int j = 0; j <= d; ++j) fout[j * plane + v] = f[j];
+      }
+    }
+    return;
+  }
+  int64_t l = 0;
+  int32_t pl = POS(0);
+  double hl = H(0);
+  for (int64_t i = 0; i < L; ++i) {
+    double tt = (double)(pl - (int32_t)i) * sd;
+    double d1 = hl + tt * tt;
+    while (l < top) {
+      const int32_t pn = POS(l + 1);
+      const double hn = H(l + 1);
+      double t2 = (double)(pn - (int32_t)i) * sd;
+      double d2 = hn + t2 * t2;
+      if (d1 <= d2) break;
+      d1 = d2;
+      ++l;
+      pl = pn;
+      hl = hn;
+    }
+    const int64_t site = base + (int64_t)pl * step;
+    for (int j = 0; j < d; ++j) f[j] = fin[j * plane + site];
+    f[d] = pl;
+    const int64_t v = base + i * step;
+    if (FINAL) {
+      for (int j = d + 1; j < sh.ndim; ++j) f[j] = (int32_t)c[j];
+      c[d] = i;
+      edt_store(o, sh.ndim, v, c, f, sp);
+    } else {
+      for (int j = 0; j <= d; ++j) fout[j * plane + v] = f[j];
+    }
+  }
+}
+
+static bool unit_sampling(int ndim, const double* s) {
+  if (!s) return true;
+  for (int j = 0; j < ndim; ++j)
+    if (s[j] != 1.0) return false;
+  return true;
+}
+
+static bool use_band_path(int ndim, const int64_t* shape, const double* s) {
+  return ndim == 2 && unit_sampling(ndim, s) && shape[1] <= 1024 && shape[0] <= 16384;
+}
+
+template <typename TIn>
+static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* shape,
+                        const double* sampling, EdtOut o, void* ws, size_t ws_bytes,
+                        hipStream_t st) {
+  Shape sh = make_shape(ndim, shape);
+  Samp sp;
+  for (int j = 0; j < kMaxNdim; ++j) sp.s[j] = (sampling && j < ndim) ? sampling[j] : 1.0;
+  o.plane = batch * sh.numel;
+
+  if (use_band_path(ndim, shape, sampling)) {
+    const int H = (int)shape[0], W = (int)shape[1];
+    constexpr int R = 8;
+    dim3 grid((H + R - 1) / R, (unsigned)batch);
+    size_t lds = (size_t)R * W * 2 * sizeof(int32_t);
+    hipLaunchKernelGGL((edt2d_band_kernel<TIn, R>), grid, dim3(256), lds, st, in, H, W, o);
+    TM_LAUNCH_CHECK();
+    return TM_OK;
+  }
+
+  const int64_t N = o.plane;
+  const size_t need = tm_edt_workspace_bytes(batch, ndim, shape, sampling);
+  if (ws_bytes < need || (need && !ws)) {
+    set_error("tm_edt: workspace of %zu bytes is smaller than the %zu required", ws_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  const int nbuf = ndim > 1 ? ndim - 1 : 1;
+  int32_t* bufA = (int32_t*)ws;
+  int32_t* bufB = bufA + (size_t)nbuf * N;
+  int32_t* gpos = bufB + (size_t)nbuf * N;
+  double* gh = (double*)(((uintptr_t)(gpos + N) + 255) & ~(uintptr_t)255);
+
+  // axis 0
+  {
+    const int64_t nlines = batch * sh.stride[0];
+    if (ndim == 1)
+      hipLaunchKernelGGL((edt_pass0_kernel<TIn, true>), dim3(grid_for(nlines, 256)), dim3(256), 0,
+                         st, in, bufA, sh, nlines, sp, o);
+    else
+      hipLaunchKernelGGL((edt_pass0_kernel<TIn, false>), dim3(grid_for(nlines, 256)), dim3(256),
+                         0, st, in, bufA, sh, nlines, sp, o);
+    TM_LAUNCH_CHECK();
+  }
+  int32_t* fin = bufA;
+  int32_t* fout = bufB;
+  for (int d = 1; d < ndim; ++d) {
+    const int64_t nlines = batch * (sh.numel / sh.size[d]);
+    const bool fin_pass = (d == ndim - 1);
+    const dim3 g(grid_for(nlines, 64)), blk(64);
+    const bool lds = sh.size[d] <= 128;
+#define TM_EDT_STACK(FINAL, DEPTH)                                                              \
+  hipLaunchKernelGGL((edt_stack_kernel<FINAL, DEPTH>), g, blk, 0, st, fin, fout, N, sh, d, nlines, \
+                     sp, gpos, gh, o)
+    if (fin_pass) {
+      if (lds) TM_EDT_STACK(true, 128);
+      else TM_EDT_STACK(true, 0);
+    } else {
+      if (lds) TM_EDT_STACK(false, 128);
+      else TM_EDT_STACK(false, 0);
+    }
+#undef TM_EDT_STACK
+    TM_LAUNCH_CHECK();
+    int32_t* tmp = fin;
+    fin = fout;
+    fout = tmp;
+  }
+  return TM_OK;
+}
+
+}  // namespace tmh
+
+extern "C" size_t tm_edt_workspace_bytes(int64_t batch, int ndim, const int64_t* shape,
+                                         const double* sampling) {
+  if (ndim < 1 || ndim > tmh::kMaxNdim || batch < 1) return 0;
+  if (tmh::use_band_path(ndim, shape, sampling)) return 0;
+  int64_t n = batch;
+  for (int j = 0; j < ndim; ++j) n *= shape[j];
+  const int nbuf = ndim > 1 ? ndim - 1 : 1;
+  return (size_t)n * 4 * (2 * nbuf + 1) + 256 + (size_t)n * 8;
+}
+
+extern "C" int tm_edt(const void* input, int input_dtype, int64_t batch, int ndim,
+                      const int64_t* shape, const double* sampling, float* distances,
+                      void* indices, int index_dtype, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(input != nullptr, "tm_edt: input is NULL");
+  TM_CHECK_ARG(ndim >= 1 && ndim <= kMaxNdim, "tm_edt: ndim must be in 1..8, got %d", ndim);
+  TM_CHECK_ARG(batch >= 1, "tm_edt: batch must be >= 1");
+  for (int j = 0; j < ndim; ++j) {
+    TM_CHECK_ARG(shape[j] >= 1 && shape[j] <= 65535, "tm_edt: axis %d size %lld out of range 1..65535",
+                 j, (long long)shape[j]);
+    if (sampling)
+      TM_CHECK_ARG(sampling[j] > 0.0 && sampling[j] < 1e300, "tm_edt: sampling must be finite > 0");
+  }
+  TM_CHECK_ARG(distances || indices, "tm_edt: at least one output must be requested");
+  TM_CHECK_ARG(!indices || index_dtype == TM_I32 || index_dtype == TM_I64,
+               "tm_edt: index dtype must be int32 or int64");
+  EdtOut o{distances, indices, index_dtype == TM_I64 ? 1 : 0, 0};
+  hipStream_t st = (hipStream_t)stream;
+  if (input_dtype == TM_U8)
+    return edt_dispatch((const uint8_t*)input, batch, ndim, shape, sampling, o, workspace,
+                        workspace_bytes, st);
+  if (input_dtype == TM_F32)
+    return edt_dispatch((const float*)input, batch, ndim, shape, sampling, o, workspace,
+                        workspace_bytes, st);
+  set_error("tm_edt: input dtype must be uint8/bool or float32");
+  return TM_ERR_ARG;
+}
