@@ -61,6 +61,23 @@ int tm_edt(const void* input, int input_dtype, int64_t batch, int ndim, const in
            const double* sampling, float* distances, void* indices, int index_dtype,
            void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------- CDT --- */
+
+/* Workspace bytes tm_cdt needs (int32 distance + int32 feature per voxel). */
+size_t tm_cdt_workspace_bytes(int64_t batch, int ndim, const int64_t* shape);
+
+/* Chamfer distance transform with SciPy's two-pass raster semantics, values
+ * and indices bit-identical to scipy.ndimage.distance_transform_cdt.
+ * Replaces torchmorph._C.cdt_cuda (reference torchmorph.cpp:42-47,
+ * cdt_kernel.cu:269-463).
+ *   metric      0 = chessboard, 1 = taxicab
+ *   distances   float32, -1 where no background is reachable, or NULL
+ *   indices     (ndim, batch, spatial...) int32/int64 feature coordinates
+ */
+int tm_cdt(const void* input, int input_dtype, int64_t batch, int ndim, const int64_t* shape,
+           int metric, float* distances, void* indices, int index_dtype, void* workspace,
+           size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,8 @@ SIGNATURES: dict[str, tuple] = {
     "tm_version": (ctypes.c_char_p, []),
     "tm_edt_workspace_bytes": (_SZ, [_I64, _I, _I64P, _DP]),
     "tm_edt": (_I, [_P, _I, _I64, _I, _I64P, _DP, _P, _P, _I, _P, _SZ, _P]),
+    "tm_cdt_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
+    "tm_cdt": (_I, [_P, _I, _I64, _I, _I64P, _I, _P, _P, _I, _P, _SZ, _P]),
 }
 
 _lib = None

@@ -12,13 +12,11 @@
 // feature deltas in axis order and rounded once to float32.
 //
 // Kernels:
-//  * edt2d_band_kernel  -- 2-D, unit sampling.  One workgroup per band of R
-//    rows of one image.  Axis-0 pass: nearest background per column found by
-//    chunked scans of the mask (coalesced along the contiguous axis).
-//    Axis-1 pass: the band's squared column distances sit in LDS and every
-//    pixel runs an exact expanding-window argmin with the lower-index tie
-//    rule.  With unit sampling every quantity is an exact integer, so this
-//    argmin equals the envelope result bit for bit.
+//  * edt2d_cols_kernel / edt2d_rows_kernel -- 2-D, unit sampling, H <= 2048,
+//    W <= 4096.  Axis 0: nearest background row per pixel from bit-packed
+//    column segments (one load round), int16 into the workspace.  Axis 1:
+//    band of rows staged in LDS, exact expanding-window argmin per pixel with
+//    the lower-index tie rule (exact integers, so equal to the envelope).
 //  * edt_pass0_kernel   -- N-D, axis 0 from the mask: two sweeps per line.
 //  * edt_stack_kernel   -- N-D, axis d >= 1: one lane per line runs the
 //    double-precision envelope (stack of sites with their partial squared
@@ -65,69 +63,123 @@ __device__ __forceinline__ void edt_store(const EdtOut& o, int ndim, int64_t g, 
   }
 }
 
-// ------------------------------------------------------------ 2-D band --
-// LDS per workgroup: R*W int32 squared column distances + R*W int32 rows.
-template <typename TIn, int R>
-__global__ __launch_bounds__(256) void edt2d_band_kernel(const TIn* __restrict__ in, int H, int W,
-                                                         EdtOut o) {
-  extern __shared__ int32_t lds[];
-  int32_t* hs = lds;          // [R][W] squared distance to column feature, -1 = none
-  int32_t* fr = lds + R * W;  // [R][W] feature row
+// ------------------------------------------------------------- 2-D path --
+// Axis-0 pass.  A workgroup covers 64 columns of one image with 16 column
+// groups (4 adjacent columns each) x 16 row segments.  Every thread loads its
+// 4 x seglen mask bytes in one round, keeps them as background bit masks, and
+// the segments exchange their first/last background rows through LDS, so the
+// kernel has a single dependent memory round trip.  Output: nearest
+// background row per pixel (int16, -1 = none), lower row on ties.
+constexpr int kColSegs = 16;
+constexpr int kColChunks = 4;  // 32-row chunks per segment: H <= 16*4*32 = 2048
+
+template <typename TIn>
+__global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__ in, int H, int W,
+                                                         int16_t* __restrict__ f0) {
+  __shared__ int16_t sfirst[kColSegs][64];
+  __shared__ int16_t slast[kColSegs][64];
+  const int cg = threadIdx.x & 15;
+  const int seg = threadIdx.x >> 4;
   const int img = blockIdx.y;
-  const int y0 = blockIdx.x * R;
-  const int rows = min(R, H - y0);
+  const int xb = blockIdx.x * 64 + cg * 4;
+  const int seglen = (H + kColSegs - 1) / kColSegs;
+  const int ys = seg * seglen;
+  const int ye = min(H, ys + seglen);
   const TIn* im = in + (int64_t)img * H * W;
 
-  for (int x = threadIdx.x; x < W; x += blockDim.x) {
-    // nearest background strictly above the band
-    int prev = -1;
-    for (int yb = y0 - 1; yb >= 0 && prev < 0; yb -= 16) {
-      uint32_t bits = 0;
+  uint32_t m[4][kColChunks];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        int y = yb - k;
-        if (y >= 0 && !is_fg(im, (int64_t)y * W + x)) bits |= 1u << k;
-      }
-      if (bits) prev = yb - __builtin_ctz(bits);
-    }
-    // nearest background strictly below the band
-    int next = -1;
-    for (int yb = y0 + rows; yb < H && next < 0; yb += 16) {
-      uint32_t bits = 0;
+  for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        int y = yb + k;
-        if (y < H && !is_fg(im, (int64_t)y * W + x)) bits |= 1u << k;
-      }
-      if (bits) next = yb + __builtin_ctz(bits);
-    }
-    bool bg[R];
+    for (int k = 0; k < kColChunks; ++k) m[c][k] = 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) bg[r] = (r < rows) && !is_fg(im, (int64_t)(y0 + r) * W + x);
-    int nxt[R];
-    int nx = next;
+  for (int k = 0; k < kColChunks; ++k) {
+#pragma unroll 8
+    for (int b = 0; b < 32; ++b) {
+      const int y = ys + k * 32 + b;
+      if (y < ye) {
 #pragma unroll
-    for (int r = R - 1; r >= 0; --r) {
-      if (bg[r]) nx = y0 + r;
-      nxt[r] = nx;
-    }
-    int pv = prev;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (r < rows) {
-        const int y = y0 + r;
-        if (bg[r]) pv = y;
-        int f = -1;
-        if (pv >= 0 && (nxt[r] < 0 || (y - pv) <= (nxt[r] - y))) f = pv;
-        else if (nxt[r] >= 0) f = nxt[r];
-        fr[r * W + x] = f;
-        hs[r * W + x] = f >= 0 ? (y - f) * (y - f) : -1;
+        for (int c = 0; c < 4; ++c) {
+          const int x = xb + c;
+          if (x < W && !is_fg(im, (int64_t)y * W + x)) m[c][k] |= 1u << b;
+        }
       }
     }
   }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    int first = -1, last = -1;
+#pragma unroll
+    for (int k = kColChunks - 1; k >= 0; --k)
+      if (m[c][k]) first = ys + k * 32 + __builtin_ctz(m[c][k]);
+#pragma unroll
+    for (int k = 0; k < kColChunks; ++k)
+      if (m[c][k]) last = ys + k * 32 + 31 - __builtin_clz(m[c][k]);
+    sfirst[seg][cg * 4 + c] = (int16_t)first;
+    slast[seg][cg * 4 + c] = (int16_t)last;
+  }
   __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int x = xb + c;
+    if (x >= W) continue;
+    int before = -1, after = -1;
+    for (int s = seg - 1; s >= 0 && before < 0; --s) before = slast[s][cg * 4 + c];
+    for (int s = seg + 1; s < kColSegs && after < 0; ++s) after = sfirst[s][cg * 4 + c];
+    for (int y = ys; y < ye; ++y) {
+      const int k = (y - ys) >> 5, b = (y - ys) & 31;
+      // nearest background at or above y
+      int pv = -1;
+      uint32_t mm = m[c][k] & (0xffffffffu >> (31 - b));
+      if (mm) {
+        pv = ys + k * 32 + 31 - __builtin_clz(mm);
+      } else {
+        for (int kk = k - 1; kk >= 0 && pv < 0; --kk)
+          if (m[c][kk]) pv = ys + kk * 32 + 31 - __builtin_clz(m[c][kk]);
+        if (pv < 0) pv = before;
+      }
+      // nearest background at or below y
+      int nx = -1;
+      mm = m[c][k] >> b;
+      if (mm) {
+        nx = y + __builtin_ctz(mm);
+      } else {
+        for (int kk = k + 1; kk < kColChunks && nx < 0; ++kk)
+          if (m[c][kk]) nx = ys + kk * 32 + __builtin_ctz(m[c][kk]);
+        if (nx < 0) nx = after;
+      }
+      int f = -1;
+      if (pv >= 0 && (nx < 0 || (y - pv) <= (nx - y))) f = pv;
+      else if (nx >= 0) f = nx;
+      f0[((int64_t)img * H + y) * W + x] = (int16_t)f;
+    }
+  }
+}
 
+// Axis-1 pass.  One workgroup per band of R rows: the band's squared column
+// distances and feature rows are staged in LDS in one load round, then every
+// pixel runs an exact expanding-window argmin over its row (the lower column
+// wins ties).  With unit sampling all quantities are exact integers, so this
+// equals SciPy's double-precision envelope result bit for bit.
+template <int R>
+__global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restrict__ f0, int H,
+                                                         int W, EdtOut o) {
+  extern __shared__ int32_t lds[];
+  int32_t* hs = lds;                       // [R][W] squared column distance, -1 = none
+  int16_t* fr = (int16_t*)(lds + R * W);   // [R][W] feature row
+  const int img = blockIdx.y;
+  const int y0 = blockIdx.x * R;
+  const int rows = min(R, H - y0);
   const int npx = rows * W;
+  const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
+  for (int p = threadIdx.x; p < npx; p += blockDim.x) {
+    const int r = p / W;
+    const int f = src[p];
+    const int dy = y0 + r - f;
+    fr[p] = (int16_t)f;
+    hs[p] = f >= 0 ? dy * dy : -1;
+  }
+  __syncthreads();
   for (int p = threadIdx.x; p < npx; p += blockDim.x) {
     const int r = p / W;
     const int x = p - r * W;
@@ -139,14 +191,14 @@ __global__ __launch_bounds__(256) void edt2d_band_kernel(const TIn* __restrict__
       const int kk = k * k;
       if (best >= 0 && kk > best) break;
       if (x - k >= 0) {
-        int hv = hr[x - k];
+        const int hv = hr[x - k];
         if (hv >= 0 && (best < 0 || hv + kk <= best)) {
           best = hv + kk;
           bp = x - k;
         }
       }
       if (x + k < W) {
-        int hv = hr[x + k];
+        const int hv = hr[x + k];
         if (hv >= 0 && (best < 0 || hv + kk < best)) {
           best = hv + kk;
           bp = x + k;
@@ -155,34 +207,28 @@ __global__ __launch_bounds__(256) void edt2d_band_kernel(const TIn* __restrict__
     }
     const int y = y0 + r;
     const int64_t g = ((int64_t)img * H + y) * W + x;
-    int32_t f[2];
+    int32_t fy, fx;
+    double acc;
     if (best >= 0) {
-      f[0] = fr[r * W + bp];
-      f[1] = bp;
+      fy = fr[r * W + bp];
+      fx = bp;
+      acc = (double)best;  // exact integer: same as the double delta/square/sum
     } else {
-      f[0] = -1;  // no background in the whole image
-      f[1] = 0;
+      fy = -1;  // no background in the whole image
+      fx = 0;
+      const double a = (double)(-1 - y), b = (double)(-x);
+      acc = a * a + b * b;
     }
-    if (o.dist) {
-      // exact integers: identical to the double delta/square/sum in edt_store
-      double acc;
-      if (best >= 0) {
-        acc = (double)best;
-      } else {
-        double a = (double)(-1 - y), b = (double)(-x);
-        acc = a * a + b * b;
-      }
-      o.dist[g] = (float)sqrt(acc);
-    }
+    if (o.dist) o.dist[g] = (float)sqrt(acc);
     if (o.idx) {
       if (o.idx64) {
         int64_t* ip = (int64_t*)o.idx;
-        ip[g] = f[0];
-        ip[o.plane + g] = f[1];
+        ip[g] = fy;
+        ip[o.plane + g] = fx;
       } else {
         int32_t* ip = (int32_t*)o.idx;
-        ip[g] = f[0];
-        ip[o.plane + g] = f[1];
+        ip[g] = fy;
+        ip[o.plane + g] = fx;
       }
     }
   }
@@ -368,8 +414,9 @@ static bool unit_sampling(int ndim, const double* s) {
   return true;
 }
 
-static bool use_band_path(int ndim, const int64_t* shape, const double* s) {
-  return ndim == 2 && unit_sampling(ndim, s) && shape[1] <= 1024 && shape[0] <= 16384;
+static bool use_2d_path(int ndim, const int64_t* shape, const double* s) {
+  return ndim == 2 && unit_sampling(ndim, s) && shape[0] <= kColSegs * kColChunks * 32 &&
+         shape[1] <= 4096;
 }
 
 template <typename TIn>
@@ -381,12 +428,26 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   for (int j = 0; j < kMaxNdim; ++j) sp.s[j] = (sampling && j < ndim) ? sampling[j] : 1.0;
   o.plane = batch * sh.numel;
 
-  if (use_band_path(ndim, shape, sampling)) {
+  if (use_2d_path(ndim, shape, sampling)) {
     const int H = (int)shape[0], W = (int)shape[1];
-    constexpr int R = 8;
-    dim3 grid((H + R - 1) / R, (unsigned)batch);
-    size_t lds = (size_t)R * W * 2 * sizeof(int32_t);
-    hipLaunchKernelGGL((edt2d_band_kernel<TIn, R>), grid, dim3(256), lds, st, in, H, W, o);
+    const size_t need = (size_t)o.plane * sizeof(int16_t);
+    if (!ws || ws_bytes < need) {
+      set_error("tm_edt: workspace of %zu bytes is smaller than the %zu required", ws_bytes, need);
+      return TM_ERR_WORKSPACE;
+    }
+    int16_t* f0 = (int16_t*)ws;
+    hipLaunchKernelGGL((edt2d_cols_kernel<TIn>), dim3((W + 63) / 64, (unsigned)batch), dim3(256),
+                       0, st, in, H, W, f0);
+    TM_LAUNCH_CHECK();
+    if (W <= 1024) {
+      constexpr int R = 8;
+      hipLaunchKernelGGL((edt2d_rows_kernel<R>), dim3((H + R - 1) / R, (unsigned)batch),
+                         dim3(256), (size_t)R * W * 6, st, f0, H, W, o);
+    } else {
+      constexpr int R = 2;
+      hipLaunchKernelGGL((edt2d_rows_kernel<R>), dim3((H + R - 1) / R, (unsigned)batch),
+                         dim3(256), (size_t)R * W * 6, st, f0, H, W, o);
+    }
     TM_LAUNCH_CHECK();
     return TM_OK;
   }
@@ -445,9 +506,9 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
 extern "C" size_t tm_edt_workspace_bytes(int64_t batch, int ndim, const int64_t* shape,
                                          const double* sampling) {
   if (ndim < 1 || ndim > tmh::kMaxNdim || batch < 1) return 0;
-  if (tmh::use_band_path(ndim, shape, sampling)) return 0;
   int64_t n = batch;
   for (int j = 0; j < ndim; ++j) n *= shape[j];
+  if (tmh::use_2d_path(ndim, shape, sampling)) return (size_t)n * sizeof(int16_t);
   const int nbuf = ndim > 1 ? ndim - 1 : 1;
   return (size_t)n * 4 * (2 * nbuf + 1) + 256 + (size_t)n * 8;
 }
