@@ -13,8 +13,31 @@ from .distance_transform import (
     distance_transform_edt,
     euclidean_distance_transform,
 )
+from .morphology import (
+    binary_closing,
+    binary_dilation,
+    binary_erosion,
+    binary_fill_holes,
+    binary_hit_or_miss,
+    binary_opening,
+    binary_propagation,
+    black_tophat,
+    generate_binary_structure,
+    grey_closing,
+    grey_dilation,
+    grey_erosion,
+    grey_opening,
+    iterate_structure,
+    morphological_gradient,
+    morphological_laplace,
+    white_tophat,
+)
 
 __all__ = [
+    "generate_binary_structure", "iterate_structure", "binary_erosion", "binary_dilation",
+    "binary_fill_holes", "binary_hit_or_miss", "binary_opening", "binary_closing",
+    "binary_propagation", "grey_erosion", "grey_dilation", "grey_opening", "grey_closing",
+    "morphological_gradient", "morphological_laplace", "white_tophat", "black_tophat",
     "euclidean_distance_transform",
     "chamfer_distance_transform",
     "brute_force_distance_transform",

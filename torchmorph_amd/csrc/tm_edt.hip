@@ -12,7 +12,7 @@
 // feature deltas in axis order and rounded once to float32.
 //
 // Kernels:
-//  * edt2d_cols_kernel / edt2d_rows_kernel -- 2-D, unit sampling, H <= 2048,
+//  * edt2d_cols_kernel / edt2d_rows_kernel -- 2-D, unit sampling, H <= 1024,
 //    W <= 4096.  Axis 0: nearest background row per pixel from bit-packed
 //    column segments (one load round), int16 into the workspace.  Axis 1:
 //    band of rows staged in LDS, exact expanding-window argmin per pixel with
@@ -65,17 +65,39 @@ __device__ __forceinline__ void edt_store(const EdtOut& o, int ndim, int64_t g, 
 
 // ------------------------------------------------------------- 2-D path --
 // Axis-0 pass.  A workgroup covers 64 columns of one image with 16 column
-// groups (4 adjacent columns each) x 16 row segments.  Every thread loads its
-// 4 x seglen mask bytes in one round, keeps them as background bit masks, and
-// the segments exchange their first/last background rows through LDS, so the
-// kernel has a single dependent memory round trip.  Output: nearest
-// background row per pixel (int16, -1 = none), lower row on ties.
+// groups (4 adjacent columns each) x 16 row segments.  Every thread issues all
+// of its mask loads up front (one 4-column word per row, NCH chunks of 16
+// rows), keeps them as background bit masks, and the segments exchange their
+// first/last background rows through LDS: the kernel has one dependent
+// memory round trip.  Output: nearest background row per pixel (int16,
+// -1 = none), the lower row on ties.
 constexpr int kColSegs = 16;
-constexpr int kColChunks = 4;  // 32-row chunks per segment: H <= 16*4*32 = 2048
+constexpr int kChunkRows = 16;
 
 template <typename TIn>
+__device__ __forceinline__ uint32_t bg4(const TIn* __restrict__ p, int x, int W, bool vec) {
+  // background bits of columns x..x+3 of one row
+  if (vec) {
+    if constexpr (sizeof(TIn) == 1) {
+      const uint32_t v = *(const uint32_t*)(p + x);
+      return (uint32_t)((v & 0xffu) == 0) | ((uint32_t)((v & 0xff00u) == 0) << 1) |
+             ((uint32_t)((v & 0xff0000u) == 0) << 2) | ((uint32_t)((v & 0xff000000u) == 0) << 3);
+    } else {
+      const float4 v = *(const float4*)(p + x);
+      return (uint32_t)(v.x == 0.0f) | ((uint32_t)(v.y == 0.0f) << 1) |
+             ((uint32_t)(v.z == 0.0f) << 2) | ((uint32_t)(v.w == 0.0f) << 3);
+    }
+  }
+  uint32_t r = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    if (x + c < W && p[x + c] == TIn(0)) r |= 1u << c;
+  return r;
+}
+
+template <typename TIn, int NCH>
 __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__ in, int H, int W,
-                                                         int16_t* __restrict__ f0) {
+                                                         int16_t* __restrict__ f0, int aligned) {
   __shared__ int16_t sfirst[kColSegs][64];
   __shared__ int16_t slast[kColSegs][64];
   const int cg = threadIdx.x & 15;
@@ -86,81 +108,113 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
   const int ys = seg * seglen;
   const int ye = min(H, ys + seglen);
   const TIn* im = in + (int64_t)img * H * W;
+  const bool vec = aligned && (W % 4 == 0) && (xb + 4 <= W);
+  const bool live = xb < W && ys < ye;
 
-  uint32_t m[4][kColChunks];
+  // m[c][k]: background bits of column xb+c, rows ys+16k .. ys+16k+15
+  uint32_t raw[NCH][kChunkRows];
+  if (live) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+    for (int k = 0; k < NCH; ++k)
 #pragma unroll
-    for (int k = 0; k < kColChunks; ++k) m[c][k] = 0;
-#pragma unroll
-  for (int k = 0; k < kColChunks; ++k) {
-#pragma unroll 8
-    for (int b = 0; b < 32; ++b) {
-      const int y = ys + k * 32 + b;
-      if (y < ye) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int x = xb + c;
-          if (x < W && !is_fg(im, (int64_t)y * W + x)) m[c][k] |= 1u << b;
-        }
+      for (int b = 0; b < kChunkRows; ++b) {
+        const int y = min(ys + k * kChunkRows + b, ye - 1);
+        raw[k][b] = bg4(im + (int64_t)y * W, xb, W, vec);
       }
+  }
+  uint32_t m[4][NCH];
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) {
+    uint32_t mk[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < kChunkRows; ++b) {
+      const bool rowok = live && (ys + k * kChunkRows + b < ye);
+      const uint32_t v = rowok ? raw[k][b] : 0u;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) mk[c] |= ((v >> c) & 1u) << b;
     }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) m[c][k] = mk[c];
   }
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     int first = -1, last = -1;
 #pragma unroll
-    for (int k = kColChunks - 1; k >= 0; --k)
-      if (m[c][k]) first = ys + k * 32 + __builtin_ctz(m[c][k]);
+    for (int k = NCH - 1; k >= 0; --k)
+      if (m[c][k]) first = ys + k * kChunkRows + __builtin_ctz(m[c][k]);
 #pragma unroll
-    for (int k = 0; k < kColChunks; ++k)
-      if (m[c][k]) last = ys + k * 32 + 31 - __builtin_clz(m[c][k]);
+    for (int k = 0; k < NCH; ++k)
+      if (m[c][k]) last = ys + k * kChunkRows + 31 - __builtin_clz(m[c][k]);
     sfirst[seg][cg * 4 + c] = (int16_t)first;
     slast[seg][cg * 4 + c] = (int16_t)last;
   }
   __syncthreads();
+  if (!live) return;
+  int before[4], after[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int x = xb + c;
-    if (x >= W) continue;
-    int before = -1, after = -1;
-    for (int s = seg - 1; s >= 0 && before < 0; --s) before = slast[s][cg * 4 + c];
-    for (int s = seg + 1; s < kColSegs && after < 0; ++s) after = sfirst[s][cg * 4 + c];
-    for (int y = ys; y < ye; ++y) {
-      const int k = (y - ys) >> 5, b = (y - ys) & 31;
-      // nearest background at or above y
-      int pv = -1;
-      uint32_t mm = m[c][k] & (0xffffffffu >> (31 - b));
-      if (mm) {
-        pv = ys + k * 32 + 31 - __builtin_clz(mm);
-      } else {
-        for (int kk = k - 1; kk >= 0 && pv < 0; --kk)
-          if (m[c][kk]) pv = ys + kk * 32 + 31 - __builtin_clz(m[c][kk]);
-        if (pv < 0) pv = before;
+    before[c] = -1;
+    after[c] = -1;
+    for (int s = seg - 1; s >= 0 && before[c] < 0; --s) before[c] = slast[s][cg * 4 + c];
+    for (int s = seg + 1; s < kColSegs && after[c] < 0; ++s) after[c] = sfirst[s][cg * 4 + c];
+  }
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) {
+#pragma unroll
+    for (int b = 0; b < kChunkRows; ++b) {
+      const int y = ys + k * kChunkRows + b;
+      if (y >= ye) break;
+      int16_t out[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        // nearest background at or above y
+        int pv = -1;
+        uint32_t mm = m[c][k] & (0xffffffffu >> (31 - b));
+        if (mm) {
+          pv = ys + k * kChunkRows + 31 - __builtin_clz(mm);
+        } else {
+#pragma unroll
+          for (int kk = k - 1; kk >= 0; --kk)
+            if (pv < 0 && m[c][kk]) pv = ys + kk * kChunkRows + 31 - __builtin_clz(m[c][kk]);
+          if (pv < 0) pv = before[c];
+        }
+        // nearest background at or below y
+        int nx = -1;
+        mm = m[c][k] >> b;
+        if (mm) {
+          nx = y + __builtin_ctz(mm);
+        } else {
+#pragma unroll
+          for (int kk = k + 1; kk < NCH; ++kk)
+            if (nx < 0 && m[c][kk]) nx = ys + kk * kChunkRows + __builtin_ctz(m[c][kk]);
+          if (nx < 0) nx = after[c];
+        }
+        int f = -1;
+        if (pv >= 0 && (nx < 0 || (y - pv) <= (nx - y))) f = pv;
+        else if (nx >= 0) f = nx;
+        out[c] = (int16_t)f;
       }
-      // nearest background at or below y
-      int nx = -1;
-      mm = m[c][k] >> b;
-      if (mm) {
-        nx = y + __builtin_ctz(mm);
+      int16_t* dst = f0 + ((int64_t)img * H + y) * W + xb;
+      if (vec) {
+        uint2 pk;
+        pk.x = (uint32_t)(uint16_t)out[0] | ((uint32_t)(uint16_t)out[1] << 16);
+        pk.y = (uint32_t)(uint16_t)out[2] | ((uint32_t)(uint16_t)out[3] << 16);
+        *(uint2*)dst = pk;
       } else {
-        for (int kk = k + 1; kk < kColChunks && nx < 0; ++kk)
-          if (m[c][kk]) nx = ys + kk * 32 + __builtin_ctz(m[c][kk]);
-        if (nx < 0) nx = after;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (xb + c < W) dst[c] = out[c];
       }
-      int f = -1;
-      if (pv >= 0 && (nx < 0 || (y - pv) <= (nx - y))) f = pv;
-      else if (nx >= 0) f = nx;
-      f0[((int64_t)img * H + y) * W + x] = (int16_t)f;
     }
   }
 }
 
-// Axis-1 pass.  One workgroup per band of R rows: the band's squared column
-// distances and feature rows are staged in LDS in one load round, then every
-// pixel runs an exact expanding-window argmin over its row (the lower column
-// wins ties).  With unit sampling all quantities are exact integers, so this
-// equals SciPy's double-precision envelope result bit for bit.
+// Axis-1 pass.  One workgroup per band of R rows: the band's column features
+// are loaded in one round (8 int16 per thread per round), turned into squared
+// column distances in LDS, then every pixel runs an exact expanding-window
+// argmin over its row (the lower column wins ties).  With unit sampling all
+// quantities are exact integers, so this equals SciPy's double-precision
+// envelope result bit for bit.
 template <int R>
 __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restrict__ f0, int H,
                                                          int W, EdtOut o) {
@@ -172,12 +226,31 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
   const int rows = min(R, H - y0);
   const int npx = rows * W;
   const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
-  for (int p = threadIdx.x; p < npx; p += blockDim.x) {
-    const int r = p / W;
-    const int f = src[p];
-    const int dy = y0 + r - f;
-    fr[p] = (int16_t)f;
-    hs[p] = f >= 0 ? dy * dy : -1;
+  const bool vec = (W % 8 == 0);
+  for (int base = 0; base < npx; base += 256 * 8) {
+    const int p0 = base + threadIdx.x * 8;
+    int16_t v[8];
+    if (vec && p0 + 8 <= npx) {
+      const uint4 q = *(const uint4*)(src + p0);
+      v[0] = (int16_t)(q.x & 0xffff); v[1] = (int16_t)(q.x >> 16);
+      v[2] = (int16_t)(q.y & 0xffff); v[3] = (int16_t)(q.y >> 16);
+      v[4] = (int16_t)(q.z & 0xffff); v[5] = (int16_t)(q.z >> 16);
+      v[6] = (int16_t)(q.w & 0xffff); v[7] = (int16_t)(q.w >> 16);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = p0 + i < npx ? src[p0 + i] : (int16_t)-1;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int p = p0 + i;
+      if (p < npx) {
+        const int r = p / W;
+        const int f = v[i];
+        const int dy = y0 + r - f;
+        fr[p] = (int16_t)f;
+        hs[p] = f >= 0 ? dy * dy : -1;
+      }
+    }
   }
   __syncthreads();
   for (int p = threadIdx.x; p < npx; p += blockDim.x) {
@@ -415,7 +488,7 @@ static bool unit_sampling(int ndim, const double* s) {
 }
 
 static bool use_2d_path(int ndim, const int64_t* shape, const double* s) {
-  return ndim == 2 && unit_sampling(ndim, s) && shape[0] <= kColSegs * kColChunks * 32 &&
+  return ndim == 2 && unit_sampling(ndim, s) && shape[0] <= kColSegs * 4 * kChunkRows &&
          shape[1] <= 4096;
 }
 
@@ -436,8 +509,22 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
       return TM_ERR_WORKSPACE;
     }
     int16_t* f0 = (int16_t*)ws;
-    hipLaunchKernelGGL((edt2d_cols_kernel<TIn>), dim3((W + 63) / 64, (unsigned)batch), dim3(256),
-                       0, st, in, H, W, f0);
+    const dim3 cgrid((W + 63) / 64, (unsigned)batch);
+    const int seglen = (H + kColSegs - 1) / kColSegs;
+    const int aligned = ((uintptr_t)in % 16) == 0;
+    if (((uintptr_t)ws % 16) != 0) {
+      set_error("tm_edt: workspace must be 16-byte aligned");
+      return TM_ERR_ARG;
+    }
+    if (seglen <= 16)
+      hipLaunchKernelGGL((edt2d_cols_kernel<TIn, 1>), cgrid, dim3(256), 0, st, in, H, W, f0,
+                         aligned);
+    else if (seglen <= 32)
+      hipLaunchKernelGGL((edt2d_cols_kernel<TIn, 2>), cgrid, dim3(256), 0, st, in, H, W, f0,
+                         aligned);
+    else
+      hipLaunchKernelGGL((edt2d_cols_kernel<TIn, 4>), cgrid, dim3(256), 0, st, in, H, W, f0,
+                         aligned);
     TM_LAUNCH_CHECK();
     if (W <= 1024) {
       constexpr int R = 8;

@@ -1,0 +1,441 @@
+// Grey-scale erosion / dilation for gfx950.
+//
+// Semantics follow the reference gather formulation (grey_kernel.cu:101-170,
+// offsets grey_kernel.cu:247-292, border modes grey_kernel.cu:63-97):
+//   erosion   out[p] = min_i ( E(p + d_i) - s_i )
+//   dilation  out[p] = max_i ( E(p - d_i) + s_i ),   d_i = coord_i - (size/2 + origin)
+// over the active elements i of the structuring element, where E extends the
+// input by the border mode (constant -> cval).  min/max are exact, so every
+// evaluation order below returns the same bits as the reference (and SciPy).
+//
+// Kernels (float32 in/out, items of up to 3 spatial dims padded to 3-D):
+//  * grey_box3_kernel   flat box SE.  2.5-D streaming: a workgroup owns a
+//    64 x 16 (x, y) output tile and a chunk of z planes; each input plane's
+//    halo tile is staged in LDS once (coalesced along x), reduced along x then
+//    y in LDS, and pushed into a ring of kz reduced planes; every output plane
+//    is the min/max over the ring.  HBM traffic is one read + one write per
+//    voxel plus halo re-reads that mostly hit L2.
+//  * grey_table3_kernel non-flat or non-box SE (<= 256 active elements).
+//    Same streaming, but the ring holds raw input halo planes and each output
+//    folds the element table (offset, value) held in kernel arguments.
+//  * grey_axis_kernel   flat box fallback for any ndim: one 1-D window pass
+//    per axis (ping-pong through the workspace).
+//  * grey_gather_kernel general fallback for any ndim and any SE: one thread
+//    per output folds the element table from device memory.
+#include "tm_common.h"
+
+namespace tmh {
+
+__device__ __forceinline__ int64_t gmap(int64_t c, int64_t size, int mode, bool* use_cval) {
+  *use_cval = false;
+  if (c >= 0 && c < size) return c;
+  switch (mode) {
+    case 0: *use_cval = true; return 0;  // constant
+    case 2: return c < 0 ? 0 : size - 1;  // nearest
+    case 1: {                               // reflect (half-sample symmetric)
+      const int64_t period = 2 * size;
+      int64_t m = ((c % period) + period) % period;
+      return m >= size ? period - 1 - m : m;
+    }
+    case 3: {  // mirror (whole-sample symmetric)
+      const int64_t period = 2 * (size - 1);
+      if (period <= 0) return 0;
+      int64_t m = ((c % period) + period) % period;
+      return m >= size ? period - m : m;
+    }
+    case 4: return ((c % size) + size) % size;  // wrap
+    default: *use_cval = true; return 0;
+  }
+}
+
+template <bool MAX>
+__device__ __forceinline__ float gop(float a, float b) {
+  return MAX ? fmaxf(a, b) : fminf(a, b);
+}
+
+template <bool MAX>
+__device__ __forceinline__ float gid() {
+  return MAX ? -INFINITY : INFINITY;
+}
+
+constexpr int kTX = 64, kTY = 16;
+
+// Loads one mode-extended input plane halo tile [SY][SX] starting at (y0, x0).
+// `bias` is added to every staged value: +0.0f for flat dilation (the
+// reference adds its zero structure, which turns -0.0 into +0.0), -0.0f
+// otherwise (an exact identity).
+__device__ __forceinline__ void load_plane(const float* __restrict__ vol, float* stage, int zi,
+                                           int D, int H, int W, int y0, int x0, int SY, int SX,
+                                           int mode, float cval, float bias) {
+  bool zc;
+  const int64_t zm = gmap(zi, D, mode, &zc);
+  for (int e = threadIdx.x; e < SY * SX; e += blockDim.x) {
+    const int r = e / SX, c = e - r * SX;
+    bool yc, xc;
+    const int64_t ym = gmap(y0 + r, H, mode, &yc);
+    const int64_t xm = gmap(x0 + c, W, mode, &xc);
+    stage[e] = ((zc || yc || xc) ? cval : vol[(zm * H + ym) * W + xm]) + bias;
+  }
+}
+
+template <bool MAX>
+__global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict__ in,
+                                                        float* __restrict__ out, int D, int H,
+                                                        int W, int kz, int ky, int kx, int sz,
+                                                        int sy, int sx, int mode, float cval,
+                                                        int zchunk) {
+  extern __shared__ float sm[];
+  const int SX = kTX + kx - 1, SY = kTY + ky - 1;
+  float* stage = sm;
+  float* xr = stage + SY * SX;
+  float* ring = xr + SY * kTX;
+  const int ntx = (W + kTX - 1) / kTX;
+  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int x0 = tx * kTX, y0 = ty * kTY;
+  const int zb = blockIdx.y * zchunk;
+  const int ze = min(D, zb + zchunk);
+  const int64_t item = blockIdx.z;
+  const float* vol = in + item * D * H * W;
+  float* ovol = out + item * D * H * W;
+  const float ID = gid<MAX>();
+  const int nplanes = (ze - zb) + kz - 1;
+  for (int n = 0; n < nplanes; ++n) {
+    const int zi = zb + sz + n;
+    load_plane(vol, stage, zi, D, H, W, y0 + sy, x0 + sx, SY, SX, mode, cval,
+               MAX ? 0.0f : -0.0f);
+    __syncthreads();
+    for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
+      const int r = e / kTX, c = e - r * kTX;
+      const float* s = stage + r * SX + c;
+      float acc = ID;
+      for (int l = 0; l < kx; ++l) acc = gop<MAX>(acc, s[l]);
+      xr[e] = acc;
+    }
+    __syncthreads();
+    float* slot = ring + (n % kz) * (kTY * kTX);
+    for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
+      const int r = e / kTX, c = e - r * kTX;
+      float acc = ID;
+      for (int j = 0; j < ky; ++j) acc = gop<MAX>(acc, xr[(r + j) * kTX + c]);
+      slot[e] = acc;
+    }
+    __syncthreads();
+    const int z = zb + n - (kz - 1);
+    if (z >= zb) {
+      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
+        const int r = e / kTX, c = e - r * kTX;
+        const int y = y0 + r, x = x0 + c;
+        if (y < H && x < W) {
+          float acc = ID;
+          for (int i = 0; i < kz; ++i) acc = gop<MAX>(acc, ring[i * (kTY * kTX) + e]);
+          ovol[((int64_t)z * H + y) * W + x] = acc;
+        }
+      }
+    }
+  }
+}
+
+constexpr int kMaxTable = 256;
+struct GreyTable {
+  int n;
+  int8_t dz[kMaxTable], dy[kMaxTable], dx[kMaxTable];
+  float v[kMaxTable];
+};
+
+// Ring of raw input halo planes; zlo..zhi, ylo..yhi, xlo..xhi bound the table
+// offsets.
+template <bool MAX>
+__global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restrict__ in,
+                                                          float* __restrict__ out, int D, int H,
+                                                          int W, GreyTable tab, int zlo, int zhi,
+                                                          int ylo, int yhi, int xlo, int xhi,
+                                                          int mode, float cval, int zchunk) {
+  extern __shared__ float sm[];
+  const int kz = zhi - zlo + 1;
+  const int SX = kTX + xhi - xlo, SY = kTY + yhi - ylo;
+  const int plane = SY * SX;
+  const int ntx = (W + kTX - 1) / kTX;
+  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int x0 = tx * kTX, y0 = ty * kTY;
+  const int zb = blockIdx.y * zchunk;
+  const int ze = min(D, zb + zchunk);
+  const int64_t item = blockIdx.z;
+  const float* vol = in + item * D * H * W;
+  float* ovol = out + item * D * H * W;
+  const float ID = gid<MAX>();
+  const int nplanes = (ze - zb) + kz - 1;
+  for (int n = 0; n < nplanes; ++n) {
+    const int zi = zb + zlo + n;
+    float* slot = sm + (n % kz) * plane;
+    load_plane(vol, slot, zi, D, H, W, y0 + ylo, x0 + xlo, SY, SX, mode, cval, -0.0f);
+    __syncthreads();
+    const int z = zb + n - (kz - 1);
+    if (z >= zb) {
+      // plane of offset dz sits at ring index (n - (zhi - dz)) % kz
+      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
+        const int r = e / kTX, c = e - r * kTX;
+        const int y = y0 + r, x = x0 + c;
+        float acc = ID;
+        for (int t = 0; t < tab.n; ++t) {
+          const int ridx = (n - (zhi - tab.dz[t])) % kz;
+          const float val =
+              sm[ridx * plane + (r + tab.dy[t] - ylo) * SX + (c + tab.dx[t] - xlo)];
+          acc = gop<MAX>(acc, val + tab.v[t]);
+        }
+        if (y < H && x < W) ovol[((int64_t)z * H + y) * W + x] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One 1-D window pass along `axis` (flat SE): out = op over in[p + (start+l)e_axis].
+template <bool MAX>
+__global__ __launch_bounds__(256) void grey_axis_kernel(const float* __restrict__ in,
+                                                        float* __restrict__ out, Shape sh,
+                                                        int64_t total, int axis, int k, int start,
+                                                        int mode, float cval) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const int64_t stride = sh.stride[axis], size = sh.size[axis];
+  const int64_t c = (g / stride) % size;
+  const int64_t base = g - c * stride;
+  float acc = gid<MAX>();
+  for (int l = 0; l < k; ++l) {
+    bool uc;
+    const int64_t m = gmap(c + start + l, size, mode, &uc);
+    acc = gop<MAX>(acc, (uc ? cval : in[base + m * stride]) + (MAX ? 0.0f : -0.0f));
+  }
+  out[g] = acc;
+}
+
+// General fallback: table rows are (ndim offsets, value) in device memory.
+template <bool MAX>
+__global__ __launch_bounds__(256) void grey_gather_kernel(const float* __restrict__ in,
+                                                          float* __restrict__ out, Shape sh,
+                                                          int64_t total,
+                                                          const int32_t* __restrict__ toff,
+                                                          const float* __restrict__ tval, int n,
+                                                          int mode, float cval) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const int nd = sh.ndim;
+  const int64_t item = g / sh.numel;
+  int64_t rem = g - item * sh.numel;
+  int64_t c[kMaxNdim];
+  for (int j = 0; j < nd; ++j) {
+    c[j] = rem / sh.stride[j];
+    rem -= c[j] * sh.stride[j];
+  }
+  const float* vol = in + item * sh.numel;
+  float acc = gid<MAX>();
+  for (int t = 0; t < n; ++t) {
+    bool use_c = false;
+    int64_t flat = 0;
+    for (int j = 0; j < nd && !use_c; ++j) {
+      bool uc;
+      const int64_t m = gmap(c[j] + toff[t * nd + j], sh.size[j], mode, &uc);
+      use_c |= uc;
+      flat += m * sh.stride[j];
+    }
+    acc = gop<MAX>(acc, (use_c ? cval : vol[flat]) + tval[t]);
+  }
+  out[g] = acc;
+}
+
+static size_t box3_lds(int kz, int ky, int kx) {
+  const size_t SX = kTX + kx - 1, SY = kTY + ky - 1;
+  return (SY * SX + SY * kTX + (size_t)kz * kTY * kTX) * sizeof(float);
+}
+
+static size_t table3_lds(int kz, int ky, int kx) {
+  return (size_t)kz * (kTY + ky - 1) * (kTX + kx - 1) * sizeof(float);
+}
+
+constexpr size_t kLdsBudget = 120 * 1024;
+
+struct GreyPlan {
+  int nd;
+  int64_t shape3[3];
+  bool box;                 // flat and every element active
+  int64_t ksize[kMaxNdim];  // SE shape
+  int64_t start[kMaxNdim];  // window start per axis (box)
+  int n;                    // active elements
+  // table: per element offsets (ndim) and value
+  int32_t* off;
+  float* val;
+};
+
+}  // namespace tmh
+
+extern "C" size_t tm_grey_workspace_bytes(int64_t batch, int ndim, const int64_t* shape,
+                                          const int64_t* se_shape) {
+  if (ndim < 1 || ndim > tmh::kMaxNdim || batch < 1) return 0;
+  int64_t n = batch, ne = 1;
+  for (int j = 0; j < ndim; ++j) {
+    n *= shape[j];
+    ne *= se_shape[j];
+  }
+  // ping-pong volume for the per-axis fallback + an element table
+  return (size_t)n * sizeof(float) + 256 + (size_t)ne * (ndim + 1) * sizeof(int32_t);
+}
+
+extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_t* shape,
+                       const float* structure, const uint8_t* footprint, const int64_t* se_shape,
+                       const int64_t* origin, int mode, float cval, int dilation, float* output,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(input && output, "tm_grey: input/output is NULL");
+  TM_CHECK_ARG(input != output, "tm_grey: output must not alias input");
+  TM_CHECK_ARG(ndim >= 1 && ndim <= kMaxNdim, "tm_grey: ndim must be in 1..8, got %d", ndim);
+  TM_CHECK_ARG(mode >= 0 && mode <= 4, "tm_grey: mode must be in 0..4");
+  TM_CHECK_ARG(batch >= 1, "tm_grey: batch must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  Shape sh = make_shape(ndim, shape);
+  const int64_t total = batch * sh.numel;
+  int64_t ne = 1;
+  for (int j = 0; j < ndim; ++j) {
+    TM_CHECK_ARG(se_shape[j] >= 1, "tm_grey: structuring element axis %d is empty", j);
+    TM_CHECK_ARG(shape[j] >= 1, "tm_grey: input axis %d is empty", j);
+    ne *= se_shape[j];
+  }
+  // host-side element table, in the reference's element order
+  bool flat = true, full = true;
+  int n = 0;
+  std::string tbl_err;
+  static thread_local int32_t* hoff = nullptr;
+  static thread_local float* hval = nullptr;
+  static thread_local int64_t hcap = 0;
+  if (hcap < ne) {
+    free(hoff);
+    free(hval);
+    hoff = (int32_t*)malloc(sizeof(int32_t) * ne * ndim);
+    hval = (float*)malloc(sizeof(float) * ne);
+    hcap = ne;
+  }
+  int64_t lo[kMaxNdim], hi[kMaxNdim];
+  for (int j = 0; j < ndim; ++j) {
+    lo[j] = INT64_MAX;
+    hi[j] = INT64_MIN;
+  }
+  for (int64_t e = 0; e < ne; ++e) {
+    if (footprint && !footprint[e]) {
+      full = false;
+      continue;
+    }
+    const float s = structure ? structure[e] : 0.0f;
+    if (s != 0.0f || (structure && std::signbit(s))) flat = false;
+    int64_t rem = e;
+    int64_t r[kMaxNdim];
+    for (int j = ndim - 1; j >= 0; --j) {
+      r[j] = rem % se_shape[j];
+      rem /= se_shape[j];
+    }
+    for (int j = 0; j < ndim; ++j) {
+      int64_t d = r[j] - (se_shape[j] / 2 + origin[j]);
+      if (dilation) d = -d;
+      hoff[n * ndim + j] = (int32_t)d;
+      lo[j] = d < lo[j] ? d : lo[j];
+      hi[j] = d > hi[j] ? d : hi[j];
+    }
+    hval[n] = dilation ? s : -s;
+    ++n;
+  }
+  TM_CHECK_ARG(n > 0, "tm_grey: footprint must contain at least one active position");
+
+  const bool is_max = dilation != 0;
+  if (ndim <= 3) {
+    int64_t s3[3] = {1, 1, 1};
+    int64_t lo3[3] = {0, 0, 0}, hi3[3] = {0, 0, 0};
+    for (int j = 0; j < ndim; ++j) {
+      s3[3 - ndim + j] = shape[j];
+      lo3[3 - ndim + j] = lo[j];
+      hi3[3 - ndim + j] = hi[j];
+    }
+    const int D = (int)s3[0], H = (int)s3[1], W = (int)s3[2];
+    const int kz = (int)(hi3[0] - lo3[0] + 1), ky = (int)(hi3[1] - lo3[1] + 1),
+              kx = (int)(hi3[2] - lo3[2] + 1);
+    const int zchunk = D <= 32 ? D : 32;
+    const unsigned tiles = (unsigned)(((W + kTX - 1) / kTX) * ((H + kTY - 1) / kTY));
+    const dim3 grid(tiles, (unsigned)((D + zchunk - 1) / zchunk), (unsigned)batch);
+    if (flat && full && box3_lds(kz, ky, kx) <= kLdsBudget) {
+      const size_t lds = box3_lds(kz, ky, kx);
+      if (is_max)
+        hipLaunchKernelGGL((grey_box3_kernel<true>), grid, dim3(256), lds, st, input, output, D,
+                           H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1], (int)lo3[2], mode, cval,
+                           zchunk);
+      else
+        hipLaunchKernelGGL((grey_box3_kernel<false>), grid, dim3(256), lds, st, input, output, D,
+                           H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1], (int)lo3[2], mode, cval,
+                           zchunk);
+      TM_LAUNCH_CHECK();
+      return TM_OK;
+    }
+    bool small = n <= kMaxTable;
+    for (int j = 0; j < 3; ++j) small = small && lo3[j] >= -128 && hi3[j] <= 127;
+    if (small && table3_lds(kz, ky, kx) <= kLdsBudget) {
+      GreyTable tab{};
+      tab.n = n;
+      for (int t = 0; t < n; ++t) {
+        int32_t o3[3] = {0, 0, 0};
+        for (int j = 0; j < ndim; ++j) o3[3 - ndim + j] = hoff[t * ndim + j];
+        tab.dz[t] = (int8_t)o3[0];
+        tab.dy[t] = (int8_t)o3[1];
+        tab.dx[t] = (int8_t)o3[2];
+        tab.v[t] = hval[t];
+      }
+      const size_t lds = table3_lds(kz, ky, kx);
+      if (is_max)
+        hipLaunchKernelGGL((grey_table3_kernel<true>), grid, dim3(256), lds, st, input, output, D,
+                           H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1], (int)hi3[1],
+                           (int)lo3[2], (int)hi3[2], mode, cval, zchunk);
+      else
+        hipLaunchKernelGGL((grey_table3_kernel<false>), grid, dim3(256), lds, st, input, output,
+                           D, H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1], (int)hi3[1],
+                           (int)lo3[2], (int)hi3[2], mode, cval, zchunk);
+      TM_LAUNCH_CHECK();
+      return TM_OK;
+    }
+  }
+  const size_t need = tm_grey_workspace_bytes(batch, ndim, shape, se_shape);
+  if (!workspace || workspace_bytes < need) {
+    set_error("tm_grey: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  if (flat && full) {
+    // separable passes: axis 0 .. ndim-1, alternating workspace / output so
+    // that the last pass lands in output
+    float* tmp = (float*)workspace;
+    const float* src = input;
+    for (int j = 0; j < ndim; ++j) {
+      float* dst = ((ndim - 1 - j) % 2 == 0) ? output : tmp;
+      const int start = (int)lo[j];
+      const int k = (int)(hi[j] - lo[j] + 1);
+      if (is_max)
+        hipLaunchKernelGGL((grey_axis_kernel<true>), dim3(grid_for(total, 256)), dim3(256), 0, st,
+                           src, dst, sh, total, j, k, start, mode, cval);
+      else
+        hipLaunchKernelGGL((grey_axis_kernel<false>), dim3(grid_for(total, 256)), dim3(256), 0, st,
+                           src, dst, sh, total, j, k, start, mode, cval);
+      TM_LAUNCH_CHECK();
+      src = dst;
+    }
+    return TM_OK;
+  }
+  int32_t* doff = (int32_t*)(((uintptr_t)workspace + (size_t)total * sizeof(float) + 255) &
+                             ~(uintptr_t)255);
+  float* dval = (float*)(doff + (size_t)n * ndim);
+  TM_CHECK_HIP(hipMemcpyAsync(doff, hoff, sizeof(int32_t) * n * ndim, hipMemcpyHostToDevice, st));
+  TM_CHECK_HIP(hipMemcpyAsync(dval, hval, sizeof(float) * n, hipMemcpyHostToDevice, st));
+  if (is_max)
+    hipLaunchKernelGGL((grey_gather_kernel<true>), dim3(grid_for(total, 256)), dim3(256), 0, st,
+                       input, output, sh, total, doff, dval, n, mode, cval);
+  else
+    hipLaunchKernelGGL((grey_gather_kernel<false>), dim3(grid_for(total, 256)), dim3(256), 0, st,
+                       input, output, sh, total, doff, dval, n, mode, cval);
+  TM_LAUNCH_CHECK();
+  // the pageable copies above are staged synchronously, so the host table
+  // may be reused once this call returns
+  return TM_OK;
+}

@@ -1,0 +1,199 @@
+"""Grey-scale morphology on (B, C, spatial...) GPU tensors.
+
+Drop-in for reference torchmorph/morphology/grey.py:29-289 (same names,
+arguments, structuring-element resolution, border modes and messages).  The
+kernels read float32 input directly (other dtypes are converted, like the
+reference's ``.float()``), write float32 and are bit-identical to SciPy's
+ndimage on every border mode.
+
+``differentiable=True`` (an opt-in the reference does not have) routes
+erosion/dilation through hand-written backward kernels; the default keeps the
+reference's non-differentiable behaviour (reference test_grey.py:218-222).
+"""
+from __future__ import annotations
+
+import torch
+from torch import Tensor
+
+from .. import _lib
+from .._validation import validate_bcs_input, validate_output
+from .structure import _normalize_origin, _validate_origin
+
+_MODE_MAP = {"constant": 0, "reflect": 1, "nearest": 2, "mirror": 3, "wrap": 4}
+
+
+def _element_kwargs(size, footprint, structure, mode, cval, origin) -> dict:
+    return dict(size=size, footprint=footprint, structure=structure, mode=mode, cval=cval,
+                origin=origin)
+
+
+def _resolve_element(input: Tensor, size, footprint, structure, mode, origin):
+    """Reference grey.py:42-89: structure > footprint > size, flat when only a
+    size or footprint is given; returns CPU (structure f32, footprint u8 or
+    None, origin tuple, mode int)."""
+    spatial_ndim = validate_bcs_input(input)
+    footprint_cpu = None
+    if structure is not None:
+        struct = structure.detach().to(device="cpu", dtype=torch.float32)
+        if footprint is not None:
+            mask = footprint.to(torch.bool)
+            if tuple(mask.shape) != tuple(struct.shape):
+                raise ValueError(f"Footprint shape {tuple(mask.shape)} must match structure "
+                                 f"shape {tuple(struct.shape)}.")
+            if not mask.any():
+                raise ValueError("All-zero footprint is not supported.")
+            footprint_cpu = mask.detach().to("cpu").contiguous()
+    elif footprint is not None:
+        mask = footprint.to(torch.bool)
+        if not mask.any():
+            raise ValueError("All-zero footprint is not supported.")
+        struct = torch.zeros(tuple(mask.shape), dtype=torch.float32)
+        footprint_cpu = mask.detach().to("cpu").contiguous()
+    elif size is not None:
+        if isinstance(size, int):
+            size = (size,) * spatial_ndim
+        if any(v <= 0 for v in size):
+            raise ValueError("size values must be greater than zero")
+        struct = torch.zeros(tuple(size), dtype=torch.float32)
+    else:
+        raise ValueError("At least one of size, footprint, or structure must be specified.")
+    if struct.ndim != spatial_ndim:
+        raise ValueError(f"Structure dimension {struct.ndim} must match spatial dimension "
+                         f"{spatial_ndim}.")
+    origin_tuple = _normalize_origin(origin, spatial_ndim)
+    _validate_origin(origin_tuple, struct)
+    if mode not in _MODE_MAP:
+        raise ValueError(f"Unknown mode '{mode}'. Must be one of {list(_MODE_MAP.keys())}.")
+    flat_box = structure is None and footprint_cpu is None
+    return (None if flat_box else struct.contiguous(), footprint_cpu, origin_tuple,
+            _MODE_MAP[mode], tuple(struct.shape))
+
+
+def _launch(x: Tensor, struct, fp, se_shape, origin, mode_int, cval, dilation: bool,
+            out: Tensor) -> None:
+    L = _lib.lib()
+    nd = x.ndim - 2
+    batch = x.shape[0] * x.shape[1]
+    shp = _lib.i64_array(x.shape[2:])
+    ses = _lib.i64_array(se_shape)
+    st_ptr = None
+    fp_ptr = None
+    st_np = fp_np = None
+    if struct is not None:
+        st_np = struct.numpy()
+        st_ptr = st_np.ctypes.data_as(_lib.ctypes.c_void_p)
+    if fp is not None:
+        fp_np = fp.to(torch.uint8).numpy()
+        fp_ptr = fp_np.ctypes.data_as(_lib.ctypes.c_void_p)
+    nbytes = L.tm_grey_workspace_bytes(batch, nd, shp, ses)
+    org = _lib.i64_array(origin)
+    with torch.cuda.device(x.device):
+        stream = _lib.stream_of(x.device)
+        # the fused kernels need no workspace; the library reports
+        # TM_ERR_WORKSPACE (-3) before launching anything when a fallback does
+        status = L.tm_grey(_lib.ptr(x), batch, nd, shp, st_ptr, fp_ptr, ses, org, mode_int,
+                           float(cval), int(dilation), _lib.ptr(out), None, 0, stream)
+        if status == -3:
+            ws = _lib.workspace(nbytes, x.device)
+            status = L.tm_grey(_lib.ptr(x), batch, nd, shp, st_ptr, fp_ptr, ses, org, mode_int,
+                               float(cval), int(dilation), _lib.ptr(out), _lib.ptr(ws), nbytes,
+                               stream)
+        _lib.check(status, "tm_grey")
+
+
+def _grey_morphology(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                     mode: str = "reflect", cval: float = 0.0, origin=0, *, operation: str,
+                     differentiable: bool = False) -> Tensor:
+    validate_bcs_input(input)
+    validate_output(input, output)
+    struct, fp, origin_t, mode_int, se_shape = _resolve_element(input, size, footprint,
+                                                                structure, mode, origin)
+    if differentiable and input.requires_grad:
+        from .grad import grey_autograd
+
+        result = grey_autograd(input, struct, fp, se_shape, origin_t, mode_int, cval,
+                               operation == "dilation")
+    else:
+        x = input.detach()
+        x = x.contiguous() if x.dtype == torch.float32 else x.float().contiguous()
+        result = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+        _launch(x, struct, fp, se_shape, origin_t, mode_int, cval, operation == "dilation",
+                result)
+    if output is not None:
+        output.copy_(result)
+        return output
+    return result
+
+
+def grey_erosion(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                 mode: str = "reflect", cval: float = 0.0, origin=0,
+                 differentiable: bool = False) -> Tensor:
+    """N-D grey erosion (reference grey.py:108-132); float32 result."""
+    return _grey_morphology(input, size, footprint, structure, output, mode, cval, origin,
+                            operation="erosion", differentiable=differentiable)
+
+
+def grey_dilation(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                  mode: str = "reflect", cval: float = 0.0, origin=0,
+                  differentiable: bool = False) -> Tensor:
+    """N-D grey dilation (reference grey.py:135-159); float32 result."""
+    return _grey_morphology(input, size, footprint, structure, output, mode, cval, origin,
+                            operation="dilation", differentiable=differentiable)
+
+
+def grey_opening(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                 mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
+    """Dilation of the erosion (reference grey.py:162-178)."""
+    kw = _element_kwargs(size, footprint, structure, mode, cval, origin)
+    return grey_dilation(grey_erosion(input, **kw), output=output, **kw)
+
+
+def grey_closing(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                 mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
+    """Erosion of the dilation (reference grey.py:181-197)."""
+    kw = _element_kwargs(size, footprint, structure, mode, cval, origin)
+    return grey_erosion(grey_dilation(input, **kw), output=output, **kw)
+
+
+def _finish(result, output):
+    if output is not None:
+        output.copy_(result)
+        return output
+    return result
+
+
+def morphological_gradient(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                           mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
+    """dilation - erosion (reference grey.py:200-221)."""
+    validate_bcs_input(input)
+    validate_output(input, output)
+    kw = _element_kwargs(size, footprint, structure, mode, cval, origin)
+    return _finish(grey_dilation(input, **kw) - grey_erosion(input, **kw), output)
+
+
+def white_tophat(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                 mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
+    """input - opening (reference grey.py:224-243)."""
+    validate_bcs_input(input)
+    validate_output(input, output)
+    opened = grey_opening(input, **_element_kwargs(size, footprint, structure, mode, cval, origin))
+    return _finish(input.detach().float() - opened, output)
+
+
+def black_tophat(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                 mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
+    """closing - input (reference grey.py:246-265)."""
+    validate_bcs_input(input)
+    validate_output(input, output)
+    closed = grey_closing(input, **_element_kwargs(size, footprint, structure, mode, cval, origin))
+    return _finish(closed - input.detach().float(), output)
+
+
+def morphological_laplace(input: Tensor, size=None, footprint=None, structure=None, output=None,
+                          mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
+    """dilation + erosion - 2 input (reference grey.py:268-289)."""
+    validate_bcs_input(input)
+    validate_output(input, output)
+    kw = _element_kwargs(size, footprint, structure, mode, cval, origin)
+    return _finish(grey_dilation(input, **kw) + grey_erosion(input, **kw)
+                   - 2 * input.detach().float(), output)
