@@ -136,6 +136,9 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline([m.cpu().numpy() for m in masks[:4]], args.cpu_seconds)
 
+    from torchmorph_amd import autotune
+
+    tuned = autotune.export()
     out = {
         "metric": "EDT Gvox/s (bit-exact); Sinkhorn solves/s at 1e-5 marginal err, warm vs cold",
         "value": round(value, 3),
@@ -159,8 +162,9 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "edt2d_band_kernel", "bytes_per_launch": nvox * BYTES_PER_VOX,
-            "event_us_per_launch": round(per_launch_s * 1e6, 3),
+            "kernel": "edt2d_cols_kernel + edt2d_rows_kernel (one step)", "bytes_per_launch": nvox * BYTES_PER_VOX,
+            "event_us_per_step": round(per_launch_s * 1e6, 3),
+            "autotune": tuned,
         },
         "cpu_baseline": cpu,
     }

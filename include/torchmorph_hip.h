@@ -42,6 +42,20 @@ int tm_last_error(char* buf, size_t len);
 /* Library version string, e.g. "torchmorph_hip 0.1 gfx950". */
 const char* tm_version(void);
 
+/* ----------------------------------------------------------- autotune --- */
+/* No reference counterpart (the reference hard-codes its launch constants,
+ * edt_kernel.cu:18-20, ot_kernel.cu:8-9).  Launch variants (tile, rows per
+ * band, columns per thread, lines per wave, LDS depth) are timed per shape
+ * class on first use and cached; every variant returns identical results.
+ * Tuning is skipped while the stream is capturing a graph. */
+void tm_autotune_enable(int enabled);
+void tm_autotune_clear(void);
+/* Text form of the cache, one "op key... = variant" per line; returns the
+ * full length. */
+int tm_autotune_export(char* buf, size_t len);
+/* Loads entries in the export format; returns the number read. */
+int tm_autotune_import(const char* text);
+
 /* ---------------------------------------------------------------- EDT --- */
 
 /* Workspace bytes tm_edt needs for this problem (0 when none). */

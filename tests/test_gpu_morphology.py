@@ -114,7 +114,7 @@ def test_grey_large_box_uses_fallback(tm, oracle_mod):
 def test_grey_signed_zero_follows_reference(tm, oracle_mod):
     x = np.zeros((1, 1, 6, 6), np.float32)
     x[0, 0, ::2] = -0.0
-    _grey_vs_oracle(tm, oracle_mod, x, True, size=3)
+    _grey_vs_oracle(tm, oracle_mod, x, True, size=(3, 3))
     got = tm.grey_dilation(torch.as_tensor(x, device="cuda"), size=3).cpu().numpy()
     assert not np.signbit(got).any()  # the reference's "+ 0" turns -0 into +0
 

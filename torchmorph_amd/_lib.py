@@ -30,6 +30,10 @@ _DP = ctypes.POINTER(ctypes.c_double)
 SIGNATURES: dict[str, tuple] = {
     "tm_last_error": (_I, [ctypes.c_char_p, _SZ]),
     "tm_version": (ctypes.c_char_p, []),
+    "tm_autotune_enable": (None, [_I]),
+    "tm_autotune_clear": (None, []),
+    "tm_autotune_export": (_I, [ctypes.c_char_p, _SZ]),
+    "tm_autotune_import": (_I, [ctypes.c_char_p]),
     "tm_edt_workspace_bytes": (_SZ, [_I64, _I, _I64P, _DP]),
     "tm_edt": (_I, [_P, _I, _I64, _I, _I64P, _DP, _P, _P, _I, _P, _SZ, _P]),
     "tm_cdt_workspace_bytes": (_SZ, [_I64, _I, _I64P]),

@@ -26,6 +26,7 @@
 // Build with -ffp-contract=off: every double multiply and add must round
 // separately, as SciPy's C code does.
 #include "tm_common.h"
+#include "tm_tune.h"
 
 namespace tmh {
 
@@ -74,10 +75,10 @@ __device__ __forceinline__ void edt_store(const EdtOut& o, int ndim, int64_t g, 
 constexpr int kColSegs = 16;
 constexpr int kChunkRows = 16;
 
-template <typename TIn>
-__device__ __forceinline__ uint32_t bg4(const TIn* __restrict__ p, int x, int W, bool vec) {
-  // background bits of columns x..x+3 of one row
-  if (vec) {
+template <typename TIn, int CPT>
+__device__ __forceinline__ uint32_t bgbits(const TIn* __restrict__ p, int x, int W, bool vec) {
+  // background bits of columns x..x+CPT-1 of one row
+  if (CPT == 4 && vec) {
     if constexpr (sizeof(TIn) == 1) {
       const uint32_t v = *(const uint32_t*)(p + x);
       return (uint32_t)((v & 0xffu) == 0) | ((uint32_t)((v & 0xff00u) == 0) << 1) |
@@ -90,28 +91,30 @@ __device__ __forceinline__ uint32_t bg4(const TIn* __restrict__ p, int x, int W,
   }
   uint32_t r = 0;
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int c = 0; c < CPT; ++c)
     if (x + c < W && p[x + c] == TIn(0)) r |= 1u << c;
   return r;
 }
 
-template <typename TIn, int NCH>
+// CPT adjacent columns per thread (1 or 4), 16 column groups x 16 row
+// segments per workgroup, NCH chunks of 16 rows per segment.
+template <typename TIn, int NCH, int CPT>
 __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__ in, int H, int W,
                                                          int16_t* __restrict__ f0, int aligned) {
-  __shared__ int16_t sfirst[kColSegs][64];
-  __shared__ int16_t slast[kColSegs][64];
+  constexpr int WGC = 16 * CPT;  // columns per workgroup
+  __shared__ int16_t sfirst[kColSegs][WGC];
+  __shared__ int16_t slast[kColSegs][WGC];
   const int cg = threadIdx.x & 15;
   const int seg = threadIdx.x >> 4;
   const int img = blockIdx.y;
-  const int xb = blockIdx.x * 64 + cg * 4;
+  const int xb = blockIdx.x * WGC + cg * CPT;
   const int seglen = (H + kColSegs - 1) / kColSegs;
   const int ys = seg * seglen;
   const int ye = min(H, ys + seglen);
   const TIn* im = in + (int64_t)img * H * W;
-  const bool vec = aligned && (W % 4 == 0) && (xb + 4 <= W);
+  const bool vec = aligned && (W % 4 == 0) && (xb + CPT <= W);
   const bool live = xb < W && ys < ye;
 
-  // m[c][k]: background bits of column xb+c, rows ys+16k .. ys+16k+15
   uint32_t raw[NCH][kChunkRows];
   if (live) {
 #pragma unroll
@@ -119,25 +122,27 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
 #pragma unroll
       for (int b = 0; b < kChunkRows; ++b) {
         const int y = min(ys + k * kChunkRows + b, ye - 1);
-        raw[k][b] = bg4(im + (int64_t)y * W, xb, W, vec);
+        raw[k][b] = bgbits<TIn, CPT>(im + (int64_t)y * W, xb, W, vec);
       }
   }
-  uint32_t m[4][NCH];
+  uint32_t m[CPT][NCH];
 #pragma unroll
   for (int k = 0; k < NCH; ++k) {
-    uint32_t mk[4] = {0, 0, 0, 0};
+    uint32_t mk[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) mk[c] = 0;
 #pragma unroll
     for (int b = 0; b < kChunkRows; ++b) {
       const bool rowok = live && (ys + k * kChunkRows + b < ye);
       const uint32_t v = rowok ? raw[k][b] : 0u;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) mk[c] |= ((v >> c) & 1u) << b;
+      for (int c = 0; c < CPT; ++c) mk[c] |= ((v >> c) & 1u) << b;
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) m[c][k] = mk[c];
+    for (int c = 0; c < CPT; ++c) m[c][k] = mk[c];
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < CPT; ++c) {
     int first = -1, last = -1;
 #pragma unroll
     for (int k = NCH - 1; k >= 0; --k)
@@ -145,18 +150,18 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
 #pragma unroll
     for (int k = 0; k < NCH; ++k)
       if (m[c][k]) last = ys + k * kChunkRows + 31 - __builtin_clz(m[c][k]);
-    sfirst[seg][cg * 4 + c] = (int16_t)first;
-    slast[seg][cg * 4 + c] = (int16_t)last;
+    sfirst[seg][cg * CPT + c] = (int16_t)first;
+    slast[seg][cg * CPT + c] = (int16_t)last;
   }
   __syncthreads();
   if (!live) return;
-  int before[4], after[4];
+  int before[CPT], after[CPT];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < CPT; ++c) {
     before[c] = -1;
     after[c] = -1;
-    for (int s = seg - 1; s >= 0 && before[c] < 0; --s) before[c] = slast[s][cg * 4 + c];
-    for (int s = seg + 1; s < kColSegs && after[c] < 0; ++s) after[c] = sfirst[s][cg * 4 + c];
+    for (int s = seg - 1; s >= 0 && before[c] < 0; --s) before[c] = slast[s][cg * CPT + c];
+    for (int s = seg + 1; s < kColSegs && after[c] < 0; ++s) after[c] = sfirst[s][cg * CPT + c];
   }
 #pragma unroll
   for (int k = 0; k < NCH; ++k) {
@@ -164,9 +169,9 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
     for (int b = 0; b < kChunkRows; ++b) {
       const int y = ys + k * kChunkRows + b;
       if (y >= ye) break;
-      int16_t out[4];
+      int16_t out[CPT];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < CPT; ++c) {
         // nearest background at or above y
         int pv = -1;
         uint32_t mm = m[c][k] & (0xffffffffu >> (31 - b));
@@ -195,14 +200,14 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
         out[c] = (int16_t)f;
       }
       int16_t* dst = f0 + ((int64_t)img * H + y) * W + xb;
-      if (vec) {
+      if (CPT == 4 && vec) {
         uint2 pk;
-        pk.x = (uint32_t)(uint16_t)out[0] | ((uint32_t)(uint16_t)out[1] << 16);
-        pk.y = (uint32_t)(uint16_t)out[2] | ((uint32_t)(uint16_t)out[3] << 16);
+        pk.x = (uint32_t)(uint16_t)out[0] | ((uint32_t)(uint16_t)out[1 % CPT] << 16);
+        pk.y = (uint32_t)(uint16_t)out[2 % CPT] | ((uint32_t)(uint16_t)out[3 % CPT] << 16);
         *(uint2*)dst = pk;
       } else {
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < CPT; ++c)
           if (xb + c < W) dst[c] = out[c];
       }
     }
@@ -508,35 +513,54 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
       set_error("tm_edt: workspace of %zu bytes is smaller than the %zu required", ws_bytes, need);
       return TM_ERR_WORKSPACE;
     }
-    int16_t* f0 = (int16_t*)ws;
-    const dim3 cgrid((W + 63) / 64, (unsigned)batch);
-    const int seglen = (H + kColSegs - 1) / kColSegs;
-    const int aligned = ((uintptr_t)in % 16) == 0;
     if (((uintptr_t)ws % 16) != 0) {
       set_error("tm_edt: workspace must be 16-byte aligned");
       return TM_ERR_ARG;
     }
-    if (seglen <= 16)
-      hipLaunchKernelGGL((edt2d_cols_kernel<TIn, 1>), cgrid, dim3(256), 0, st, in, H, W, f0,
-                         aligned);
-    else if (seglen <= 32)
-      hipLaunchKernelGGL((edt2d_cols_kernel<TIn, 2>), cgrid, dim3(256), 0, st, in, H, W, f0,
-                         aligned);
-    else
-      hipLaunchKernelGGL((edt2d_cols_kernel<TIn, 4>), cgrid, dim3(256), 0, st, in, H, W, f0,
-                         aligned);
-    TM_LAUNCH_CHECK();
-    if (W <= 1024) {
-      constexpr int R = 8;
-      hipLaunchKernelGGL((edt2d_rows_kernel<R>), dim3((H + R - 1) / R, (unsigned)batch),
-                         dim3(256), (size_t)R * W * 6, st, f0, H, W, o);
-    } else {
-      constexpr int R = 2;
-      hipLaunchKernelGGL((edt2d_rows_kernel<R>), dim3((H + R - 1) / R, (unsigned)batch),
-                         dim3(256), (size_t)R * W * 6, st, f0, H, W, o);
-    }
-    TM_LAUNCH_CHECK();
-    return TM_OK;
+    int16_t* f0 = (int16_t*)ws;
+    const int seglen = (H + kColSegs - 1) / kColSegs;
+    const int aligned = ((uintptr_t)in % 16) == 0;
+    // variant v: columns per thread = (v & 1) ? 4 : 1, rows per band = 1 << (v >> 1)
+    auto run = [&](int v) -> int {
+      const int cpt = (v & 1) ? 4 : 1;
+      const int R = 1 << (v >> 1);
+      if ((size_t)R * W * 6 > 96 * 1024) {
+        set_error("tm_edt: band does not fit LDS");
+        return TM_ERR_ARG;
+      }
+      const dim3 cgrid((W + 16 * cpt - 1) / (16 * cpt), (unsigned)batch);
+#define TM_COLS(NCH, CPT)                                                                         \
+  hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, CPT>), cgrid, dim3(256), 0, st, in, H, W, f0, \
+                     aligned)
+      if (cpt == 4) {
+        if (seglen <= 16) TM_COLS(1, 4);
+        else if (seglen <= 32) TM_COLS(2, 4);
+        else TM_COLS(4, 4);
+      } else {
+        if (seglen <= 16) TM_COLS(1, 1);
+        else if (seglen <= 32) TM_COLS(2, 1);
+        else TM_COLS(4, 1);
+      }
+#undef TM_COLS
+      TM_LAUNCH_CHECK();
+      const dim3 rgrid((H + R - 1) / R, (unsigned)batch);
+      const size_t lds = (size_t)R * W * 6;
+      switch (R) {
+        case 1: hipLaunchKernelGGL((edt2d_rows_kernel<1>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
+        case 2: hipLaunchKernelGGL((edt2d_rows_kernel<2>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
+        case 4: hipLaunchKernelGGL((edt2d_rows_kernel<4>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
+        default: hipLaunchKernelGGL((edt2d_rows_kernel<8>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
+      }
+      TM_LAUNCH_CHECK();
+      return TM_OK;
+    };
+    auto p2 = [](int64_t v) { int64_t p = 1; while (p < v) p <<= 1; return p; };
+    const int64_t key[4] = {p2(H), p2(W), o.idx ? (o.idx64 ? 64 : 32) : 0, o.dist ? 1 : 0};
+    const int dflt = W >= 512 ? 1 : 3;  // 4 columns/thread; 1 or 2 rows per band
+    bool ran = false;
+    const int v = tune_pick("edt2d", key, 4, 8, dflt, run, st, &ran);
+    if (ran) return TM_OK;
+    return run(v);
   }
 
   const int64_t N = o.plane;

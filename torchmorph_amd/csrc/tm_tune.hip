@@ -1,0 +1,131 @@
+// Autotuner: one cached variant index per (operator, shape class).
+//
+// Shape classes are chosen by each operator (e.g. the EDT 2-D path keys on
+// rounded-up powers of two of H and W and on the index width).  The cache is
+// process-wide behind a mutex; tm_autotune_export/tm_autotune_import move it
+// to and from a text form ("op k0 k1 ... = idx" per line) that the Python
+// layer persists as JSON.
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "tm_tune.h"
+
+namespace tmh {
+
+static std::mutex g_mu;
+static std::map<std::string, int>& cache() {
+  static std::map<std::string, int> c;
+  return c;
+}
+static int g_enabled = -1;
+
+static bool tuning_enabled() {
+  if (g_enabled < 0) {
+    const char* e = getenv("TORCHMORPH_AMD_AUTOTUNE");
+    g_enabled = (e && e[0] == '0') ? 0 : 1;
+  }
+  return g_enabled == 1;
+}
+
+static std::string make_key(const char* op, const int64_t* key, int nkey) {
+  std::ostringstream os;
+  os << op;
+  for (int i = 0; i < nkey; ++i) os << ' ' << key[i];
+  return os.str();
+}
+
+int tune_pick(const char* op, const int64_t* key, int nkey, int n, int dflt,
+              const std::function<int(int)>& run, hipStream_t st, bool* ran) {
+  *ran = false;
+  const std::string k = make_key(op, key, nkey);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = cache().find(k);
+    if (it != cache().end()) return it->second;
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone ||
+      !tuning_enabled() || n <= 1)
+    return dflt;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return dflt;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return dflt;
+  }
+  int best = -1;
+  float best_ms = 1e30f;
+  int last_ok = -1;
+  for (int v = 0; v < n; ++v) {
+    if (run(v) != TM_OK) continue;  // variant not applicable (e.g. LDS)
+    constexpr int reps = 5;
+    hipEventRecord(e0, st);
+    bool ok = true;
+    for (int r = 0; r < reps && ok; ++r) ok = run(v) == TM_OK;
+    hipEventRecord(e1, st);
+    if (!ok || hipEventSynchronize(e1) != hipSuccess) continue;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    last_ok = v;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (best < 0) return dflt;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    cache()[k] = best;
+  }
+  if (last_ok != best) {
+    // outputs currently hold the last variant's (identical) result; they are
+    // complete, so nothing to redo
+  }
+  *ran = last_ok >= 0;
+  return best;
+}
+
+}  // namespace tmh
+
+extern "C" void tm_autotune_enable(int enabled) { tmh::g_enabled = enabled ? 1 : 0; }
+
+extern "C" void tm_autotune_clear(void) {
+  std::lock_guard<std::mutex> lk(tmh::g_mu);
+  tmh::cache().clear();
+}
+
+extern "C" int tm_autotune_export(char* buf, size_t len) {
+  std::ostringstream os;
+  {
+    std::lock_guard<std::mutex> lk(tmh::g_mu);
+    for (auto& kv : tmh::cache()) os << kv.first << " = " << kv.second << '\n';
+  }
+  const std::string s = os.str();
+  if (buf && len) {
+    const size_t k = s.size() < len - 1 ? s.size() : len - 1;
+    memcpy(buf, s.data(), k);
+    buf[k] = 0;
+  }
+  return (int)s.size();
+}
+
+extern "C" int tm_autotune_import(const char* text) {
+  if (!text) return TM_ERR_ARG;
+  std::istringstream is(text);
+  std::string line;
+  int n = 0;
+  std::lock_guard<std::mutex> lk(tmh::g_mu);
+  while (std::getline(is, line)) {
+    const size_t eq = line.rfind(" = ");
+    if (eq == std::string::npos) continue;
+    tmh::cache()[line.substr(0, eq)] = atoi(line.c_str() + eq + 3);
+    ++n;
+  }
+  return n;
+}
