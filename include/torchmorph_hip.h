@@ -92,6 +92,38 @@ int tm_cdt(const void* input, int input_dtype, int64_t batch, int ndim, const in
            int metric, float* distances, void* indices, int index_dtype, void* workspace,
            size_t workspace_bytes, void* stream);
 
+/* ----------------------------------------------------------- Sinkhorn --- */
+
+/* Histogram Sinkhorn, n_iter iterations on (nb, d) rows sharing one (d, d)
+ * cost.  Replaces torchmorph._C.sinkhorn_logiter (log_space=1: num = log a /
+ * log b, M = cost, u/v = log-scalings) and torchmorph._C.sinkhorn_fastiter
+ * (log_space=0: num = a / b, M = K = exp(-C/eps), u/v = scalings)
+ * (reference torchmorph.cpp:58-77, ot_kernel.cu:188-283).  u and v are
+ * updated in place.  dtype TM_F32 or TM_F64.  n_iter >= 100 replays a
+ * captured 25-iteration hipGraph (reference optimal_transport.py:213-243);
+ * *used_fallback is set when capture failed and plain launches ran. */
+int tm_sinkhorn_hist(int dtype, int log_space, const void* num_a, const void* num_b,
+                     const void* M, const void* M_T, void* u, void* v, int64_t nb, int d,
+                     int n_iter, double epsilon, int* used_fallback, void* stream);
+
+/* Workspace bytes for tm_sinkhorn_points. */
+size_t tm_sinkhorn_points_workspace_bytes(int npairs, const int32_t* n, const int32_t* m);
+
+/* No reference counterpart (the reference solves histograms only).  Entropic
+ * OT between npairs point-cloud pairs with cost |x - y|^2: x (sum n, dim),
+ * y (sum m, dim), weights a (sum n), b (sum m), per-pair eps (host array),
+ * n/m per-pair sizes (host arrays, each <= 4096), dim 1..4.  f/g: initial
+ * potentials in (zeros = cold start; a warm start sets f, g is recomputed)
+ * and the solution out.  Each pair stops at the first state whose L1
+ * row-marginal error after a g-step is <= tol, or after max_iter iterations;
+ * iters/err (device, per pair) report it.  The loop runs as a hipGraph of
+ * `chunk` iterations with a device-side active-pair count read between
+ * replays (the call returns when every pair has stopped). */
+int tm_sinkhorn_points(const float* x, const float* y, const float* a, const float* b, int dim,
+                       int npairs, const int32_t* n, const int32_t* m, const float* eps,
+                       float tol, int max_iter, int chunk, float* f, float* g, int32_t* iters,
+                       float* err, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,12 @@ from .morphology import (
     morphological_laplace,
     white_tophat,
 )
+from .optimal_transport import (
+    SinkhornPointCloud,
+    SinkhornSolver,
+    build_cost_matrix,
+    sinkhorn_points,
+)
 
 __all__ = [
     "generate_binary_structure", "iterate_structure", "binary_erosion", "binary_dilation",
@@ -43,6 +49,10 @@ __all__ = [
     "brute_force_distance_transform",
     "distance_transform_edt",
     "distance_transform_cdt",
+    "SinkhornSolver",
+    "build_cost_matrix",
+    "sinkhorn_points",
+    "SinkhornPointCloud",
 ]
 
 __version__ = "0.1.0"

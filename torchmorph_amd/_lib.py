@@ -40,6 +40,13 @@ SIGNATURES: dict[str, tuple] = {
     "tm_cdt": (_I, [_P, _I, _I64, _I, _I64P, _I, _P, _P, _I, _P, _SZ, _P]),
     "tm_grey_workspace_bytes": (_SZ, [_I64, _I, _I64P, _I64P]),
     "tm_grey": (_I, [_P, _I64, _I, _I64P, _P, _P, _I64P, _I64P, _I, _F, _I, _P, _P, _SZ, _P]),
+    "tm_sinkhorn_hist": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _I64, _I, _I, _D,
+                              ctypes.POINTER(ctypes.c_int), _P]),
+    "tm_sinkhorn_points_workspace_bytes": (_SZ, [_I, ctypes.POINTER(ctypes.c_int32),
+                                                 ctypes.POINTER(ctypes.c_int32)]),
+    "tm_sinkhorn_points": (_I, [_P, _P, _P, _P, _I, _I, ctypes.POINTER(ctypes.c_int32),
+                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float),
+                                _F, _I, _I, _P, _P, _P, _P, _P, _SZ, _P]),
     "tm_binary_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
     "tm_binary": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _I, _I,
                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _P, _P, _P,

@@ -1,0 +1,572 @@
+// Log-domain Sinkhorn for gfx950: ragged batches of point-cloud pairs with the
+// squared-Euclidean cost computed on the fly, and the reference's histogram
+// entry point (shared (d, d) cost).
+//
+// Point clouds (no reference counterpart: the reference only solves
+// histograms sharing one cost, optimal_transport.py:189-213).  Dual potentials
+// f (rows, points x, weights a) and g (columns, points y, weights b):
+//   f_i <- -eps * log sum_j b_j exp((g_j - C_ij) / eps)
+//   g_j <- -eps * log sum_i a_i exp((f_i - C_ij) / eps),  C_ij = |x_i - y_j|^2.
+// Kernel arithmetic is in base 2: with L = log2(e),
+//   (g_j - C_ij) L/eps + log2 b_j = h_j + xt_i . y_j - c_i,
+//   h_j = L/eps (g_j - |y_j|^2) + log2 b_j,  xt_i = 2L/eps x_i,  c_i = L/eps |x_i|^2,
+// so each term is three FMAs and one v_exp_f32; clouds are centred by the
+// caller, which keeps the expansion accurate.  The sum for row i is taken
+// relative to s_i = c_i - L/eps f_i(old); the resulting sum S_i is exactly
+// the row-marginal ratio r_i / a_i of the current plan, so the L1 marginal
+// error sum_i a_i |S_i - 1| of the state after the previous g-step comes for
+// free with every f-update.  If S_i leaves [2^-60, 2^60] the workgroup
+// recomputes with an exact per-row maximum.
+//
+// One iteration = f-update (into the other half of a ping-pong buffer, with
+// the marginal error) -> per-pair check -> g-update (in place).  A pair whose
+// error is <= tol (or that reached max_iter) is frozen at the state the error
+// was measured on: its f, g and iteration count are copied out and later
+// launches skip it.  The host replays a captured hipGraph of K iterations and
+// reads the number of active pairs between replays.
+//
+// Work distribution: a device work-list of (pair, first row) tiles of 64 rows
+// per half-step covers ragged pairs in one launch; a workgroup of 256 lanes
+// stages the pair's column points and h_j in LDS and splits each row's sum
+// over 4 lanes.
+#include <vector>
+
+#include "tm_common.h"
+
+namespace tmh {
+
+constexpr float kL2E = 1.4426950408889634f;  // log2(e)
+constexpr int kRowsPerTile = 64;
+constexpr int kMaxCols = 4096;
+
+struct SkPair {
+  int64_t xoff, yoff;  // first row of the pair in the concatenated x / y
+  int32_t n, m;
+  float eps;
+  int32_t pad;
+};
+
+struct SkTile {
+  int32_t pair;
+  int32_t r0;
+};
+
+template <int D>
+__device__ __forceinline__ float dotp(const float* a, const float* b) {
+  float s = 0.f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) s = fmaf(a[d], b[d], s);
+  return s;
+}
+
+// One half-step for every tile of the work-list.
+//   rows/cols: concatenated points (N x D); lwc: log2 of the column weights;
+//   pc: column potentials; pin/pout: row potentials (old/new, may alias);
+//   lwr/wr: log2 / plain row weights (for the marginal error).
+// ROWS_ARE_X selects which offsets of SkPair index rows.
+template <int D, bool ROWS_ARE_X>
+__global__ __launch_bounds__(256) void sk_half_kernel(
+    const float* __restrict__ rows, const float* __restrict__ cols,
+    const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ pin,
+    float* __restrict__ pout, const float* __restrict__ wr, const SkPair* __restrict__ pairs,
+    const SkTile* __restrict__ tiles, const int* __restrict__ active, float* __restrict__ errpart,
+    int maxc) {
+  extern __shared__ float dyn[];
+  float* sy = dyn;              // [maxc][D] column points
+  float* sh = dyn + maxc * D;   // [maxc] h_j
+  __shared__ float ssum[4][kRowsPerTile];
+  __shared__ float smax[4][kRowsPerTile];
+  __shared__ int sbad;
+  const SkTile tile = tiles[blockIdx.x];
+  if (!active[tile.pair]) return;  // uniform per workgroup
+  const SkPair pr = pairs[tile.pair];
+  const int64_t roff = ROWS_ARE_X ? pr.xoff : pr.yoff;
+  const int64_t coff = ROWS_ARE_X ? pr.yoff : pr.xoff;
+  const int nr = ROWS_ARE_X ? pr.n : pr.m;
+  const int nc = ROWS_ARE_X ? pr.m : pr.n;
+  const float k1 = kL2E / pr.eps;  // L / eps
+  // stage the pair's columns
+  for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+    float y[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) y[d] = cols[(coff + j) * D + d];
+    float yy = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      sy[j * D + d] = y[d];
+      yy = fmaf(y[d], y[d], yy);
+    }
+    sh[j] = fmaf(k1, pc[coff + j] - yy, lwc[coff + j]);
+  }
+  if (threadIdx.x == 0) sbad = 0;
+  __syncthreads();
+  const int r = threadIdx.x & (kRowsPerTile - 1);
+  const int q = threadIdx.x >> 6;  // column quarter
+  const int i = tile.r0 + r;
+  const bool live = i < nr;
+  float xt[D];
+  float ci = 0.f, s = 0.f, fold = 0.f;
+  if (live) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float x = rows[(roff + i) * D + d];
+      xt[d] = 2.f * k1 * x;
+      ci = fmaf(x, x, ci);
+    }
+    ci *= k1;
+    fold = pin[roff + i];
+    s = ci - k1 * fold;
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d) xt[d] = 0.f;
+  }
+  float acc = 0.f;
+  if (live) {
+    for (int j = q; j < nc; j += 4) {
+      const float e = sh[j] + dotp<D>(xt, &sy[j * D]) - s;
+      acc += __builtin_amdgcn_exp2f(e);
+    }
+  }
+  ssum[q][r] = acc;
+  __syncthreads();
+  float S = ssum[0][r] + ssum[1][r] + ssum[2][r] + ssum[3][r];
+  const bool bad = live && !(S >= 0x1p-60f && S <= 0x1p60f);
+  if (bad) sbad = 1;
+  __syncthreads();
+  if (sbad) {
+    // exact path for the whole workgroup: per-row maximum, then the sum
+    float mx = -INFINITY;
+    if (live)
+      for (int j = q; j < nc; j += 4) mx = fmaxf(mx, sh[j] + dotp<D>(xt, &sy[j * D]));
+    smax[q][r] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(smax[0][r], smax[1][r]), fmaxf(smax[2][r], smax[3][r]));
+    float a2 = 0.f;
+    if (live)
+      for (int j = q; j < nc; j += 4)
+        a2 += __builtin_amdgcn_exp2f(sh[j] + dotp<D>(xt, &sy[j * D]) - mx);
+    __syncthreads();
+    ssum[q][r] = a2;
+    __syncthreads();
+    const float S2 = ssum[0][r] + ssum[1][r] + ssum[2][r] + ssum[3][r];
+    // rebase to the stale shift: log2 S = mx - s + log2 S2
+    if (live) {
+      const float l2 = mx - s + __builtin_amdgcn_logf(S2);
+      S = exp2f(l2);  // may be inf/0 for the error only
+      if (q == 0) pout[roff + i] = fold - l2 / k1;
+    }
+  } else if (live && q == 0) {
+    pout[roff + i] = fold - __builtin_amdgcn_logf(S) / k1;
+  }
+  if (errpart) {
+    // deterministic tile partial of sum_i a_i |S_i - 1| (row-marginal L1 error)
+    __syncthreads();
+    float e = (live && q == 0) ? wr[roff + i] * fabsf(S - 1.f) : 0.f;
+    if (q == 0) ssum[0][r] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float t = 0.f;
+      for (int k = 0; k < kRowsPerTile; ++k) t += ssum[0][k];
+      errpart[blockIdx.x] = t;
+    }
+  }
+}
+
+// Per-pair convergence check after an f-update of iteration k (1-based):
+// the error is that of the state after iteration k-1, whose f is still in
+// fold and whose g is in g.
+__global__ __launch_bounds__(256) void sk_check_kernel(
+    const SkPair* __restrict__ pairs, const int32_t* __restrict__ tile_first,
+    const float* __restrict__ errpart, int* __restrict__ active, int* __restrict__ n_active,
+    const int* __restrict__ iter_base, int k_in_chunk, int max_iter, float tol,
+    const float* __restrict__ fold, const float* __restrict__ g, float* __restrict__ fout,
+    float* __restrict__ gout, int32_t* __restrict__ iters, float* __restrict__ err) {
+  const int p = blockIdx.x;
+  if (!active[p]) return;
+  __shared__ int done;
+  const int k = *iter_base + k_in_chunk;  // iteration whose f-update just ran
+  if (threadIdx.x == 0) {
+    float e = 0.f;
+    for (int t = tile_first[p]; t < tile_first[p + 1]; ++t) e += errpart[t];
+    err[p] = e;
+    done = (e <= tol || k - 1 >= max_iter) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!done) return;
+  const SkPair pr = pairs[p];
+  for (int i = threadIdx.x; i < pr.n; i += blockDim.x) fout[pr.xoff + i] = fold[pr.xoff + i];
+  for (int j = threadIdx.x; j < pr.m; j += blockDim.x) gout[pr.yoff + j] = g[pr.yoff + j];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    iters[p] = k - 1;
+    active[p] = 0;
+    atomicSub(n_active, 1);
+  }
+}
+
+__global__ void sk_init_kernel(int* active, int npairs, int* n_active, int* iter_base) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x)
+    active[p] = 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *n_active = npairs;
+    *iter_base = 0;
+  }
+}
+
+__global__ void sk_log2_kernel(const float* __restrict__ w, float* __restrict__ lw, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lw[i] = __builtin_amdgcn_logf(w[i]);
+}
+
+__global__ void sk_advance_kernel(int* iter_base, int k) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *iter_base += k;
+}
+
+__global__ void sk_copy_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// ------------------------------------------------------------- histograms --
+// Reference sinkhorn_logiter (ot_kernel.cu:235-283): out[b, i] =
+//   log_num[b, i] - LSE_j(-M[i, j] / eps + log_scale[b, j]); one lane per
+// (b, i), two passes (max, then sum) over j with the cost row in cache.
+template <typename T>
+__global__ __launch_bounds__(256) void hist_log_update(const T* __restrict__ lognum,
+                                                       const T* __restrict__ M,
+                                                       const T* __restrict__ lscale,
+                                                       T* __restrict__ out, int64_t nb, int d,
+                                                       T inv_eps) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * d) return;
+  const int64_t b = t / d;
+  const int i = (int)(t - b * d);
+  const T* row = M + (int64_t)i * d;
+  const T* ls = lscale + b * d;
+  T mx = -INFINITY;
+  for (int j = 0; j < d; ++j) mx = fmax(mx, -row[j] * inv_eps + ls[j]);
+  T s = 0;
+  if (mx > -INFINITY)
+    for (int j = 0; j < d; ++j) s += exp(-row[j] * inv_eps + ls[j] - mx);
+  const T lse = (mx > -INFINITY) ? mx + log(s) : mx;
+  out[t] = lognum[t] - lse;
+}
+
+// Reference sinkhorn_fastiter (ot_kernel.cu:188-233): out = num / (K scale + 1e-12).
+template <typename T>
+__global__ __launch_bounds__(256) void hist_scaling_update(const T* __restrict__ num,
+                                                           const T* __restrict__ K,
+                                                           const T* __restrict__ scale,
+                                                           T* __restrict__ out, int64_t nb,
+                                                           int d) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * d) return;
+  const int64_t b = t / d;
+  const int i = (int)(t - b * d);
+  const T* row = K + (int64_t)i * d;
+  const T* sc = scale + b * d;
+  T s = 0;
+  for (int j = 0; j < d; ++j) s += row[j] * sc[j];
+  out[t] = num[t] / (s + (T)1e-12);
+}
+
+// Runs `launch(k)` (k iterations enqueued on st) n_iter times in total; for
+// n_iter >= 100 a chunk of 25 iterations is captured once as a hipGraph and
+// replayed (reference optimal_transport.py:213-243).  A capture failure
+// falls back to plain launches; extra iterations are never run.
+template <typename F>
+static int run_chunked(F&& launch, int n_iter, hipStream_t st, hipStream_t* lst,
+                       int* fallback_out) {
+  constexpr int kChunk = 25;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  const bool capturing = hipStreamIsCapturing(st, &cs) == hipSuccess &&
+                         cs != hipStreamCaptureStatusNone;
+  const char* force = getenv("TORCHMORPH_AMD_FORCE_GRAPH_FAILURE");
+  if (n_iter < 100 || capturing) return launch(n_iter);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  bool ok = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) == hipSuccess;
+  if (ok) ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  int status = TM_OK;
+  if (ok) {
+    *lst = cap;
+    status = launch(kChunk);
+    *lst = st;
+    ok = hipStreamEndCapture(cap, &graph) == hipSuccess && status == TM_OK;
+  }
+  if (ok && force && force[0] == '1') ok = false;
+  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (!ok) {
+    if (graph) hipGraphDestroy(graph);
+    if (cap) hipStreamDestroy(cap);
+    (void)hipGetLastError();
+    if (fallback_out) *fallback_out = 1;
+    return launch(n_iter);
+  }
+  const int reps = n_iter / kChunk;
+  for (int r = 0; r < reps && status == TM_OK; ++r)
+    if (hipGraphLaunch(exec, st) != hipSuccess) status = TM_ERR_HIP;
+  if (status == TM_OK && n_iter % kChunk) status = launch(n_iter % kChunk);
+  hipGraphExecDestroy(exec);
+  hipGraphDestroy(graph);
+  hipStreamDestroy(cap);
+  return status;
+}
+
+}  // namespace tmh
+
+// ------------------------------------------------------------ C ABI: points --
+
+extern "C" size_t tm_sinkhorn_points_workspace_bytes(int npairs, const int32_t* n,
+                                                     const int32_t* m) {
+  int64_t sx = 0, sy = 0, tx = 0, ty = 0;
+  for (int p = 0; p < npairs; ++p) {
+    sx += n[p];
+    sy += m[p];
+    tx += (n[p] + tmh::kRowsPerTile - 1) / tmh::kRowsPerTile;
+    ty += (m[p] + tmh::kRowsPerTile - 1) / tmh::kRowsPerTile;
+  }
+  size_t b = 0;
+  auto add = [&](size_t x) { b += (x + 255) & ~(size_t)255; };
+  add(sizeof(tmh::SkPair) * npairs);
+  add(sizeof(tmh::SkTile) * (tx + ty));
+  add(sizeof(int32_t) * (npairs + 1));
+  add(sizeof(float) * tx);        // errpart
+  add(sizeof(int) * npairs);      // active
+  add(sizeof(int) * 4);           // n_active, iter_base
+  add(sizeof(float) * (sx + sy)); // log2 weights
+  add(sizeof(float) * sx * 2);    // f ping-pong
+  add(sizeof(float) * sy);        // g working copy
+  return b;
+}
+
+// Solves every pair to `tol` (L1 row-marginal error after a g-step) or
+// `max_iter`.  f/g hold the initial potentials on entry (zeros = cold start;
+// a warm start passes predicted f and any g, which is recomputed from f
+// before the first iteration) and the solution on return.
+extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a, const float* b,
+                                  int dim, int npairs, const int32_t* n, const int32_t* m,
+                                  const float* eps, float tol, int max_iter, int chunk, float* f,
+                                  float* g, int32_t* iters, float* err, void* workspace,
+                                  size_t workspace_bytes, void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(x && y && a && b && f && g && iters && err, "tm_sinkhorn_points: NULL argument");
+  TM_CHECK_ARG(dim >= 1 && dim <= 4, "tm_sinkhorn_points: dim must be 1..4, got %d", dim);
+  TM_CHECK_ARG(npairs >= 1, "tm_sinkhorn_points: npairs must be >= 1");
+  TM_CHECK_ARG(max_iter >= 1 && chunk >= 1, "tm_sinkhorn_points: max_iter and chunk must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<SkPair> hp(npairs);
+  std::vector<SkTile> htile;
+  std::vector<int32_t> hfirst(npairs + 1);
+  int64_t sx = 0, sy = 0;
+  int maxc = 1;
+  for (int p = 0; p < npairs; ++p) {
+    TM_CHECK_ARG(n[p] >= 1 && m[p] >= 1, "tm_sinkhorn_points: empty cloud in pair %d", p);
+    maxc = n[p] > maxc ? n[p] : maxc;
+    maxc = m[p] > maxc ? m[p] : maxc;
+    TM_CHECK_ARG(n[p] <= kMaxCols && m[p] <= kMaxCols,
+                 "tm_sinkhorn_points: clouds are limited to %d points (pair %d)", kMaxCols, p);
+    TM_CHECK_ARG(eps[p] > 0.f, "tm_sinkhorn_points: eps must be > 0 (pair %d)", p);
+    hp[p] = SkPair{sx, sy, n[p], m[p], eps[p], 0};
+    sx += n[p];
+    sy += m[p];
+  }
+  for (int p = 0; p < npairs; ++p) {
+    hfirst[p] = (int32_t)htile.size();
+    for (int r = 0; r < n[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+  }
+  hfirst[npairs] = (int32_t)htile.size();
+  const int64_t tx = (int64_t)htile.size();
+  for (int p = 0; p < npairs; ++p)
+    for (int r = 0; r < m[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+  const int64_t ty = (int64_t)htile.size() - tx;
+  const size_t need = tm_sinkhorn_points_workspace_bytes(npairs, n, m);
+  if (!workspace || workspace_bytes < need) {
+    set_error("tm_sinkhorn_points: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  char* w = (char*)workspace;
+  auto take = [&](size_t bytes) {
+    char* p = w;
+    w += (bytes + 255) & ~(size_t)255;
+    return (void*)p;
+  };
+  SkPair* dpairs = (SkPair*)take(sizeof(SkPair) * npairs);
+  SkTile* dtiles = (SkTile*)take(sizeof(SkTile) * (tx + ty));
+  int32_t* dfirst = (int32_t*)take(sizeof(int32_t) * (npairs + 1));
+  float* errpart = (float*)take(sizeof(float) * tx);
+  int* active = (int*)take(sizeof(int) * npairs);
+  int* ctr = (int*)take(sizeof(int) * 4);
+  int* n_active = ctr;
+  int* iter_base = ctr + 1;
+  float* lw = (float*)take(sizeof(float) * (sx + sy));
+  float* la = lw;
+  float* lb = lw + sx;
+  float* fbuf = (float*)take(sizeof(float) * sx * 2);
+  float* gw = (float*)take(sizeof(float) * sy);
+  // the small tables travel with the stream (pageable copies are staged
+  // before this call returns)
+  TM_CHECK_HIP(hipMemcpyAsync(dpairs, hp.data(), sizeof(SkPair) * npairs, hipMemcpyHostToDevice, st));
+  TM_CHECK_HIP(hipMemcpyAsync(dtiles, htile.data(), sizeof(SkTile) * (tx + ty),
+                              hipMemcpyHostToDevice, st));
+  TM_CHECK_HIP(hipMemcpyAsync(dfirst, hfirst.data(), sizeof(int32_t) * (npairs + 1),
+                              hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(sk_init_kernel, dim3(grid_for(npairs, 256)), dim3(256), 0, st, active, npairs,
+                     n_active, iter_base);
+  hipLaunchKernelGGL(sk_log2_kernel, dim3(grid_for(sx, 256)), dim3(256), 0, st, a, la, sx);
+  hipLaunchKernelGGL(sk_log2_kernel, dim3(grid_for(sy, 256)), dim3(256), 0, st, b, lb, sy);
+  hipLaunchKernelGGL(sk_copy_kernel, dim3(grid_for(sx, 256)), dim3(256), 0, st, f, fbuf, sx);
+  TM_LAUNCH_CHECK();
+  float* fA = fbuf;
+  float* fB = fbuf + sx;
+  hipStream_t lst = st;  // launch stream: a private stream while capturing
+
+#define SK_HALF(D, RX, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, NT, ERR)                     \
+  hipLaunchKernelGGL((sk_half_kernel<D, RX>), dim3((unsigned)(NT)), dim3(256),                  \
+                     (size_t)maxc * (D + 1) * sizeof(float), lst, ROWS, COLS, LWC, PC, PIN, POUT, \
+                     WR, dpairs, TILES, active, ERR, maxc)
+  auto gstep = [&](const float* fcur) -> int {
+    switch (dim) {
+      case 1: SK_HALF(1, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
+      case 2: SK_HALF(2, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
+      case 3: SK_HALF(3, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
+      default: SK_HALF(4, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
+    }
+    TM_LAUNCH_CHECK();
+    return TM_OK;
+  };
+  auto fstep = [&](const float* fin, float* fo) -> int {
+    switch (dim) {
+      case 1: SK_HALF(1, true, x, y, lb, gw, fin, fo, a, dtiles, tx, errpart); break;
+      case 2: SK_HALF(2, true, x, y, lb, gw, fin, fo, a, dtiles, tx, errpart); break;
+      case 3: SK_HALF(3, true, x, y, lb, gw, fin, fo, a, dtiles, tx, errpart); break;
+      default: SK_HALF(4, true, x, y, lb, gw, fin, fo, a, dtiles, tx, errpart); break;
+    }
+    TM_LAUNCH_CHECK();
+    return TM_OK;
+  };
+#undef SK_HALF
+  // state 0: g from the initial f (g's own initial values only seed the shift)
+  hipLaunchKernelGGL(sk_copy_kernel, dim3(grid_for(sy, 256)), dim3(256), 0, st, g, gw, sy);
+  TM_LAUNCH_CHECK();
+  if (gstep(fA) != TM_OK) return TM_ERR_HIP;
+
+  // one chunk = `chunk` iterations (chunk even keeps the f ping-pong aligned)
+  const int K = chunk % 2 ? chunk + 1 : chunk;
+  auto enqueue_chunk = [&]() -> int {
+    float* fo = fA;
+    float* fn = fB;
+    for (int k = 1; k <= K; ++k) {
+      if (fstep(fo, fn) != TM_OK) return TM_ERR_HIP;
+      hipLaunchKernelGGL(sk_check_kernel, dim3(npairs), dim3(256), 0, lst, dpairs, dfirst, errpart,
+                         active, n_active, iter_base, k, max_iter, tol, fo, gw, f, g, iters, err);
+      if (gstep(fn) != TM_OK) return TM_ERR_HIP;
+      float* t = fo;
+      fo = fn;
+      fn = t;
+    }
+    hipLaunchKernelGGL(sk_advance_kernel, dim3(1), dim3(64), 0, lst, iter_base, K);
+    TM_LAUNCH_CHECK();
+    return TM_OK;
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  const bool outer_capture = hipStreamIsCapturing(st, &cs) == hipSuccess &&
+                             cs != hipStreamCaptureStatusNone;
+  TM_CHECK_ARG(!outer_capture, "tm_sinkhorn_points synchronises on convergence and cannot be "
+                               "captured; capture tm_sinkhorn_points_iterate instead");
+  // capture one chunk on a private stream (the caller's may be the null
+  // stream, which cannot capture), replay it on the caller's stream
+  hipStream_t cap = nullptr;
+  bool use_graph = getenv("TORCHMORPH_AMD_NO_GRAPH") == nullptr &&
+                   hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) == hipSuccess;
+  if (use_graph) use_graph = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (use_graph) {
+    lst = cap;
+    const int s1 = enqueue_chunk();
+    lst = st;
+    use_graph = hipStreamEndCapture(cap, &graph) == hipSuccess && s1 == TM_OK;
+    if (use_graph) use_graph = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (!use_graph) {
+      if (graph) hipGraphDestroy(graph);
+      graph = nullptr;
+      (void)hipGetLastError();
+    }
+  }
+  int status = TM_OK;
+  int host_active = npairs;
+  for (int done = 0; done < max_iter + K && host_active > 0; done += K) {
+    if (use_graph) {
+      if (hipGraphLaunch(exec, st) != hipSuccess) { status = TM_ERR_HIP; break; }
+    } else if ((status = enqueue_chunk()) != TM_OK) {
+      break;
+    }
+    if (hipMemcpyAsync(&host_active, n_active, sizeof(int), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      status = TM_ERR_HIP;
+      break;
+    }
+  }
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  if (cap) hipStreamDestroy(cap);
+  if (status != TM_OK) {
+    set_error("tm_sinkhorn_points: HIP failure in the iteration loop");
+    return status;
+  }
+  return TM_OK;
+}
+
+// --------------------------------------------------------- C ABI: histograms --
+
+extern "C" int tm_sinkhorn_hist(int dtype, int log_space, const void* num_a, const void* num_b,
+                                const void* M, const void* M_T, void* u, void* v, int64_t nb,
+                                int d, int n_iter, double epsilon, int* used_fallback,
+                                void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(num_a && num_b && M && M_T && u && v, "tm_sinkhorn_hist: NULL argument");
+  TM_CHECK_ARG(nb >= 1 && d >= 1 && n_iter >= 0, "tm_sinkhorn_hist: bad sizes");
+  TM_CHECK_ARG(dtype == TM_F32 || dtype == TM_F64, "tm_sinkhorn_hist: dtype must be f32 or f64");
+  hipStream_t st = (hipStream_t)stream;
+  hipStream_t lst = st;
+  const dim3 grid(grid_for(nb * d, 256)), blk(256);
+  if (used_fallback) *used_fallback = 0;
+  auto launch = [&](int k) -> int {
+    for (int it = 0; it < k; ++it) {
+      if (dtype == TM_F32) {
+        if (log_space) {
+          const float ie = (float)(1.0 / epsilon);
+          hipLaunchKernelGGL(hist_log_update<float>, grid, blk, 0, lst, (const float*)num_a,
+                             (const float*)M, (const float*)v, (float*)u, nb, d, ie);
+          hipLaunchKernelGGL(hist_log_update<float>, grid, blk, 0, lst, (const float*)num_b,
+                             (const float*)M_T, (const float*)u, (float*)v, nb, d, ie);
+        } else {
+          hipLaunchKernelGGL(hist_scaling_update<float>, grid, blk, 0, lst, (const float*)num_a,
+                             (const float*)M, (const float*)v, (float*)u, nb, d);
+          hipLaunchKernelGGL(hist_scaling_update<float>, grid, blk, 0, lst, (const float*)num_b,
+                             (const float*)M_T, (const float*)u, (float*)v, nb, d);
+        }
+      } else {
+        if (log_space) {
+          const double ie = 1.0 / epsilon;
+          hipLaunchKernelGGL(hist_log_update<double>, grid, blk, 0, lst, (const double*)num_a,
+                             (const double*)M, (const double*)v, (double*)u, nb, d, ie);
+          hipLaunchKernelGGL(hist_log_update<double>, grid, blk, 0, lst, (const double*)num_b,
+                             (const double*)M_T, (const double*)u, (double*)v, nb, d, ie);
+        } else {
+          hipLaunchKernelGGL(hist_scaling_update<double>, grid, blk, 0, lst, (const double*)num_a,
+                             (const double*)M, (const double*)v, (double*)u, nb, d);
+          hipLaunchKernelGGL(hist_scaling_update<double>, grid, blk, 0, lst, (const double*)num_b,
+                             (const double*)M_T, (const double*)u, (double*)v, nb, d);
+        }
+      }
+      TM_LAUNCH_CHECK();
+    }
+    return TM_OK;
+  };
+  return run_chunked(launch, n_iter, st, &lst, used_fallback);
+}

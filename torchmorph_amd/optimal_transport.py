@@ -1,0 +1,340 @@
+"""Entropic optimal transport (log-domain Sinkhorn) on MI355X.
+
+Two entry points:
+
+* :class:`SinkhornSolver` / :func:`build_cost_matrix` -- the reference's
+  histogram API (reference torchmorph/optimal_transport.py:182-424): (n, d)
+  histograms sharing one (d, d) cost, same arguments, preprocessing,
+  envelope-theorem backward and messages.  GPU tensors (float32 and float64)
+  run the HIP kernels of libtorchmorph_hip (``tm_sinkhorn_hist``: one update
+  kernel per half-iteration, chunks of 25 iterations captured as a hipGraph
+  for max_iter >= 100, plain launches below, as the reference does).  CPU
+  tensors run the same iterations with torch operations, as the reference's
+  CPU path does (optimal_transport.py:259-300); that path never handles GPU
+  tensors.
+* :func:`sinkhorn_points` / :class:`SinkhornPointCloud` -- no reference
+  counterpart: ragged batches of weighted point-cloud pairs with the squared
+  Euclidean cost computed on the fly, solved to a row-marginal L1 error
+  tolerance with device-side convergence (``tm_sinkhorn_points``), optional
+  warm-start potentials, and a differentiable OT value (gradients from the
+  converged plan, see :class:`SinkhornPointCloud`).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import torch
+from torch import Tensor, nn
+
+from . import _lib
+
+# ------------------------------------------------------------------ histograms
+
+
+def build_cost_matrix(shape, p=2, device=None) -> Tensor:
+    """Pairwise L^p distances between the points of a grid (reference
+    optimal_transport.py:7-18)."""
+    coords = torch.stack(
+        torch.meshgrid([torch.arange(s, device=device) for s in shape], indexing="ij"), dim=-1)
+    coords = coords.reshape(-1, len(shape)).float()
+    return torch.cdist(coords, coords, p=p)
+
+
+def _transpose_or_self(matrix: Tensor) -> Tensor:
+    return matrix if torch.equal(matrix, matrix.mT) else matrix.mT.contiguous()
+
+
+def _hip_iterations(a: Tensor, b: Tensor, cost_matrix: Tensor, epsilon: float, max_iter: int,
+                    log_space: bool):
+    """HIP path for GPU tensors: returns (log_u, log_v)."""
+    L = _lib.lib()
+    dtype = {torch.float32: _lib.TM_F32, torch.float64: _lib.TM_F64}.get(a.dtype)
+    if dtype is None:
+        raise RuntimeError(f"SinkhornSolver: unsupported GPU dtype {a.dtype}")
+    a, b = a.contiguous(), b.contiguous()
+    n, d = a.shape
+    with torch.cuda.device(a.device):
+        if log_space:
+            num_a, num_b = a.log().contiguous(), b.log().contiguous()
+            M = cost_matrix.contiguous()
+            u = torch.zeros_like(a)
+            v = torch.zeros_like(b)
+        else:
+            num_a, num_b = a, b
+            M = torch.exp(-cost_matrix / epsilon).contiguous()
+            u = torch.ones_like(a)
+            v = torch.ones_like(b)
+        MT = _transpose_or_self(M)
+        fb = _lib.ctypes.c_int(0)
+        _lib.check(L.tm_sinkhorn_hist(dtype, int(log_space), _lib.ptr(num_a), _lib.ptr(num_b),
+                                      _lib.ptr(M), _lib.ptr(MT), _lib.ptr(u), _lib.ptr(v), n, d,
+                                      int(max_iter), float(epsilon), _lib.ctypes.byref(fb),
+                                      _lib.stream_of(a.device)), "tm_sinkhorn_hist")
+    if log_space:
+        return u, v
+    return u.log(), v.log()
+
+
+def _torch_iterations(a, b, cost_matrix, epsilon, max_iter, threshold, log_space):
+    """CPU tensors: the reference's torch iterations (optimal_transport.py:259-300)."""
+    if log_space:
+        log_a, log_b = a.log(), b.log()
+        log_K = -cost_matrix / epsilon
+        log_u = torch.zeros_like(a)
+        log_v = torch.zeros_like(b)
+        for i in range(max_iter):
+            prev = log_u
+            log_u = log_a - torch.logsumexp(log_K + log_v.unsqueeze(-2), dim=-1)
+            log_v = log_b - torch.logsumexp(log_K.mT + log_u.unsqueeze(-2), dim=-1)
+            if threshold > 0 and i % 10 == 0:
+                if (log_u - prev).abs().sum(dim=-1).max() <= threshold:
+                    break
+        return log_u, log_v
+    K = torch.exp(-cost_matrix / epsilon)
+    u = torch.ones_like(a)
+    v = torch.ones_like(b)
+    for i in range(max_iter):
+        prev = u
+        u = a / (v @ K.mT + 1e-12)
+        v = b / (u @ K + 1e-12)
+        if threshold > 0 and i % 10 == 0:
+            if (u - prev).abs().sum(dim=-1).max() <= threshold:
+                break
+    return u.log(), v.log()
+
+
+def _iterate(a, b, cost_matrix, epsilon, max_iter, threshold, log_space):
+    if a.is_cuda:
+        return _hip_iterations(a, b, cost_matrix, epsilon, max_iter, log_space)
+    return _torch_iterations(a, b, cost_matrix, epsilon, max_iter, threshold, log_space)
+
+
+class _SinkhornDistance(torch.autograd.Function):
+    """<P, C> with the centred dual potentials as gradients w.r.t. the
+    marginals (reference optimal_transport.py:303-325)."""
+
+    @staticmethod
+    def forward(ctx, a, b, cost_matrix, epsilon, max_iter, threshold, log_space):
+        log_u, log_v = _iterate(a, b, cost_matrix, epsilon, max_iter, threshold, log_space)
+        plan = torch.exp(log_u.unsqueeze(-1) - cost_matrix / epsilon + log_v.unsqueeze(-2))
+        distance = (plan * cost_matrix).sum(dim=(-2, -1))
+        f = epsilon * log_u
+        g = epsilon * log_v
+        ctx.save_for_backward(f - f.mean(dim=-1, keepdim=True), g - g.mean(dim=-1, keepdim=True))
+        return distance
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        f, g = ctx.saved_tensors
+        grad = grad_output.unsqueeze(-1)
+        return grad * f, grad * g, None, None, None, None, None
+
+
+class SinkhornSolver(nn.Module):
+    """Entropy-regularised balanced OT between (n, d) histograms sharing one
+    (d, d) cost (reference optimal_transport.py:328-424)."""
+
+    def __init__(self, epsilon=1.0, max_iter=100, threshold=0.0, p=2, log_space=False):
+        super().__init__()
+        if epsilon <= 0:
+            raise ValueError("epsilon must be positive.")
+        if max_iter <= 0:
+            raise ValueError("max_iter must be positive.")
+        if threshold < 0:
+            raise ValueError("threshold must be non-negative.")
+        self.epsilon = epsilon
+        self.max_iter = max_iter
+        self.threshold = threshold
+        self.p = p
+        self.log_space = log_space
+
+    def extra_repr(self):
+        return (f"epsilon={self.epsilon}, max_iter={self.max_iter}, "
+                f"threshold={self.threshold}, log_space={self.log_space}")
+
+    def data_preprocess(self, source, target, cost_matrix: Optional[Tensor] = None):
+        if source.ndim != 2 or source.shape != target.shape:
+            raise ValueError("source and target must be (n, d) tensors of the same shape.")
+        if not source.is_floating_point():
+            raise ValueError("source and target must be floating-point tensors.")
+        d = source.shape[1]
+        if cost_matrix is None:
+            cost_matrix = build_cost_matrix((d,), self.p, source.device)
+        elif cost_matrix.shape != (d, d):
+            raise ValueError(
+                f"cost_matrix must have shape ({d}, {d}), got {tuple(cost_matrix.shape)}.")
+        cost_matrix = cost_matrix.to(device=source.device, dtype=source.dtype).contiguous()
+        source = source.clamp(min=0)
+        target = target.clamp(min=0)
+        source = source / source.sum(dim=-1, keepdim=True).clamp(min=1e-12)
+        target = target / target.sum(dim=-1, keepdim=True).clamp(min=1e-12)
+        return source, target, cost_matrix
+
+    def forward(self, source: Tensor, target: Tensor, cost_matrix: Optional[Tensor] = None):
+        source, target, cost_matrix = self.data_preprocess(source, target, cost_matrix)
+        return _SinkhornDistance.apply(source, target, cost_matrix, self.epsilon, self.max_iter,
+                                       self.threshold, self.log_space)
+
+    @torch.no_grad()
+    def plan(self, source: Tensor, target: Tensor, cost_matrix: Optional[Tensor] = None):
+        source, target, cost_matrix = self.data_preprocess(source, target, cost_matrix)
+        log_u, log_v = self._iterate(source, target, cost_matrix)
+        return torch.exp(log_u.unsqueeze(-1) - cost_matrix / self.epsilon + log_v.unsqueeze(-2))
+
+    @torch.no_grad()
+    def potentials(self, source: Tensor, target: Tensor, cost_matrix: Optional[Tensor] = None):
+        source, target, cost_matrix = self.data_preprocess(source, target, cost_matrix)
+        log_u, log_v = self._iterate(source, target, cost_matrix)
+        f = self.epsilon * log_u
+        g = self.epsilon * log_v
+        return f - f.mean(dim=-1, keepdim=True), g - g.mean(dim=-1, keepdim=True)
+
+    def _iterate(self, a, b, cost_matrix):
+        return _iterate(a, b, cost_matrix, self.epsilon, self.max_iter, self.threshold,
+                        self.log_space)
+
+
+# ---------------------------------------------------------------- point clouds
+
+
+@dataclass
+class SinkhornResult:
+    """Solution of a batch of point-cloud pairs (concatenated layout).
+
+    f, g: potentials of every point (sum n_p,), (sum m_p,); iterations and
+    err: per pair.  split_f()/split_g() return per-pair views."""
+
+    f: Tensor
+    g: Tensor
+    iterations: Tensor
+    err: Tensor
+    n: list
+    m: list
+    eps: Tensor
+
+    def split_f(self):
+        return list(torch.split(self.f, self.n))
+
+    def split_g(self):
+        return list(torch.split(self.g, self.m))
+
+
+def _as_ragged(clouds) -> tuple[Tensor, list]:
+    if isinstance(clouds, Tensor):
+        if clouds.ndim != 3:
+            raise ValueError("point clouds must be (B, n, d) or a list of (n_p, d) tensors")
+        return clouds.reshape(-1, clouds.shape[-1]), [clouds.shape[1]] * clouds.shape[0]
+    sizes = [int(c.shape[0]) for c in clouds]
+    return torch.cat(list(clouds), dim=0), sizes
+
+
+def _weights(w, sizes, device) -> Tensor:
+    if w is None:
+        return torch.cat([torch.full((s,), 1.0 / s, device=device) for s in sizes])
+    w = torch.cat(list(w)) if not isinstance(w, Tensor) else w.reshape(-1)
+    return w.to(device=device, dtype=torch.float32)
+
+
+def _center(x: Tensor, y: Tensor, n: list, m: list) -> tuple[Tensor, Tensor]:
+    """Shift both clouds of every pair by the mean of its y cloud (the cost is
+    invariant to a common translation; centring keeps |x|^2 small)."""
+    pid_y = torch.repeat_interleave(torch.arange(len(m), device=y.device),
+                                    torch.tensor(m, device=y.device))
+    sums = torch.zeros((len(m), y.shape[1]), device=y.device, dtype=y.dtype)
+    sums.index_add_(0, pid_y, y)
+    mean = sums / torch.tensor(m, device=y.device, dtype=y.dtype)[:, None]
+    pid_x = torch.repeat_interleave(torch.arange(len(n), device=x.device),
+                                    torch.tensor(n, device=x.device))
+    return x - mean[pid_x], y - mean[pid_y]
+
+
+def sinkhorn_points(xs, ys, eps, a=None, b=None, tol: float = 1e-5, max_iter: int = 20000,
+                    f_init: Optional[Tensor] = None, chunk: int = 10) -> SinkhornResult:
+    """Solves entropic OT between every pair (xs[p], ys[p]) with cost
+    |x - y|^2, to an L1 row-marginal error <= tol measured after a g-step.
+
+    xs, ys: (B, n, d) / (B, m, d) tensors or lists of (n_p, d) / (m_p, d);
+    eps: float or (B,) tensor; a, b: weights (default uniform), same layouts;
+    f_init: concatenated initial f (warm start), default zeros.
+    Runs entirely on the GPU (tm_sinkhorn_points): one launch per half-step
+    for the whole ragged batch, convergence checked on the device, the
+    iteration loop replayed as a hipGraph of `chunk` iterations.
+    """
+    x, n = _as_ragged(xs)
+    y, m = _as_ragged(ys)
+    if len(n) != len(m):
+        raise ValueError("xs and ys must hold the same number of clouds")
+    if not x.is_cuda:
+        raise ValueError("sinkhorn_points runs on GPU tensors")
+    dev = x.device
+    x = x.detach().to(torch.float32)
+    y = y.detach().to(torch.float32)
+    if x.shape[1] != y.shape[1] or not 1 <= x.shape[1] <= 4:
+        raise ValueError("point dimension must match and be in 1..4")
+    x, y = _center(x, y, n, m)
+    x, y = x.contiguous(), y.contiguous()
+    P = len(n)
+    aw = _weights(a, n, dev).contiguous()
+    bw = _weights(b, m, dev).contiguous()
+    if isinstance(eps, Tensor):
+        eps_t = eps.detach().to(torch.float32).reshape(-1).cpu()
+    else:
+        eps_t = torch.full((P,), float(eps))
+    if eps_t.numel() != P:
+        raise ValueError("eps must be a scalar or hold one value per pair")
+    f = (torch.zeros(sum(n), device=dev) if f_init is None
+         else f_init.detach().to(device=dev, dtype=torch.float32).reshape(-1).clone())
+    g = torch.zeros(sum(m), device=dev)
+    iters = torch.zeros(P, dtype=torch.int32, device=dev)
+    err = torch.zeros(P, dtype=torch.float32, device=dev)
+    L = _lib.lib()
+    n_arr = (_lib.ctypes.c_int32 * P)(*n)
+    m_arr = (_lib.ctypes.c_int32 * P)(*m)
+    e_arr = (_lib.ctypes.c_float * P)(*[float(v) for v in eps_t.tolist()])
+    with torch.cuda.device(dev):
+        nbytes = L.tm_sinkhorn_points_workspace_bytes(P, n_arr, m_arr)
+        ws = _lib.workspace(nbytes, dev)
+        _lib.check(L.tm_sinkhorn_points(_lib.ptr(x), _lib.ptr(y), _lib.ptr(aw), _lib.ptr(bw),
+                                        x.shape[1], P, n_arr, m_arr, e_arr, float(tol),
+                                        int(max_iter), int(chunk), _lib.ptr(f), _lib.ptr(g),
+                                        _lib.ptr(iters), _lib.ptr(err), _lib.ptr(ws), nbytes,
+                                        _lib.stream_of(dev)), "tm_sinkhorn_points")
+    return SinkhornResult(f, g, iters, err, n, m, eps_t)
+
+
+def median_sqeuclid(x: Tensor, y: Tensor) -> Tensor:
+    """Median of |x_i - y_j|^2 over all pairs of one (n, d) / (m, d) pair."""
+    return torch.cdist(x.float(), y.float()).square().flatten().median()
+
+
+class SinkhornPointCloud(torch.autograd.Function):
+    """Entropic OT value <f, a> + <g, b> between two point clouds, with
+    gradients from the converged solution: d/da = f, d/db = g (each centred,
+    the mass-preserving direction), d/dx_i = 2 sum_j P_ij (x_i - y_j),
+    d/dy_j = 2 sum_i P_ij (y_j - x_i).  At the fixed point these are the exact
+    derivatives of the dual value (envelope theorem); tests check them against
+    finite differences and autograd through an unrolled FP64 solver."""
+
+    @staticmethod
+    def forward(ctx, x, y, a, b, eps, tol, max_iter):
+        res = sinkhorn_points(x[None], y[None], eps, a[None], b[None], tol=tol, max_iter=max_iter)
+        f, g = res.f, res.g
+        value = (f * a.float()).sum() + (g * b.float()).sum()
+        ctx.save_for_backward(x, y, a, b, f, g)
+        ctx.eps = eps
+        return value.to(x.dtype)
+
+    @staticmethod
+    def backward(ctx, grad):
+        x, y, a, b, f, g = ctx.saved_tensors
+        eps = ctx.eps
+        xf, yf = x.float(), y.float()
+        C = torch.cdist(xf, yf).square()
+        P = torch.exp((f[:, None] + g[None, :] - C) / eps) * a.float()[:, None] * b.float()[None, :]
+        gx = 2 * (P.sum(1, keepdim=True) * xf - P @ yf)
+        gy = 2 * (P.sum(0)[:, None] * yf - P.T @ xf)
+        ga = f - f.mean()
+        gb = g - g.mean()
+        return (grad * gx).to(x.dtype), (grad * gy).to(y.dtype), (grad * ga).to(a.dtype), \
+            (grad * gb).to(b.dtype), None, None, None
