@@ -92,6 +92,58 @@ int tm_cdt(const void* input, int input_dtype, int64_t batch, int ndim, const in
            int metric, float* distances, void* indices, int index_dtype, void* workspace,
            size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------- grey morphology --- */
+
+/* Workspace bytes for tm_grey (0 is enough for the fused kernels; the per-axis
+ * and general fallbacks need it and report TM_ERR_WORKSPACE before launching
+ * anything when it is missing). */
+size_t tm_grey_workspace_bytes(int64_t batch, int ndim, const int64_t* shape,
+                               const int64_t* se_shape);
+
+/* Grey erosion (dilation = 0) / dilation (dilation = 1) of float32 items with
+ * the reference's element offsets, border modes (0 constant, 1 reflect,
+ * 2 nearest, 3 mirror, 4 wrap) and cval.  structure (float32) and footprint
+ * (uint8) are HOST arrays of prod(se_shape) values, either may be NULL
+ * (flat / all active).  Replaces torchmorph._C.grey_erosion_cuda /
+ * grey_dilation_cuda (reference torchmorph.cpp:3-19, grey_kernel.cu:172-330). */
+int tm_grey(const float* input, int64_t batch, int ndim, const int64_t* shape,
+            const float* structure, const uint8_t* footprint, const int64_t* se_shape,
+            const int64_t* origin, int mode, float cval, int dilation, float* output,
+            void* workspace, size_t workspace_bytes, void* stream);
+
+size_t tm_grey_backward_workspace_bytes(int64_t batch, int ndim, const int64_t* shape);
+
+/* No reference counterpart (the reference ops are not differentiable).
+ * Gradient of tm_grey w.r.t. its input: grad_out of every output is routed to
+ * the input voxel read by the first active element (element order) attaining
+ * the min/max; deterministic, no atomics.  At most 256 active elements. */
+int tm_grey_backward(const float* input, int64_t batch, int ndim, const int64_t* shape,
+                     const float* structure, const uint8_t* footprint, const int64_t* se_shape,
+                     const int64_t* origin, int mode, float cval, int dilation,
+                     const float* grad_out, float* grad_in, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* -------------------------------------------------- binary morphology --- */
+
+/* Workspace bytes for tm_binary (packed ping/pong/orig/mask + flags). */
+size_t tm_binary_workspace_bytes(int64_t batch, int ndim, const int64_t* shape);
+
+/* Binary morphology as one bit-packed pipeline: runs nops operations in
+ * order, op k being erosion (op_dilation[k] = 0) or dilation (1) with
+ * op_iterations[k] iterations (< 1: until an iteration changes nothing,
+ * synchronising the host every 8 iterations).  structure: HOST uint8 array of
+ * prod(se_shape); border_value for reads outside the item; mask (device
+ * uint8, (batch, spatial...), may be NULL) keeps each step's input where it is
+ * 0.  input uint8/bool or float32 (nonzero = set); output uint8 (0/1).
+ * Replaces torchmorph._C.binary_erosion_cuda / binary_dilation_cuda and the
+ * Python iteration loop around them (reference torchmorph.cpp:21-33,
+ * binary_kernel.cu:129-250, binary.py:28-88). */
+int tm_binary(const void* input, int input_dtype, int64_t batch, int ndim, const int64_t* shape,
+              const uint8_t* structure, const int64_t* se_shape, const int64_t* origin,
+              int border_value, int nops, const int* op_dilation, const int* op_iterations,
+              const uint8_t* mask, uint8_t* output, void* workspace, size_t workspace_bytes,
+              void* stream);
+
 /* ----------------------------------------------------------- Sinkhorn --- */
 
 /* Histogram Sinkhorn, n_iter iterations on (nb, d) rows sharing one (d, d)
@@ -123,6 +175,20 @@ int tm_sinkhorn_points(const float* x, const float* y, const float* a, const flo
                        int npairs, const int32_t* n, const int32_t* m, const float* eps,
                        float tol, int max_iter, int chunk, float* f, float* g, int32_t* iters,
                        float* err, void* workspace, size_t workspace_bytes, void* stream);
+
+size_t tm_sinkhorn_points_grad_workspace_bytes(int npairs, const int32_t* n, const int32_t* m);
+
+/* No reference counterpart (the reference's backward only returns the
+ * potentials as marginal gradients, optimal_transport.py:321-325).  Gradient
+ * of every pair's entropic OT value <f,a> + <g,b> w.r.t. its points from the
+ * converged potentials (envelope theorem at the fixed point):
+ * gx_i = 2 sum_j P_ij (x_i - y_j), gy_j = 2 sum_i P_ij (y_j - x_i), with the
+ * plan P formed on the fly (never materialised).  Layouts as for
+ * tm_sinkhorn_points. */
+int tm_sinkhorn_points_grad(const float* x, const float* y, const float* a, const float* b,
+                            int dim, int npairs, const int32_t* n, const int32_t* m,
+                            const float* eps, const float* f, const float* g, float* gx,
+                            float* gy, void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

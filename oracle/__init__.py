@@ -263,3 +263,84 @@ def sinkhorn_points(x, y, a, b, eps, tol=1e-5, max_iter=100000, f0=None, g0=None
 def plan_points(x, y, a, b, f, g, eps):
     c = sqeuclid(x, y)
     return np.exp((f[:, None] + g[None, :] - c) / eps) * np.asarray(a)[:, None] * np.asarray(b)[None, :]
+
+
+# ------------------------------------------------ grey morphology (numpy) --
+def _map_coord_np(c, size, mode):
+    """Border modes of reference grey_kernel.cu:63-97, vectorised; returns
+    (mapped, use_cval)."""
+    c = np.asarray(c, np.int64)
+    inb = (c >= 0) & (c < size)
+    if mode == "constant":
+        return np.where(inb, c, 0), ~inb
+    if mode == "nearest":
+        return np.clip(c, 0, size - 1), np.zeros_like(inb)
+    if mode == "reflect":
+        period = 2 * size
+        m = np.mod(c, period)
+        return np.where(m >= size, period - 1 - m, m), np.zeros_like(inb)
+    if mode == "mirror":
+        period = 2 * (size - 1)
+        if period <= 0:
+            return np.zeros_like(c), np.zeros_like(inb)
+        m = np.mod(c, period)
+        return np.where(m >= size, period - m, m), np.zeros_like(inb)
+    if mode == "wrap":
+        return np.mod(c, size), np.zeros_like(inb)
+    raise ValueError(mode)
+
+
+def _grey_table(structure, footprint, se_shape, origin, dilation):
+    offs, vals = [], []
+    for e in np.ndindex(*se_shape):
+        if footprint is not None and not footprint[e]:
+            continue
+        d = [e[j] - (se_shape[j] // 2 + origin[j]) for j in range(len(se_shape))]
+        s = 0.0 if structure is None else float(structure[e])
+        offs.append([-v for v in d] if dilation else d)
+        vals.append(s if dilation else -s)
+    return np.array(offs, np.int64), np.array(vals, np.float64)
+
+
+def _grey_candidates(x, offs, vals, mode, cval):
+    """Per element t and output p: (value, flat source index or -1)."""
+    shape = x.shape
+    coords = np.indices(shape).reshape(len(shape), -1)
+    n = coords.shape[1]
+    V = np.empty((len(offs), n), np.float64)
+    SRC = np.empty((len(offs), n), np.int64)
+    xf = x.reshape(-1).astype(np.float64)
+    for t, (o, v) in enumerate(zip(offs, vals)):
+        flat = np.zeros(n, np.int64)
+        usec = np.zeros(n, bool)
+        for j in range(len(shape)):
+            m, uc = _map_coord_np(coords[j] + o[j], shape[j], mode)
+            flat = flat * shape[j] + m
+            usec |= uc
+        V[t] = np.where(usec, cval, xf[flat]) + v
+        SRC[t] = np.where(usec, -1, flat)
+    return V, SRC
+
+
+def grey_fp64(x, structure, footprint, se_shape, origin, mode, cval, dilation):
+    """Grey erosion/dilation of one item in float64 (same element semantics
+    as tmo_grey), for finite-difference checks."""
+    offs, vals = _grey_table(structure, footprint, se_shape, origin, dilation)
+    V, _ = _grey_candidates(x, offs, vals, mode, cval)
+    return (V.max(0) if dilation else V.min(0)).reshape(x.shape)
+
+
+def grey_backward(x, structure, footprint, se_shape, origin, mode, cval, dilation, dy):
+    """Gradient of one item: dy[p] goes to the source of the first element
+    attaining the extremum (np.argmax/argmin return the first occurrence),
+    nothing for cval reads.  Same rule as the HIP backward kernels."""
+    offs, vals = _grey_table(structure, footprint, se_shape, origin, dilation)
+    V, SRC = _grey_candidates(x.astype(np.float32), offs, vals.astype(np.float32), mode,
+                              np.float32(cval))
+    V = V.astype(np.float32)
+    arg = V.argmax(0) if dilation else V.argmin(0)
+    src = SRC[arg, np.arange(V.shape[1])]
+    w = np.asarray(dy, np.float64).reshape(-1)
+    keep = src >= 0
+    dx = np.bincount(src[keep], weights=w[keep], minlength=x.size)
+    return dx.reshape(x.shape)

@@ -1,0 +1,100 @@
+"""Hand-written grey erosion/dilation backward: routing rule against the CPU
+oracle, and FP64 finite differences at tie-free points."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tm():
+    import torchmorph_amd
+
+    return torchmorph_amd
+
+
+def _grad(tm, x, dy, dilation, **kw):
+    xt = torch.tensor(x, device="cuda", requires_grad=True)
+    fn = tm.grey_dilation if dilation else tm.grey_erosion
+    y = fn(xt, differentiable=True, **kw)
+    assert y.requires_grad
+    y.backward(torch.tensor(dy, device="cuda"))
+    return y.detach().cpu().numpy(), xt.grad.cpu().numpy()
+
+
+CASES = [
+    dict(size=3, mode="reflect"),
+    dict(size=(3, 5), mode="constant", cval=0.3),
+    dict(size=(2, 3), mode="nearest", origin=(0, 1)),
+    dict(size=3, mode="mirror"),
+    dict(size=(3, 3), mode="wrap"),
+]
+
+
+@pytest.mark.parametrize("kw", CASES, ids=lambda k: f"{k['mode']}")
+@pytest.mark.parametrize("dilation", [False, True])
+def test_grey_backward_matches_oracle_with_ties(tm, oracle_mod, kw, dilation):
+    """Quantised input (many ties): exact routing rule vs the oracle."""
+    rng = np.random.default_rng(0)
+    x = (np.floor(rng.random((2, 1, 23, 31)) * 8) / 8).astype(np.float32)
+    dy = rng.normal(size=x.shape).astype(np.float32)
+    y, dx = _grad(tm, x, dy, dilation, **kw)
+    size = kw["size"]
+    se = (size, size) if isinstance(size, int) else size
+    origin = kw.get("origin", (0, 0))
+    for b in range(2):
+        want = oracle_mod.grey_backward(x[b, 0], None, None, se, origin, kw["mode"],
+                                        kw.get("cval", 0.0), dilation, dy[b, 0])
+        np.testing.assert_allclose(dx[b, 0], want, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("dilation", [False, True])
+def test_grey_backward_nonflat_3d_vs_oracle(tm, oracle_mod, dilation):
+    rng = np.random.default_rng(1)
+    x = rng.random((1, 1, 9, 10, 12)).astype(np.float32)
+    zz, yy, xx = np.mgrid[-1:2, -1:2, -1:2]
+    fp = (zz ** 2 + yy ** 2 + xx ** 2) <= 2
+    st = rng.integers(0, 4, size=fp.shape).astype(np.float32)
+    dy = rng.normal(size=x.shape).astype(np.float32)
+    _, dx = _grad(tm, x, dy, dilation, footprint=torch.tensor(fp),
+                  structure=torch.tensor(st), mode="reflect")
+    want = oracle_mod.grey_backward(x[0, 0], st, fp, (3, 3, 3), (0, 0, 0), "reflect", 0.0,
+                                    dilation, dy[0, 0])
+    np.testing.assert_allclose(dx[0, 0], want, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("dilation", [False, True])
+def test_grey_backward_fp64_finite_differences(tm, oracle_mod, dilation):
+    """Tie-free input: the routed gradient is the exact derivative; check
+    directional derivatives against central differences of an FP64 forward."""
+    rng = np.random.default_rng(2)
+    x = rng.permutation(20 * 24).reshape(1, 1, 20, 24).astype(np.float32) / 7.0
+    dy = rng.normal(size=x.shape)
+    _, dx = _grad(tm, x, dy.astype(np.float32), dilation, size=3, mode="reflect")
+    for k in range(3):
+        v = rng.normal(size=x.shape)
+        h = 1e-6
+        fp = oracle_mod.grey_fp64(x[0, 0].astype(np.float64) + h * v[0, 0], None, None, (3, 3),
+                                  (0, 0), "reflect", 0.0, dilation)
+        fm = oracle_mod.grey_fp64(x[0, 0].astype(np.float64) - h * v[0, 0], None, None, (3, 3),
+                                  (0, 0), "reflect", 0.0, dilation)
+        fd = ((fp - fm) / (2 * h) * dy[0, 0]).sum()
+        an = (dx.astype(np.float64) * v).sum()
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd)), (fd, an)
+
+
+def test_grey_backward_config4_shape(tm, oracle_mod):
+    """16x1x512x512 float32, flat size 3: full-size run, one slice checked."""
+    g = torch.Generator().manual_seed(4)
+    x = torch.rand((16, 1, 512, 512), generator=g)
+    dy = torch.randn((16, 1, 512, 512), generator=g)
+    xt = x.cuda().requires_grad_(True)
+    tm.grey_dilation(xt, size=3, differentiable=True).backward(dy.cuda())
+    want = oracle_mod.grey_backward(x[3, 0, :64, :64].numpy(), None, None, (3, 3), (0, 0),
+                                    "reflect", 0.0, True, dy[3, 0, :64, :64].numpy())
+    # interior of the crop (away from the crop's own border) must match
+    np.testing.assert_allclose(xt.grad[3, 0, 1:63, 1:63].cpu().numpy(), want[1:63, 1:63],
+                               rtol=1e-6, atol=1e-6)
+    # total routed gradient equals the total upstream gradient (no cval reads)
+    assert abs(float(xt.grad.double().sum()) - float(dy.double().sum())) < 1e-2

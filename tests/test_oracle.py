@@ -253,3 +253,31 @@ def test_oracle_matches_scipy_on_fresh_random_inputs(oracle_mod):
             d, i = oracle_mod.cdt(m[None, None], metric)
             sd, si = ndi.distance_transform_cdt(m, metric=metric, return_indices=True)
             assert np.array_equal(d[0, 0], sd) and np.array_equal(i[:, 0, 0], si)
+
+
+def test_grey_backward_oracle_fd_and_mass(oracle_mod):
+    """The oracle's routing rule is the derivative at tie-free points and
+    conserves the upstream gradient when no cval is read."""
+    rng = np.random.default_rng(5)
+    x = rng.permutation(11 * 13).reshape(11, 13).astype(np.float64) / 5.0
+    dy = rng.normal(size=x.shape)
+    for dil in (False, True):
+        dx = oracle_mod.grey_backward(x.astype(np.float32), None, None, (3, 3), (0, 0),
+                                      "reflect", 0.0, dil, dy)
+        assert abs(dx.sum() - dy.sum()) < 1e-9
+        v = rng.normal(size=x.shape)
+        h = 1e-7
+        fp = oracle_mod.grey_fp64(x + h * v, None, None, (3, 3), (0, 0), "reflect", 0.0, dil)
+        fm = oracle_mod.grey_fp64(x - h * v, None, None, (3, 3), (0, 0), "reflect", 0.0, dil)
+        assert abs(((fp - fm) / (2 * h) * dy).sum() - (dx * v).sum()) < 1e-6
+    # first element in element order wins a tie: flat 1-D size 3 dilation of a
+    # constant line routes everything to the element reading p+1 (offset -d
+    # of the first element d=-1)
+    dx = oracle_mod.grey_backward(np.zeros((1, 5), np.float32), None, None, (1, 3), (0, 0),
+                                  "nearest", 0.0, True, np.ones((1, 5)))
+    np.testing.assert_array_equal(dx, [[0, 1, 1, 1, 2]])
+    # the float32 forward of the oracle agrees with the float64 one on exact data
+    xq = np.floor(rng.random((9, 9)) * 16).astype(np.float32)
+    np.testing.assert_array_equal(
+        oracle_mod.grey(xq[None, None], None, None, (3, 3), (0, 0), "mirror", 0.0, True)[0, 0],
+        oracle_mod.grey_fp64(xq, None, None, (3, 3), (0, 0), "mirror", 0.0, True))

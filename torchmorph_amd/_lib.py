@@ -47,6 +47,15 @@ SIGNATURES: dict[str, tuple] = {
     "tm_sinkhorn_points": (_I, [_P, _P, _P, _P, _I, _I, ctypes.POINTER(ctypes.c_int32),
                                 ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float),
                                 _F, _I, _I, _P, _P, _P, _P, _P, _SZ, _P]),
+    "tm_grey_backward_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
+    "tm_grey_backward": (_I, [_P, _I64, _I, _I64P, _P, _P, _I64P, _I64P, _I, _F, _I, _P, _P, _P,
+                              _SZ, _P]),
+    "tm_sinkhorn_points_grad_workspace_bytes": (_SZ, [_I, ctypes.POINTER(ctypes.c_int32),
+                                                      ctypes.POINTER(ctypes.c_int32)]),
+    "tm_sinkhorn_points_grad": (_I, [_P, _P, _P, _P, _I, _I, ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_float), _P, _P, _P, _P, _P, _SZ,
+                                     _P]),
     "tm_binary_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
     "tm_binary": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _I, _I,
                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _P, _P, _P,

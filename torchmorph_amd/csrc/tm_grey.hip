@@ -439,3 +439,195 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
   // may be reused once this call returns
   return TM_OK;
 }
+
+namespace tmh {
+
+// ------------------------------------------------------------- backward --
+// Hand-written backward of grey erosion / dilation (no reference
+// counterpart: the reference ops are not differentiable, test_grey.py:218-222).
+// Unified form y[p] = OP_i E(p + o_i) + v_i (erosion: OP = min, o = d,
+// v = -s; dilation: OP = max, o = -d, v = s).  The gradient of y[p] goes to
+// the input voxel read by the FIRST element (table order = the reference's
+// element order) attaining the extremum; elements reading cval route nothing.
+// Two kernels, no atomics: arg[p] = winning element, then every input voxel
+// gathers dy[p] over the outputs whose winning element reads it, in a fixed
+// order (deterministic).
+
+struct GreyTabN {
+  int n;
+  int32_t ext[kMaxNdim];  // max |offset| per axis
+  int8_t o[kMaxTable][kMaxNdim];
+  float v[kMaxTable];
+};
+
+template <bool MAX>
+__global__ __launch_bounds__(256) void grey_arg_kernel(const float* __restrict__ in,
+                                                       int16_t* __restrict__ arg, Shape sh,
+                                                       int64_t total, GreyTabN tab, int mode,
+                                                       float cval) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const int nd = sh.ndim;
+  const int64_t item = g / sh.numel;
+  int64_t rem = g - item * sh.numel;
+  int64_t c[kMaxNdim];
+  for (int j = 0; j < nd; ++j) {
+    c[j] = rem / sh.stride[j];
+    rem -= c[j] * sh.stride[j];
+  }
+  const float* vol = in + item * sh.numel;
+  float best = 0.f;
+  int bi = 0;
+  for (int t = 0; t < tab.n; ++t) {
+    bool use_c = false;
+    int64_t flat = 0;
+    for (int j = 0; j < nd && !use_c; ++j) {
+      bool uc;
+      const int64_t m = gmap(c[j] + tab.o[t][j], sh.size[j], mode, &uc);
+      use_c |= uc;
+      flat += m * sh.stride[j];
+    }
+    const float v = (use_c ? cval : vol[flat]) + tab.v[t];
+    if (t == 0 || (MAX ? v > best : v < best)) {
+      best = v;
+      bi = t;
+    }
+  }
+  arg[g] = (int16_t)bi;
+}
+
+__global__ __launch_bounds__(256) void grey_grad_gather_kernel(
+    const int16_t* __restrict__ arg, const float* __restrict__ dy, float* __restrict__ dx,
+    Shape sh, int64_t total, GreyTabN tab, int mode) {
+  const int32_t* ext = tab.ext;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const int nd = sh.ndim;
+  const int64_t item = g / sh.numel;
+  int64_t rem = g - item * sh.numel;
+  int64_t q[kMaxNdim];
+  bool interior = true;
+  for (int j = 0; j < nd; ++j) {
+    q[j] = rem / sh.stride[j];
+    rem -= q[j] * sh.stride[j];
+    if (q[j] < ext[j] || q[j] > sh.size[j] - 1 - ext[j]) interior = false;
+  }
+  const int64_t base = item * sh.numel;
+  float acc = 0.f;
+  if (interior) {
+    // every output reading q through element t is p = q - o_t, in bounds
+    for (int t = 0; t < tab.n; ++t) {
+      int64_t p = 0;
+      for (int j = 0; j < nd; ++j) p += (q[j] - tab.o[t][j]) * sh.stride[j];
+      if (arg[base + p] == t) acc += dy[base + p];
+    }
+  } else {
+    // border voxel: scan the outputs within the element extent of q (per
+    // axis q-ext..q+ext, wrapped around for mode wrap, clipped otherwise), in
+    // raster order; each output routes to exactly one input, checked here
+    int64_t cnt[kMaxNdim], k[kMaxNdim];
+    bool wrapd[kMaxNdim];
+    for (int j = 0; j < nd; ++j) {
+      const int64_t n = sh.size[j], e = ext[j];
+      wrapd[j] = (mode == 4) && (2 * e + 1 < n);
+      cnt[j] = (mode == 4) ? (wrapd[j] ? 2 * e + 1 : n)
+                           : (q[j] + e > n - 1 ? n - 1 : q[j] + e) - (q[j] - e < 0 ? 0 : q[j] - e) + 1;
+      k[j] = 0;
+    }
+    for (;;) {
+      int64_t flat = 0;
+      int64_t p[kMaxNdim];
+      for (int j = 0; j < nd; ++j) {
+        const int64_t n = sh.size[j], e = ext[j];
+        if (mode == 4) p[j] = wrapd[j] ? (((q[j] + k[j] - e) % n) + n) % n : k[j];
+        else p[j] = (q[j] - e < 0 ? 0 : q[j] - e) + k[j];
+        flat += p[j] * sh.stride[j];
+      }
+      const int t = arg[base + flat];
+      bool hit = true;
+      for (int j = 0; j < nd && hit; ++j) {
+        bool uc;
+        const int64_t m = gmap(p[j] + tab.o[t][j], sh.size[j], mode, &uc);
+        hit = !uc && m == q[j];
+      }
+      if (hit) acc += dy[base + flat];
+      int j = nd - 1;
+      for (; j >= 0; --j) {
+        if (++k[j] < cnt[j]) break;
+        k[j] = 0;
+      }
+      if (j < 0) break;
+    }
+  }
+  dx[g] = acc;
+}
+
+}  // namespace tmh
+
+extern "C" size_t tm_grey_backward_workspace_bytes(int64_t batch, int ndim, const int64_t* shape) {
+  int64_t n = batch;
+  for (int j = 0; j < ndim; ++j) n *= shape[j];
+  return (size_t)n * sizeof(int16_t) + 256 + 64;
+}
+
+extern "C" int tm_grey_backward(const float* input, int64_t batch, int ndim, const int64_t* shape,
+                                const float* structure, const uint8_t* footprint,
+                                const int64_t* se_shape, const int64_t* origin, int mode,
+                                float cval, int dilation, const float* grad_out, float* grad_in,
+                                void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(input && grad_out && grad_in, "tm_grey_backward: NULL argument");
+  TM_CHECK_ARG(ndim >= 1 && ndim <= kMaxNdim, "tm_grey_backward: ndim must be in 1..8");
+  TM_CHECK_ARG(mode >= 0 && mode <= 4, "tm_grey_backward: mode must be in 0..4");
+  hipStream_t st = (hipStream_t)stream;
+  Shape sh = make_shape(ndim, shape);
+  const int64_t total = batch * sh.numel;
+  int64_t ne = 1;
+  for (int j = 0; j < ndim; ++j) ne *= se_shape[j];
+  GreyTabN tab{};
+  int32_t ext[kMaxNdim] = {0};
+  int n = 0;
+  for (int64_t e = 0; e < ne; ++e) {
+    if (footprint && !footprint[e]) continue;
+    TM_CHECK_ARG(n < kMaxTable, "tm_grey_backward: more than %d active elements", kMaxTable);
+    int64_t rem = e;
+    int64_t r[kMaxNdim];
+    for (int j = ndim - 1; j >= 0; --j) {
+      r[j] = rem % se_shape[j];
+      rem /= se_shape[j];
+    }
+    for (int j = 0; j < ndim; ++j) {
+      int64_t d = r[j] - (se_shape[j] / 2 + origin[j]);
+      if (dilation) d = -d;
+      TM_CHECK_ARG(d >= -127 && d <= 127, "tm_grey_backward: element offsets beyond +-127");
+      tab.o[n][j] = (int8_t)d;
+      const int32_t ad = (int32_t)(d < 0 ? -d : d);
+      ext[j] = ad > ext[j] ? ad : ext[j];
+    }
+    const float s = structure ? structure[e] : 0.0f;
+    tab.v[n] = dilation ? s : -s;
+    ++n;
+  }
+  TM_CHECK_ARG(n > 0, "tm_grey_backward: footprint must contain at least one active position");
+  tab.n = n;
+  const size_t need = tm_grey_backward_workspace_bytes(batch, ndim, shape);
+  if (!workspace || workspace_bytes < need) {
+    set_error("tm_grey_backward: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  int16_t* arg = (int16_t*)workspace;
+  for (int j = 0; j < kMaxNdim; ++j) tab.ext[j] = j < ndim ? ext[j] : 0;
+  const dim3 grid(grid_for(total, 256)), blk(256);
+  if (dilation)
+    hipLaunchKernelGGL(grey_arg_kernel<true>, grid, blk, 0, st, input, arg, sh, total, tab, mode,
+                       cval);
+  else
+    hipLaunchKernelGGL(grey_arg_kernel<false>, grid, blk, 0, st, input, arg, sh, total, tab, mode,
+                       cval);
+  TM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(grey_grad_gather_kernel, grid, blk, 0, st, arg, grad_out, grad_in, sh, total,
+                     tab, mode);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}

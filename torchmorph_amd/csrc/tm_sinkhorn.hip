@@ -46,6 +46,12 @@ struct SkPair {
   int32_t pad;
 };
 
+// Per-pair centre (mean of the y cloud) subtracted from both clouds when
+// points are loaded: the cost is invariant to a common translation, and
+// centred coordinates keep the |x|^2 + |y|^2 - 2 x.y expansion accurate.
+// Computed by a fixed-order tree reduction, so results are deterministic.
+constexpr int kCtrStride = 4;
+
 struct SkTile {
   int32_t pair;
   int32_t r0;
@@ -70,7 +76,7 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
     const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ pin,
     float* __restrict__ pout, const float* __restrict__ wr, const SkPair* __restrict__ pairs,
     const SkTile* __restrict__ tiles, const int* __restrict__ active, float* __restrict__ errpart,
-    int maxc) {
+    int maxc, const float* __restrict__ centre) {
   extern __shared__ float dyn[];
   float* sy = dyn;              // [maxc][D] column points
   float* sh = dyn + maxc * D;   // [maxc] h_j
@@ -85,11 +91,14 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
   const int nr = ROWS_ARE_X ? pr.n : pr.m;
   const int nc = ROWS_ARE_X ? pr.m : pr.n;
   const float k1 = kL2E / pr.eps;  // L / eps
+  float ctr[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) ctr[d] = centre[tile.pair * kCtrStride + d];
   // stage the pair's columns
   for (int j = threadIdx.x; j < nc; j += blockDim.x) {
     float y[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) y[d] = cols[(coff + j) * D + d];
+    for (int d = 0; d < D; ++d) y[d] = cols[(coff + j) * D + d] - ctr[d];
     float yy = 0.f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -109,7 +118,7 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
   if (live) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const float x = rows[(roff + i) * D + d];
+      const float x = rows[(roff + i) * D + d] - ctr[d];
       xt[d] = 2.f * k1 * x;
       ci = fmaf(x, x, ci);
     }
@@ -172,6 +181,83 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
   }
 }
 
+
+// Gradient of the OT value <f, a> + <g, b> w.r.t. the row points at the
+// solution (envelope theorem): grad_i = 2 (S0_i x_i - S1_i) with
+// S0_i = sum_j P_ij, S1_i = sum_j P_ij y_j (centred coordinates; the
+// difference is translation invariant), P_ij = a_i 2^(e_ij - s_i).
+template <int D, bool ROWS_ARE_X>
+__global__ __launch_bounds__(256) void sk_grad_kernel(
+    const float* __restrict__ rows, const float* __restrict__ cols,
+    const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ prow,
+    const float* __restrict__ wr, const SkPair* __restrict__ pairs,
+    const SkTile* __restrict__ tiles, int maxc, const float* __restrict__ centre,
+    float* __restrict__ grad) {
+  extern __shared__ float dyn[];
+  float* sy = dyn;
+  float* sh = dyn + maxc * D;
+  __shared__ float s0[4][kRowsPerTile];
+  __shared__ float s1[4][kRowsPerTile][D];
+  const SkTile tile = tiles[blockIdx.x];
+  const SkPair pr = pairs[tile.pair];
+  const int64_t roff = ROWS_ARE_X ? pr.xoff : pr.yoff;
+  const int64_t coff = ROWS_ARE_X ? pr.yoff : pr.xoff;
+  const int nr = ROWS_ARE_X ? pr.n : pr.m;
+  const int nc = ROWS_ARE_X ? pr.m : pr.n;
+  const float k1 = kL2E / pr.eps;
+  float ctr[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) ctr[d] = centre[tile.pair * kCtrStride + d];
+  for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+    float yy = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float y = cols[(coff + j) * D + d] - ctr[d];
+      sy[j * D + d] = y;
+      yy = fmaf(y, y, yy);
+    }
+    sh[j] = fmaf(k1, pc[coff + j] - yy, lwc[coff + j]);
+  }
+  __syncthreads();
+  const int r = threadIdx.x & (kRowsPerTile - 1);
+  const int q = threadIdx.x >> 6;
+  const int i = tile.r0 + r;
+  const bool live = i < nr;
+  float x[D], xt[D];
+  float ci = 0.f, s = 0.f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    x[d] = live ? rows[(roff + i) * D + d] - ctr[d] : 0.f;
+    xt[d] = 2.f * k1 * x[d];
+    ci = fmaf(x[d], x[d], ci);
+  }
+  if (live) s = k1 * ci - k1 * prow[roff + i];
+  float a0 = 0.f, a1[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) a1[d] = 0.f;
+  if (live) {
+    for (int j = q; j < nc; j += 4) {
+      const float w = __builtin_amdgcn_exp2f(sh[j] + dotp<D>(xt, &sy[j * D]) - s);
+      a0 += w;
+#pragma unroll
+      for (int d = 0; d < D; ++d) a1[d] = fmaf(w, sy[j * D + d], a1[d]);
+    }
+  }
+  s0[q][r] = a0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) s1[q][r][d] = a1[d];
+  __syncthreads();
+  if (live && q == 0) {
+    const float wi = wr[roff + i];
+    const float S0 = (s0[0][r] + s0[1][r] + s0[2][r] + s0[3][r]) * wi;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float S1 = (s1[0][r][d] + s1[1][r][d] + s1[2][r][d] + s1[3][r][d]) * wi;
+      grad[(roff + i) * D + d] = 2.f * (S0 * x[d] - S1);
+    }
+  }
+}
+
 // Per-pair convergence check after an f-update of iteration k (1-based):
 // the error is that of the state after iteration k-1, whose f is still in
 // fold and whose g is in g.
@@ -201,6 +287,27 @@ __global__ __launch_bounds__(256) void sk_check_kernel(
     iters[p] = k - 1;
     active[p] = 0;
     atomicSub(n_active, 1);
+  }
+}
+
+// One workgroup per pair: centre[p] = mean of the pair's y points.
+__global__ __launch_bounds__(256) void sk_centre_kernel(const float* __restrict__ y, int dim,
+                                                        const SkPair* __restrict__ pairs,
+                                                        float* __restrict__ centre) {
+  __shared__ float red[256];
+  const SkPair pr = pairs[blockIdx.x];
+  for (int d = 0; d < kCtrStride; ++d) {
+    float s = 0.f;
+    if (d < dim)
+      for (int j = threadIdx.x; j < pr.m; j += blockDim.x) s += y[(pr.yoff + j) * dim + d];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) centre[blockIdx.x * kCtrStride + d] = d < dim ? red[0] / (float)pr.m : 0.f;
+    __syncthreads();
   }
 }
 
@@ -338,6 +445,7 @@ extern "C" size_t tm_sinkhorn_points_workspace_bytes(int npairs, const int32_t* 
   add(sizeof(float) * (sx + sy)); // log2 weights
   add(sizeof(float) * sx * 2);    // f ping-pong
   add(sizeof(float) * sy);        // g working copy
+  add(sizeof(float) * tmh::kCtrStride * npairs);  // centres
   return b;
 }
 
@@ -406,6 +514,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   float* lb = lw + sx;
   float* fbuf = (float*)take(sizeof(float) * sx * 2);
   float* gw = (float*)take(sizeof(float) * sy);
+  float* centre = (float*)take(sizeof(float) * kCtrStride * npairs);
   // the small tables travel with the stream (pageable copies are staged
   // before this call returns)
   TM_CHECK_HIP(hipMemcpyAsync(dpairs, hp.data(), sizeof(SkPair) * npairs, hipMemcpyHostToDevice, st));
@@ -413,6 +522,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
                               hipMemcpyHostToDevice, st));
   TM_CHECK_HIP(hipMemcpyAsync(dfirst, hfirst.data(), sizeof(int32_t) * (npairs + 1),
                               hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(sk_centre_kernel, dim3(npairs), dim3(256), 0, st, y, dim, dpairs, centre);
   hipLaunchKernelGGL(sk_init_kernel, dim3(grid_for(npairs, 256)), dim3(256), 0, st, active, npairs,
                      n_active, iter_base);
   hipLaunchKernelGGL(sk_log2_kernel, dim3(grid_for(sx, 256)), dim3(256), 0, st, a, la, sx);
@@ -426,7 +536,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
 #define SK_HALF(D, RX, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, NT, ERR)                     \
   hipLaunchKernelGGL((sk_half_kernel<D, RX>), dim3((unsigned)(NT)), dim3(256),                  \
                      (size_t)maxc * (D + 1) * sizeof(float), lst, ROWS, COLS, LWC, PC, PIN, POUT, \
-                     WR, dpairs, TILES, active, ERR, maxc)
+                     WR, dpairs, TILES, active, ERR, maxc, centre)
   auto gstep = [&](const float* fcur) -> int {
     switch (dim) {
       case 1: SK_HALF(1, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
@@ -518,6 +628,87 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
     set_error("tm_sinkhorn_points: HIP failure in the iteration loop");
     return status;
   }
+  return TM_OK;
+}
+
+
+extern "C" size_t tm_sinkhorn_points_grad_workspace_bytes(int npairs, const int32_t* n,
+                                                          const int32_t* m) {
+  return tm_sinkhorn_points_workspace_bytes(npairs, n, m);
+}
+
+// Gradients of the entropic OT value <f, a> + <g, b> of every pair w.r.t.
+// its points, from converged potentials f, g (as returned by
+// tm_sinkhorn_points): gx (sum n, dim), gy (sum m, dim).
+extern "C" int tm_sinkhorn_points_grad(const float* x, const float* y, const float* a,
+                                       const float* b, int dim, int npairs, const int32_t* n,
+                                       const int32_t* m, const float* eps, const float* f,
+                                       const float* g, float* gx, float* gy, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(x && y && a && b && f && g && gx && gy, "tm_sinkhorn_points_grad: NULL argument");
+  TM_CHECK_ARG(dim >= 1 && dim <= 4, "tm_sinkhorn_points_grad: dim must be 1..4");
+  TM_CHECK_ARG(npairs >= 1, "tm_sinkhorn_points_grad: npairs must be >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<SkPair> hp(npairs);
+  std::vector<SkTile> htile;
+  int64_t sx = 0, sy = 0;
+  int maxc = 1;
+  for (int p = 0; p < npairs; ++p) {
+    TM_CHECK_ARG(n[p] >= 1 && m[p] >= 1 && n[p] <= kMaxCols && m[p] <= kMaxCols,
+                 "tm_sinkhorn_points_grad: cloud sizes must be in 1..%d", kMaxCols);
+    TM_CHECK_ARG(eps[p] > 0.f, "tm_sinkhorn_points_grad: eps must be > 0");
+    hp[p] = SkPair{sx, sy, n[p], m[p], eps[p], 0};
+    sx += n[p];
+    sy += m[p];
+    maxc = n[p] > maxc ? n[p] : maxc;
+    maxc = m[p] > maxc ? m[p] : maxc;
+  }
+  for (int p = 0; p < npairs; ++p)
+    for (int r = 0; r < n[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+  const int64_t tx = (int64_t)htile.size();
+  for (int p = 0; p < npairs; ++p)
+    for (int r = 0; r < m[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+  const int64_t ty = (int64_t)htile.size() - tx;
+  const size_t need = tm_sinkhorn_points_grad_workspace_bytes(npairs, n, m);
+  if (!workspace || workspace_bytes < need) {
+    set_error("tm_sinkhorn_points_grad: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  char* w = (char*)workspace;
+  auto take = [&](size_t bytes) {
+    char* p = w;
+    w += (bytes + 255) & ~(size_t)255;
+    return (void*)p;
+  };
+  SkPair* dpairs = (SkPair*)take(sizeof(SkPair) * npairs);
+  SkTile* dtiles = (SkTile*)take(sizeof(SkTile) * (tx + ty));
+  float* lw = (float*)take(sizeof(float) * (sx + sy));
+  float* centre = (float*)take(sizeof(float) * kCtrStride * npairs);
+  float* la = lw;
+  float* lb = lw + sx;
+  TM_CHECK_HIP(hipMemcpyAsync(dpairs, hp.data(), sizeof(SkPair) * npairs, hipMemcpyHostToDevice, st));
+  TM_CHECK_HIP(hipMemcpyAsync(dtiles, htile.data(), sizeof(SkTile) * (tx + ty),
+                              hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(sk_centre_kernel, dim3(npairs), dim3(256), 0, st, y, dim, dpairs, centre);
+  hipLaunchKernelGGL(sk_log2_kernel, dim3(grid_for(sx, 256)), dim3(256), 0, st, a, la, sx);
+  hipLaunchKernelGGL(sk_log2_kernel, dim3(grid_for(sy, 256)), dim3(256), 0, st, b, lb, sy);
+  TM_LAUNCH_CHECK();
+  const size_t lds = (size_t)maxc * (dim + 1) * sizeof(float);
+#define SK_GRAD(D)                                                                                \
+  hipLaunchKernelGGL((sk_grad_kernel<D, true>), dim3((unsigned)tx), dim3(256), lds, st, x, y, lb, \
+                     g, f, a, dpairs, dtiles, maxc, centre, gx);                                  \
+  hipLaunchKernelGGL((sk_grad_kernel<D, false>), dim3((unsigned)ty), dim3(256), lds, st, y, x,    \
+                     la, f, g, b, dpairs, dtiles + tx, maxc, centre, gy)
+  switch (dim) {
+    case 1: SK_GRAD(1); break;
+    case 2: SK_GRAD(2); break;
+    case 3: SK_GRAD(3); break;
+    default: SK_GRAD(4); break;
+  }
+#undef SK_GRAD
+  TM_LAUNCH_CHECK();
   return TM_OK;
 }
 

@@ -236,19 +236,6 @@ def _weights(w, sizes, device) -> Tensor:
     return w.to(device=device, dtype=torch.float32)
 
 
-def _center(x: Tensor, y: Tensor, n: list, m: list) -> tuple[Tensor, Tensor]:
-    """Shift both clouds of every pair by the mean of its y cloud (the cost is
-    invariant to a common translation; centring keeps |x|^2 small)."""
-    pid_y = torch.repeat_interleave(torch.arange(len(m), device=y.device),
-                                    torch.tensor(m, device=y.device))
-    sums = torch.zeros((len(m), y.shape[1]), device=y.device, dtype=y.dtype)
-    sums.index_add_(0, pid_y, y)
-    mean = sums / torch.tensor(m, device=y.device, dtype=y.dtype)[:, None]
-    pid_x = torch.repeat_interleave(torch.arange(len(n), device=x.device),
-                                    torch.tensor(n, device=x.device))
-    return x - mean[pid_x], y - mean[pid_y]
-
-
 def sinkhorn_points(xs, ys, eps, a=None, b=None, tol: float = 1e-5, max_iter: int = 20000,
                     f_init: Optional[Tensor] = None, chunk: int = 10) -> SinkhornResult:
     """Solves entropic OT between every pair (xs[p], ys[p]) with cost
@@ -272,7 +259,6 @@ def sinkhorn_points(xs, ys, eps, a=None, b=None, tol: float = 1e-5, max_iter: in
     y = y.detach().to(torch.float32)
     if x.shape[1] != y.shape[1] or not 1 <= x.shape[1] <= 4:
         raise ValueError("point dimension must match and be in 1..4")
-    x, y = _center(x, y, n, m)
     x, y = x.contiguous(), y.contiguous()
     P = len(n)
     aw = _weights(a, n, dev).contiguous()
@@ -308,32 +294,59 @@ def median_sqeuclid(x: Tensor, y: Tensor) -> Tensor:
     return torch.cdist(x.float(), y.float()).square().flatten().median()
 
 
+def sinkhorn_points_grad(xs, ys, res: SinkhornResult, a=None, b=None):
+    """Gradients of every pair's OT value <f,a> + <g,b> w.r.t. its points,
+    from a converged :class:`SinkhornResult` (HIP kernel, plan formed on the
+    fly).  Returns concatenated (gx, gy)."""
+    x, n = _as_ragged(xs)
+    y, m = _as_ragged(ys)
+    dev = x.device
+    x = x.detach().to(torch.float32).contiguous()
+    y = y.detach().to(torch.float32).contiguous()
+    aw = _weights(a, n, dev).contiguous()
+    bw = _weights(b, m, dev).contiguous()
+    P = len(n)
+    gx = torch.empty_like(x)
+    gy = torch.empty_like(y)
+    L = _lib.lib()
+    n_arr = (_lib.ctypes.c_int32 * P)(*n)
+    m_arr = (_lib.ctypes.c_int32 * P)(*m)
+    e_arr = (_lib.ctypes.c_float * P)(*[float(v) for v in res.eps.tolist()])
+    with torch.cuda.device(dev):
+        nbytes = L.tm_sinkhorn_points_grad_workspace_bytes(P, n_arr, m_arr)
+        ws = _lib.workspace(nbytes, dev)
+        _lib.check(L.tm_sinkhorn_points_grad(
+            _lib.ptr(x), _lib.ptr(y), _lib.ptr(aw), _lib.ptr(bw), x.shape[1], P, n_arr, m_arr,
+            e_arr, _lib.ptr(res.f.contiguous()), _lib.ptr(res.g.contiguous()), _lib.ptr(gx),
+            _lib.ptr(gy), _lib.ptr(ws), nbytes, _lib.stream_of(dev)), "tm_sinkhorn_points_grad")
+    return gx, gy
+
+
 class SinkhornPointCloud(torch.autograd.Function):
-    """Entropic OT value <f, a> + <g, b> between two point clouds, with
-    gradients from the converged solution: d/da = f, d/db = g (each centred,
-    the mass-preserving direction), d/dx_i = 2 sum_j P_ij (x_i - y_j),
-    d/dy_j = 2 sum_i P_ij (y_j - x_i).  At the fixed point these are the exact
-    derivatives of the dual value (envelope theorem); tests check them against
-    finite differences and autograd through an unrolled FP64 solver."""
+    """Entropic OT value <f, a> + <g, b> between two point clouds (one pair),
+    differentiable w.r.t. the points and the weights.
+
+    Backward (HIP, tm_sinkhorn_points_grad): at the converged fixed point the
+    value's derivatives are d/da = f, d/db = g (centred: the mass-preserving
+    direction), d/dx_i = 2 sum_j P_ij (x_i - y_j), d/dy_j = 2 sum_i P_ij
+    (y_j - x_i) -- the envelope theorem applied to the dual problem; tests
+    check them against FP64 central differences of a tightly converged
+    solve."""
 
     @staticmethod
     def forward(ctx, x, y, a, b, eps, tol, max_iter):
         res = sinkhorn_points(x[None], y[None], eps, a[None], b[None], tol=tol, max_iter=max_iter)
-        f, g = res.f, res.g
-        value = (f * a.float()).sum() + (g * b.float()).sum()
-        ctx.save_for_backward(x, y, a, b, f, g)
-        ctx.eps = eps
+        value = (res.f * a.float()).sum() + (res.g * b.float()).sum()
+        ctx.save_for_backward(x, y, a, b)
+        ctx.res = res
         return value.to(x.dtype)
 
     @staticmethod
     def backward(ctx, grad):
-        x, y, a, b, f, g = ctx.saved_tensors
-        eps = ctx.eps
-        xf, yf = x.float(), y.float()
-        C = torch.cdist(xf, yf).square()
-        P = torch.exp((f[:, None] + g[None, :] - C) / eps) * a.float()[:, None] * b.float()[None, :]
-        gx = 2 * (P.sum(1, keepdim=True) * xf - P @ yf)
-        gy = 2 * (P.sum(0)[:, None] * yf - P.T @ xf)
+        x, y, a, b = ctx.saved_tensors
+        res = ctx.res
+        gx, gy = sinkhorn_points_grad(x[None], y[None], res, a[None], b[None])
+        f, g = res.f, res.g
         ga = f - f.mean()
         gb = g - g.mean()
         return (grad * gx).to(x.dtype), (grad * gy).to(y.dtype), (grad * ga).to(a.dtype), \
