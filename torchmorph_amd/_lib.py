@@ -56,6 +56,8 @@ SIGNATURES: dict[str, tuple] = {
                                      ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_float), _P, _P, _P, _P, _P, _SZ,
                                      _P]),
+    "tm_mlp_gemm_bf16": (_I, [_P, _I64, _I64, _P, _I64, _I64, _P, _I, _I, _I, _P, _SZ, _P]),
+    "tm_cast_bf16": (_I, [_P, _P, _I64, _P]),
     "tm_binary_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
     "tm_binary": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _I, _I,
                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _P, _P, _P,

@@ -510,7 +510,8 @@ __global__ __launch_bounds__(256) void grey_grad_gather_kernel(
   for (int j = 0; j < nd; ++j) {
     q[j] = rem / sh.stride[j];
     rem -= q[j] * sh.stride[j];
-    if (q[j] < ext[j] || q[j] > sh.size[j] - 1 - ext[j]) interior = false;
+    // mirror maps a read at -k onto k (not k-1): its border zone is one wider
+    if (q[j] < ext[j] + 1 || q[j] > sh.size[j] - 2 - ext[j]) interior = false;
   }
   const int64_t base = item * sh.numel;
   float acc = 0.f;
@@ -528,7 +529,7 @@ __global__ __launch_bounds__(256) void grey_grad_gather_kernel(
     int64_t cnt[kMaxNdim], k[kMaxNdim];
     bool wrapd[kMaxNdim];
     for (int j = 0; j < nd; ++j) {
-      const int64_t n = sh.size[j], e = ext[j];
+      const int64_t n = sh.size[j], e = ext[j] + 1;
       wrapd[j] = (mode == 4) && (2 * e + 1 < n);
       cnt[j] = (mode == 4) ? (wrapd[j] ? 2 * e + 1 : n)
                            : (q[j] + e > n - 1 ? n - 1 : q[j] + e) - (q[j] - e < 0 ? 0 : q[j] - e) + 1;
@@ -538,7 +539,7 @@ __global__ __launch_bounds__(256) void grey_grad_gather_kernel(
       int64_t flat = 0;
       int64_t p[kMaxNdim];
       for (int j = 0; j < nd; ++j) {
-        const int64_t n = sh.size[j], e = ext[j];
+        const int64_t n = sh.size[j], e = ext[j] + 1;
         if (mode == 4) p[j] = wrapd[j] ? (((q[j] + k[j] - e) % n) + n) % n : k[j];
         else p[j] = (q[j] - e < 0 ? 0 : q[j] - e) + k[j];
         flat += p[j] * sh.stride[j];
