@@ -281,3 +281,32 @@ def test_grey_backward_oracle_fd_and_mass(oracle_mod):
     np.testing.assert_array_equal(
         oracle_mod.grey(xq[None, None], None, None, (3, 3), (0, 0), "mirror", 0.0, True)[0, 0],
         oracle_mod.grey_fp64(xq, None, None, (3, 3), (0, 0), "mirror", 0.0, True))
+
+
+def test_deepsets_oracle_gradient_matches_central_differences():
+    from oracle import deepsets as ds
+
+    rng = np.random.default_rng(7)
+    params = ds.init_params(3) * 0.7
+    pairs, targets = [], []
+    for n, m in [(9, 7), (5, 11)]:
+        x = rng.normal(size=(n, 3))
+        y = rng.normal(size=(m, 3)) + 0.5
+        pairs.append((x, y, 0.3))
+        targets.append(rng.normal(size=n))
+    loss, grad = ds.loss_and_grad(params, pairs, targets)
+    assert loss > 0
+    h = 1e-6
+    idx = rng.choice(ds.N_PARAMS, size=40, replace=False)
+    # include biases and pair-level weights explicitly
+    idx = np.concatenate([idx, [ds.N_PARAMS - 1, 4 * 128 + 3, ds.N_PARAMS - 300]])
+    for i in idx:
+        e = np.zeros_like(params)
+        e[i] = h
+        lp, _ = ds.loss_and_grad(params + e, pairs, targets)
+        lm, _ = ds.loss_and_grad(params - e, pairs, targets)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, grad[i])
+    # loss is invariant to a constant shift of the targets (dual gauge)
+    l2, _ = ds.loss_and_grad(params, pairs, [t + 3.0 for t in targets])
+    assert abs(l2 - loss) < 1e-12

@@ -13,7 +13,7 @@
 //    ReLU to bf16.
 //  * small fp32 kernels around it (casts, ReLU masks, column sums, segment
 //    means, Adam).
-#include "tm_common.h"
+#include "tm_mlp.h"
 
 namespace tmh {
 
@@ -34,22 +34,6 @@ enum GemmEpi {
   EPI_F32 = 0,           // C (f32) = acc  (+ beta * C)
   EPI_F32_BIAS = 1,      // C (f32) = acc + bias[n]
   EPI_BF16_BIAS_RELU = 2 // C (bf16) = relu(acc + bias[n] + rowadd[group(m)][n])
-};
-
-struct GemmArgs {
-  const uint16_t* A;
-  int64_t sam, sak;
-  const uint16_t* B;
-  int64_t sbk, sbn;
-  void* C;
-  int64_t ldc;
-  int M, N, K;
-  int epi;
-  const float* bias;
-  const float* rowadd;       // [groups][N] fp32 or NULL
-  const int32_t* rowgroup;   // group of each row or NULL
-  float beta;                // EPI_F32: C = acc + beta * C
-  int ksplit;                // > 1: partial sums to C + z * M * ldc (fp32)
 };
 
 constexpr int kBM = 64, kBN = 64, kBK = 32;
