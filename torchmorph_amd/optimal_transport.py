@@ -265,6 +265,8 @@ def sinkhorn_points(xs, ys, eps, a=None, b=None, tol: float = 1e-5, max_iter: in
     bw = _weights(b, m, dev).contiguous()
     if isinstance(eps, Tensor):
         eps_t = eps.detach().to(torch.float32).reshape(-1).cpu()
+    elif isinstance(eps, (list, tuple)) or hasattr(eps, "__len__"):
+        eps_t = torch.tensor([float(v) for v in eps], dtype=torch.float32)
     else:
         eps_t = torch.full((P,), float(eps))
     if eps_t.numel() != P:
