@@ -190,6 +190,39 @@ int tm_sinkhorn_points_grad(const float* x, const float* y, const float* a, cons
                             const float* eps, const float* f, const float* g, float* gx,
                             float* gy, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ----------------------------------------------------------- warm start --- */
+/* No reference counterpart.  DeepSets network predicting the dual potential f
+ * of every x point of a point-cloud pair (dim 3) from both clouds and eps:
+ * per-point MLP 4 -> 128 -> 128 (shared), mean pooling, head
+ * [phi(x_i), mean phi(x), mean phi(y), log(eps/s2)] -> 256 -> 256 -> 1,
+ * BF16 MFMA GEMMs with fp32 accumulation.  params: flat fp32 vector of
+ * tm_warmstart_param_count() values (layout in tm_warmstart.hip).  Pair
+ * sizes n, m and eps are HOST arrays. */
+int64_t tm_warmstart_param_count(void);
+size_t tm_warmstart_workspace_bytes(int npairs, const int32_t* n, const int32_t* m);
+/* f: predicted potentials (sum n), in cost units. */
+int tm_warmstart_predict(const float* params, const float* x, const float* y, int npairs,
+                         const int32_t* n, const int32_t* m, const float* eps, float* f,
+                         void* workspace, size_t workspace_bytes, void* stream);
+/* Loss (device scalar) and parameter gradient for targets f_target (sum n,
+ * cost units); the per-pair loss is weighted by (s2/eps)^loss_alpha. */
+int tm_warmstart_grad(const float* params, const float* x, const float* y, int npairs,
+                      const int32_t* n, const int32_t* m, const float* eps,
+                      const float* f_target, float loss_alpha, float* grad, float* loss,
+                      void* workspace, size_t workspace_bytes, void* stream);
+/* Adam (bias-corrected, decoupled weight decay) on n parameters. */
+int tm_warmstart_adam(float* params, const float* grad, float* m1, float* m2, int64_t n, int step,
+                      float lr, float beta1, float beta2, float eps, float weight_decay,
+                      void* stream);
+/* BF16 x BF16 -> fp32 GEMM with strided operands (the network's building
+ * block, exported for tests): A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk +
+ * n*sbn], C dense row-major M x N; splitk_ws (may be NULL) enables split-K. */
+int tm_mlp_gemm_bf16(const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+                     int64_t sbk, int64_t sbn, float* C, int M, int N, int K, float* splitk_ws,
+                     size_t splitk_ws_elems, void* stream);
+/* float32 -> bfloat16 (round to nearest even). */
+int tm_cast_bf16(const float* x, uint16_t* y, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

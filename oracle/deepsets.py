@@ -81,9 +81,9 @@ def predict(params, pairs, round_bf16=False):
     return [o * c["s2"] for o, c in zip(outs, cache)]
 
 
-def loss_and_grad(params, pairs, targets, round_bf16=False):
-    """Centred MSE (mean over pairs of the per-pair mean of squared,
-    mean-removed residuals in units of s2) and its gradient."""
+def loss_and_grad(params, pairs, targets, round_bf16=False, alpha=0.0):
+    """Centred MSE (mean over pairs of (s2/eps)^alpha times the per-pair mean
+    of squared, mean-removed residuals in units of s2) and its gradient."""
     R = bf16 if round_bf16 else (lambda a: np.asarray(a, np.float64))
     P = unpack(params)
     W1, W2a, W3 = R(P["W1"]), R(P["W2a"]), R(P["W3"])
@@ -91,12 +91,13 @@ def loss_and_grad(params, pairs, targets, round_bf16=False):
     npairs = len(pairs)
     g = {name: np.zeros(shape) for name, shape in LAYOUT}
     loss = 0.0
-    for out, cc, t in zip(outs, cache, targets):
+    for out, cc, t, pr in zip(outs, cache, targets, pairs):
         res = out - np.asarray(t, np.float64) / cc["s2"]
         res = res - res.mean()
         n = len(out)
-        loss += (res ** 2).sum() / n / npairs
-        dout = 2.0 * res / (n * npairs)
+        wp = (cc["s2"] / pr[2]) ** alpha
+        loss += wp * (res ** 2).sum() / n / npairs
+        dout = 2.0 * wp * res / (n * npairs)
         g["w4"] += cc["A2"].T @ dout
         g["b4"] += dout.sum()
         dA2 = np.outer(dout, P["w4"])

@@ -62,9 +62,11 @@ def test_gradient_matches_oracle(ws):
     net = ws.WarmStartNet.create(seed=2)
     loss, grad = net.loss_and_grad([torch.tensor(x, device="cuda") for x in xs],
                                    [torch.tensor(y, device="cuda") for y in ys], eps,
-                                   torch.tensor(np.concatenate(targets), device="cuda"))
+                                   torch.tensor(np.concatenate(targets), device="cuda"),
+                                   loss_alpha=0.5)
     params = net.params.cpu().double().numpy()
-    oloss, ograd = ds.loss_and_grad(params, list(zip(xs, ys, eps)), targets, round_bf16=True)
+    oloss, ograd = ds.loss_and_grad(params, list(zip(xs, ys, eps)), targets, round_bf16=True,
+                                    alpha=0.5)
     assert abs(float(loss) - oloss) <= 1e-3 * oloss
     g = grad.cpu().double().numpy()
     o = 0

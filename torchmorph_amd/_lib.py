@@ -66,7 +66,7 @@ SIGNATURES: dict[str, tuple] = {
                                   _P, _P, _SZ, _P]),
     "tm_warmstart_grad": (_I, [_P, _P, _P, _I, ctypes.POINTER(ctypes.c_int32),
                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float), _P,
-                               _P, _P, _P, _SZ, _P]),
+                               _F, _P, _P, _P, _SZ, _P]),
     "tm_warmstart_adam": (_I, [_P, _P, _P, _P, _I64, _I, _F, _F, _F, _F, _F, _P]),
     "tm_binary_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
     "tm_binary": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _I, _I,

@@ -15,9 +15,10 @@
 // The pair-constant part of the head's first layer (ex, ey, r, bias) is one
 // 256-vector per pair added in the GEMM epilogue.  182,017 parameters.
 //
-// Training loss (per batch of pairs): mean over pairs of the mean squared
-// error of (prediction - target) after removing its per-pair mean (the dual
-// potentials are defined up to a constant shift between f and g).
+// Training loss (per batch of pairs): mean over pairs of w_p times the mean
+// squared error of (prediction - target) / s2 after removing its per-pair
+// mean (the dual potentials are defined up to a constant shift between f and
+// g), w_p = (s2 / eps)^alpha.
 #include <vector>
 
 #include "tm_mlp.h"
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void ws_loss_kernel(const float* __restrict__ 
                                                       const float* __restrict__ target,
                                                       const WsPair* __restrict__ pairs,
                                                       const float* __restrict__ stats, int npairs,
-                                                      float* __restrict__ dout,
+                                                      float alpha, float* __restrict__ dout,
                                                       float* __restrict__ lpair) {
   __shared__ float red[256];
   const WsPair pr = pairs[blockIdx.x];
@@ -182,7 +183,10 @@ __global__ __launch_bounds__(256) void ws_loss_kernel(const float* __restrict__ 
   const float mean = red[0] / (float)pr.n;
   __syncthreads();
   float l = 0.f;
-  const float gs = 2.f / ((float)pr.n * (float)npairs);
+  // pair weight (s2 / eps)^alpha: alpha = 1 measures errors in units of eps
+  // (what the Sinkhorn iteration count depends on), alpha = 0 in units of s2
+  const float wp = powf(s2 / pr.eps, alpha);
+  const float gs = 2.f * wp / ((float)pr.n * (float)npairs);
   for (int i = threadIdx.x; i < pr.n; i += blockDim.x) {
     const float rr = out[pr.xoff + i] - target[pr.xoff + i] / s2 - mean;
     l = fmaf(rr, rr, l);
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(256) void ws_loss_kernel(const float* __restrict__ 
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) lpair[blockIdx.x] = red[0] / (float)pr.n / (float)npairs;
+  if (threadIdx.x == 0) lpair[blockIdx.x] = wp * red[0] / (float)pr.n / (float)npairs;
 }
 
 __global__ void ws_sum_kernel(const float* __restrict__ v, int n, float* __restrict__ out) {
@@ -521,7 +525,8 @@ static int dgrad(WsCtx& c, const uint16_t* dZ, int nout, const uint16_t* W, int 
   return launch_gemm(g, nullptr, 0, c.st);
 }
 
-static int ws_backward(WsCtx& c, const float* P, const float* target, float* grad, float* loss) {
+static int ws_backward(WsCtx& c, const float* P, const float* target, float alpha, float* grad,
+                       float* loss) {
   hipStream_t st = c.st;
   const WsLayout& L = c.L;
   const uint16_t* Wb = c.at<uint16_t>(L.Wb);
@@ -529,7 +534,7 @@ static int ws_backward(WsCtx& c, const float* P, const float* target, float* gra
   int s = TM_OK;
   float* dout = c.at<float>(L.dout);
   hipLaunchKernelGGL(ws_loss_kernel, dim3(c.np), dim3(256), 0, st, c.at<float>(L.out), target,
-                     c.at<WsPair>(L.pairs), c.at<float>(L.stats), c.np, dout,
+                     c.at<WsPair>(L.pairs), c.at<float>(L.stats), c.np, alpha, dout,
                      c.at<float>(L.lpair));
   hipLaunchKernelGGL(ws_sum_kernel, dim3(1), dim3(64), 0, st, c.at<float>(L.lpair), c.np, loss);
   TM_LAUNCH_CHECK();
@@ -635,15 +640,15 @@ extern "C" int tm_warmstart_predict(const float* params, const float* x, const f
 // point; grad has tm_warmstart_param_count() floats, loss is one float.
 extern "C" int tm_warmstart_grad(const float* params, const float* x, const float* y, int npairs,
                                  const int32_t* n, const int32_t* m, const float* eps,
-                                 const float* f_target, float* grad, float* loss, void* workspace,
-                                 size_t workspace_bytes, void* stream) {
+                                 const float* f_target, float loss_alpha, float* grad, float* loss,
+                                 void* workspace, size_t workspace_bytes, void* stream) {
   using namespace tmh;
   TM_CHECK_ARG(params && x && y && f_target && grad && loss, "tm_warmstart_grad: NULL argument");
   WsCtx c{};
   int s = ws_setup(c, x, y, npairs, n, m, eps, workspace, workspace_bytes, (hipStream_t)stream);
   if (s != TM_OK) return s;
   if ((s = ws_forward(c, params, x, y)) != TM_OK) return s;
-  return ws_backward(c, params, f_target, grad, loss);
+  return ws_backward(c, params, f_target, loss_alpha, grad, loss);
 }
 
 // Adam step with bias correction and decoupled weight decay.

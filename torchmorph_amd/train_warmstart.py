@@ -61,7 +61,8 @@ def generate(count, seed, chunk=1024, tol=1e-5, max_iter=5000, log=print):
     return data
 
 
-def train(net: WarmStartNet, data, epochs=8, batch=64, lr=1e-3, lr_min=1e-5, seed=0, log=print):
+def train(net: WarmStartNet, data, epochs=8, batch=64, lr=1e-3, lr_min=1e-5, seed=0,
+          loss_alpha=1.0, log=print):
     rng = np.random.default_rng(seed)
     steps_per_epoch = math.ceil(len(data) / batch)
     total = epochs * steps_per_epoch
@@ -78,7 +79,7 @@ def train(net: WarmStartNet, data, epochs=8, batch=64, lr=1e-3, lr_min=1e-5, see
             eps = [data[i][2] for i in idx]
             f = torch.cat([data[i][3] for i in idx])
             cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * step / total))
-            losses.append(net.train_step(xs, ys, eps, f, lr=cur))
+            losses.append(net.train_step(xs, ys, eps, f, lr=cur, loss_alpha=loss_alpha))
             step += 1
         ep_loss = float(torch.stack(losses).mean())
         hist.append(ep_loss)
@@ -131,6 +132,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--loss-alpha", type=float, default=1.0)
     ap.add_argument("--eval-pairs", type=int, default=1024)
     ap.add_argument("--out", default="warmstart.pt")
     ap.add_argument("--summary", default=None)
@@ -138,7 +140,8 @@ def main():
     log = lambda s: print(s, flush=True)
     data = generate(args.pairs, args.seed, log=log)
     net = WarmStartNet.create(seed=args.seed)
-    hist = train(net, data, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+    hist = train(net, data, epochs=args.epochs, batch=args.batch, lr=args.lr,
+                 loss_alpha=args.loss_alpha, log=log)
     net.save(args.out)
     ev = evaluate(net, count=args.eval_pairs, log=log)
     summary = {"train_pairs": len(data), "epochs": args.epochs, "loss_history": hist,

@@ -102,7 +102,7 @@ class WarmStartNet:
         return f
 
     # ------------------------------------------------------------ training
-    def loss_and_grad(self, xs, ys, eps, f_target):
+    def loss_and_grad(self, xs, ys, eps, f_target, loss_alpha: float = 1.0):
         x, y, n, m = _sizes(xs, ys)
         P = len(n)
         L = _lib.lib()
@@ -116,14 +116,15 @@ class WarmStartNet:
             nb = L.tm_warmstart_workspace_bytes(P, na, ma)
             ws = _lib.workspace(nb, x.device)
             _lib.check(L.tm_warmstart_grad(_lib.ptr(self.params), _lib.ptr(x), _lib.ptr(y), P,
-                                           na, ma, ea, _lib.ptr(t), _lib.ptr(grad),
+                                           na, ma, ea, _lib.ptr(t), float(loss_alpha),
+                                           _lib.ptr(grad),
                                            _lib.ptr(loss), _lib.ptr(ws), nb,
                                            _lib.stream_of(x.device)), "tm_warmstart_grad")
         return loss, grad
 
     def train_step(self, xs, ys, eps, f_target, lr=1e-3, betas=(0.9, 0.999), adam_eps=1e-8,
-                   weight_decay=0.0) -> torch.Tensor:
-        loss, grad = self.loss_and_grad(xs, ys, eps, f_target)
+                   weight_decay=0.0, loss_alpha: float = 1.0) -> torch.Tensor:
+        loss, grad = self.loss_and_grad(xs, ys, eps, f_target, loss_alpha)
         self.step += 1
         _lib.check(_lib.lib().tm_warmstart_adam(
             _lib.ptr(self.params), _lib.ptr(grad), _lib.ptr(self.m1), _lib.ptr(self.m2),
