@@ -92,6 +92,51 @@ int tm_cdt(const void* input, int input_dtype, int64_t batch, int ndim, const in
            int metric, float* distances, void* indices, int index_dtype, void* workspace,
            size_t workspace_bytes, void* stream);
 
+/* ------------------------------------------------------------- ragged --- */
+
+/* No reference counterpart: the reference takes dense (B, C, spatial...)
+ * tensors only (distance_transform.py:10-138, edt_kernel.cu:904-958,
+ * cdt_kernel.cu:269-463), so a list of differently shaped masks costs one
+ * call per shape there.  Here every (item, op) pair is a job; all jobs run in
+ * one persistent launch that pulls per-pass line blocks from a device
+ * work-list, and each job's result is bit-identical to tm_edt / tm_cdt on
+ * that item alone. */
+enum tm_ragged_op { TM_RG_EDT = 0, TM_RG_CDT_CHESSBOARD = 1, TM_RG_CDT_TAXICAB = 2 };
+
+typedef struct tm_ragged_job {
+  int32_t op;          /* tm_ragged_op */
+  int32_t ndim;        /* 1..8 */
+  int64_t shape[8];    /* each axis 1..65535, product below 2^31 */
+  double sampling[8];  /* EDT spacing per axis, finite > 0; ignored by the chamfer ops */
+  int64_t in_offset;   /* element offset of the C-contiguous item in `input` */
+  int64_t out_offset;  /* element offset of its float32 distances, or -1 */
+  int64_t idx_offset;  /* element offset of its indices (ndim planes of prod(shape)), or -1 */
+} tm_ragged_job;
+
+/* Workspace bytes for these jobs (0 on invalid jobs). */
+size_t tm_ragged_dt_workspace_bytes(const tm_ragged_job* jobs, int64_t njobs);
+
+/* Runs every job.  input: TM_U8 or TM_F32 masks (nonzero = foreground) shared
+ * by all jobs; distances / indices: flat outputs addressed by the jobs'
+ * offsets (NULL when no job uses them); index_dtype TM_I32 / TM_I64.
+ *   schedule   0 = one persistent launch with dependency counters,
+ *              1 = one launch per pass level (for comparison)
+ *   upload     1 = copy the job and task descriptors into the workspace
+ *              (pinned staging, asynchronous); 0 = the workspace already holds
+ *              them from an earlier call with the same jobs, only launch
+ *   workspace  256-byte aligned, tm_ragged_dt_workspace_bytes() bytes
+ * A chamfer job with indices runs SciPy's raster passes on one workgroup; a
+ * chamfer job without indices runs separably (same values).  The persistent
+ * schedule bounds every dependency wait; tm_ragged_dt_check() reports a wait
+ * that hit its bound.  Not for graph capture when upload = 1. */
+int tm_ragged_dt(const tm_ragged_job* jobs, int64_t njobs, const void* input, int input_dtype,
+                 float* distances, void* indices, int index_dtype, int schedule, int upload,
+                 void* workspace, size_t workspace_bytes, void* stream);
+
+/* Synchronises `stream` and returns TM_OK, or TM_ERR_HIP when the last
+ * tm_ragged_dt on this workspace aborted a dependency wait. */
+int tm_ragged_dt_check(const void* workspace, void* stream);
+
 /* ---------------------------------------------------- grey morphology --- */
 
 /* Workspace bytes for tm_grey (0 is enough for the fused kernels; the per-axis

@@ -77,3 +77,31 @@ def test_cdt_is_deterministic(tm):
     a = tm.chamfer_distance_transform(x, return_indices=True)
     b = tm.chamfer_distance_transform(x, return_indices=True)
     assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("metric", ["chessboard", "taxicab"])
+@pytest.mark.parametrize("shape,p", [((3, 1, 130, 77), 0.9), ((1, 2, 64, 200), 0.5),
+                                     ((2, 1, 300), 0.97), ((1, 1, 300, 1), 0.9),
+                                     ((2, 1, 17, 19, 23), 0.9), ((1, 1, 40, 3, 70), 0.995),
+                                     ((1, 2, 5, 4, 6, 7), 0.85), ((1, 1, 9, 11), 1.0),
+                                     ((1, 1, 9, 11), 0.0), ((2, 1, 1, 1), 1.0)])
+def test_cdt_distances_only_vs_oracle(tm, oracle_mod, metric, shape, p):
+    """Without indices the values come from the separable per-axis passes;
+    they must equal SciPy's raster result exactly (-1 on all-foreground)."""
+    rng = np.random.default_rng(sum(shape) + int(p * 100))
+    mask = rng.random(shape) < p
+    for dtype in (torch.bool, torch.float32):
+        x = torch.as_tensor(mask, device="cuda").to(dtype)
+        d = tm.chamfer_distance_transform(x, metric=metric)
+        od, _ = oracle_mod.cdt(mask, metric)
+        assert torch.equal(d.cpu(), torch.from_numpy(od.astype(np.float32)))
+
+
+@pytest.mark.parametrize("metric", ["chessboard", "taxicab"])
+def test_cdt_distances_only_single_site(tm, oracle_mod, metric):
+    mask = np.ones((2, 1, 150, 130), bool)
+    mask[0, 0, 0, 0] = False
+    mask[1, 0, 149, 7] = False
+    d = tm.chamfer_distance_transform(torch.as_tensor(mask, device="cuda"), metric=metric)
+    od, _ = oracle_mod.cdt(mask, metric)
+    assert torch.equal(d.cpu(), torch.from_numpy(od.astype(np.float32)))

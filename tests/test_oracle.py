@@ -114,6 +114,30 @@ def test_cdt_oracle_closed_forms(oracle_mod):
     np.testing.assert_array_equal(i[:, 0, 0], np.indices((3, 4)))
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_cdt_oracle_values_are_exact_metric_distances(oracle_mod, seed):
+    """SciPy's two raster passes give exactly the taxicab / chessboard
+    distance to the nearest background voxel (-1 with no background).  The
+    GPU's distances-only path is built on this fact, so it is pinned here by
+    brute force on 1-D..3-D masks of every density."""
+    rng = np.random.default_rng(100 + seed)
+    for trial in range(40):
+        nd = int(rng.integers(1, 4))
+        shape = tuple(int(s) for s in rng.integers(1, 9 if nd == 3 else 14, size=nd))
+        m = rng.random(shape) < rng.choice([0.3, 0.7, 0.95, 1.0])
+        bg = np.argwhere(~m)
+        pts = np.indices(shape).reshape(nd, -1).T
+        for metric in ("chessboard", "taxicab"):
+            d, _ = oracle_mod.cdt(m[None, None], metric)
+            if len(bg) == 0:
+                want = np.full(shape, -1)
+            else:
+                diff = np.abs(pts[:, None, :] - bg[None, :, :])
+                dd = diff.sum(-1) if metric == "taxicab" else diff.max(-1)
+                want = dd.min(1).reshape(shape)
+            np.testing.assert_array_equal(d[0, 0], want, err_msg=f"{shape} {metric}")
+
+
 def _grey_kwargs(meta, x):
     nd = x.ndim - 2
     st = meta.get("structure")

@@ -68,6 +68,9 @@ SIGNATURES: dict[str, tuple] = {
                                ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_float), _P,
                                _F, _P, _P, _P, _SZ, _P]),
     "tm_warmstart_adam": (_I, [_P, _P, _P, _P, _I64, _I, _F, _F, _F, _F, _F, _P]),
+    "tm_ragged_dt_workspace_bytes": (_SZ, [_P, _I64]),
+    "tm_ragged_dt": (_I, [_P, _I64, _P, _I, _P, _P, _I, _I, _I, _P, _SZ, _P]),
+    "tm_ragged_dt_check": (_I, [_P, _P]),
     "tm_binary_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
     "tm_binary": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _I, _I,
                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _P, _P, _P,

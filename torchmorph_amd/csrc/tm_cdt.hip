@@ -23,171 +23,92 @@
 //    row in 64-element slots (element x in lane x % 64), so loads and stores
 //    are coalesced along the contiguous axis.
 // Pass 2 runs the same code on reversed coordinates, in place, as SciPy does.
-#include "tm_common.h"
+//
+// Without indices the values are computed separably instead (one lane per
+// line, every line of an axis in parallel; tm_dt_lines.h shows why the result
+// is the same): the float output holds the int32 intermediates in place, the
+// chessboard envelope keeps its stack in the workspace.
+#include "tm_cdt_dev.h"
+#include "tm_dt_lines.h"
 
 namespace tmh {
 
-constexpr int32_t kInf = 1 << 29;  // "unreached" inside the scan
-
-struct CdtOut {
-  float* dist;
-  void* idx;
-  int idx64;
-  int64_t plane;
-};
-
-__device__ __forceinline__ int64_t phys(const Shape& sh, const int64_t* lc, bool rev) {
-  int64_t p = 0;
-  for (int j = 0; j < sh.ndim; ++j) {
-    const int64_t c = rev ? sh.size[j] - 1 - lc[j] : lc[j];
-    p += c * sh.stride[j];
-  }
-  return p;
-}
-
-// Processes one row (leading logical coordinates lc[0..ndim-2]) with one wave.
-template <typename TIn>
-__device__ void cdt_row(const TIn* __restrict__ in, int32_t* __restrict__ dt,
-                        int32_t* __restrict__ ft, const Shape& sh, int64_t item, int64_t* lc,
-                        bool rev, bool pass2, int metric, CdtOut o) {
-  const int lane = threadIdx.x & 63;
-  const int nd = sh.ndim;
-  const int64_t W = sh.size[nd - 1];
-  const int64_t ibase = item * sh.numel;
-  // number of structure elements strictly before the centre, minus the row
-  // predecessor (the last of them)
-  int half = 1;
-  for (int j = 0; j < nd; ++j) half *= 3;
-  half = half / 2 - 1;
-
-  int32_t carry_w = kInf;
-  int32_t carry_s = -1;
-  for (int64_t x0 = 0; x0 < W; x0 += 64) {
-    const int64_t x = x0 + lane;
-    const bool live = x < W;
-    int32_t a = -1, src = -1;
-    int64_t p = 0;
-    if (live) {
-      lc[nd - 1] = x;
-      p = phys(sh, lc, rev);
-      int32_t own;
-      if (pass2) {
-        own = dt[ibase + p];
-        src = ft[ibase + p];
-      } else {
-        own = is_fg(in, ibase + p) ? -1 : 0;
-        src = (int32_t)p;
-      }
-      a = own;
-      if (own != 0) {
-        for (int e = 0; e < half; ++e) {
-          // decode structure element e (raster order over {-1,0,1}^nd)
-          int rem = e, nz = 0;
-          int d[kMaxNdim];
-          for (int j = nd - 1; j >= 0; --j) {
-            d[j] = rem % 3 - 1;
-            rem /= 3;
-            nz += d[j] != 0;
-          }
-          if (metric == 1 && nz > 1) continue;
-          bool inb = true;
-          int64_t q = 0;
-          for (int j = 0; j < nd; ++j) {
-            const int64_t c = lc[j] + d[j];
-            if (c < 0 || c >= sh.size[j]) { inb = false; break; }
-            q += (rev ? sh.size[j] - 1 - c : c) * sh.stride[j];
-          }
-          if (!inb) continue;
-          const int32_t tt = dt[ibase + q];
-          if (tt >= 0 && (a < 0 || tt + 1 < a)) {
-            a = tt + 1;
-            src = ft[ibase + q];
-          }
-        }
-      }
-    }
-    // inclusive min-plus scan over the slot; the later element wins ties
-    int32_t w = (live && a >= 0) ? a - (int32_t)x : kInf;
-    int32_t s = src;
-#pragma unroll
-    for (int dlt = 1; dlt < 64; dlt <<= 1) {
-      const int32_t ow = __shfl_up(w, dlt);
-      const int32_t os = __shfl_up(s, dlt);
-      if (lane >= dlt && ow < w) {
-        w = ow;
-        s = os;
-      }
-    }
-    if (carry_w < w) {
-      w = carry_w;
-      s = carry_s;
-    }
-    carry_w = __shfl(w, 63);
-    carry_s = __shfl(s, 63);
-    if (live) {
-      const int32_t v = (w >= kInf / 2) ? -1 : w + (int32_t)x;
-      const int32_t f = (w >= kInf / 2) ? src : s;
-      dt[ibase + p] = v;
-      ft[ibase + p] = f;
-      if (pass2) {
-        const int64_t g = ibase + p;
-        if (o.dist) o.dist[g] = (float)v;
-        if (o.idx) {
-          int64_t rem = f;
-          for (int j = nd - 1; j >= 0; --j) {
-            const int64_t c = rem % sh.size[j];
-            rem /= sh.size[j];
-            if (o.idx64) ((int64_t*)o.idx)[j * o.plane + g] = c;
-            else ((int32_t*)o.idx)[j * o.plane + g] = (int32_t)c;
-          }
-        }
-      }
-    }
-  }
-}
-
-// One workgroup per item.  mode 0: rows one per step (raster order);
-// mode 1: 3-D wavefront t = 2z + y.
+// One workgroup per item of a dense batch.
 template <typename TIn>
 __global__ __launch_bounds__(256) void cdt_pass_kernel(const TIn* __restrict__ in,
                                                        int32_t* __restrict__ dt,
                                                        int32_t* __restrict__ ft, Shape sh,
                                                        int mode, int metric, int pass2,
                                                        CdtOut o) {
-  const int64_t item = blockIdx.x;
-  const int wave = threadIdx.x >> 6;
-  const int nwaves = blockDim.x >> 6;
-  const int nd = sh.ndim;
-  const bool rev = pass2 != 0;
-  int64_t lc[kMaxNdim];
-  if (mode == 1) {
-    const int64_t D = sh.size[0], H = sh.size[1];
-    const int64_t T = 2 * (D - 1) + (H - 1);
-    for (int64_t t = 0; t <= T; ++t) {
-      int64_t zlo = t - (H - 1) > 0 ? (t - (H - 1) + 1) / 2 : 0;
-      int64_t zhi = t / 2 < D - 1 ? t / 2 : D - 1;
-      for (int64_t z = zlo + wave; z <= zhi; z += nwaves) {
-        lc[0] = z;
-        lc[1] = t - 2 * z;
-        cdt_row(in, dt, ft, sh, item, lc, rev, rev, metric, o);
-      }
-      __syncthreads();
-    }
-  } else {
-    const int64_t W = sh.size[nd - 1];
-    const int64_t nrows = sh.numel / W;
-    for (int64_t r = 0; r < nrows; ++r) {
-      if (wave == 0) {
-        int64_t rem = r;
-        for (int j = nd - 2; j >= 0; --j) {
-          lc[j] = rem % sh.size[j];
-          rem /= sh.size[j];
-        }
-        cdt_row(in, dt, ft, sh, item, lc, rev, rev, metric, o);
-      }
-      __syncthreads();
-    }
+  const int64_t base = (int64_t)blockIdx.x * sh.numel;
+  CdtOut oi = o;
+  if (oi.dist) oi.dist += base;
+  // index planes keep the batch-wide plane stride; offset the item in them
+  if (oi.idx) oi.idx = (char*)oi.idx + base * (oi.idx64 ? 8 : 4);
+  cdt_item_pass(in + base, dt + base, ft + base, sh, mode, metric, pass2, oi);
+}
+
+// Separable distances, axis 0 from the mask.
+template <typename TIn, bool FINAL>
+__global__ __launch_bounds__(256) void cdt_sep0_kernel(const TIn* __restrict__ in,
+                                                       int32_t* __restrict__ w, Shape sh,
+                                                       int64_t nlines) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlines) return;
+  int64_t c[kMaxNdim], base, b;
+  decode_line(sh, 0, t, c, &base, &b);
+  cdt_line_axis0<TIn, FINAL>(in, w, base, sh.stride[0], sh.size[0]);
+}
+
+// Separable distances, axis d >= 1 (metric 0 chessboard, 1 taxicab).  The
+// chessboard stack entry k of line t sits at k * nlines + t.
+template <int METRIC, bool FINAL>
+__global__ __launch_bounds__(256) void cdt_sepd_kernel(int32_t* __restrict__ w, Shape sh, int d,
+                                                       int64_t nlines, uint16_t* __restrict__ S,
+                                                       uint16_t* __restrict__ T,
+                                                       int32_t* __restrict__ G) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlines) return;
+  int64_t c[kMaxNdim], base, b;
+  decode_line(sh, d, t, c, &base, &b);
+  if (METRIC == 1)
+    cdt_line_taxicab<FINAL>(w, base, sh.stride[d], sh.size[d]);
+  else
+    cdt_line_chessboard<FINAL>(w, base, sh.stride[d], sh.size[d], S + t, T + t, G + t, nlines);
+}
+
+template <typename TIn>
+static int cdt_separable(const TIn* in, int64_t batch, const Shape& sh, int metric, float* dist,
+                         void* ws, hipStream_t st) {
+  const int64_t N = batch * sh.numel;
+  int32_t* w = (int32_t*)dist;
+  uint16_t* S = (uint16_t*)ws;
+  uint16_t* T = S + N;
+  int32_t* G = (int32_t*)(T + N);
+  {
+    const int64_t nlines = batch * (sh.numel / sh.size[0]);
+    if (sh.ndim == 1)
+      hipLaunchKernelGGL((cdt_sep0_kernel<TIn, true>), dim3(grid_for(nlines, 256)), dim3(256), 0,
+                         st, in, w, sh, nlines);
+    else
+      hipLaunchKernelGGL((cdt_sep0_kernel<TIn, false>), dim3(grid_for(nlines, 256)), dim3(256), 0,
+                         st, in, w, sh, nlines);
+    TM_LAUNCH_CHECK();
   }
+  for (int d = 1; d < sh.ndim; ++d) {
+    const int64_t nlines = batch * (sh.numel / sh.size[d]);
+    const dim3 g(grid_for(nlines, 256)), blk(256);
+    const bool fin = d == sh.ndim - 1;
+    if (metric == 1) {
+      if (fin) hipLaunchKernelGGL((cdt_sepd_kernel<1, true>), g, blk, 0, st, w, sh, d, nlines, S, T, G);
+      else hipLaunchKernelGGL((cdt_sepd_kernel<1, false>), g, blk, 0, st, w, sh, d, nlines, S, T, G);
+    } else {
+      if (fin) hipLaunchKernelGGL((cdt_sepd_kernel<0, true>), g, blk, 0, st, w, sh, d, nlines, S, T, G);
+      else hipLaunchKernelGGL((cdt_sepd_kernel<0, false>), g, blk, 0, st, w, sh, d, nlines, S, T, G);
+    }
+    TM_LAUNCH_CHECK();
+  }
+  return TM_OK;
 }
 
 template <typename TIn>
@@ -200,6 +121,9 @@ static int cdt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     set_error("tm_cdt: workspace of %zu bytes is smaller than the %zu required", ws_bytes, need);
     return TM_ERR_WORKSPACE;
   }
+  bool short_axes = true;
+  for (int j = 0; j < ndim; ++j) short_axes = short_axes && shape[j] <= 65535;
+  if (!o.idx && short_axes) return cdt_separable(in, batch, sh, metric, o.dist, ws, st);
   int32_t* dt = (int32_t*)ws;
   int32_t* ft = dt + o.plane;
   const int mode = ndim == 3 ? 1 : 0;

@@ -25,44 +25,10 @@
 //
 // Build with -ffp-contract=off: every double multiply and add must round
 // separately, as SciPy's C code does.
-#include "tm_common.h"
+#include "tm_dt_lines.h"
 #include "tm_tune.h"
 
 namespace tmh {
-
-struct EdtOut {
-  float* dist;
-  void* idx;
-  int idx64;
-  int64_t plane;  // elements per index plane = batch * numel
-};
-
-struct Samp {
-  double s[kMaxNdim];
-};
-
-__device__ __forceinline__ void edt_store(const EdtOut& o, int ndim, int64_t g, const int64_t* c,
-                                          const int32_t* f, const Samp& sp) {
-  if (o.dist) {
-    double acc = 0.0;
-    for (int j = 0; j < ndim; ++j) {
-      int32_t delta = f[j] - (int32_t)c[j];
-      double t = (double)delta * sp.s[j];
-      t = t * t;
-      acc = (j == 0) ? t : acc + t;
-    }
-    o.dist[g] = (float)sqrt(acc);
-  }
-  if (o.idx) {
-    if (o.idx64) {
-      int64_t* p = (int64_t*)o.idx;
-      for (int j = 0; j < ndim; ++j) p[j * o.plane + g] = f[j];
-    } else {
-      int32_t* p = (int32_t*)o.idx;
-      for (int j = 0; j < ndim; ++j) p[j * o.plane + g] = f[j];
-    }
-  }
-}
 
 // ------------------------------------------------------------- 2-D path --
 // Axis-0 pass.  A workgroup covers 64 columns of one image with 16 column
@@ -313,72 +279,19 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
 }
 
 // --------------------------------------------------------------- N-D --
-__device__ __forceinline__ void decode_line(const Shape& sh, int d, int64_t t, int64_t* c,
-                                            int64_t* base, int64_t* b) {
-  // t enumerates (batch, c_0..c_{d-1}, c_{d+1}..c_{n-1}) with the inner axes fastest.
-  const int64_t inner = sh.stride[d];
-  int64_t ob = t / inner;
-  int64_t in = t - ob * inner;
-  int64_t rem = in;
-  for (int j = sh.ndim - 1; j > d; --j) {
-    c[j] = rem % sh.size[j];
-    rem /= sh.size[j];
-  }
-  rem = ob;
-  for (int j = d - 1; j >= 0; --j) {
-    c[j] = rem % sh.size[j];
-    rem /= sh.size[j];
-  }
-  *b = rem;
-  c[d] = 0;
-  *base = ob * sh.size[d] * inner + in;
-}
-
-// Axis-0 pass from the mask: nearest background along the line, lower index
-// on ties (with all partial distances zero every site stays on the envelope,
-// so SciPy's query reduces to this).  Writes the feature row into f0, or the
-// final outputs when ndim == 1.
+// Axis 0 from the mask, one lane per line (tm_dt_lines.h).
 template <typename TIn, bool FINAL>
 __global__ __launch_bounds__(256) void edt_pass0_kernel(const TIn* __restrict__ in,
                                                         int32_t* __restrict__ f0, Shape sh,
                                                         int64_t nlines, Samp sp, EdtOut o) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nlines) return;
-  int64_t c[kMaxNdim], base, b;
-  decode_line(sh, 0, t, c, &base, &b);
-  const int64_t step = sh.stride[0];
-  const int64_t L = sh.size[0];
-  int32_t last = -1;
-  for (int64_t i = 0; i < L; ++i) {
-    const int64_t v = base + i * step;
-    if (!is_fg(in, v)) last = (int32_t)i;
-    f0[v] = last;
-  }
-  int32_t nx = -1;
-  for (int64_t i = L - 1; i >= 0; --i) {
-    const int64_t v = base + i * step;
-    if (!is_fg(in, v)) nx = (int32_t)i;
-    const int32_t pv = f0[v];
-    int32_t r;
-    if (pv < 0) r = nx;
-    else if (nx < 0) r = pv;
-    else r = ((int32_t)i - pv <= nx - (int32_t)i) ? pv : nx;
-    if (FINAL) {
-      c[0] = i;
-      int32_t f[kMaxNdim];
-      f[0] = r;
-      for (int j = 1; j < sh.ndim; ++j) f[j] = r >= 0 ? (int32_t)c[j] : 0;
-      edt_store(o, sh.ndim, v, c, f, sp);
-    } else {
-      f0[v] = r;
-    }
-  }
+  edt_line_axis0<TIn, FINAL>(in, f0, sh, t, sp, o);
 }
 
-// Axis-d pass (d >= 1): envelope of the sites on each line.  fin holds the d
-// feature planes of the previous pass, fout receives d+1 planes (or the final
-// outputs when FINAL).  LDS_DEPTH > 0: stack in LDS (requires L <= LDS_DEPTH);
-// LDS_DEPTH == 0: stack in the interleaved global scratch gpos/gh.
+// Axis d >= 1, one lane per line.  LDS_DEPTH > 0: stack in LDS (requires
+// L <= LDS_DEPTH); LDS_DEPTH == 0: stack in the interleaved global scratch
+// gpos/gh (entry k of line t at k * nlines + t, coalesced across lanes).
 template <bool FINAL, int LDS_DEPTH>
 __global__ __launch_bounds__(64) void edt_stack_kernel(const int32_t* __restrict__ fin,
                                                        int32_t* __restrict__ fout, int64_t plane,
@@ -389,100 +302,11 @@ __global__ __launch_bounds__(64) void edt_stack_kernel(const int32_t* __restrict
   __shared__ double lh[LDS_DEPTH > 0 ? LDS_DEPTH * 64 : 1];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nlines) return;
-  int64_t c[kMaxNdim], base, b;
-  decode_line(sh, d, t, c, &base, &b);
-  const int64_t step = sh.stride[d];
-  const int64_t L = sh.size[d];
-  const double sd = sp.s[d];
-  const int lane = threadIdx.x;
-
-  auto POS = [&](int64_t k) -> int32_t {
-    if (LDS_DEPTH > 0) return (int32_t)lpos[k * 64 + lane];
-    return gpos[k * nlines + t];
-  };
-  auto H = [&](int64_t k) -> double {
-    if (LDS_DEPTH > 0) return lh[k * 64 + lane];
-    return gh[k * nlines + t];
-  };
-
-  int64_t top = -1;
-  for (int64_t i = 0; i < L; ++i) {
-    const int64_t v = base + i * step;
-    const int32_t f0 = fin[v];
-    if (f0 < 0) continue;
-    double wR = 0.0;
-    {
-      double tt = (double)(f0 - (int32_t)c[0]) * sp.s[0];
-      wR += tt * tt;
-      for (int j = 1; j < d; ++j) {
-        double u = (double)(fin[j * plane + v] - (int32_t)c[j]) * sp.s[j];
-        wR += u * u;
-      }
-    }
-    while (top >= 1) {
-      const int32_t pv = POS(top), pu = POS(top - 1);
-      const double a = (double)(pv - pu) * sd;
-      const double bb = (double)((int32_t)i - pv) * sd;
-      const double cc = a + bb;
-      const double uR = H(top - 1), vR = H(top);
-      if (cc * vR - bb * uR - a * wR - a * bb * cc <= 0.0) break;
-      --top;
-    }
-    ++top;
-    if (LDS_DEPTH > 0) {
-      lpos[top * 64 + lane] = (uint16_t)i;
-      lh[top * 64 + lane] = wR;
-    } else {
-      gpos[top * nlines + t] = (int32_t)i;
-      gh[top * nlines + t] = wR;
-    }
-  }
-
-  int32_t f[kMaxNdim];
-  if (top < 0) {
-    // no site on this line: every voxel keeps the feature (-1, 0, ..., 0)
-    for (int j = 0; j < sh.ndim; ++j) f[j] = 0;
-    f[0] = -1;
-    for (int64_t i = 0; i < L; ++i) {
-      const int64_t v = base + i * step;
-      if (FINAL) {
-        c[d] = i;
-        edt_store(o, sh.ndim, v, c, f, sp);
-      } else {
-        for (int j = 0; j <= d; ++j) fout[j * plane + v] = f[j];
-      }
-    }
-    return;
-  }
-  int64_t l = 0;
-  int32_t pl = POS(0);
-  double hl = H(0);
-  for (int64_t i = 0; i < L; ++i) {
-    double tt = (double)(pl - (int32_t)i) * sd;
-    double d1 = hl + tt * tt;
-    while (l < top) {
-      const int32_t pn = POS(l + 1);
-      const double hn = H(l + 1);
-      double t2 = (double)(pn - (int32_t)i) * sd;
-      double d2 = hn + t2 * t2;
-      if (d1 <= d2) break;
-      d1 = d2;
-      ++l;
-      pl = pn;
-      hl = hn;
-    }
-    const int64_t site = base + (int64_t)pl * step;
-    for (int j = 0; j < d; ++j) f[j] = fin[j * plane + site];
-    f[d] = pl;
-    const int64_t v = base + i * step;
-    if (FINAL) {
-      for (int j = d + 1; j < sh.ndim; ++j) f[j] = (int32_t)c[j];
-      c[d] = i;
-      edt_store(o, sh.ndim, v, c, f, sp);
-    } else {
-      for (int j = 0; j <= d; ++j) fout[j * plane + v] = f[j];
-    }
-  }
+  if (LDS_DEPTH > 0)
+    edt_line_stack<FINAL>(fin, fout, plane, sh, d, t, sp, lpos + threadIdx.x, lh + threadIdx.x,
+                          64, o);
+  else
+    edt_line_stack<FINAL>(fin, fout, plane, sh, d, t, sp, gpos + t, gh + t, nlines, o);
 }
 
 static bool unit_sampling(int ndim, const double* s) {
