@@ -121,7 +121,10 @@ class RaggedDistanceTransform:
             codes = [_OPS[op] for op in self.ops]
         except KeyError as e:
             raise ValueError(f"unknown op {e.args[0]!r}; choose from {sorted(_OPS)}") from None
-        self.device = torch.device(device) if device is not None else torch.device(
+        dev = torch.device(device) if device is not None else torch.device("cuda")
+        if dev.type != "cuda":
+            raise ValueError("the ragged distance transforms run on CUDA devices only")
+        self.device = dev if dev.index is not None else torch.device(
             "cuda", torch.cuda.current_device())
         self.return_distances = return_distances
         self.return_indices = return_indices
