@@ -11,7 +11,17 @@ steps run as replays of a HIP graph holding one pass over all buffer sets.
 Printed: one JSON line (contract of the build driver), with ``roofline`` for
 the dominant kernel (algorithmic bytes per launch / event-timed launch
 duration) and ``cpu_baseline`` (the CPU oracle, one core, on a bounded
-sample of the same workload).
+sample of the same workload).  ``extras`` holds the other workloads, each
+timed with HIP events on its own stream and parity-checked:
+  morph3d   -- grey dilation (flat cubes 3/7/15, non-flat ball r=3 with
+               heights 0..3) and binary erosion x4 on 4x2x128^3;
+  ragged    -- 512 2-D masks (sides log-uniform 32..1024) + 64 3-D volumes
+               (48..256), smoothed-noise masks, anisotropic sampling; EDT,
+               taxicab and chessboard chamfer in one persistent launch;
+  grad      -- grey dilation backward on 16x1x512x512, and point-cloud
+               Sinkhorn B=256, n=m=1024, d=3, eps=0.05*median, to 1e-5;
+  warmstart -- held-out ragged pairs (256..4096 points, eps 0.1/0.01 x
+               median) solved to 1e-5 cold and warm-started.
 """
 from __future__ import annotations
 
@@ -38,6 +48,16 @@ def parse():
     ap.add_argument("--sets", type=int, default=48, help="distinct buffer sets (>=256 MB total)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--only", choices=["morph3d", "ragged", "grad", "warmstart"], default=None,
+                    help="run one extra workload alone and print its JSON (for profiling)")
+    ap.add_argument("--tuning", default=os.path.join(ROOT, "torchmorph_amd", "tuning", "mi355x.json"),
+                    help="autotune table to preload ('' = tune live)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches instead of graph replays (for counter collection)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="only run a plain device-to-device copy of MB megabytes 8 times "
+                         "(reference for the HBM byte counters) and exit")
     return ap.parse_args()
 
 
@@ -70,6 +90,26 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.tuning:
+        # launch variants chosen by an offline sweep of every variant on MI355X
+        # (the live tuner's timings of near-equal variants are noisy)
+        from torchmorph_amd import autotune
+
+        autotune.load_json(args.tuning)
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy * 1_000_000
+        src = torch.empty(nbytes, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(8):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 8}), flush=True)
+        return
+    if args.only:
+        fn = {"morph3d": extra_morph3d, "ragged": extra_ragged, "grad": extra_grad_and_sinkhorn,
+              "warmstart": extra_warmstart}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return
     B, C, H, W, p = C1["B"], C1["C"], C1["H"], C1["W"], C1["p"]
     nvox = B * C * H * W
     S = args.sets
@@ -120,10 +160,14 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
-    for _ in range(reps):
-        graph.replay()
-    for k in range(rem):
-        step(k)
+    if args.no_graph:
+        for k in range(K):
+            step(k)
+    else:
+        for _ in range(reps):
+            graph.replay()
+        for k in range(rem):
+            step(k)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -135,6 +179,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline([m.cpu().numpy() for m in masks[:4]], args.cpu_seconds)
+
+    extras = None
+    if not args.no_extras:
+        extras = {}
+        for name, fn in (("morph3d", extra_morph3d), ("ragged", extra_ragged),
+                         ("grad", extra_grad_and_sinkhorn), ("warmstart", extra_warmstart)):
+            t_x = time.perf_counter()
+            extras[name] = fn()
+            extras[name]["wall_s"] = round(time.perf_counter() - t_x, 1)
+            print(f"# extra {name} done in {extras[name]['wall_s']} s", file=sys.stderr,
+                  flush=True)
 
     from torchmorph_amd import autotune
 
@@ -156,7 +211,7 @@ def main():
             "workload": "plain exact EDT: bool masks B=8 C=1 256x256 p_fg=0.9, unit sampling, "
                         "f32 distances + int64 indices in place",
             "batch": B * C, "shape": [H, W], "buffer_sets": S,
-            "rotating_footprint_MB": round(footprint_mb, 1), "timing": "HIP graph replays",
+            "rotating_footprint_MB": round(footprint_mb, 1), "timing": "eager launches" if args.no_graph else "HIP graph replays",
         },
         "parity": "bit-exact vs oracle" if parity else "MISMATCH",
         "roofline": {
@@ -167,8 +222,232 @@ def main():
             "autotune": tuned,
         },
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
+
+
+# ------------------------------------------------------------------ extras --
+
+def _timed(fn, reps, stream):
+    """Mean milliseconds of fn() over reps, HIP events on `stream`."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _gb(bytes_, ms):
+    return round(bytes_ / ms / 1e6, 1)
+
+
+def extra_morph3d():
+    """4x2x128^3 float32 quantised to 256 levels; parity on item 0."""
+    import numpy as np
+    import torch
+    import torch.nn.functional as F
+
+    import oracle
+    import torchmorph_amd as tm
+
+    g = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.floor(torch.rand((4, 2, 128, 128, 128), generator=g, device="cuda") * 256) / 256
+    mask = torch.rand((4, 2, 128, 128, 128), generator=g, device="cuda") < 0.7
+    n = x.numel()
+    st = torch.cuda.current_stream()
+    out = {}
+    x0 = x[:1, :1].cpu().numpy()
+    for size in (3, 7, 15):
+        y = tm.grey_dilation(x, size=size)
+        ms = _timed(lambda: tm.grey_dilation(x, size=size), 5, st)
+        # flat cube, reflect border: the max over the in-image part of the window
+        ref = F.max_pool3d(x.view(8, 1, 128, 128, 128), size, 1, size // 2).view_as(x)
+        out[f"grey_dilation_cube{size}"] = {"ms": round(ms, 3), "Gvox_s": round(n / ms / 1e6, 2),
+                                            "GB_s": _gb(8 * n, ms),
+                                            "parity": bool(torch.equal(y, ref))}
+    r = np.sqrt(((np.indices((7, 7, 7)) - 3) ** 2).sum(0))
+    fp = r <= 3.0
+    heights = np.where(fp, np.floor(3.0 - r), 0.0).astype(np.float32)
+    s_t = torch.from_numpy(heights)
+    f_t = torch.from_numpy(fp)
+    y = tm.grey_dilation(x, structure=s_t, footprint=f_t)
+    ms = _timed(lambda: tm.grey_dilation(x, structure=s_t, footprint=f_t), 3, st)
+    want = oracle.grey(x0, heights, fp, (7, 7, 7), (0, 0, 0), dilation=True)
+    out["grey_dilation_ball_r3"] = {"ms": round(ms, 3), "Gvox_s": round(n / ms / 1e6, 2),
+                                    "GB_s": _gb(8 * n, ms),
+                                    "parity": bool(np.array_equal(y[:1, :1].cpu().numpy(), want))}
+    yb = tm.binary_erosion(mask, iterations=4)
+    ms = _timed(lambda: tm.binary_erosion(mask, iterations=4), 5, st)
+    cross = np.zeros((3, 3, 3), bool)
+    cross[1, 1, :] = cross[1, :, 1] = cross[:, 1, 1] = True
+    want = oracle.binary(mask[:1, :1].cpu().numpy(), cross, (0, 0, 0), iterations=4)
+    out["binary_erosion_x4"] = {"ms": round(ms, 3), "Gvox_s": round(n / ms / 1e6, 2),
+                                "GB_s": _gb(2 * n, ms),
+                                "parity": bool(np.array_equal(yb[:1, :1].cpu().numpy(),
+                                                              want.astype(bool)))}
+    out["bytes_per_vox"] = "grey 4 in + 4 out; binary 1 in + 1 out (algorithmic)"
+    return out
+
+
+def _smooth_noise_mask(shape, gen, sigma=4.0):
+    """N(0,1) noise, Gaussian filter (sigma 4, truncate 4, reflect), > 0."""
+    import torch
+    import torch.nn.functional as F
+
+    x = torch.randn(shape, generator=gen, device="cuda")
+    rad = int(4 * sigma + 0.5)
+    w = torch.exp(-0.5 * (torch.arange(-rad, rad + 1, dtype=torch.float64) / sigma) ** 2)
+    w = (w / w.sum()).tolist()
+    for ax in range(x.dim()):
+        L = x.shape[ax]
+        y = x.movedim(ax, -1)
+        lead = y.shape[:-1]
+        y = F.pad(y.reshape(1, -1, L), (rad, rad), mode="reflect" if L > rad else "replicate")
+        acc = torch.zeros(y.shape[:-1] + (L,), device="cuda")
+        for j, wj in enumerate(w):
+            acc += wj * y[..., j:j + L]
+        x = acc.reshape(lead + (L,)).movedim(-1, ax)
+    return x > 0
+
+
+def extra_ragged():
+    import math
+
+    import numpy as np
+    import torch
+
+    import oracle
+    import torchmorph_amd as tm
+    from torchmorph_amd.ragged import RaggedDistanceTransform
+
+    rng = np.random.default_rng(0)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    shapes = [tuple(int(round(math.exp(rng.uniform(math.log(32), math.log(1024)))))
+                    for _ in range(2)) for _ in range(512)]
+    shapes += [tuple(int(round(math.exp(rng.uniform(math.log(48), math.log(256)))))
+                     for _ in range(3)) for _ in range(64)]
+    masks = [_smooth_noise_mask(sh, gen) for sh in shapes]
+    samp = [list(rng.uniform(0.5, 2.0, size=len(sh))) for sh in shapes]
+    nvox = sum(math.prod(sh) for sh in shapes)
+    ops = ("edt", "cdt_taxicab", "cdt_chessboard")
+    st = torch.cuda.current_stream()
+    res = {"items": len(shapes), "voxels": nvox, "ops": list(ops)}
+    for sched in ("persistent", "per_level"):
+        plan = RaggedDistanceTransform(shapes, ops, samp, schedule=sched, device="cuda")
+        flat = plan.pack(masks)
+        plan.run(flat)
+        ms = _timed(lambda: plan.run(flat, check=False), 5, st)
+        plan.run(flat)  # checks the dependency-wait flag
+        res[sched] = {"ms": round(ms, 3), "Gvox_op_s": round(3 * nvox / ms / 1e6, 2),
+                      "GB_s": _gb(3 * 5 * nvox, ms)}
+    out = plan.results()
+    # dense per-item calls for comparison (the reference's only option)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dense = []
+    for k, m in enumerate(masks):
+        xk = m[None, None]
+        dense.append((tm.euclidean_distance_transform(xk, sampling=samp[k]),
+                      tm.chamfer_distance_transform(xk, metric="taxicab"),
+                      tm.chamfer_distance_transform(xk, metric="chessboard")))
+    torch.cuda.synchronize()
+    res["dense_per_item_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    same = all(torch.equal(out[o][k], dense[k][o][0, 0]) for k in range(len(masks))
+               for o in range(3))
+    ok = True
+    for k in (0, 1, 2, 512, 513):
+        if math.prod(shapes[k]) > 3_000_000:
+            continue
+        mk = masks[k].cpu().numpy()[None, None]
+        od, _ = oracle.edt(mk, samp[k])
+        ok &= np.array_equal(out[0][k].cpu().numpy(), od[0, 0].astype(np.float32))
+        for o, metric in ((1, "taxicab"), (2, "chessboard")):
+            cd, _ = oracle.cdt(mk, metric)
+            ok &= np.array_equal(out[o][k].cpu().numpy(), cd[0, 0].astype(np.float32))
+    res["parity"] = {"equal_to_dense_per_item": bool(same), "oracle_items_ok": bool(ok)}
+    res["bytes_per_vox_op"] = "1 mask in + 4 f32 out (algorithmic)"
+    return res
+
+
+def extra_grad_and_sinkhorn():
+    import numpy as np
+    import torch
+
+    import torchmorph_amd as tm
+    from torchmorph_amd.warmstart import gmm_clouds
+
+    st = torch.cuda.current_stream()
+    out = {}
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand((16, 1, 512, 512), generator=g, device="cuda").requires_grad_(True)
+    y = tm.grey_dilation(x, size=3, differentiable=True)
+    go = torch.rand_like(y)
+    n = x.numel()
+    ms = _timed(lambda: torch.autograd.grad(y, x, go, retain_graph=True), 10, st)
+    out["grey_dilation_backward"] = {"ms": round(ms, 3), "Gvox_s": round(n / ms / 1e6, 2),
+                                     "GB_s": _gb(12 * n, ms),
+                                     "bytes_per_vox": "x 4 + grad_out 4 + grad_in 4"}
+
+    rng = np.random.default_rng(11)
+    B, N = 256, 1024
+    xs = torch.from_numpy(np.stack(gmm_clouds(rng, [N] * B))).cuda()
+    ys = torch.from_numpy(np.stack(gmm_clouds(rng, [N] * B))).cuda()
+    eps = torch.stack([0.05 * tm.optimal_transport.median_sqeuclid(xs[p], ys[p])
+                       for p in range(B)]).cpu()
+    tol = 8e-6  # float32 error estimate; FP64 re-measurement reported below
+    res = tm.sinkhorn_points(xs, ys, eps, tol=tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        res = tm.sinkhorn_points(xs, ys, eps, tol=tol)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    # FP64 re-measurement of the row/column marginal errors on 8 pairs
+    errs = []
+    f = res.f.view(B, N).double()
+    gg = res.g.view(B, N).double()
+    for p in range(0, B, 32):
+        xp, yp = xs[p].double(), ys[p].double()
+        C = ((xp[:, None] - yp[None]) ** 2).sum(-1)
+        P = torch.exp((f[p][:, None] + gg[p][None] - C) / float(eps[p])) / (N * N)
+        errs.append(max(float((P.sum(1) - 1.0 / N).abs().sum()),
+                        float((P.sum(0) - 1.0 / N).abs().sum())))
+    it = res.iterations.float()
+    out["sinkhorn_points"] = {"pairs": B, "n": N, "m": N, "solves_s": round(B / dt, 1),
+                              "ms": round(dt * 1e3, 2), "iters_mean": round(float(it.mean()), 1),
+                              "iters_max": int(it.max()), "tol_f32": tol,
+                              "fp64_marginal_err_max_of_8": float(max(errs)),
+                              "all_converged": bool((res.err <= tol).all())}
+    return out
+
+
+def extra_warmstart():
+    import os as _os
+
+    import numpy as np
+    import torch
+
+    from torchmorph_amd import train_warmstart
+    from torchmorph_amd.warmstart import WarmStartNet
+
+    path = _os.path.join(ROOT, "torchmorph_amd", "weights", "warmstart_gmm3d.pt")
+    if not _os.path.exists(path):
+        return {"skipped": "no trained weights"}
+    net = WarmStartNet.load(path)
+    ev = train_warmstart.evaluate(net, count=256, seed=4242, tol=1e-5, chunk=256,
+                                  log=lambda s: None)
+    keys = ("pairs", "cold_iters", "warm_iters", "iter_ratio", "cold_solves_per_s",
+            "warm_solves_per_s", "speedup", "cold_err_max", "warm_err_max", "warm_fallbacks")
+    return {k: (round(ev[k], 4) if isinstance(ev[k], float) else ev[k]) for k in keys}
 
 
 if __name__ == "__main__":

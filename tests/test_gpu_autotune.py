@@ -1,0 +1,123 @@
+"""Every launch variant the autotuner can choose gives bit-identical results:
+each variant is forced through the cache and compared with the oracle."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _p2(v):
+    p = 1
+    while p < v:
+        p <<= 1
+    return p
+
+
+@pytest.fixture(scope="module")
+def tm():
+    import torchmorph_amd
+
+    return torchmorph_amd
+
+
+@pytest.mark.parametrize("shape", [(8, 1, 256, 256), (2, 1, 100, 300), (1, 2, 17, 33),
+                                   (3, 1, 256, 512), (1, 1, 5, 512), (2, 1, 300, 700)])
+@pytest.mark.parametrize("idx", ["i64", "i32", "none"])
+def test_edt2d_every_variant_matches_oracle(tm, oracle_mod, shape, idx):
+    from torchmorph_amd import autotune
+
+    rng = np.random.default_rng(sum(shape))
+    mask = rng.random(shape) < 0.9
+    mask[0, 0, : shape[2] // 3, :] = True  # long foreground runs along columns
+    od, oi = oracle_mod.edt(mask)
+    want_d = torch.from_numpy(od.astype(np.float32))
+    H, W = shape[2:]
+    bits = {"i64": 64, "i32": 32, "none": 0}[idx]
+    key = f"edt2d {_p2(H)} {_p2(W)} {bits} 1"
+    x = torch.as_tensor(mask, device="cuda")
+    try:
+        for v in range(8):
+            autotune.clear()
+            autotune.load({key: v})
+            if idx == "none":
+                d = tm.euclidean_distance_transform(x)
+                assert torch.equal(d.cpu(), want_d), v
+            else:
+                dt = torch.int64 if idx == "i64" else torch.int32
+                ind = torch.empty((2, *shape), dtype=dt, device="cuda")
+                d = tm.euclidean_distance_transform(x, indices=ind)
+                assert torch.equal(d.cpu(), want_d), v
+                assert np.array_equal(ind.cpu().numpy(), oi), v
+    finally:
+        autotune.clear()
+
+
+def test_edt2d_variants_all_foreground_and_background(tm, oracle_mod):
+    from torchmorph_amd import autotune
+
+    try:
+        for v in range(8):
+            autotune.clear()
+            autotune.load({"edt2d 64 128 64 1": v})
+            for fill in (True, False):
+                mask = np.full((2, 1, 40, 100), fill)
+                mask[1, 0, 39, 99] = not fill
+                x = torch.as_tensor(mask, device="cuda")
+                ind = torch.empty((2, 2, 1, 40, 100), dtype=torch.int64, device="cuda")
+                d = tm.euclidean_distance_transform(x, indices=ind)
+                od, oi = oracle_mod.edt(mask)
+                assert torch.equal(d.cpu(), torch.from_numpy(od.astype(np.float32))), (v, fill)
+                assert np.array_equal(ind.cpu().numpy(), oi), (v, fill)
+    finally:
+        autotune.clear()
+
+
+def test_inapplicable_cached_variant_falls_back(tm, oracle_mod):
+    """A cached choice that cannot run here (variant 7 = 8-row bands needs
+    more LDS than a 4096-wide row allows) falls back to the default."""
+    from torchmorph_amd import autotune
+
+    rng = np.random.default_rng(9)
+    mask = rng.random((1, 1, 16, 4096)) < 0.9
+    try:
+        autotune.clear()
+        autotune.load({"edt2d 16 4096 0 1": 7})
+        d = tm.euclidean_distance_transform(torch.as_tensor(mask, device="cuda"))
+        od, _ = oracle_mod.edt(mask)
+        assert torch.equal(d.cpu(), torch.from_numpy(od.astype(np.float32)))
+    finally:
+        autotune.clear()
+
+
+@pytest.mark.parametrize("se", ["cube3", "cube7", "ball2"])
+@pytest.mark.parametrize("mode", ["reflect", "constant", "wrap"])
+def test_grey3_every_zchunk_variant_matches_oracle(tm, oracle_mod, se, mode):
+    """The streaming 3-D grey kernels give the same bits for every z-chunk."""
+    from torchmorph_amd import autotune
+
+    rng = np.random.default_rng(len(se) + len(mode))
+    x = (np.floor(rng.random((2, 1, 37, 21, 70)) * 16) / 16).astype(np.float32)
+    if se.startswith("cube"):
+        k = int(se[4:])
+        kw = dict(size=k)
+        st, fp, shape = None, None, (k, k, k)
+        key_kind, kk = 1, (k, k, k)
+    else:
+        r = np.sqrt(((np.indices((5, 5, 5)) - 2) ** 2).sum(0))
+        fp = r <= 2.0
+        st = np.where(fp, np.floor(2.0 - r), 0.0).astype(np.float32)
+        kw = dict(structure=torch.from_numpy(st), footprint=torch.from_numpy(fp))
+        shape = (5, 5, 5)
+        key_kind, kk = 2, (5, 5, 5)
+    want = oracle_mod.grey(x, st, fp, shape, (0, 0, 0), mode=mode, cval=0.25, dilation=True)
+    xt = torch.as_tensor(x, device="cuda")
+    key = "grey3 {} {} {} {} 64 32 128 2".format(key_kind, *kk)
+    try:
+        for v in range(4):
+            autotune.clear()
+            autotune.load({key: v})
+            y = tm.grey_dilation(xt, mode=mode, cval=0.25, **kw)
+            assert np.array_equal(y.cpu().numpy(), want), v
+    finally:
+        autotune.clear()

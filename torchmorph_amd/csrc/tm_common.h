@@ -70,6 +70,16 @@ inline unsigned grid_for(int64_t n, int block) {
   return (unsigned)(g < 1 ? 1 : g);
 }
 
+// MI355X dispatches workgroup i of a launch to XCD i % 8, and each XCD has
+// its own L2.  xcd_swizzle maps the launch-order id to a logical block index
+// so that every XCD receives one contiguous range of logical blocks: blocks
+// that share cache lines (neighbouring tiles of one image) then share an L2
+// instead of being fetched once per XCD.  A bijection of [0, G).
+__device__ __forceinline__ int xcd_swizzle(int id, int G) {
+  const int x = id & 7, s = id >> 3, q = G >> 3, r = G & 7;
+  return x * q + (x < r ? x : r) + s;
+}
+
 // Reads a mask element: nonzero means foreground (reference feeds
 // input.float(); any nonzero value is foreground).
 template <typename T>

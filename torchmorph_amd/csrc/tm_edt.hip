@@ -72,8 +72,12 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
   __shared__ int16_t slast[kColSegs][WGC];
   const int cg = threadIdx.x & 15;
   const int seg = threadIdx.x >> 4;
-  const int img = blockIdx.y;
-  const int xb = blockIdx.x * WGC + cg * CPT;
+  // 1-D grid of (image, column block); the column blocks of one image stay on
+  // one XCD (they share the image's cache lines)
+  const int ncb = (W + WGC - 1) / WGC;
+  const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int img = lb / ncb;
+  const int xb = (lb - img * ncb) * WGC + cg * CPT;
   const int seglen = (H + kColSegs - 1) / kColSegs;
   const int ys = seg * seglen;
   const int ye = min(H, ys + seglen);
@@ -180,50 +184,11 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
   }
 }
 
-// Axis-1 pass.  One workgroup per band of R rows: the band's column features
-// are loaded in one round (8 int16 per thread per round), turned into squared
-// column distances in LDS, then every pixel runs an exact expanding-window
-// argmin over its row (the lower column wins ties).  With unit sampling all
-// quantities are exact integers, so this equals SciPy's double-precision
-// envelope result bit for bit.
-template <int R>
-__global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restrict__ f0, int H,
-                                                         int W, EdtOut o) {
-  extern __shared__ int32_t lds[];
-  int32_t* hs = lds;                       // [R][W] squared column distance, -1 = none
-  int16_t* fr = (int16_t*)(lds + R * W);   // [R][W] feature row
-  const int img = blockIdx.y;
-  const int y0 = blockIdx.x * R;
-  const int rows = min(R, H - y0);
-  const int npx = rows * W;
-  const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
-  const bool vec = (W % 8 == 0);
-  for (int base = 0; base < npx; base += 256 * 8) {
-    const int p0 = base + threadIdx.x * 8;
-    int16_t v[8];
-    if (vec && p0 + 8 <= npx) {
-      const uint4 q = *(const uint4*)(src + p0);
-      v[0] = (int16_t)(q.x & 0xffff); v[1] = (int16_t)(q.x >> 16);
-      v[2] = (int16_t)(q.y & 0xffff); v[3] = (int16_t)(q.y >> 16);
-      v[4] = (int16_t)(q.z & 0xffff); v[5] = (int16_t)(q.z >> 16);
-      v[6] = (int16_t)(q.w & 0xffff); v[7] = (int16_t)(q.w >> 16);
-    } else {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = p0 + i < npx ? src[p0 + i] : (int16_t)-1;
-    }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int p = p0 + i;
-      if (p < npx) {
-        const int r = p / W;
-        const int f = v[i];
-        const int dy = y0 + r - f;
-        fr[p] = (int16_t)f;
-        hs[p] = f >= 0 ? dy * dy : -1;
-      }
-    }
-  }
-  __syncthreads();
+// Row pass of one band of `rows` rows starting at y0: squared column
+// distances hs and feature rows fr in LDS, exact expanding-window argmin per
+// pixel (the lower column wins ties), outputs written to HBM.
+__device__ __forceinline__ void edt2d_band_rows(const int32_t* hs, const int16_t* fr, int npx,
+                                                int img, int H, int W, int y0, const EdtOut& o) {
   for (int p = threadIdx.x; p < npx; p += blockDim.x) {
     const int r = p / W;
     const int x = p - r * W;
@@ -276,6 +241,53 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
       }
     }
   }
+}
+
+// Axis-1 pass.  One workgroup per band of R rows: the band's column features
+// are loaded in one round (8 int16 per thread per round), turned into squared
+// column distances in LDS, then every pixel runs an exact expanding-window
+// argmin over its row (the lower column wins ties).  With unit sampling all
+// quantities are exact integers, so this equals SciPy's double-precision
+// envelope result bit for bit.
+template <int R>
+__global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restrict__ f0, int H,
+                                                         int W, EdtOut o) {
+  extern __shared__ int32_t lds[];
+  int32_t* hs = lds;                       // [R][W] squared column distance, -1 = none
+  int16_t* fr = (int16_t*)(lds + R * W);   // [R][W] feature row
+  const int img = blockIdx.y;
+  const int y0 = blockIdx.x * R;
+  const int rows = min(R, H - y0);
+  const int npx = rows * W;
+  const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
+  const bool vec = (W % 8 == 0);
+  for (int base = 0; base < npx; base += 256 * 8) {
+    const int p0 = base + threadIdx.x * 8;
+    int16_t v[8];
+    if (vec && p0 + 8 <= npx) {
+      const uint4 q = *(const uint4*)(src + p0);
+      v[0] = (int16_t)(q.x & 0xffff); v[1] = (int16_t)(q.x >> 16);
+      v[2] = (int16_t)(q.y & 0xffff); v[3] = (int16_t)(q.y >> 16);
+      v[4] = (int16_t)(q.z & 0xffff); v[5] = (int16_t)(q.z >> 16);
+      v[6] = (int16_t)(q.w & 0xffff); v[7] = (int16_t)(q.w >> 16);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = p0 + i < npx ? src[p0 + i] : (int16_t)-1;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int p = p0 + i;
+      if (p < npx) {
+        const int r = p / W;
+        const int f = v[i];
+        const int dy = y0 + r - f;
+        fr[p] = (int16_t)f;
+        hs[p] = f >= 0 ? dy * dy : -1;
+      }
+    }
+  }
+  __syncthreads();
+  edt2d_band_rows(hs, fr, npx, img, H, W, y0, o);
 }
 
 // --------------------------------------------------------------- N-D --
@@ -352,7 +364,12 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
         set_error("tm_edt: band does not fit LDS");
         return TM_ERR_ARG;
       }
-      const dim3 cgrid((W + 16 * cpt - 1) / (16 * cpt), (unsigned)batch);
+      const int64_t ncb = (W + 16 * cpt - 1) / (16 * cpt);
+      if (ncb * batch >= ((int64_t)1 << 31)) {
+        set_error("tm_edt: batch too large for one launch");
+        return TM_ERR_ARG;
+      }
+      const dim3 cgrid((unsigned)(ncb * batch));
 #define TM_COLS(NCH, CPT)                                                                         \
   hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, CPT>), cgrid, dim3(256), 0, st, in, H, W, f0, \
                      aligned)
@@ -384,7 +401,11 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     bool ran = false;
     const int v = tune_pick("edt2d", key, 4, 8, dflt, run, st, &ran);
     if (ran) return TM_OK;
-    return run(v);
+    const int rc = run(v);
+    // a cached choice that does not apply here (imported from another
+    // device, or an LDS limit) falls back to the default variant
+    if (rc == TM_ERR_ARG && v != dflt) return run(dflt);
+    return rc;
   }
 
   const int64_t N = o.plane;

@@ -22,7 +22,10 @@
 //    per axis (ping-pong through the workspace).
 //  * grey_gather_kernel general fallback for any ndim and any SE: one thread
 //    per output folds the element table from device memory.
+#include <algorithm>
+
 #include "tm_common.h"
+#include "tm_tune.h"
 
 namespace tmh {
 
@@ -59,23 +62,113 @@ __device__ __forceinline__ float gid() {
 }
 
 constexpr int kTX = 64, kTY = 16;
+constexpr int kPlaneRows = 12;  // halo rows a thread keeps in registers (4 rows per pass)
 
-// Loads one mode-extended input plane halo tile [SY][SX] starting at (y0, x0).
-// `bias` is added to every staged value: +0.0f for flat dilation (the
-// reference adds its zero structure, which turns -0.0 into +0.0), -0.0f
-// otherwise (an exact identity).
-__device__ __forceinline__ void load_plane(const float* __restrict__ vol, float* stage, int zi,
-                                           int D, int H, int W, int y0, int x0, int SY, int SX,
-                                           int mode, float cval, float bias) {
-  bool zc;
-  const int64_t zm = gmap(zi, D, mode, &zc);
-  for (int e = threadIdx.x; e < SY * SX; e += blockDim.x) {
-    const int r = e / SX, c = e - r * SX;
-    bool yc, xc;
-    const int64_t ym = gmap(y0 + r, H, mode, &yc);
-    const int64_t xm = gmap(x0 + c, W, mode, &xc);
-    stage[e] = ((zc || yc || xc) ? cval : vol[(zm * H + ym) * W + xm]) + bias;
+// One mode-extended input plane halo tile [SY][SX] starting at (y0, x0),
+// distributed over the workgroup as 4 rows x 64 columns per pass: thread
+// (rr, cc) holds rows rr + 4k and columns cc, cc + 64 (coalesced along x, no
+// divisions).  Rows beyond 4 * kPlaneRows are staged directly (large SEs).
+// The registers of plane n + 1 are filled while plane n is reduced.
+// `bias` is added to every value: +0.0f for flat dilation (the reference
+// adds its zero structure, which turns -0.0 into +0.0), -0.0f otherwise (an
+// exact identity).
+struct PlaneGeom {
+  int D, H, W, SY, SX, mode;
+  float cval, bias;
+};
+
+__device__ __forceinline__ float plane_value(const float* __restrict__ vol, const PlaneGeom& g,
+                                             int64_t zm, bool zc, int y, int x) {
+  if (zc) return g.cval + g.bias;
+  bool yc, xc;
+  const int64_t ym = gmap(y, g.H, g.mode, &yc);
+  const int64_t xm = gmap(x, g.W, g.mode, &xc);
+  return ((yc || xc) ? g.cval : vol[(zm * g.H + ym) * g.W + xm]) + g.bias;
+}
+
+// The rows and columns a thread loads are the same for every plane of its
+// tile: their mode-mapped coordinates (-1 = constant border) are computed
+// once per workgroup.
+struct PlaneMap {
+  int32_t ym[kPlaneRows];
+  int32_t xm[2];
+};
+
+__device__ __forceinline__ PlaneMap plane_map(const PlaneGeom& g, int y0, int x0) {
+  PlaneMap m;
+  const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < kPlaneRows; ++k) {
+    bool c;
+    const int64_t v = gmap(y0 + rr + 4 * k, g.H, g.mode, &c);
+    m.ym[k] = c ? -1 : (int32_t)v;
   }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    bool c;
+    const int64_t v = gmap(x0 + cc + 64 * h, g.W, g.mode, &c);
+    m.xm[h] = c ? -1 : (int32_t)v;
+  }
+  return m;
+}
+
+struct PlaneRegs {
+  float v[kPlaneRows][2];
+};
+
+__device__ __forceinline__ void plane_fetch(PlaneRegs& R, const float* __restrict__ vol,
+                                            const PlaneGeom& g, const PlaneMap& m, int zi) {
+  bool zc;
+  const int64_t zm = gmap(zi, g.D, g.mode, &zc);
+  const float* pl = vol + zm * g.H * g.W;
+  const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < kPlaneRows; ++k) {
+    const int r = rr + 4 * k;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c = cc + 64 * h;
+      if (r < g.SY && c < g.SX) {
+        const bool border = zc || m.ym[k] < 0 || m.xm[h] < 0;
+        R.v[k][h] = (border ? g.cval : pl[(int64_t)m.ym[k] * g.W + m.xm[h]]) + g.bias;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void plane_store(const PlaneRegs& R, float* stage,
+                                            const float* __restrict__ vol, const PlaneGeom& g,
+                                            int zi, int y0, int x0) {
+  const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < kPlaneRows; ++k) {
+    const int r = rr + 4 * k;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int c = cc + 64 * h;
+      if (r < g.SY && c < g.SX) stage[r * g.SX + c] = R.v[k][h];
+    }
+  }
+  if (g.SY > 4 * kPlaneRows) {
+    bool zc;
+    const int64_t zm = gmap(zi, g.D, g.mode, &zc);
+    for (int r = 4 * kPlaneRows + rr; r < g.SY; r += 4)
+      for (int c = cc; c < g.SX; c += 64)
+        stage[r * g.SX + c] = plane_value(vol, g, zm, zc, y0 + r, x0 + c);
+  }
+}
+
+// Logical block (tile fastest, then z chunk, then item) of a 1-D launch,
+// XCD-swizzled so neighbouring tiles (shared halos) share an L2.
+__device__ __forceinline__ void grey_block(int ntx, int ntiles, int nz, int* tx, int* ty, int* zc,
+                                           int64_t* item) {
+  const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = lb % ntiles;
+  const int rest = lb / ntiles;
+  *tx = t % ntx;
+  *ty = t / ntx;
+  *zc = rest % nz;
+  *item = rest / nz;
 }
 
 template <bool MAX>
@@ -89,23 +182,30 @@ __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict_
   float* stage = sm;
   float* xr = stage + SY * SX;
   float* ring = xr + SY * kTX;
-  const int ntx = (W + kTX - 1) / kTX;
-  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
+  const int nz = (D + zchunk - 1) / zchunk;
+  int tx, ty, zci;
+  int64_t item;
+  grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
   const int x0 = tx * kTX, y0 = ty * kTY;
-  const int zb = blockIdx.y * zchunk;
+  const int zb = zci * zchunk;
   const int ze = min(D, zb + zchunk);
-  const int64_t item = blockIdx.z;
   const float* vol = in + item * D * H * W;
   float* ovol = out + item * D * H * W;
   const float ID = gid<MAX>();
+  const PlaneGeom g{D, H, W, SY, SX, mode, cval, MAX ? 0.0f : -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
+  const PlaneMap pm = plane_map(g, y0 + sy, x0 + sx);
+  PlaneRegs R;
+  plane_fetch(R, vol, g, pm, zb + sz);
   for (int n = 0; n < nplanes; ++n) {
     const int zi = zb + sz + n;
-    load_plane(vol, stage, zi, D, H, W, y0 + sy, x0 + sx, SY, SX, mode, cval,
-               MAX ? 0.0f : -0.0f);
+    plane_store(R, stage, vol, g, zi, y0 + sy, x0 + sx);
     __syncthreads();
+    // the next plane's loads are in flight while this one is reduced
+    if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
     for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
-      const int r = e / kTX, c = e - r * kTX;
+      const int r = e >> 6, c = e & 63;
       const float* s = stage + r * SX + c;
       float acc = ID;
       for (int l = 0; l < kx; ++l) acc = gop<MAX>(acc, s[l]);
@@ -114,7 +214,7 @@ __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict_
     __syncthreads();
     float* slot = ring + (n % kz) * (kTY * kTX);
     for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
-      const int r = e / kTX, c = e - r * kTX;
+      const int r = e >> 6, c = e & 63;
       float acc = ID;
       for (int j = 0; j < ky; ++j) acc = gop<MAX>(acc, xr[(r + j) * kTX + c]);
       slot[e] = acc;
@@ -123,7 +223,7 @@ __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict_
     const int z = zb + n - (kz - 1);
     if (z >= zb) {
       for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
-        const int r = e / kTX, c = e - r * kTX;
+        const int r = e >> 6, c = e & 63;
         const int y = y0 + r, x = x0 + c;
         if (y < H && x < W) {
           float acc = ID;
@@ -143,7 +243,9 @@ struct GreyTable {
 };
 
 // Ring of raw input halo planes; zlo..zhi, ylo..yhi, xlo..xhi bound the table
-// offsets.
+// offsets.  Per plane step the LDS offset of every table element (ring slot
+// of its dz, then dy, dx) is computed once, so the inner loop is one LDS
+// read, one add and one min/max per element.
 template <bool MAX>
 __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restrict__ in,
                                                           float* __restrict__ out, int D, int H,
@@ -151,37 +253,44 @@ __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restric
                                                           int ylo, int yhi, int xlo, int xhi,
                                                           int mode, float cval, int zchunk) {
   extern __shared__ float sm[];
+  __shared__ int soff[kMaxTable];
   const int kz = zhi - zlo + 1;
   const int SX = kTX + xhi - xlo, SY = kTY + yhi - ylo;
   const int plane = SY * SX;
-  const int ntx = (W + kTX - 1) / kTX;
-  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
+  const int nz = (D + zchunk - 1) / zchunk;
+  int tx, ty, zci;
+  int64_t item;
+  grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
   const int x0 = tx * kTX, y0 = ty * kTY;
-  const int zb = blockIdx.y * zchunk;
+  const int zb = zci * zchunk;
   const int ze = min(D, zb + zchunk);
-  const int64_t item = blockIdx.z;
   const float* vol = in + item * D * H * W;
   float* ovol = out + item * D * H * W;
   const float ID = gid<MAX>();
+  const PlaneGeom g{D, H, W, SY, SX, mode, cval, -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
+  const PlaneMap pm = plane_map(g, y0 + ylo, x0 + xlo);
+  PlaneRegs R;
+  plane_fetch(R, vol, g, pm, zb + zlo);
   for (int n = 0; n < nplanes; ++n) {
     const int zi = zb + zlo + n;
-    float* slot = sm + (n % kz) * plane;
-    load_plane(vol, slot, zi, D, H, W, y0 + ylo, x0 + xlo, SY, SX, mode, cval, -0.0f);
+    plane_store(R, sm + (n % kz) * plane, vol, g, zi, y0 + ylo, x0 + xlo);
+    // plane of offset dz sits at ring index (n - (zhi - dz)) mod kz
+    for (int t = threadIdx.x; t < tab.n; t += blockDim.x) {
+      const int ridx = ((n - (zhi - tab.dz[t])) % kz + kz) % kz;
+      soff[t] = ridx * plane + (tab.dy[t] - ylo) * SX + (tab.dx[t] - xlo);
+    }
     __syncthreads();
+    if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
     const int z = zb + n - (kz - 1);
     if (z >= zb) {
-      // plane of offset dz sits at ring index (n - (zhi - dz)) % kz
       for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
-        const int r = e / kTX, c = e - r * kTX;
+        const int r = e >> 6, c = e & 63;
         const int y = y0 + r, x = x0 + c;
+        const float* base = sm + r * SX + c;
         float acc = ID;
-        for (int t = 0; t < tab.n; ++t) {
-          const int ridx = (n - (zhi - tab.dz[t])) % kz;
-          const float val =
-              sm[ridx * plane + (r + tab.dy[t] - ylo) * SX + (c + tab.dx[t] - xlo)];
-          acc = gop<MAX>(acc, val + tab.v[t]);
-        }
+        for (int t = 0; t < tab.n; ++t) acc = gop<MAX>(acc, base[soff[t]] + tab.v[t]);
         if (y < H && x < W) ovol[((int64_t)z * H + y) * W + x] = acc;
       }
     }
@@ -355,21 +464,39 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
     const int D = (int)s3[0], H = (int)s3[1], W = (int)s3[2];
     const int kz = (int)(hi3[0] - lo3[0] + 1), ky = (int)(hi3[1] - lo3[1] + 1),
               kx = (int)(hi3[2] - lo3[2] + 1);
-    const int zchunk = D <= 32 ? D : 32;
-    const unsigned tiles = (unsigned)(((W + kTX - 1) / kTX) * ((H + kTY - 1) / kTY));
-    const dim3 grid(tiles, (unsigned)((D + zchunk - 1) / zchunk), (unsigned)batch);
+    const int64_t ntiles = ((W + kTX - 1) / kTX) * (int64_t)((H + kTY - 1) / kTY);
+    // launch variant v: z planes per workgroup = 8 << v (capped at D); the
+    // autotuner trades workgroup count against the kz - 1 halo planes every
+    // chunk re-reads
+    auto grid_of = [&](int zchunk) -> int64_t {
+      return ntiles * ((D + zchunk - 1) / zchunk) * batch;
+    };
+    auto zchunk_of = [&](int v) { return std::min(D, 8 << v); };
+    auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
     if (flat && full && box3_lds(kz, ky, kx) <= kLdsBudget) {
       const size_t lds = box3_lds(kz, ky, kx);
-      if (is_max)
-        hipLaunchKernelGGL((grey_box3_kernel<true>), grid, dim3(256), lds, st, input, output, D,
-                           H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1], (int)lo3[2], mode, cval,
-                           zchunk);
-      else
-        hipLaunchKernelGGL((grey_box3_kernel<false>), grid, dim3(256), lds, st, input, output, D,
-                           H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1], (int)lo3[2], mode, cval,
-                           zchunk);
-      TM_LAUNCH_CHECK();
-      return TM_OK;
+      auto run = [&](int v) -> int {
+        const int zchunk = zchunk_of(v);
+        const int64_t G = grid_of(zchunk);
+        if (G >= ((int64_t)1 << 31)) {
+          set_error("tm_grey: batch too large for one launch");
+          return TM_ERR_ARG;
+        }
+        if (is_max)
+          hipLaunchKernelGGL((grey_box3_kernel<true>), dim3((unsigned)G), dim3(256), lds, st, input,
+                             output, D, H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1], (int)lo3[2],
+                             mode, cval, zchunk);
+        else
+          hipLaunchKernelGGL((grey_box3_kernel<false>), dim3((unsigned)G), dim3(256), lds, st,
+                             input, output, D, H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1],
+                             (int)lo3[2], mode, cval, zchunk);
+        TM_LAUNCH_CHECK();
+        return TM_OK;
+      };
+      const int64_t key[8] = {1, kz, ky, kx, p2(D), p2(H), p2(W), p2(batch)};
+      bool ran = false;
+      const int v = tune_pick("grey3", key, 8, 4, 1, run, st, &ran);
+      return ran ? TM_OK : run(v);
     }
     bool small = n <= kMaxTable;
     for (int j = 0; j < 3; ++j) small = small && lo3[j] >= -128 && hi3[j] <= 127;
@@ -385,16 +512,28 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
         tab.v[t] = hval[t];
       }
       const size_t lds = table3_lds(kz, ky, kx);
-      if (is_max)
-        hipLaunchKernelGGL((grey_table3_kernel<true>), grid, dim3(256), lds, st, input, output, D,
-                           H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1], (int)hi3[1],
-                           (int)lo3[2], (int)hi3[2], mode, cval, zchunk);
-      else
-        hipLaunchKernelGGL((grey_table3_kernel<false>), grid, dim3(256), lds, st, input, output,
-                           D, H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1], (int)hi3[1],
-                           (int)lo3[2], (int)hi3[2], mode, cval, zchunk);
-      TM_LAUNCH_CHECK();
-      return TM_OK;
+      auto run = [&](int v) -> int {
+        const int zchunk = zchunk_of(v);
+        const int64_t G = grid_of(zchunk);
+        if (G >= ((int64_t)1 << 31)) {
+          set_error("tm_grey: batch too large for one launch");
+          return TM_ERR_ARG;
+        }
+        if (is_max)
+          hipLaunchKernelGGL((grey_table3_kernel<true>), dim3((unsigned)G), dim3(256), lds, st,
+                             input, output, D, H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1],
+                             (int)hi3[1], (int)lo3[2], (int)hi3[2], mode, cval, zchunk);
+        else
+          hipLaunchKernelGGL((grey_table3_kernel<false>), dim3((unsigned)G), dim3(256), lds, st,
+                             input, output, D, H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1],
+                             (int)hi3[1], (int)lo3[2], (int)hi3[2], mode, cval, zchunk);
+        TM_LAUNCH_CHECK();
+        return TM_OK;
+      };
+      const int64_t key[8] = {2, kz, ky, kx, p2(D), p2(H), p2(W), p2(batch)};
+      bool ran = false;
+      const int v = tune_pick("grey3", key, 8, 4, 1, run, st, &ran);
+      return ran ? TM_OK : run(v);
     }
   }
   const size_t need = tm_grey_workspace_bytes(batch, ndim, shape, se_shape);
