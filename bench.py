@@ -447,7 +447,7 @@ def extra_warmstart():
                                   log=lambda s: None)
     keys = ("pairs", "cold_iters", "warm_iters", "iter_ratio", "cold_solves_per_s",
             "warm_solves_per_s", "speedup", "cold_err_max", "warm_err_max", "warm_fallbacks")
-    return {k: (round(ev[k], 4) if isinstance(ev[k], float) else ev[k]) for k in keys}
+    return {k: (float(f"{ev[k]:.5g}") if isinstance(ev[k], float) else ev[k]) for k in keys}
 
 
 if __name__ == "__main__":

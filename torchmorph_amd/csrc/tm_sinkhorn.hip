@@ -77,9 +77,13 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
     float* __restrict__ pout, const float* __restrict__ wr, const SkPair* __restrict__ pairs,
     const SkTile* __restrict__ tiles, const int* __restrict__ active, float* __restrict__ errpart,
     int maxc, const float* __restrict__ centre) {
-  extern __shared__ float dyn[];
-  float* sy = dyn;              // [maxc][D] column points
-  float* sh = dyn + maxc * D;   // [maxc] h_j
+  // column j in LDS as one float4: (y_0, .., y_{D-1}, h_j) for D <= 3, so a
+  // term is one broadcast LDS read, three FMAs (h_j as the addend), one
+  // subtraction, one v_exp_f32 and one add; D = 4 keeps h_j in its own array
+  constexpr bool PACK = D <= 3;
+  extern __shared__ float4 dyn4[];
+  float4* s4 = dyn4;                                 // [maxc] points (+ h_j when PACK)
+  float* shx = (float*)(dyn4 + maxc);                // [maxc] h_j when !PACK
   __shared__ float ssum[4][kRowsPerTile];
   __shared__ float smax[4][kRowsPerTile];
   __shared__ int sbad;
@@ -96,24 +100,28 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
   for (int d = 0; d < D; ++d) ctr[d] = centre[tile.pair * kCtrStride + d];
   // stage the pair's columns
   for (int j = threadIdx.x; j < nc; j += blockDim.x) {
-    float y[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) y[d] = cols[(coff + j) * D + d] - ctr[d];
+    float y[4] = {0.f, 0.f, 0.f, 0.f};
     float yy = 0.f;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      sy[j * D + d] = y[d];
+      y[d] = cols[(coff + j) * D + d] - ctr[d];
       yy = fmaf(y[d], y[d], yy);
     }
-    sh[j] = fmaf(k1, pc[coff + j] - yy, lwc[coff + j]);
+    const float h = fmaf(k1, pc[coff + j] - yy, lwc[coff + j]);
+    if (PACK) {
+      s4[j] = make_float4(y[0], y[1], y[2], h);
+    } else {
+      s4[j] = make_float4(y[0], y[1], y[2], y[3]);
+      shx[j] = h;
+    }
   }
   if (threadIdx.x == 0) sbad = 0;
   __syncthreads();
   const int r = threadIdx.x & (kRowsPerTile - 1);
-  const int q = threadIdx.x >> 6;  // column quarter
+  const int q = threadIdx.x >> 6;  // column quarter (one per wavefront)
   const int i = tile.r0 + r;
   const bool live = i < nr;
-  float xt[D];
+  float xt[4] = {0.f, 0.f, 0.f, 0.f};
   float ci = 0.f, s = 0.f, fold = 0.f;
   if (live) {
 #pragma unroll
@@ -125,16 +133,21 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
     ci *= k1;
     fold = pin[roff + i];
     s = ci - k1 * fold;
-  } else {
-#pragma unroll
-    for (int d = 0; d < D; ++d) xt[d] = 0.f;
   }
+  // t_j = h_j + xt . y_j
+  auto term = [&](int j) -> float {
+    const float4 v = s4[j];
+    float t = PACK ? v.w : shx[j];
+    t = fmaf(xt[0], v.x, t);
+    if (D >= 2) t = fmaf(xt[1], v.y, t);
+    if (D >= 3) t = fmaf(xt[2], v.z, t);
+    if (D >= 4) t = fmaf(xt[3], v.w, t);
+    return t;
+  };
   float acc = 0.f;
   if (live) {
-    for (int j = q; j < nc; j += 4) {
-      const float e = sh[j] + dotp<D>(xt, &sy[j * D]) - s;
-      acc += __builtin_amdgcn_exp2f(e);
-    }
+#pragma unroll 4
+    for (int j = q; j < nc; j += 4) acc += __builtin_amdgcn_exp2f(term(j) - s);
   }
   ssum[q][r] = acc;
   __syncthreads();
@@ -146,14 +159,13 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
     // exact path for the whole workgroup: per-row maximum, then the sum
     float mx = -INFINITY;
     if (live)
-      for (int j = q; j < nc; j += 4) mx = fmaxf(mx, sh[j] + dotp<D>(xt, &sy[j * D]));
+      for (int j = q; j < nc; j += 4) mx = fmaxf(mx, term(j));
     smax[q][r] = mx;
     __syncthreads();
     mx = fmaxf(fmaxf(smax[0][r], smax[1][r]), fmaxf(smax[2][r], smax[3][r]));
     float a2 = 0.f;
     if (live)
-      for (int j = q; j < nc; j += 4)
-        a2 += __builtin_amdgcn_exp2f(sh[j] + dotp<D>(xt, &sy[j * D]) - mx);
+      for (int j = q; j < nc; j += 4) a2 += __builtin_amdgcn_exp2f(term(j) - mx);
     __syncthreads();
     ssum[q][r] = a2;
     __syncthreads();
@@ -535,7 +547,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
 
 #define SK_HALF(D, RX, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, NT, ERR)                     \
   hipLaunchKernelGGL((sk_half_kernel<D, RX>), dim3((unsigned)(NT)), dim3(256),                  \
-                     (size_t)maxc * (D + 1) * sizeof(float), lst, ROWS, COLS, LWC, PC, PIN, POUT, \
+                     (size_t)maxc * ((D) <= 3 ? 16 : 20), lst, ROWS, COLS, LWC, PC, PIN, POUT,   \
                      WR, dpairs, TILES, active, ERR, maxc, centre)
   auto gstep = [&](const float* fcur) -> int {
     switch (dim) {
