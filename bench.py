@@ -38,6 +38,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 C1 = dict(B=8, C=1, H=256, W=256, p=0.9)
 BYTES_PER_VOX = 1 + 4 + 16  # bool mask in, f32 distance + 2 x int64 index out
+# HBM bytes per step measured with the FETCH_SIZE / WRITE_SIZE counters (copy-calibrated):
+# profiles/edt2d_b8_256x256_hbm_counters.md
+TRAFFIC_PER_STEP = 13_260_000
 
 
 def parse():
@@ -216,7 +219,9 @@ def main():
         "parity": "bit-exact vs oracle" if parity else "MISMATCH",
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            # the counter figure belongs to the profiled launch variant
+            "traffic": TRAFFIC_PER_STEP if tuned.get("edt2d 256 256 64 1") == 0 else None,
             "kernel": "edt2d_cols_kernel + edt2d_rows_kernel (one step)", "bytes_per_launch": nvox * BYTES_PER_VOX,
             "event_us_per_step": round(per_launch_s * 1e6, 3),
             "autotune": tuned,
