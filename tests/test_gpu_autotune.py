@@ -22,7 +22,8 @@ def tm():
 
 
 @pytest.mark.parametrize("shape", [(8, 1, 256, 256), (2, 1, 100, 300), (1, 2, 17, 33),
-                                   (3, 1, 256, 512), (1, 1, 5, 512), (2, 1, 300, 700)])
+                                   (3, 1, 256, 512), (1, 1, 5, 512), (2, 1, 300, 700),
+                                   (1, 1, 1000, 61), (2, 1, 130, 1)])
 @pytest.mark.parametrize("idx", ["i64", "i32", "none"])
 def test_edt2d_every_variant_matches_oracle(tm, oracle_mod, shape, idx):
     from torchmorph_amd import autotune
