@@ -597,7 +597,24 @@ struct GreyTabN {
   int32_t ext[kMaxNdim];  // max |offset| per axis
   int8_t o[kMaxTable][kMaxNdim];
   float v[kMaxTable];
+  int32_t flat[kMaxTable];  // sum_j o[t][j] * stride[j]: the element's offset inside an item
 };
+
+// Coordinates of element g of a batch; true when every read at an offset of
+// the table (|o_j| <= ext_j + pad) stays inside the item.
+__device__ __forceinline__ bool grey_decode(const Shape& sh, int64_t g, const int32_t* ext, int pad,
+                                            int64_t* item, int64_t* local, int64_t* c) {
+  *item = g / sh.numel;
+  int64_t rem = g - *item * sh.numel;
+  *local = rem;
+  bool interior = true;
+  for (int j = 0; j < sh.ndim; ++j) {
+    c[j] = rem / sh.stride[j];
+    rem -= c[j] * sh.stride[j];
+    if (c[j] < ext[j] + pad || c[j] > sh.size[j] - 1 - ext[j] - pad) interior = false;
+  }
+  return interior;
+}
 
 template <bool MAX>
 __global__ __launch_bounds__(256) void grey_arg_kernel(const float* __restrict__ in,
@@ -607,16 +624,24 @@ __global__ __launch_bounds__(256) void grey_arg_kernel(const float* __restrict__
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const int nd = sh.ndim;
-  const int64_t item = g / sh.numel;
-  int64_t rem = g - item * sh.numel;
-  int64_t c[kMaxNdim];
-  for (int j = 0; j < nd; ++j) {
-    c[j] = rem / sh.stride[j];
-    rem -= c[j] * sh.stride[j];
-  }
+  int64_t item, local, c[kMaxNdim];
+  const bool interior = grey_decode(sh, g, tab.ext, 0, &item, &local, c);
   const float* vol = in + item * sh.numel;
   float best = 0.f;
   int bi = 0;
+  if (interior) {
+    // every read in bounds: one load per element at a precomputed offset
+    const float* vp = vol + local;
+    for (int t = 0; t < tab.n; ++t) {
+      const float v = vp[tab.flat[t]] + tab.v[t];
+      if (t == 0 || (MAX ? v > best : v < best)) {
+        best = v;
+        bi = t;
+      }
+    }
+    arg[g] = (int16_t)bi;
+    return;
+  }
   for (int t = 0; t < tab.n; ++t) {
     bool use_c = false;
     int64_t flat = 0;
@@ -642,24 +667,18 @@ __global__ __launch_bounds__(256) void grey_grad_gather_kernel(
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const int nd = sh.ndim;
-  const int64_t item = g / sh.numel;
-  int64_t rem = g - item * sh.numel;
-  int64_t q[kMaxNdim];
-  bool interior = true;
-  for (int j = 0; j < nd; ++j) {
-    q[j] = rem / sh.stride[j];
-    rem -= q[j] * sh.stride[j];
-    // mirror maps a read at -k onto k (not k-1): its border zone is one wider
-    if (q[j] < ext[j] + 1 || q[j] > sh.size[j] - 2 - ext[j]) interior = false;
-  }
+  int64_t item, local, q[kMaxNdim];
+  // mirror maps a read at -k onto k (not k-1): its border zone is one wider
+  const bool interior = grey_decode(sh, g, ext, 1, &item, &local, q);
   const int64_t base = item * sh.numel;
   float acc = 0.f;
   if (interior) {
     // every output reading q through element t is p = q - o_t, in bounds
+    const int16_t* ap = arg + g;
+    const float* dp = dy + g;
     for (int t = 0; t < tab.n; ++t) {
-      int64_t p = 0;
-      for (int j = 0; j < nd; ++j) p += (q[j] - tab.o[t][j]) * sh.stride[j];
-      if (arg[base + p] == t) acc += dy[base + p];
+      const int32_t off = tab.flat[t];
+      if (ap[-off] == t) acc += dp[-off];
     }
   } else {
     // border voxel: scan the outputs within the element extent of q (per
@@ -722,6 +741,7 @@ extern "C" int tm_grey_backward(const float* input, int64_t batch, int ndim, con
   hipStream_t st = (hipStream_t)stream;
   Shape sh = make_shape(ndim, shape);
   const int64_t total = batch * sh.numel;
+  TM_CHECK_ARG(sh.numel < ((int64_t)1 << 31), "tm_grey_backward: items above 2^31 voxels");
   int64_t ne = 1;
   for (int j = 0; j < ndim; ++j) ne *= se_shape[j];
   GreyTabN tab{};
@@ -758,6 +778,11 @@ extern "C" int tm_grey_backward(const float* input, int64_t batch, int ndim, con
   }
   int16_t* arg = (int16_t*)workspace;
   for (int j = 0; j < kMaxNdim; ++j) tab.ext[j] = j < ndim ? ext[j] : 0;
+  for (int t = 0; t < tab.n; ++t) {
+    int64_t f = 0;
+    for (int j = 0; j < ndim; ++j) f += (int64_t)tab.o[t][j] * sh.stride[j];
+    tab.flat[t] = (int32_t)f;  // |f| < numel of one item
+  }
   const dim3 grid(grid_for(total, 256)), blk(256);
   if (dilation)
     hipLaunchKernelGGL(grey_arg_kernel<true>, grid, blk, 0, st, input, arg, sh, total, tab, mode,
