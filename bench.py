@@ -299,6 +299,17 @@ def extra_morph3d():
                                 "parity": bool(np.array_equal(yb[:1, :1].cpu().numpy(),
                                                               want.astype(bool)))}
     out["bytes_per_vox"] = "grey 4 in + 4 out; binary 1 in + 1 out (algorithmic)"
+    # CPU oracle beside it: one 128^3 item, single thread
+    t0 = time.perf_counter()
+    oracle.grey(x0, None, None, (3, 3, 3), (0, 0, 0), dilation=True)
+    t_grey = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.binary(mask[:1, :1].cpu().numpy(), cross, (0, 0, 0), iterations=4)
+    t_bin = time.perf_counter() - t0
+    out["cpu_baseline"] = {"grey_dilation_cube3_Mvox_s": round(x0.size / t_grey / 1e6, 2),
+                           "binary_erosion_x4_Mvox_s": round(x0.size / t_bin / 1e6, 2),
+                           "cores": 1, "kind": "port",
+                           "sample": "one 128^3 item through oracle/tm_oracle.c"}
     return out
 
 
@@ -378,6 +389,20 @@ def extra_ragged():
             cd, _ = oracle.cdt(mk, metric)
             ok &= np.array_equal(out[o][k].cpu().numpy(), cd[0, 0].astype(np.float32))
     res["parity"] = {"equal_to_dense_per_item": bool(same), "oracle_items_ok": bool(ok)}
+    # CPU oracle beside it: the three ops on the first 2-D items until ~3 s
+    t0 = time.perf_counter()
+    done_vox = 0
+    k = 0
+    while time.perf_counter() - t0 < 3.0 and k < 512:
+        mk = masks[k].cpu().numpy()[None, None]
+        oracle.edt(mk, samp[k])
+        oracle.cdt(mk, "taxicab")
+        oracle.cdt(mk, "chessboard")
+        done_vox += 3 * mk.size
+        k += 1
+    dt = time.perf_counter() - t0
+    res["cpu_baseline"] = {"value": round(done_vox / dt / 1e6, 2), "unit": "Mvox-op/s", "cores": 1,
+                           "kind": "port", "sample": f"{k} 2-D items x 3 ops, oracle/tm_oracle.c"}
     res["bytes_per_vox_op"] = "1 mask in + 4 f32 out (algorithmic)"
     return res
 
@@ -426,6 +451,20 @@ def extra_grad_and_sinkhorn():
         P = torch.exp((f[p][:, None] + gg[p][None] - C) / float(eps[p])) / (N * N)
         errs.append(max(float((P.sum(1) - 1.0 / N).abs().sum()),
                         float((P.sum(0) - 1.0 / N).abs().sum())))
+    # CPU beside it: the FP64 numpy oracle solving two of the pairs to the same tolerance
+    import oracle
+
+    t0 = time.perf_counter()
+    for p in (0, 1):
+        xp = xs[p].double().cpu().numpy()
+        yp = ys[p].double().cpu().numpy()
+        oracle.sinkhorn_points(xp, yp, np.full(N, 1.0 / N), np.full(N, 1.0 / N), float(eps[p]),
+                               tol=1e-5)
+    cpu_rate = 2 / (time.perf_counter() - t0)
+    out["cpu_baseline"] = {"value": round(cpu_rate, 3), "unit": "solves/s", "cores": 1,
+                           "kind": "port",
+                           "sample": "2 pairs, FP64 numpy log-domain oracle (elementwise numpy, "
+                                     "one thread)"}
     it = res.iterations.float()
     out["sinkhorn_points"] = {"pairs": B, "n": N, "m": N, "solves_s": round(B / dt, 1),
                               "ms": round(dt * 1e3, 2), "iters_mean": round(float(it.mean()), 1),
