@@ -62,12 +62,14 @@ __device__ __forceinline__ float gid() {
 }
 
 constexpr int kTX = 64, kTY = 16;
-constexpr int kPlaneRows = 12;  // halo rows a thread keeps in registers (4 rows per pass)
+// Halo rows a thread keeps in registers (4 rows per pass): NR = 6 covers
+// structuring elements up to 9 high (more waves per SIMD), NR = 12 up to 33.
+constexpr int kPlaneRowsSmall = 6, kPlaneRowsLarge = 12;
 
 // One mode-extended input plane halo tile [SY][SX] starting at (y0, x0),
 // distributed over the workgroup as 4 rows x 64 columns per pass: thread
 // (rr, cc) holds rows rr + 4k and columns cc, cc + 64 (coalesced along x, no
-// divisions).  Rows beyond 4 * kPlaneRows are staged directly (large SEs).
+// divisions).  Rows beyond 4 * NR are staged directly (large SEs).
 // The registers of plane n + 1 are filled while plane n is reduced.
 // `bias` is added to every value: +0.0f for flat dilation (the reference
 // adds its zero structure, which turns -0.0 into +0.0), -0.0f otherwise (an
@@ -89,16 +91,18 @@ __device__ __forceinline__ float plane_value(const float* __restrict__ vol, cons
 // The rows and columns a thread loads are the same for every plane of its
 // tile: their mode-mapped coordinates (-1 = constant border) are computed
 // once per workgroup.
+template <int NR>
 struct PlaneMap {
-  int32_t ym[kPlaneRows];
+  int32_t ym[NR];
   int32_t xm[2];
 };
 
-__device__ __forceinline__ PlaneMap plane_map(const PlaneGeom& g, int y0, int x0) {
-  PlaneMap m;
+template <int NR>
+__device__ __forceinline__ PlaneMap<NR> plane_map(const PlaneGeom& g, int y0, int x0) {
+  PlaneMap<NR> m;
   const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
 #pragma unroll
-  for (int k = 0; k < kPlaneRows; ++k) {
+  for (int k = 0; k < NR; ++k) {
     bool c;
     const int64_t v = gmap(y0 + rr + 4 * k, g.H, g.mode, &c);
     m.ym[k] = c ? -1 : (int32_t)v;
@@ -112,18 +116,20 @@ __device__ __forceinline__ PlaneMap plane_map(const PlaneGeom& g, int y0, int x0
   return m;
 }
 
+template <int NR>
 struct PlaneRegs {
-  float v[kPlaneRows][2];
+  float v[NR][2];
 };
 
-__device__ __forceinline__ void plane_fetch(PlaneRegs& R, const float* __restrict__ vol,
-                                            const PlaneGeom& g, const PlaneMap& m, int zi) {
+template <int NR>
+__device__ __forceinline__ void plane_fetch(PlaneRegs<NR>& R, const float* __restrict__ vol,
+                                            const PlaneGeom& g, const PlaneMap<NR>& m, int zi) {
   bool zc;
   const int64_t zm = gmap(zi, g.D, g.mode, &zc);
   const float* pl = vol + zm * g.H * g.W;
   const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
 #pragma unroll
-  for (int k = 0; k < kPlaneRows; ++k) {
+  for (int k = 0; k < NR; ++k) {
     const int r = rr + 4 * k;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -136,12 +142,13 @@ __device__ __forceinline__ void plane_fetch(PlaneRegs& R, const float* __restric
   }
 }
 
-__device__ __forceinline__ void plane_store(const PlaneRegs& R, float* stage,
+template <int NR>
+__device__ __forceinline__ void plane_store(const PlaneRegs<NR>& R, float* stage,
                                             const float* __restrict__ vol, const PlaneGeom& g,
                                             int zi, int y0, int x0) {
   const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
 #pragma unroll
-  for (int k = 0; k < kPlaneRows; ++k) {
+  for (int k = 0; k < NR; ++k) {
     const int r = rr + 4 * k;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -149,10 +156,10 @@ __device__ __forceinline__ void plane_store(const PlaneRegs& R, float* stage,
       if (r < g.SY && c < g.SX) stage[r * g.SX + c] = R.v[k][h];
     }
   }
-  if (g.SY > 4 * kPlaneRows) {
+  if (g.SY > 4 * NR) {
     bool zc;
     const int64_t zm = gmap(zi, g.D, g.mode, &zc);
-    for (int r = 4 * kPlaneRows + rr; r < g.SY; r += 4)
+    for (int r = 4 * NR + rr; r < g.SY; r += 4)
       for (int c = cc; c < g.SX; c += 64)
         stage[r * g.SX + c] = plane_value(vol, g, zm, zc, y0 + r, x0 + c);
   }
@@ -171,7 +178,7 @@ __device__ __forceinline__ void grey_block(int ntx, int ntiles, int nz, int* tx,
   *item = rest / nz;
 }
 
-template <bool MAX>
+template <bool MAX, int NR>
 __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict__ in,
                                                         float* __restrict__ out, int D, int H,
                                                         int W, int kz, int ky, int kx, int sz,
@@ -195,8 +202,8 @@ __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict_
   const float ID = gid<MAX>();
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, MAX ? 0.0f : -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
-  const PlaneMap pm = plane_map(g, y0 + sy, x0 + sx);
-  PlaneRegs R;
+  const PlaneMap<NR> pm = plane_map<NR>(g, y0 + sy, x0 + sx);
+  PlaneRegs<NR> R;
   plane_fetch(R, vol, g, pm, zb + sz);
   for (int n = 0; n < nplanes; ++n) {
     const int zi = zb + sz + n;
@@ -246,7 +253,7 @@ struct GreyTable {
 // offsets.  Per plane step the LDS offset of every table element (ring slot
 // of its dz, then dy, dx) is computed once, so the inner loop is one LDS
 // read, one add and one min/max per element.
-template <bool MAX>
+template <bool MAX, int NR>
 __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restrict__ in,
                                                           float* __restrict__ out, int D, int H,
                                                           int W, GreyTable tab, int zlo, int zhi,
@@ -270,8 +277,8 @@ __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restric
   const float ID = gid<MAX>();
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
-  const PlaneMap pm = plane_map(g, y0 + ylo, x0 + xlo);
-  PlaneRegs R;
+  const PlaneMap<NR> pm = plane_map<NR>(g, y0 + ylo, x0 + xlo);
+  PlaneRegs<NR> R;
   plane_fetch(R, vol, g, pm, zb + zlo);
   for (int n = 0; n < nplanes; ++n) {
     const int zi = zb + zlo + n;
@@ -482,14 +489,19 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
           set_error("tm_grey: batch too large for one launch");
           return TM_ERR_ARG;
         }
-        if (is_max)
-          hipLaunchKernelGGL((grey_box3_kernel<true>), dim3((unsigned)G), dim3(256), lds, st, input,
-                             output, D, H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1], (int)lo3[2],
-                             mode, cval, zchunk);
-        else
-          hipLaunchKernelGGL((grey_box3_kernel<false>), dim3((unsigned)G), dim3(256), lds, st,
-                             input, output, D, H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1],
-                             (int)lo3[2], mode, cval, zchunk);
+        const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
+#define TM_BOX(MX, NR)                                                                          \
+  hipLaunchKernelGGL((grey_box3_kernel<MX, NR>), dim3((unsigned)G), dim3(256), lds, st, input,     \
+                     output, D, H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1], (int)lo3[2], mode,     \
+                     cval, zchunk)
+        if (is_max) {
+          if (small_se) TM_BOX(true, kPlaneRowsSmall);
+          else TM_BOX(true, kPlaneRowsLarge);
+        } else {
+          if (small_se) TM_BOX(false, kPlaneRowsSmall);
+          else TM_BOX(false, kPlaneRowsLarge);
+        }
+#undef TM_BOX
         TM_LAUNCH_CHECK();
         return TM_OK;
       };
@@ -519,14 +531,19 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
           set_error("tm_grey: batch too large for one launch");
           return TM_ERR_ARG;
         }
-        if (is_max)
-          hipLaunchKernelGGL((grey_table3_kernel<true>), dim3((unsigned)G), dim3(256), lds, st,
-                             input, output, D, H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1],
-                             (int)hi3[1], (int)lo3[2], (int)hi3[2], mode, cval, zchunk);
-        else
-          hipLaunchKernelGGL((grey_table3_kernel<false>), dim3((unsigned)G), dim3(256), lds, st,
-                             input, output, D, H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1],
-                             (int)hi3[1], (int)lo3[2], (int)hi3[2], mode, cval, zchunk);
+        const bool small_se = kTY + (int)(hi3[1] - lo3[1]) <= 4 * kPlaneRowsSmall;
+#define TM_TAB(MX, NR)                                                                          \
+  hipLaunchKernelGGL((grey_table3_kernel<MX, NR>), dim3((unsigned)G), dim3(256), lds, st, input,   \
+                     output, D, H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1], (int)hi3[1],     \
+                     (int)lo3[2], (int)hi3[2], mode, cval, zchunk)
+        if (is_max) {
+          if (small_se) TM_TAB(true, kPlaneRowsSmall);
+          else TM_TAB(true, kPlaneRowsLarge);
+        } else {
+          if (small_se) TM_TAB(false, kPlaneRowsSmall);
+          else TM_TAB(false, kPlaneRowsLarge);
+        }
+#undef TM_TAB
         TM_LAUNCH_CHECK();
         return TM_OK;
       };
