@@ -38,8 +38,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 C1 = dict(B=8, C=1, H=256, W=256, p=0.9)
 BYTES_PER_VOX = 1 + 4 + 16  # bool mask in, f32 distance + 2 x int64 index out
-# HBM bytes per step measured with the FETCH_SIZE / WRITE_SIZE counters (copy-calibrated):
-# profiles/edt2d_b8_256x256_hbm_counters.md
+# HBM bytes per step of the headline launch variant, measured with the FETCH_SIZE and
+# WRITE_SIZE counters and calibrated against a plain copy (summary in profiles/)
 TRAFFIC_PER_STEP = 13_260_000
 
 

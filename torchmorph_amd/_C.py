@@ -2,11 +2,11 @@
 
 Same nine function names, argument lists and return conventions as the
 reference's pybind11 module (reference torchmorph/csrc/torchmorph.cpp:3-77,
-79-155), implemented over the C ABI of libtorchmorph_hip.  A maintainer can
-put this file in place of the compiled extension and the reference's own
-Python wrappers (grey.py, binary.py, distance_transform.py,
-optimal_transport.py) run on the gfx950 kernels unchanged; INTEGRATION.md
-shows the one-line change.
+79-155), implemented over the C ABI of libtorchmorph_hip.  With
+``sys.modules["torchmorph._C"] = torchmorph_amd._C`` set before the
+reference package imports its submodules, the reference's own Python
+wrappers (grey.py, binary.py, distance_transform.py, optimal_transport.py)
+run on the gfx950 kernels unchanged.
 """
 from __future__ import annotations
 

@@ -38,6 +38,11 @@ from .optimal_transport import (
     build_cost_matrix,
     sinkhorn_points,
 )
+from .ragged import (
+    RaggedDistanceTransform,
+    chamfer_distance_transform_ragged,
+    euclidean_distance_transform_ragged,
+)
 
 __all__ = [
     "generate_binary_structure", "iterate_structure", "binary_erosion", "binary_dilation",
@@ -53,6 +58,9 @@ __all__ = [
     "build_cost_matrix",
     "sinkhorn_points",
     "SinkhornPointCloud",
+    "RaggedDistanceTransform",
+    "euclidean_distance_transform_ragged",
+    "chamfer_distance_transform_ragged",
 ]
 
 __version__ = "0.1.0"

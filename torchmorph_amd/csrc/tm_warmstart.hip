@@ -3,7 +3,7 @@
 //
 // A DeepSets model predicts the dual potential f of every x point of a pair
 // from both clouds; plain Sinkhorn then starts from it and still runs to the
-// same marginal tolerance (it remains the judge and the cold-start fallback).
+// same marginal tolerance (it decides convergence and is the cold-start fallback).
 //
 // Normalisation (per pair, computed on the device): c = mean of y,
 // s2 = mean_ij |x_i - y_j|^2 = mean|x - c|^2 + mean|y - c|^2 (closed form),
