@@ -168,6 +168,28 @@ int tm_grey_backward(const float* input, int64_t batch, int ndim, const int64_t*
                      const float* grad_out, float* grad_in, void* workspace,
                      size_t workspace_bytes, void* stream);
 
+/* Workspace bytes for tm_ragged_grey on these jobs (0 on invalid jobs). */
+size_t tm_ragged_grey_workspace_bytes(const tm_ragged_job* jobs, int64_t njobs);
+
+/* No reference counterpart: the reference's grey_erosion / grey_dilation take
+ * one dense (B, C, spatial...) tensor (grey.py:29-105, grey_kernel.cu:172-330).
+ * Grey erosion / dilation of a ragged list of float32 items in one launch:
+ * every job is one item of ndim (1..3, the same for every job and equal to
+ * the structuring element's) spatial axes at in_offset / out_offset (float32
+ * elements) of the flat input / output; op, sampling and idx_offset are not
+ * read.  Structuring element, origin, mode and cval as tm_grey, and each
+ * item's result is bit-identical to tm_grey on that item alone.  The element
+ * must be a flat box within the LDS budget or have at most 256 active
+ * elements with offsets in -128..127 (TM_ERR_ARG otherwise).
+ *   upload     1 = copy the item descriptors and the block -> item table into
+ *              the workspace (pinned staging, asynchronous); 0 = the
+ *              workspace holds them from an earlier call with the same jobs
+ *   workspace  256-byte aligned, tm_ragged_grey_workspace_bytes() bytes */
+int tm_ragged_grey(const tm_ragged_job* jobs, int64_t njobs, const float* input,
+                   const float* structure, const uint8_t* footprint, const int64_t* se_shape,
+                   const int64_t* origin, int mode, float cval, int dilation, float* output,
+                   int upload, void* workspace, size_t workspace_bytes, void* stream);
+
 /* -------------------------------------------------- binary morphology --- */
 
 /* Workspace bytes for tm_binary (packed ping/pong/orig/mask + flags). */

@@ -36,3 +36,20 @@ def test_public_names_match_reference():
         "brute_force_distance_transform",
     }
     assert ref_names <= set(tm.__all__)
+
+
+def test_ragged_grey_validation():
+    from torchmorph_amd.ragged import RaggedGreyMorphology
+
+    with pytest.raises(ValueError, match="every item must have 2 axes"):
+        RaggedGreyMorphology([(4, 4), (4, 4, 4)], size=3, device="cuda:0")
+    with pytest.raises(ValueError, match="1..3 axes"):
+        RaggedGreyMorphology([(2, 2, 2, 2)], size=3, device="cuda:0")
+    with pytest.raises(ValueError, match="operation must be"):
+        RaggedGreyMorphology([(4, 4)], "gradient", size=3, device="cuda:0")
+    with pytest.raises(ValueError, match="Structure dimension 3 must match spatial dimension 2"):
+        RaggedGreyMorphology([(4, 4)], structure=torch.zeros(3, 3, 3), device="cuda:0")
+    with pytest.raises(ValueError, match="At least one of size"):
+        RaggedGreyMorphology([(4, 4)], device="cuda:0")
+    with pytest.raises(ValueError, match="CUDA devices only"):
+        RaggedGreyMorphology([(4, 4)], size=3, device="cpu")

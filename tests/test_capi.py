@@ -50,3 +50,60 @@ def test_version_and_error_calls_need_no_gpu():
     rc = lib.tm_edt(None, 0, 1, 2, _lib.i64_array([4, 4]), None, None, None, 2, None, 0, None)
     assert rc == -1
     assert "input is NULL" in _lib.last_error()
+
+
+def test_ragged_grey_workspace_query_needs_no_gpu():
+    from torchmorph_amd import _lib
+    from torchmorph_amd.ragged import TmRaggedJob
+
+    lib = _lib.lib()
+    jobs = (TmRaggedJob * 2)()
+    for k, sh in enumerate([(20, 33, 70), (1, 1, 1)]):
+        jobs[k].ndim = 3
+        for a, v in enumerate(sh):
+            jobs[k].shape[a] = v
+        jobs[k].in_offset = jobs[k].out_offset = 0
+    # two item descriptors (one 256-byte block) + a block -> item table of
+    # 2 x 3 x 2 + 1 tiles of 64 x 16 x 16 (one 256-byte block)
+    assert lib.tm_ragged_grey_workspace_bytes(jobs, 2) == 512
+    jobs[1].ndim = 2
+    assert lib.tm_ragged_grey_workspace_bytes(jobs, 2) == 0
+    assert "job 1 has 2 axes" in _lib.last_error()
+    jobs[1].ndim = 3
+    jobs[1].out_offset = -1
+    assert lib.tm_ragged_grey_workspace_bytes(jobs, 2) == 0
+    assert "negative offset" in _lib.last_error()
+
+
+def test_grey_element_tables_of_growing_rank_need_no_gpu():
+    # tm_grey builds its element table on the host before it needs a device:
+    # with more than 3 axes and no workspace it returns TM_ERR_WORKSPACE (-3).
+    # The same element count with more axes needs a larger offset table.
+    from torchmorph_amd import _lib
+
+    lib = _lib.lib()
+    fake_in, fake_out = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    for se in ([3, 3, 1, 1], [3, 3, 1, 1, 1], [3, 3, 1, 1, 1, 1, 1, 1], [4, 4, 1, 1],
+               [5, 5, 1, 1, 1, 1, 1, 1]):
+        nd = len(se)
+        rc = lib.tm_grey(fake_in, 1, nd, _lib.i64_array([4] * nd), None, None,
+                         _lib.i64_array(se), _lib.i64_array([0] * nd), 1, 0.0, 0, fake_out,
+                         None, 0, None)
+        assert rc == -3, _lib.last_error()
+
+
+def test_ragged_grey_oversized_element_is_reported_on_the_host():
+    from torchmorph_amd import _lib
+    from torchmorph_amd.ragged import TmRaggedJob
+
+    lib = _lib.lib()
+    jobs = (TmRaggedJob * 1)()
+    jobs[0].ndim = 2
+    jobs[0].shape[0] = jobs[0].shape[1] = 30
+    need = lib.tm_ragged_grey_workspace_bytes(jobs, 1)
+    st = (ctypes.c_float * 300)(*([1.0] * 300))
+    rc = lib.tm_ragged_grey(jobs, 1, ctypes.c_void_p(0x1000), st, None, _lib.i64_array([20, 15]),
+                            _lib.i64_array([0, 0]), 1, 0.0, 0, ctypes.c_void_p(0x2000), 1,
+                            ctypes.c_void_p(0x100000), need, None)
+    assert rc == -1
+    assert "fits neither streaming kernel" in _lib.last_error()

@@ -40,8 +40,13 @@ from .optimal_transport import (
 )
 from .ragged import (
     RaggedDistanceTransform,
+    RaggedGreyMorphology,
     chamfer_distance_transform_ragged,
     euclidean_distance_transform_ragged,
+    grey_closing_ragged,
+    grey_dilation_ragged,
+    grey_erosion_ragged,
+    grey_opening_ragged,
 )
 
 __all__ = [
@@ -61,6 +66,11 @@ __all__ = [
     "RaggedDistanceTransform",
     "euclidean_distance_transform_ragged",
     "chamfer_distance_transform_ragged",
+    "RaggedGreyMorphology",
+    "grey_erosion_ragged",
+    "grey_dilation_ragged",
+    "grey_opening_ragged",
+    "grey_closing_ragged",
 ]
 
 __version__ = "0.1.0"

@@ -71,6 +71,8 @@ SIGNATURES: dict[str, tuple] = {
     "tm_ragged_dt_workspace_bytes": (_SZ, [_P, _I64]),
     "tm_ragged_dt": (_I, [_P, _I64, _P, _I, _P, _P, _I, _I, _I, _P, _SZ, _P]),
     "tm_ragged_dt_check": (_I, [_P, _P]),
+    "tm_ragged_grey_workspace_bytes": (_SZ, [_P, _I64]),
+    "tm_ragged_grey": (_I, [_P, _I64, _P, _P, _P, _I64P, _I64P, _I, _F, _I, _P, _I, _P, _SZ, _P]),
     "tm_binary_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
     "tm_binary": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _I, _I,
                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _P, _P, _P,

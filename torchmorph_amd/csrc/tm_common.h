@@ -65,6 +65,43 @@ inline Shape make_shape(int ndim, const int64_t* s) {
     }                                                                        \
   } while (0)
 
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Pinned host staging for descriptor uploads, one per host thread and call
+// site: a buffer is reused only after its previous copy has completed.
+struct PinnedStage {
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+};
+
+// Waits for the stage's previous copy and grows the buffer to `need` bytes.
+inline int stage_acquire(PinnedStage& s, size_t need) {
+  if (s.pending) {
+    TM_CHECK_HIP(hipEventSynchronize(s.ev));
+    s.pending = false;
+  }
+  if (s.cap < need) {
+    if (s.host) TM_CHECK_HIP(hipHostFree(s.host));
+    s.host = nullptr;
+    s.cap = 0;
+    TM_CHECK_HIP(hipHostMalloc(&s.host, need, hipHostMallocDefault));
+    s.cap = need;
+  }
+  if (!s.ev) TM_CHECK_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+  return TM_OK;
+}
+
+// Copies the first `n` staged bytes to `dst` on `st`; the stage stays busy
+// until that copy completes.
+inline int stage_upload(PinnedStage& s, void* dst, size_t n, hipStream_t st) {
+  TM_CHECK_HIP(hipMemcpyAsync(dst, s.host, n, hipMemcpyHostToDevice, st));
+  TM_CHECK_HIP(hipEventRecord(s.ev, st));
+  s.pending = true;
+  return TM_OK;
+}
+
 inline unsigned grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
   return (unsigned)(g < 1 ? 1 : g);

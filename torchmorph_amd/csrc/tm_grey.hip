@@ -23,6 +23,7 @@
 //  * grey_gather_kernel general fallback for any ndim and any SE: one thread
 //    per output folds the element table from device memory.
 #include <algorithm>
+#include <vector>
 
 #include "tm_common.h"
 #include "tm_tune.h"
@@ -178,27 +179,18 @@ __device__ __forceinline__ void grey_block(int ntx, int ntiles, int nz, int* tx,
   *item = rest / nz;
 }
 
+// One (x, y) tile of one item over output planes [zb, ze).
 template <bool MAX, int NR>
-__global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict__ in,
-                                                        float* __restrict__ out, int D, int H,
-                                                        int W, int kz, int ky, int kx, int sz,
-                                                        int sy, int sx, int mode, float cval,
-                                                        int zchunk) {
-  extern __shared__ float sm[];
+__device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
+                                               float* __restrict__ ovol, int D, int H, int W,
+                                               int kz, int ky, int kx, int sz, int sy, int sx,
+                                               int mode, float cval, int tx, int ty, int zb,
+                                               int ze, float* sm) {
   const int SX = kTX + kx - 1, SY = kTY + ky - 1;
   float* stage = sm;
   float* xr = stage + SY * SX;
   float* ring = xr + SY * kTX;
-  const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
-  const int nz = (D + zchunk - 1) / zchunk;
-  int tx, ty, zci;
-  int64_t item;
-  grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
   const int x0 = tx * kTX, y0 = ty * kTY;
-  const int zb = zci * zchunk;
-  const int ze = min(D, zb + zchunk);
-  const float* vol = in + item * D * H * W;
-  float* ovol = out + item * D * H * W;
   const float ID = gid<MAX>();
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, MAX ? 0.0f : -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
@@ -242,6 +234,24 @@ __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict_
   }
 }
 
+template <bool MAX, int NR>
+__global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict__ in,
+                                                        float* __restrict__ out, int D, int H,
+                                                        int W, int kz, int ky, int kx, int sz,
+                                                        int sy, int sx, int mode, float cval,
+                                                        int zchunk) {
+  extern __shared__ float sm[];
+  const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
+  const int nz = (D + zchunk - 1) / zchunk;
+  int tx, ty, zci;
+  int64_t item;
+  grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
+  const int zb = zci * zchunk;
+  const int64_t n = (int64_t)D * H * W;
+  grey_box3_tile<MAX, NR>(in + item * n, out + item * n, D, H, W, kz, ky, kx, sz, sy, sx, mode,
+                          cval, tx, ty, zb, min(D, zb + zchunk), sm);
+}
+
 constexpr int kMaxTable = 256;
 struct GreyTable {
   int n;
@@ -254,26 +264,16 @@ struct GreyTable {
 // of its dz, then dy, dx) is computed once, so the inner loop is one LDS
 // read, one add and one min/max per element.
 template <bool MAX, int NR>
-__global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restrict__ in,
-                                                          float* __restrict__ out, int D, int H,
-                                                          int W, GreyTable tab, int zlo, int zhi,
-                                                          int ylo, int yhi, int xlo, int xhi,
-                                                          int mode, float cval, int zchunk) {
-  extern __shared__ float sm[];
-  __shared__ int soff[kMaxTable];
+__device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
+                                                 float* __restrict__ ovol, int D, int H, int W,
+                                                 const GreyTable& tab, int zlo, int zhi, int ylo,
+                                                 int yhi, int xlo, int xhi, int mode, float cval,
+                                                 int tx, int ty, int zb, int ze, float* sm,
+                                                 int* soff) {
   const int kz = zhi - zlo + 1;
   const int SX = kTX + xhi - xlo, SY = kTY + yhi - ylo;
   const int plane = SY * SX;
-  const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
-  const int nz = (D + zchunk - 1) / zchunk;
-  int tx, ty, zci;
-  int64_t item;
-  grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
   const int x0 = tx * kTX, y0 = ty * kTY;
-  const int zb = zci * zchunk;
-  const int ze = min(D, zb + zchunk);
-  const float* vol = in + item * D * H * W;
-  float* ovol = out + item * D * H * W;
   const float ID = gid<MAX>();
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
@@ -303,6 +303,77 @@ __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restric
     }
     __syncthreads();
   }
+}
+
+template <bool MAX, int NR>
+__global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restrict__ in,
+                                                          float* __restrict__ out, int D, int H,
+                                                          int W, GreyTable tab, int zlo, int zhi,
+                                                          int ylo, int yhi, int xlo, int xhi,
+                                                          int mode, float cval, int zchunk) {
+  extern __shared__ float sm[];
+  __shared__ int soff[kMaxTable];
+  const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
+  const int nz = (D + zchunk - 1) / zchunk;
+  int tx, ty, zci;
+  int64_t item;
+  grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
+  const int zb = zci * zchunk;
+  const int64_t n = (int64_t)D * H * W;
+  grey_table3_tile<MAX, NR>(in + item * n, out + item * n, D, H, W, tab, zlo, zhi, ylo, yhi, xlo,
+                            xhi, mode, cval, tx, ty, zb, min(D, zb + zchunk), sm, soff);
+}
+
+// Ragged lists (no reference counterpart): items of different (D, H, W) at
+// their own flat offsets share one launch.  Item k owns the logical blocks
+// [block0, block0 + ntiles * nz); a per-block table names the item, so a
+// workgroup finds its item with one load instead of a search.  The logical
+// block is XCD-swizzled as in the dense kernels.
+struct GreyRgItem {
+  int64_t in_off, out_off;  // float32 elements
+  int D, H, W;
+  int ntx, ntiles;  // tiles per row of tiles, per plane
+  int block0;
+};
+constexpr int kRgZChunk = 16;
+
+__device__ __forceinline__ GreyRgItem grey_rg_block(const GreyRgItem* __restrict__ items,
+                                                    const int32_t* __restrict__ blk_item, int* tx,
+                                                    int* ty, int* zb) {
+  const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
+  const GreyRgItem it = items[blk_item[lb]];
+  const int local = lb - it.block0;
+  const int t = local % it.ntiles;
+  *tx = t % it.ntx;
+  *ty = t / it.ntx;
+  *zb = (local / it.ntiles) * kRgZChunk;
+  return it;
+}
+
+template <bool MAX, int NR>
+__global__ __launch_bounds__(256) void grey_box3_ragged_kernel(
+    const float* __restrict__ in, float* __restrict__ out, const GreyRgItem* __restrict__ items,
+    const int32_t* __restrict__ blk_item, int kz, int ky, int kx, int sz, int sy, int sx, int mode,
+    float cval) {
+  extern __shared__ float sm[];
+  int tx, ty, zb;
+  const GreyRgItem it = grey_rg_block(items, blk_item, &tx, &ty, &zb);
+  grey_box3_tile<MAX, NR>(in + it.in_off, out + it.out_off, it.D, it.H, it.W, kz, ky, kx, sz, sy,
+                          sx, mode, cval, tx, ty, zb, min(it.D, zb + kRgZChunk), sm);
+}
+
+template <bool MAX, int NR>
+__global__ __launch_bounds__(256) void grey_table3_ragged_kernel(
+    const float* __restrict__ in, float* __restrict__ out, const GreyRgItem* __restrict__ items,
+    const int32_t* __restrict__ blk_item, GreyTable tab, int zlo, int zhi, int ylo, int yhi,
+    int xlo, int xhi, int mode, float cval) {
+  extern __shared__ float sm[];
+  __shared__ int soff[kMaxTable];
+  int tx, ty, zb;
+  const GreyRgItem it = grey_rg_block(items, blk_item, &tx, &ty, &zb);
+  grey_table3_tile<MAX, NR>(in + it.in_off, out + it.out_off, it.D, it.H, it.W, tab, zlo, zhi,
+                            ylo, yhi, xlo, xhi, mode, cval, tx, ty, zb,
+                            min(it.D, zb + kRgZChunk), sm, soff);
 }
 
 // One 1-D window pass along `axis` (flat SE): out = op over in[p + (start+l)e_axis].
@@ -370,17 +441,113 @@ static size_t table3_lds(int kz, int ky, int kx) {
 
 constexpr size_t kLdsBudget = 120 * 1024;
 
-struct GreyPlan {
-  int nd;
-  int64_t shape3[3];
-  bool box;                 // flat and every element active
-  int64_t ksize[kMaxNdim];  // SE shape
-  int64_t start[kMaxNdim];  // window start per axis (box)
-  int n;                    // active elements
-  // table: per element offsets (ndim) and value
-  int32_t* off;
-  float* val;
+// Active elements of a structuring element, in the reference's element order
+// (grey_kernel.cu:247-292): offsets o_i (erosion d_i, dilation -d_i) and
+// values v_i (erosion -s_i, dilation s_i).  The tables live in per-thread
+// storage that the next call reuses.
+struct GreyElems {
+  int n = 0;
+  bool flat = true;  // every value is +0.0
+  bool full = true;  // every element of the box is active
+  int64_t lo[kMaxNdim], hi[kMaxNdim];  // offset bounds per axis
+  const int32_t* off = nullptr;        // n x ndim
+  const float* val = nullptr;          // n
 };
+
+static int grey_elements(const char* who, int ndim, const float* structure,
+                         const uint8_t* footprint, const int64_t* se_shape,
+                         const int64_t* origin, int dilation, GreyElems* E) {
+  int64_t ne = 1;
+  for (int j = 0; j < ndim; ++j) {
+    TM_CHECK_ARG(se_shape[j] >= 1, "%s: structuring element axis %d is empty", who, j);
+    ne *= se_shape[j];
+  }
+  // ne x ndim offsets and ne values (sized by both: the same element count
+  // with more axes needs a larger offset table)
+  static thread_local std::vector<int32_t> hoff_buf;
+  static thread_local std::vector<float> hval_buf;
+  if (hoff_buf.size() < (size_t)(ne * ndim)) hoff_buf.resize((size_t)(ne * ndim));
+  if (hval_buf.size() < (size_t)ne) hval_buf.resize((size_t)ne);
+  int32_t* hoff = hoff_buf.data();
+  float* hval = hval_buf.data();
+  for (int j = 0; j < ndim; ++j) {
+    E->lo[j] = INT64_MAX;
+    E->hi[j] = INT64_MIN;
+  }
+  int n = 0;
+  for (int64_t e = 0; e < ne; ++e) {
+    if (footprint && !footprint[e]) {
+      E->full = false;
+      continue;
+    }
+    const float s = structure ? structure[e] : 0.0f;
+    if (s != 0.0f || (structure && std::signbit(s))) E->flat = false;
+    int64_t rem = e;
+    int64_t r[kMaxNdim];
+    for (int j = ndim - 1; j >= 0; --j) {
+      r[j] = rem % se_shape[j];
+      rem /= se_shape[j];
+    }
+    for (int j = 0; j < ndim; ++j) {
+      int64_t d = r[j] - (se_shape[j] / 2 + origin[j]);
+      if (dilation) d = -d;
+      hoff[n * ndim + j] = (int32_t)d;
+      E->lo[j] = d < E->lo[j] ? d : E->lo[j];
+      E->hi[j] = d > E->hi[j] ? d : E->hi[j];
+    }
+    hval[n] = dilation ? s : -s;
+    ++n;
+  }
+  TM_CHECK_ARG(n > 0, "%s: footprint must contain at least one active position", who);
+  E->n = n;
+  E->off = hoff;
+  E->val = hval;
+  return TM_OK;
+}
+
+// Offset bounds of an element of <= 3 axes, padded to (z, y, x).
+struct Grey3 {
+  int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  int kz, ky, kx;
+};
+
+static Grey3 grey3_of(int ndim, const GreyElems& E) {
+  Grey3 g;
+  for (int j = 0; j < ndim; ++j) {
+    g.lo[3 - ndim + j] = E.lo[j];
+    g.hi[3 - ndim + j] = E.hi[j];
+  }
+  g.kz = (int)(g.hi[0] - g.lo[0] + 1);
+  g.ky = (int)(g.hi[1] - g.lo[1] + 1);
+  g.kx = (int)(g.hi[2] - g.lo[2] + 1);
+  return g;
+}
+
+// The streaming kernels take the element as a flat box (box3) or as a table
+// of <= kMaxTable elements with int8 offsets (table3), within the LDS budget.
+static bool grey_box3_fits(const GreyElems& E, const Grey3& g) {
+  return E.flat && E.full && box3_lds(g.kz, g.ky, g.kx) <= kLdsBudget;
+}
+
+static bool grey_table3_fits(const GreyElems& E, const Grey3& g) {
+  bool ok = E.n <= kMaxTable;
+  for (int j = 0; j < 3; ++j) ok = ok && g.lo[j] >= -128 && g.hi[j] <= 127;
+  return ok && table3_lds(g.kz, g.ky, g.kx) <= kLdsBudget;
+}
+
+static GreyTable grey_table3_of(int ndim, const GreyElems& E) {
+  GreyTable tab{};
+  tab.n = E.n;
+  for (int t = 0; t < E.n; ++t) {
+    int32_t o3[3] = {0, 0, 0};
+    for (int j = 0; j < ndim; ++j) o3[3 - ndim + j] = E.off[t * ndim + j];
+    tab.dz[t] = (int8_t)o3[0];
+    tab.dy[t] = (int8_t)o3[1];
+    tab.dx[t] = (int8_t)o3[2];
+    tab.v[t] = E.val[t];
+  }
+  return tab;
+}
 
 }  // namespace tmh
 
@@ -409,68 +576,29 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
   hipStream_t st = (hipStream_t)stream;
   Shape sh = make_shape(ndim, shape);
   const int64_t total = batch * sh.numel;
-  int64_t ne = 1;
-  for (int j = 0; j < ndim; ++j) {
-    TM_CHECK_ARG(se_shape[j] >= 1, "tm_grey: structuring element axis %d is empty", j);
-    TM_CHECK_ARG(shape[j] >= 1, "tm_grey: input axis %d is empty", j);
-    ne *= se_shape[j];
+  for (int j = 0; j < ndim; ++j) TM_CHECK_ARG(shape[j] >= 1, "tm_grey: input axis %d is empty", j);
+  GreyElems E;
+  {
+    const int rc = grey_elements("tm_grey", ndim, structure, footprint, se_shape, origin,
+                                 dilation, &E);
+    if (rc != TM_OK) return rc;
   }
-  // host-side element table, in the reference's element order
-  bool flat = true, full = true;
-  int n = 0;
-  std::string tbl_err;
-  static thread_local int32_t* hoff = nullptr;
-  static thread_local float* hval = nullptr;
-  static thread_local int64_t hcap = 0;
-  if (hcap < ne) {
-    free(hoff);
-    free(hval);
-    hoff = (int32_t*)malloc(sizeof(int32_t) * ne * ndim);
-    hval = (float*)malloc(sizeof(float) * ne);
-    hcap = ne;
-  }
-  int64_t lo[kMaxNdim], hi[kMaxNdim];
-  for (int j = 0; j < ndim; ++j) {
-    lo[j] = INT64_MAX;
-    hi[j] = INT64_MIN;
-  }
-  for (int64_t e = 0; e < ne; ++e) {
-    if (footprint && !footprint[e]) {
-      full = false;
-      continue;
-    }
-    const float s = structure ? structure[e] : 0.0f;
-    if (s != 0.0f || (structure && std::signbit(s))) flat = false;
-    int64_t rem = e;
-    int64_t r[kMaxNdim];
-    for (int j = ndim - 1; j >= 0; --j) {
-      r[j] = rem % se_shape[j];
-      rem /= se_shape[j];
-    }
-    for (int j = 0; j < ndim; ++j) {
-      int64_t d = r[j] - (se_shape[j] / 2 + origin[j]);
-      if (dilation) d = -d;
-      hoff[n * ndim + j] = (int32_t)d;
-      lo[j] = d < lo[j] ? d : lo[j];
-      hi[j] = d > hi[j] ? d : hi[j];
-    }
-    hval[n] = dilation ? s : -s;
-    ++n;
-  }
-  TM_CHECK_ARG(n > 0, "tm_grey: footprint must contain at least one active position");
+  const int n = E.n;
+  const bool flat = E.flat, full = E.full;
+  const int32_t* hoff = E.off;
+  const float* hval = E.val;
+  const int64_t* lo = E.lo;
+  const int64_t* hi = E.hi;
 
   const bool is_max = dilation != 0;
   if (ndim <= 3) {
     int64_t s3[3] = {1, 1, 1};
-    int64_t lo3[3] = {0, 0, 0}, hi3[3] = {0, 0, 0};
-    for (int j = 0; j < ndim; ++j) {
-      s3[3 - ndim + j] = shape[j];
-      lo3[3 - ndim + j] = lo[j];
-      hi3[3 - ndim + j] = hi[j];
-    }
+    for (int j = 0; j < ndim; ++j) s3[3 - ndim + j] = shape[j];
+    const Grey3 g3 = grey3_of(ndim, E);
+    const int64_t* lo3 = g3.lo;
+    const int64_t* hi3 = g3.hi;
     const int D = (int)s3[0], H = (int)s3[1], W = (int)s3[2];
-    const int kz = (int)(hi3[0] - lo3[0] + 1), ky = (int)(hi3[1] - lo3[1] + 1),
-              kx = (int)(hi3[2] - lo3[2] + 1);
+    const int kz = g3.kz, ky = g3.ky, kx = g3.kx;
     const int64_t ntiles = ((W + kTX - 1) / kTX) * (int64_t)((H + kTY - 1) / kTY);
     // launch variant v: z planes per workgroup = 8 << v (capped at D); the
     // autotuner trades workgroup count against the kz - 1 halo planes every
@@ -480,7 +608,7 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
     };
     auto zchunk_of = [&](int v) { return std::min(D, 8 << v); };
     auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
-    if (flat && full && box3_lds(kz, ky, kx) <= kLdsBudget) {
+    if (grey_box3_fits(E, g3)) {
       const size_t lds = box3_lds(kz, ky, kx);
       auto run = [&](int v) -> int {
         const int zchunk = zchunk_of(v);
@@ -510,19 +638,8 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
       const int v = tune_pick("grey3", key, 8, 4, 1, run, st, &ran);
       return ran ? TM_OK : run(v);
     }
-    bool small = n <= kMaxTable;
-    for (int j = 0; j < 3; ++j) small = small && lo3[j] >= -128 && hi3[j] <= 127;
-    if (small && table3_lds(kz, ky, kx) <= kLdsBudget) {
-      GreyTable tab{};
-      tab.n = n;
-      for (int t = 0; t < n; ++t) {
-        int32_t o3[3] = {0, 0, 0};
-        for (int j = 0; j < ndim; ++j) o3[3 - ndim + j] = hoff[t * ndim + j];
-        tab.dz[t] = (int8_t)o3[0];
-        tab.dy[t] = (int8_t)o3[1];
-        tab.dx[t] = (int8_t)o3[2];
-        tab.v[t] = hval[t];
-      }
+    if (grey_table3_fits(E, g3)) {
+      const GreyTable tab = grey_table3_of(ndim, E);
       const size_t lds = table3_lds(kz, ky, kx);
       auto run = [&](int v) -> int {
         const int zchunk = zchunk_of(v);
@@ -593,6 +710,158 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
   TM_LAUNCH_CHECK();
   // the pageable copies above are staged synchronously, so the host table
   // may be reused once this call returns
+  return TM_OK;
+}
+
+// ------------------------------------------------------- ragged lists --
+namespace tmh {
+
+struct GreyRgPlan {
+  std::vector<GreyRgItem> items;
+  int nd = 0;
+  int64_t blocks = 0;
+  size_t items_bytes = 0, total = 0;
+};
+
+static int grey_rg_plan(const tm_ragged_job* jobs, int64_t njobs, GreyRgPlan& p) {
+  TM_CHECK_ARG(jobs != nullptr, "tm_ragged_grey: jobs is NULL");
+  TM_CHECK_ARG(njobs >= 1 && njobs < ((int64_t)1 << 30), "tm_ragged_grey: njobs must be in 1..2^30");
+  const int nd = jobs[0].ndim;
+  TM_CHECK_ARG(nd >= 1 && nd <= 3, "tm_ragged_grey: items must have 1..3 spatial axes, got %d",
+               nd);
+  p.nd = nd;
+  p.items.resize((size_t)njobs);
+  int64_t blocks = 0;
+  for (int64_t k = 0; k < njobs; ++k) {
+    const tm_ragged_job& j = jobs[k];
+    TM_CHECK_ARG(j.ndim == nd,
+                 "tm_ragged_grey: job %lld has %d axes but job 0 has %d (one structuring element "
+                 "serves every item)",
+                 (long long)k, j.ndim, nd);
+    TM_CHECK_ARG(j.in_offset >= 0 && j.out_offset >= 0,
+                 "tm_ragged_grey: job %lld has a negative offset", (long long)k);
+    int64_t s3[3] = {1, 1, 1};
+    for (int a = 0; a < nd; ++a) {
+      TM_CHECK_ARG(j.shape[a] >= 1 && j.shape[a] <= INT32_MAX,
+                   "tm_ragged_grey: job %lld axis %d has size %lld", (long long)k, a,
+                   (long long)j.shape[a]);
+      s3[3 - nd + a] = j.shape[a];
+    }
+    GreyRgItem& it = p.items[(size_t)k];
+    it.in_off = j.in_offset;
+    it.out_off = j.out_offset;
+    it.D = (int)s3[0];
+    it.H = (int)s3[1];
+    it.W = (int)s3[2];
+    const int64_t ntx = (s3[2] + kTX - 1) / kTX, nty = (s3[1] + kTY - 1) / kTY;
+    const int64_t nz = (s3[0] + kRgZChunk - 1) / kRgZChunk;
+    TM_CHECK_ARG(ntx * nty * nz < ((int64_t)1 << 31) - blocks,
+                 "tm_ragged_grey: more tiles than one launch holds");
+    it.ntx = (int)ntx;
+    it.ntiles = (int)(ntx * nty);
+    it.block0 = (int)blocks;
+    blocks += ntx * nty * nz;
+  }
+  p.blocks = blocks;
+  p.items_bytes = align256(p.items.size() * sizeof(GreyRgItem));
+  p.total = p.items_bytes + align256((size_t)blocks * sizeof(int32_t));
+  return TM_OK;
+}
+
+static thread_local PinnedStage g_grey_stage;
+
+}  // namespace tmh
+
+extern "C" size_t tm_ragged_grey_workspace_bytes(const tm_ragged_job* jobs, int64_t njobs) {
+  tmh::GreyRgPlan p;
+  return tmh::grey_rg_plan(jobs, njobs, p) == TM_OK ? p.total : 0;
+}
+
+extern "C" int tm_ragged_grey(const tm_ragged_job* jobs, int64_t njobs, const float* input,
+                              const float* structure, const uint8_t* footprint,
+                              const int64_t* se_shape, const int64_t* origin, int mode,
+                              float cval, int dilation, float* output, int upload,
+                              void* workspace, size_t workspace_bytes, void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(input && output, "tm_ragged_grey: input/output is NULL");
+  TM_CHECK_ARG(input != output, "tm_ragged_grey: output must not alias input");
+  TM_CHECK_ARG(mode >= 0 && mode <= 4, "tm_ragged_grey: mode must be in 0..4");
+  GreyRgPlan p;
+  int rc = grey_rg_plan(jobs, njobs, p);
+  if (rc != TM_OK) return rc;
+  if (!workspace || workspace_bytes < p.total) {
+    set_error("tm_ragged_grey: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, p.total);
+    return TM_ERR_WORKSPACE;
+  }
+  TM_CHECK_ARG(((uintptr_t)workspace % 256) == 0,
+               "tm_ragged_grey: workspace must be 256-byte aligned");
+  const int nd = p.nd;
+  GreyElems E;
+  rc = grey_elements("tm_ragged_grey", nd, structure, footprint, se_shape, origin, dilation, &E);
+  if (rc != TM_OK) return rc;
+  const Grey3 g3 = grey3_of(nd, E);
+  const bool box = grey_box3_fits(E, g3);
+  TM_CHECK_ARG(box || grey_table3_fits(E, g3),
+               "tm_ragged_grey: the structuring element fits neither streaming kernel (a flat "
+               "box within the LDS budget, or at most %d active elements with offsets in "
+               "-128..127)",
+               kMaxTable);
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  const GreyRgItem* items = (const GreyRgItem*)ws;
+  const int32_t* blk_item = (const int32_t*)(ws + p.items_bytes);
+  if (upload) {
+    const size_t need = p.items_bytes + (size_t)p.blocks * sizeof(int32_t);
+    PinnedStage& s = g_grey_stage;
+    rc = stage_acquire(s, need);
+    if (rc != TM_OK) return rc;
+    char* h = (char*)s.host;
+    memcpy(h, p.items.data(), p.items.size() * sizeof(GreyRgItem));
+    int32_t* hb = (int32_t*)(h + p.items_bytes);
+    for (size_t k = 0; k < p.items.size(); ++k) {
+      const int64_t b1 = k + 1 < p.items.size() ? p.items[k + 1].block0 : p.blocks;
+      for (int64_t b = p.items[k].block0; b < b1; ++b) hb[b] = (int32_t)k;
+    }
+    rc = stage_upload(s, ws, need, st);
+    if (rc != TM_OK) return rc;
+  }
+  const unsigned G = (unsigned)p.blocks;
+  const bool is_max = dilation != 0;
+  const int kz = g3.kz, ky = g3.ky, kx = g3.kx;
+  if (box) {
+    const size_t lds = box3_lds(kz, ky, kx);
+    const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
+#define TM_BOX(MX, NR)                                                                            \
+  hipLaunchKernelGGL((grey_box3_ragged_kernel<MX, NR>), dim3(G), dim3(256), lds, st, input, output, \
+                     items, blk_item, kz, ky, kx, (int)g3.lo[0], (int)g3.lo[1], (int)g3.lo[2],      \
+                     mode, cval)
+    if (is_max) {
+      if (small_se) TM_BOX(true, kPlaneRowsSmall);
+      else TM_BOX(true, kPlaneRowsLarge);
+    } else {
+      if (small_se) TM_BOX(false, kPlaneRowsSmall);
+      else TM_BOX(false, kPlaneRowsLarge);
+    }
+#undef TM_BOX
+  } else {
+    const GreyTable tab = grey_table3_of(nd, E);
+    const size_t lds = table3_lds(kz, ky, kx);
+    const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
+#define TM_TAB(MX, NR)                                                                             \
+  hipLaunchKernelGGL((grey_table3_ragged_kernel<MX, NR>), dim3(G), dim3(256), lds, st, input,       \
+                     output, items, blk_item, tab, (int)g3.lo[0], (int)g3.hi[0], (int)g3.lo[1],     \
+                     (int)g3.hi[1], (int)g3.lo[2], (int)g3.hi[2], mode, cval)
+    if (is_max) {
+      if (small_se) TM_TAB(true, kPlaneRowsSmall);
+      else TM_TAB(true, kPlaneRowsLarge);
+    } else {
+      if (small_se) TM_TAB(false, kPlaneRowsSmall);
+      else TM_TAB(false, kPlaneRowsLarge);
+    }
+#undef TM_TAB
+  }
+  TM_LAUNCH_CHECK();
   return TM_OK;
 }
 

@@ -231,8 +231,6 @@ struct RgPlan {
   size_t ctr_bytes, jobs_off, tasks_off, work_off, total;
 };
 
-static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
-
 static size_t job_work_bytes(const RgJob& j) {
   const int64_t n = j.sh.numel;
   const int nd = j.sh.ndim;
@@ -362,15 +360,8 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
   return TM_OK;
 }
 
-// Pinned staging for the descriptor upload, one per host thread; the event
-// guards reuse until the previous copy has been consumed.
-struct RgStage {
-  void* host = nullptr;
-  size_t cap = 0;
-  hipEvent_t ev = nullptr;
-  bool pending = false;
-};
-static thread_local RgStage g_stage;
+// Pinned staging for the descriptor upload, one per host thread.
+static thread_local PinnedStage g_stage;
 
 struct RgDevInfo {
   int cus = 0;
@@ -470,25 +461,14 @@ extern "C" int tm_ragged_dt(const tm_ragged_job* jobs, int64_t njobs, const void
     const size_t jb = p.jobs.size() * sizeof(RgJob);
     const size_t tb = p.tasks.size() * sizeof(RgTask);
     const size_t need = p.work_off - p.jobs_off;
-    RgStage& s = g_stage;
-    if (s.pending) {
-      TM_CHECK_HIP(hipEventSynchronize(s.ev));
-      s.pending = false;
-    }
-    if (s.cap < need) {
-      if (s.host) TM_CHECK_HIP(hipHostFree(s.host));
-      s.host = nullptr;
-      s.cap = 0;
-      TM_CHECK_HIP(hipHostMalloc(&s.host, need, hipHostMallocDefault));
-      s.cap = need;
-    }
-    if (!s.ev) TM_CHECK_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    PinnedStage& s = g_stage;
+    int rc2 = stage_acquire(s, need);
+    if (rc2 != TM_OK) return rc2;
     char* h = (char*)s.host;
     memcpy(h, p.jobs.data(), jb);
     memcpy(h + (p.tasks_off - p.jobs_off), p.tasks.data(), tb);
-    TM_CHECK_HIP(hipMemcpyAsync(ws + p.jobs_off, h, need, hipMemcpyHostToDevice, st));
-    TM_CHECK_HIP(hipEventRecord(s.ev, st));
-    s.pending = true;
+    rc2 = stage_upload(s, ws + p.jobs_off, need, st);
+    if (rc2 != TM_OK) return rc2;
   }
 
   RgArgs a;

@@ -31,7 +31,12 @@ def _resolve_element(input: Tensor, size, footprint, structure, mode, origin):
     """Reference grey.py:42-89: structure > footprint > size, flat when only a
     size or footprint is given; returns CPU (structure f32, footprint u8 or
     None, origin tuple, mode int)."""
-    spatial_ndim = validate_bcs_input(input)
+    return _resolve_element_nd(validate_bcs_input(input), size, footprint, structure, mode,
+                               origin)
+
+
+def _resolve_element_nd(spatial_ndim: int, size, footprint, structure, mode, origin):
+    """``_resolve_element`` for items of ``spatial_ndim`` axes."""
     footprint_cpu = None
     if structure is not None:
         struct = structure.detach().to(device="cpu", dtype=torch.float32)

@@ -12,6 +12,14 @@ alone (SciPy semantics, indices included).
                                    ops=("edt", "cdt_taxicab", "cdt_chessboard"),
                                    sampling=per_mask_spacings)
     edt, taxi, cheb = plan(masks)          # one list per op, one tensor per mask
+
+Grey morphology has the same form (``tm_ragged_grey``): a list of float items
+of one spatial rank and one structuring element, every tile of every item in
+one launch, each result bit-identical to ``grey_erosion`` / ``grey_dilation``
+on that item (reference grey.py:29-197 takes one dense tensor per call).
+
+    plan = RaggedGreyMorphology([v.shape for v in volumes], "opening", size=3)
+    opened = plan(volumes)                 # one tensor per volume
 """
 from __future__ import annotations
 
@@ -23,11 +31,17 @@ import torch
 from torch import Tensor
 
 from . import _lib
+from .morphology.grey import _resolve_element_nd
 
 __all__ = [
     "RaggedDistanceTransform",
     "euclidean_distance_transform_ragged",
     "chamfer_distance_transform_ragged",
+    "RaggedGreyMorphology",
+    "grey_erosion_ragged",
+    "grey_dilation_ragged",
+    "grey_opening_ragged",
+    "grey_closing_ragged",
 ]
 
 TM_RG_EDT, TM_RG_CDT_CHESSBOARD, TM_RG_CDT_TAXICAB = 0, 1, 2
@@ -264,3 +278,166 @@ def chamfer_distance_transform_ragged(masks: Sequence[Tensor], metric: str = "ch
     plan = RaggedDistanceTransform([x.shape for x in masks], (m,), None, return_distances,
                                    return_indices, device=masks[0].device)
     return plan(masks)[0]
+
+
+# operation -> dilation flag of each pass (opening = dilation of the erosion,
+# closing = erosion of the dilation, reference grey.py:162-197)
+_GREY_PASSES = {"erosion": (0,), "dilation": (1,), "opening": (0, 1), "closing": (1, 0)}
+
+
+class RaggedGreyMorphology:
+    """Grey morphology of a fixed list of item shapes, with its outputs and
+    workspace.
+
+    Args:
+        shapes: spatial shape of every item (1..3 axes, the same number for
+            every item; no batch or channel axes).
+        operation: ``"erosion"``, ``"dilation"``, ``"opening"`` or ``"closing"``.
+        size, footprint, structure, mode, cval, origin: as ``grey_erosion``
+            (structure > footprint > size; reference grey.py:42-89), for items
+            of that many axes.
+        device: CUDA device of the items and outputs.
+    """
+
+    def __init__(self, shapes: Sequence[Sequence[int]], operation: str = "erosion", size=None,
+                 footprint=None, structure=None, mode: str = "reflect", cval: float = 0.0,
+                 origin=0, device: torch.device | str | None = None):
+        if not shapes:
+            raise ValueError("at least one item shape is required")
+        if operation not in _GREY_PASSES:
+            raise ValueError(f"operation must be one of {sorted(_GREY_PASSES)}")
+        self.shapes = [tuple(int(v) for v in sh) for sh in shapes]
+        nd = len(self.shapes[0])
+        if not 1 <= nd <= 3:
+            raise ValueError(f"ragged grey morphology takes items of 1..3 axes, got {nd}")
+        for sh in self.shapes:
+            if len(sh) != nd:
+                raise ValueError(f"every item must have {nd} axes (one structuring element "
+                                 f"serves the list); got shape {sh}")
+            if any(v < 1 for v in sh):
+                raise ValueError(f"item shape {sh} has an empty axis")
+        dev = torch.device(device) if device is not None else torch.device("cuda")
+        if dev.type != "cuda":
+            raise ValueError("ragged grey morphology runs on CUDA devices only")
+        self.device = dev if dev.index is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        self.operation = operation
+        struct, fp, origin_t, mode_int, se_shape = _resolve_element_nd(
+            nd, size, footprint, structure, mode, origin)
+        # host arrays handed to the library on every call
+        self._struct = None if struct is None else struct.numpy()
+        self._fp = None if fp is None else fp.to(torch.uint8).numpy()
+        self._se_shape = _lib.i64_array(se_shape)
+        self._origin = _lib.i64_array(origin_t)
+        self._mode = mode_int
+        self._cval = float(cval)
+
+        self.offsets = [0]
+        for sh in self.shapes:
+            self.offsets.append(self.offsets[-1] + math.prod(sh))
+        jobs = (TmRaggedJob * len(self.shapes))()
+        for k, sh in enumerate(self.shapes):
+            j = jobs[k]
+            j.op = 0
+            j.ndim = nd
+            for a, v in enumerate(sh):
+                j.shape[a] = v
+            j.in_offset = self.offsets[k]
+            j.out_offset = self.offsets[k]
+            j.idx_offset = -1
+        self._jobs = jobs
+        L = _lib.lib()
+        need = int(L.tm_ragged_grey_workspace_bytes(jobs, len(jobs)))
+        if need == 0:
+            raise RuntimeError(f"tm_ragged_grey_workspace_bytes: {_lib.last_error()}")
+        self.workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
+        self.output = torch.empty(self.numel, dtype=torch.float32, device=self.device)
+        self._mid = torch.empty(self.numel, dtype=torch.float32, device=self.device) \
+            if len(_GREY_PASSES[operation]) == 2 else None
+        self._uploaded = False
+
+    @property
+    def numel(self) -> int:
+        return self.offsets[-1]
+
+    def pack(self, items: Sequence[Tensor]) -> Tensor:
+        """Flat float32 buffer of the items in list order."""
+        if len(items) != len(self.shapes):
+            raise ValueError(f"expected {len(self.shapes)} items, got {len(items)}")
+        for k, x in enumerate(items):
+            if tuple(x.shape) != self.shapes[k]:
+                raise ValueError(f"item {k} has shape {tuple(x.shape)}, expected {self.shapes[k]}")
+        return torch.cat([x.detach().reshape(-1).float() for x in items]).to(self.device)
+
+    def _pass(self, src: Tensor, dst: Tensor, dilation: int, st) -> None:
+        def arr(a):
+            return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+        L = _lib.lib()
+        _lib.check(L.tm_ragged_grey(
+            self._jobs, len(self._jobs), _lib.ptr(src), arr(self._struct), arr(self._fp),
+            self._se_shape, self._origin, self._mode, self._cval, dilation, _lib.ptr(dst),
+            0 if self._uploaded else 1, _lib.ptr(self.workspace), self.workspace.numel(), st),
+            "tm_ragged_grey")
+        self._uploaded = True
+
+    def run(self, flat: Tensor) -> Tensor:
+        """Runs the operation on a packed float32 input; returns ``self.output``."""
+        if not flat.is_cuda or flat.device != self.device:
+            raise ValueError(f"input must be on {self.device}")
+        if flat.dtype != torch.float32 or flat.dim() != 1 or flat.numel() != self.numel \
+                or not flat.is_contiguous():
+            raise ValueError(f"packed input must be a contiguous 1-D float32 tensor of "
+                             f"{self.numel} elements")
+        passes = _GREY_PASSES[self.operation]
+        with torch.cuda.device(self.device):
+            st = _lib.stream_of(self.device)
+            if len(passes) == 1:
+                self._pass(flat, self.output, passes[0], st)
+            else:
+                self._pass(flat, self._mid, passes[0], st)
+                self._pass(self._mid, self.output, passes[1], st)
+        return self.output
+
+    def results(self) -> list[Tensor]:
+        """One float32 view of ``self.output`` per item."""
+        return [self.output[self.offsets[k]:self.offsets[k + 1]].view(sh)
+                for k, sh in enumerate(self.shapes)]
+
+    def __call__(self, items: Sequence[Tensor] | Tensor) -> list[Tensor]:
+        flat = items if isinstance(items, Tensor) else self.pack(items)
+        self.run(flat)
+        return self.results()
+
+
+def _grey_ragged(items, operation, size, footprint, structure, mode, cval, origin):
+    if not items:
+        raise ValueError("at least one item is required")
+    plan = RaggedGreyMorphology([x.shape for x in items], operation, size, footprint, structure,
+                                mode, cval, origin, device=items[0].device)
+    return plan(items)
+
+
+def grey_erosion_ragged(items: Sequence[Tensor], size=None, footprint=None, structure=None,
+                        mode: str = "reflect", cval: float = 0.0, origin=0) -> list[Tensor]:
+    """Grey erosion of every item of a ragged list (spatial shapes only) in one
+    launch; one float32 result per item, as ``grey_erosion`` returns it."""
+    return _grey_ragged(items, "erosion", size, footprint, structure, mode, cval, origin)
+
+
+def grey_dilation_ragged(items: Sequence[Tensor], size=None, footprint=None, structure=None,
+                         mode: str = "reflect", cval: float = 0.0, origin=0) -> list[Tensor]:
+    """Grey dilation of every item of a ragged list in one launch."""
+    return _grey_ragged(items, "dilation", size, footprint, structure, mode, cval, origin)
+
+
+def grey_opening_ragged(items: Sequence[Tensor], size=None, footprint=None, structure=None,
+                        mode: str = "reflect", cval: float = 0.0, origin=0) -> list[Tensor]:
+    """Grey opening (dilation of the erosion) of every item, two launches."""
+    return _grey_ragged(items, "opening", size, footprint, structure, mode, cval, origin)
+
+
+def grey_closing_ragged(items: Sequence[Tensor], size=None, footprint=None, structure=None,
+                        mode: str = "reflect", cval: float = 0.0, origin=0) -> list[Tensor]:
+    """Grey closing (erosion of the dilation) of every item, two launches."""
+    return _grey_ragged(items, "closing", size, footprint, structure, mode, cval, origin)
