@@ -122,3 +122,39 @@ def test_grey3_every_zchunk_variant_matches_oracle(tm, oracle_mod, se, mode):
             assert np.array_equal(y.cpu().numpy(), want), v
     finally:
         autotune.clear()
+
+
+def _edtnd_key(shape, idx):
+    """Cache key of the N-D EDT on (B, C, *spatial)."""
+    sp = shape[2:]
+    return " ".join(["edtnd", str(len(sp)), str(_p2(shape[0] * shape[1])), str(int(idx))]
+                    + [str(_p2(s)) for s in sp])
+
+
+@pytest.mark.parametrize("shape", [(2, 1, 20, 33, 40), (1, 1, 7, 130, 300), (3, 1, 50, 64, 2),
+                                   (2, 1, 100, 150), (1, 1, 3, 4, 5, 6), (1, 1, 9, 1, 70)])
+@pytest.mark.parametrize("idx", ["i64", "none"])
+def test_edtnd_every_variant_matches_oracle(tm, oracle_mod, shape, idx):
+    from torchmorph_amd import autotune
+
+    rng = np.random.default_rng(sum(shape) + 7)
+    mask = rng.random(shape) < 0.93
+    samp = [1.3, 0.7, 1.9, 0.55][: len(shape) - 2]
+    od, oi = oracle_mod.edt(mask, samp)
+    want_d = torch.from_numpy(od.astype(np.float32))
+    x = torch.as_tensor(mask, device="cuda")
+    key = _edtnd_key(shape, idx != "none")
+    try:
+        for v in range(8):
+            autotune.clear()
+            autotune.load({key: v})
+            if idx == "none":
+                d = tm.euclidean_distance_transform(x, sampling=samp)
+                assert torch.equal(d.cpu(), want_d), v
+            else:
+                ind = torch.empty((len(shape) - 2, *shape), dtype=torch.int64, device="cuda")
+                d = tm.euclidean_distance_transform(x, sampling=samp, indices=ind)
+                assert torch.equal(d.cpu(), want_d), v
+                assert np.array_equal(ind.cpu().numpy(), oi), v
+    finally:
+        autotune.clear()

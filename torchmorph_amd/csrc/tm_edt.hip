@@ -20,11 +20,13 @@
 //  * edt_pass0_kernel   -- N-D, axis 0 from the mask: two sweeps per line.
 //  * edt_stack_kernel   -- N-D, axis d >= 1: one lane per line runs the
 //    double-precision envelope (stack of sites with their partial squared
-//    distances) exactly in SciPy's operation order; the stack is in LDS when
+//    distances) exactly in SciPy's operation order, with loads issued ahead
+//    and the stack tops in registers (tm_dt_fast.h); the stack is in LDS when
 //    the line fits the LDS depth, otherwise in an interleaved global scratch.
 //
 // Build with -ffp-contract=off: every double multiply and add must round
 // separately, as SciPy's C code does.
+#include "tm_dt_fast.h"
 #include "tm_dt_lines.h"
 #include "tm_tune.h"
 
@@ -301,24 +303,23 @@ __global__ __launch_bounds__(256) void edt_pass0_kernel(const TIn* __restrict__ 
   edt_line_axis0<TIn, FINAL>(in, f0, sh, t, sp, o);
 }
 
-// Axis d >= 1, one lane per line.  LDS_DEPTH > 0: stack in LDS (requires
-// L <= LDS_DEPTH); LDS_DEPTH == 0: stack in the interleaved global scratch
-// gpos/gh (entry k of line t at k * nlines + t, coalesced across lanes).
-template <bool FINAL, int LDS_DEPTH>
-__global__ __launch_bounds__(64) void edt_stack_kernel(const int32_t* __restrict__ fin,
-                                                       int32_t* __restrict__ fout, int64_t plane,
-                                                       Shape sh, int d, int64_t nlines, Samp sp,
-                                                       int32_t* __restrict__ gpos,
-                                                       double* __restrict__ gh, EdtOut o) {
-  __shared__ uint16_t lpos[LDS_DEPTH > 0 ? LDS_DEPTH * 64 : 1];
-  __shared__ double lh[LDS_DEPTH > 0 ? LDS_DEPTH * 64 : 1];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Axis d >= 1, one lane per line (tm_dt_fast.h, loads PF voxels ahead).
+// LDS_DEPTH > 0: stack in LDS (requires L <= LDS_DEPTH); LDS_DEPTH == 0:
+// stack in the interleaved global scratch gpos/gh (entry k of line t at
+// k * nlines + t, coalesced across lanes).
+template <bool FINAL, int PF, int LDS_DEPTH, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void edt_stack_kernel(
+    const int32_t* __restrict__ fin, int32_t* __restrict__ fout, int64_t plane, Shape sh, int d,
+    int64_t nlines, Samp sp, int32_t* __restrict__ gpos, double* __restrict__ gh, EdtOut o) {
+  __shared__ uint16_t lpos[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+  __shared__ double lh[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+  const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (t >= nlines) return;
   if (LDS_DEPTH > 0)
-    edt_line_stack<FINAL>(fin, fout, plane, sh, d, t, sp, lpos + threadIdx.x, lh + threadIdx.x,
-                          64, o);
+    edt_line_stack_fast<FINAL, PF>(fin, fout, plane, sh, d, t, sp, lpos + threadIdx.x,
+                                   lh + threadIdx.x, BLOCK, o);
   else
-    edt_line_stack<FINAL>(fin, fout, plane, sh, d, t, sp, gpos + t, gh + t, nlines, o);
+    edt_line_stack_fast<FINAL, PF>(fin, fout, plane, sh, d, t, sp, gpos + t, gh + t, nlines, o);
 }
 
 static bool unit_sampling(int ndim, const double* s) {
@@ -420,41 +421,83 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   int32_t* gpos = bufB + (size_t)nbuf * N;
   double* gh = (double*)(((uintptr_t)(gpos + N) + 255) & ~(uintptr_t)255);
 
-  // axis 0
-  {
-    const int64_t nlines = batch * sh.stride[0];
-    if (ndim == 1)
-      hipLaunchKernelGGL((edt_pass0_kernel<TIn, true>), dim3(grid_for(nlines, 256)), dim3(256), 0,
-                         st, in, bufA, sh, nlines, sp, o);
-    else
-      hipLaunchKernelGGL((edt_pass0_kernel<TIn, false>), dim3(grid_for(nlines, 256)), dim3(256),
-                         0, st, in, bufA, sh, nlines, sp, o);
-    TM_LAUNCH_CHECK();
-  }
-  int32_t* fin = bufA;
-  int32_t* fout = bufB;
-  for (int d = 1; d < ndim; ++d) {
-    const int64_t nlines = batch * (sh.numel / sh.size[d]);
-    const bool fin_pass = (d == ndim - 1);
-    const dim3 g(grid_for(nlines, 64)), blk(64);
-    const bool lds = sh.size[d] <= 128;
-#define TM_EDT_STACK(FINAL, DEPTH)                                                              \
-  hipLaunchKernelGGL((edt_stack_kernel<FINAL, DEPTH>), g, blk, 0, st, fin, fout, N, sh, d, nlines, \
-                     sp, gpos, gh, o)
-    if (fin_pass) {
-      if (lds) TM_EDT_STACK(true, 128);
-      else TM_EDT_STACK(true, 0);
-    } else {
-      if (lds) TM_EDT_STACK(false, 128);
-      else TM_EDT_STACK(false, 0);
+  // launch variant v (one choice for the whole call):
+  //   v & 3   where each envelope stack lives and the workgroup size:
+  //           0: LDS (128 deep, 80 KB per 64 lanes) when the line fits, else
+  //              global with 64 lanes; 1: global, 64 lanes; 2: global, 256
+  //              lanes; 3: LDS 64 deep when the line fits, else global, 256
+  //   v >> 2  loads issued 8 (0) or 16 (1) voxels ahead
+  auto run = [&](int v) -> int {
+    const int w = v & 3;
+    const bool pf16 = (v >> 2) != 0;
+    {
+      const int64_t nlines = batch * sh.stride[0];
+      const dim3 g(grid_for(nlines, 256)), blk(256);
+      if (ndim == 1)
+        hipLaunchKernelGGL((edt_pass0_kernel<TIn, true>), g, blk, 0, st, in, bufA, sh, nlines, sp,
+                           o);
+      else
+        hipLaunchKernelGGL((edt_pass0_kernel<TIn, false>), g, blk, 0, st, in, bufA, sh, nlines,
+                           sp, o);
+      TM_LAUNCH_CHECK();
     }
+    int32_t* fin = bufA;
+    int32_t* fout = bufB;
+    for (int d = 1; d < ndim; ++d) {
+      const int64_t nlines = batch * (sh.numel / sh.size[d]);
+      const int64_t L = sh.size[d];
+      const bool fin_pass = (d == ndim - 1);
+      int depth = 0, block = 256;
+      if (w == 0) {
+        depth = L <= 128 ? 128 : 0;
+        block = 64;
+      } else if (w == 1) {
+        block = 64;
+      } else if (w == 3 && L <= 64) {
+        depth = 64;
+        block = 64;
+      }
+      const dim3 g(grid_for(nlines, block)), blk(block);
+#define TM_EDT_STACK(FINAL, PF, DEPTH, B)                                                        \
+  hipLaunchKernelGGL((edt_stack_kernel<FINAL, PF, DEPTH, B>), g, blk, 0, st, fin, fout, N, sh, d, \
+                     nlines, sp, gpos, gh, o)
+#define TM_EDT_STACK_ANY(FINAL, PF)                                   \
+  if (depth == 128) TM_EDT_STACK(FINAL, PF, 128, 64);                \
+  else if (depth == 64) TM_EDT_STACK(FINAL, PF, 64, 64);             \
+  else if (block == 64) TM_EDT_STACK(FINAL, PF, 0, 64);              \
+  else TM_EDT_STACK(FINAL, PF, 0, 256);
+      if (fin_pass) {
+        if (pf16) {
+          TM_EDT_STACK_ANY(true, 16)
+        } else {
+          TM_EDT_STACK_ANY(true, 8)
+        }
+      } else {
+        if (pf16) {
+          TM_EDT_STACK_ANY(false, 16)
+        } else {
+          TM_EDT_STACK_ANY(false, 8)
+        }
+      }
+#undef TM_EDT_STACK_ANY
 #undef TM_EDT_STACK
-    TM_LAUNCH_CHECK();
-    int32_t* tmp = fin;
-    fin = fout;
-    fout = tmp;
-  }
-  return TM_OK;
+      TM_LAUNCH_CHECK();
+      int32_t* tmp = fin;
+      fin = fout;
+      fout = tmp;
+    }
+    return TM_OK;
+  };
+  auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
+  int64_t key[kMaxNdim + 3];
+  key[0] = ndim;
+  key[1] = p2(batch);
+  key[2] = o.idx ? 1 : 0;
+  for (int j = 0; j < ndim; ++j) key[3 + j] = p2(sh.size[j]);
+  bool ran = false;
+  const int v = tune_pick("edtnd", key, 3 + ndim, 8, 0, run, st, &ran);
+  if (ran) return TM_OK;
+  return run(v);
 }
 
 }  // namespace tmh
