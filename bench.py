@@ -376,7 +376,8 @@ def extra_ragged():
     ops = ("edt", "cdt_taxicab", "cdt_chessboard")
     st = torch.cuda.current_stream()
     res = {"items": len(shapes), "voxels": nvox, "ops": list(ops)}
-    for sched in ("persistent", "per_level"):
+    outs = {}
+    for sched in ("per_pass", "persistent", "per_level"):
         plan = RaggedDistanceTransform(shapes, ops, samp, schedule=sched, device="cuda")
         flat = plan.pack(masks)
         plan.run(flat)
@@ -384,8 +385,16 @@ def extra_ragged():
         plan.run(flat)  # checks the dependency-wait flag
         res[sched] = {"ms": round(ms, 3), "Gvox_op_s": round(3 * nvox / ms / 1e6, 2),
                       "GB_s": _gb(3 * 5 * nvox, ms)}
+        outs[sched] = plan.distances.clone()
+        del plan
+    res["schedules_agree"] = bool(all(torch.equal(outs["per_pass"], v) for v in outs.values()))
+    plan = RaggedDistanceTransform(shapes, ops, samp, schedule="per_pass", device="cuda")
+    plan.run(plan.pack(masks))
     out = plan.results()
-    # dense per-item calls for comparison (the reference's only option)
+    # dense per-item calls for comparison (the reference's only option), with
+    # the default launch variants (no tuning runs inside the timed loop)
+    from torchmorph_amd import autotune
+    autotune.enable(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dense = []
@@ -396,6 +405,7 @@ def extra_ragged():
                       tm.chamfer_distance_transform(xk, metric="chessboard")))
     torch.cuda.synchronize()
     res["dense_per_item_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    autotune.enable(True)
     same = all(torch.equal(out[o][k], dense[k][o][0, 0]) for k in range(len(masks))
                for o in range(3))
     ok = True

@@ -97,10 +97,10 @@ int tm_cdt(const void* input, int input_dtype, int64_t batch, int ndim, const in
 /* No reference counterpart: the reference takes dense (B, C, spatial...)
  * tensors only (distance_transform.py:10-138, edt_kernel.cu:904-958,
  * cdt_kernel.cu:269-463), so a list of differently shaped masks costs one
- * call per shape there.  Here every (item, op) pair is a job; all jobs run in
- * one persistent launch that pulls per-pass line blocks from a device
- * work-list, and each job's result is bit-identical to tm_edt / tm_cdt on
- * that item alone. */
+ * call per shape there.  Here every (item, op) pair is a job; all jobs run
+ * together (one persistent launch over a device work-list, or one launch per
+ * pass level and kind of pass), and each job's result is bit-identical to
+ * tm_edt / tm_cdt on that item alone. */
 enum tm_ragged_op { TM_RG_EDT = 0, TM_RG_CDT_CHESSBOARD = 1, TM_RG_CDT_TAXICAB = 2 };
 
 typedef struct tm_ragged_job {
@@ -120,7 +120,10 @@ size_t tm_ragged_dt_workspace_bytes(const tm_ragged_job* jobs, int64_t njobs);
  * by all jobs; distances / indices: flat outputs addressed by the jobs'
  * offsets (NULL when no job uses them); index_dtype TM_I32 / TM_I64.
  *   schedule   0 = one persistent launch with dependency counters,
- *              1 = one launch per pass level (for comparison)
+ *              1 = one launch per pass level,
+ *              2 = one launch per pass level and kind of pass, one line per
+ *                  lane (passes along the contiguous axis staged through LDS,
+ *                  chessboard passes per voxel); no dependency waits
  *   upload     1 = copy the job and task descriptors into the workspace
  *              (pinned staging, asynchronous); 0 = the workspace already holds
  *              them from an earlier call with the same jobs, only launch

@@ -26,7 +26,7 @@ SHAPES = [(37, 53), (1, 300), (300, 1), (64, 64), (9, 10, 11), (5, 1, 7), (130, 
           (1,), (17,), (3, 4, 5, 6), (1, 1), (40, 3, 70)]
 
 
-@pytest.mark.parametrize("schedule", ["persistent", "per_level"])
+@pytest.mark.parametrize("schedule", ["persistent", "per_level", "per_pass"])
 def test_ragged_edt_matches_oracle_and_dense(tm, oracle_mod, schedule):
     from torchmorph_amd.ragged import RaggedDistanceTransform
 
@@ -48,7 +48,7 @@ def test_ragged_edt_matches_oracle_and_dense(tm, oracle_mod, schedule):
         assert torch.equal(d, dd[0, 0]) and torch.equal(i, di[:, 0, 0].to(i.dtype)), k
 
 
-@pytest.mark.parametrize("schedule", ["persistent", "per_level"])
+@pytest.mark.parametrize("schedule", ["persistent", "per_level", "per_pass"])
 @pytest.mark.parametrize("indices", [False, True])
 def test_ragged_cdt_matches_oracle(tm, oracle_mod, schedule, indices):
     from torchmorph_amd.ragged import RaggedDistanceTransform

@@ -28,8 +28,21 @@
 // schedule = 1 runs the same tasks with one launch per level instead (stream
 // order provides the dependencies, no waiting) for comparison.
 //
-// The per-line code is the dense kernels' (tm_dt_lines.h), so every job's
-// result is bit-identical to the dense entry point on that item alone.
+// schedule = 2 has no work-list: one ordinary grid launch per (level, kind
+// of pass), one line per lane, and a per-block table naming each block's job
+// and first line.  Each launch is specialised to its kind of pass (EDT axis
+// 0, EDT envelope with the stack in LDS or in global memory, chamfer axis 0,
+// taxicab, chessboard, raster), and the EDT envelope uses the prefetching
+// form of tm_dt_fast.h.  Passes along an item's contiguous last axis
+// (distances-only EDT of 2-D and 3-D items, taxicab chamfer) run row-staged
+// instead (tm_dt_chunk.h): a workgroup owns 256 consecutive rows and moves
+// along them in chunks staged through LDS.  Chessboard passes along every
+// axis >= 1 are computed per voxel by exact expanding windows
+// (cheb_window), with no stack.  Stream order provides the dependencies.
+//
+// The per-line code is the dense kernels' (tm_dt_lines.h, tm_dt_fast.h), so
+// every job's result is bit-identical to the dense entry point on that item
+// alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +50,8 @@
 #include <vector>
 
 #include "tm_cdt_dev.h"
+#include "tm_dt_chunk.h"
+#include "tm_dt_fast.h"
 #include "tm_dt_lines.h"
 
 namespace tmh {
@@ -47,7 +62,12 @@ enum RgKind : int32_t {
   RG_CDT0 = 2,
   RG_CDT_TAXICAB = 3,
   RG_CDT_CHESSBOARD = 4,
-  RG_CDT_RASTER = 5
+  RG_CDT_RASTER = 5,
+  // schedule 2 only: row-staged passes along the contiguous last axis
+  RG_EDT_ROWS2 = 6,  // EDT, 2-D item, distances only
+  RG_EDT_ROWS3 = 7,  // EDT, 3-D item, distances only
+  RG_CDT_TAXICAB_ROWS = 8,
+  RG_CDT_CHEB_WINDOW = 9  // chessboard axis >= 1, one voxel per lane
 };
 
 constexpr int kRgLevels = kMaxNdim;     // levels 0..ndim-1
@@ -76,6 +96,14 @@ struct RgTask {
   int64_t nlines;  // lines in the task
   int32_t need;    // job counter value required before starting
   int32_t pad;
+};
+
+// Schedule 2: one block of lines of one job (lines line0 .. line0 + nlines - 1
+// of the job's pass at the launch's level).
+struct RgBlk {
+  int32_t job;
+  int32_t nlines;
+  int64_t line0;
 };
 
 struct RgArgs {
@@ -221,14 +249,185 @@ __global__ __launch_bounds__(kRgThreads) void rg_kernel(const TIn* __restrict__ 
   }
 }
 
+// Schedule 2: one launch per (level d, kind); lane = line.
+template <typename TIn, int KIND, int BLOCK, int LDS_DEPTH>
+__global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__ in_all,
+                                                         RgArgs a, const RgBlk* __restrict__ blks,
+                                                         int d) {
+  __shared__ uint16_t lpos[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+  __shared__ double lh[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+  const RgBlk bk = blks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  if ((int)threadIdx.x >= bk.nlines) return;
+  const RgJob& jb = a.jobs[bk.job];
+  const Shape& sh = jb.sh;
+  const TIn* in = in_all + jb.in_off;
+  char* wk = a.work + jb.ws_off;
+  const int64_t numel = sh.numel;
+  const int nd = sh.ndim;
+  const bool fin = d == nd - 1;
+  const int64_t nl_axis = numel / sh.size[d];
+  const int64_t t = bk.line0 + threadIdx.x;
+  if (KIND == RG_EDT0 || KIND == RG_EDT_STACK) {
+    EdtOut o;
+    o.dist = (a.dist && jb.out_off >= 0) ? a.dist + jb.out_off : nullptr;
+    o.idx = (a.idx && jb.idx_off >= 0) ? (char*)a.idx + jb.idx_off * (a.idx64 ? 8 : 4) : nullptr;
+    o.idx64 = a.idx64;
+    o.plane = numel;
+    const int nbuf = nd > 1 ? nd - 1 : 1;
+    int32_t* A = (int32_t*)wk;
+    int32_t* B = A + (int64_t)nbuf * numel;
+    if (KIND == RG_EDT0) {
+      if (fin) edt_line_axis0<TIn, true>(in, A, sh, t, jb.sp, o);
+      else edt_line_axis0<TIn, false>(in, A, sh, t, jb.sp, o);
+    } else {
+      const int32_t* src = (d & 1) ? A : B;
+      int32_t* dst = (d & 1) ? B : A;
+      if (LDS_DEPTH > 0) {
+        uint16_t* pp = lpos + threadIdx.x;
+        double* hp = lh + threadIdx.x;
+        if (fin) edt_line_stack_fast<true, 8>(src, dst, numel, sh, d, t, jb.sp, pp, hp, BLOCK, o);
+        else edt_line_stack_fast<false, 8>(src, dst, numel, sh, d, t, jb.sp, pp, hp, BLOCK, o);
+      } else {
+        int32_t* gpos = B + (int64_t)nbuf * numel;
+        double* gh = (double*)(((uintptr_t)(gpos + numel) + 255) & ~(uintptr_t)255);
+        if (fin)
+          edt_line_stack_fast<true, 8>(src, dst, numel, sh, d, t, jb.sp, gpos + t, gh + t,
+                                       nl_axis, o);
+        else
+          edt_line_stack_fast<false, 8>(src, dst, numel, sh, d, t, jb.sp, gpos + t, gh + t,
+                                        nl_axis, o);
+      }
+    }
+  } else {
+    int32_t* w = (int32_t*)(a.dist + jb.out_off);
+    // chessboard passes alternate between the work area and the output so
+    // that the last one lands in the output (cheb_window is not in place)
+    if (KIND == RG_CDT0 && jb.op == TM_RG_CDT_CHESSBOARD && ((nd - 1) & 1)) w = (int32_t*)wk;
+    int64_t c[kMaxNdim], base, b;
+    decode_line(sh, d, t, c, &base, &b);
+    const int64_t step = sh.stride[d], L = sh.size[d];
+    if (KIND == RG_CDT0) {
+      if (fin) cdt_line_axis0<TIn, true>(in, w, base, step, L);
+      else cdt_line_axis0<TIn, false>(in, w, base, step, L);
+    } else if (KIND == RG_CDT_TAXICAB) {
+      if (fin) cdt_line_taxicab<true>(w, base, step, L);
+      else cdt_line_taxicab<false>(w, base, step, L);
+    } else {
+      uint16_t* S = (uint16_t*)wk;
+      uint16_t* T = S + numel;
+      int32_t* G = (int32_t*)(T + numel);
+      if (fin) cdt_line_chessboard<true>(w, base, step, L, S + t, T + t, G + t, nl_axis);
+      else cdt_line_chessboard<false>(w, base, step, L, S + t, T + t, G + t, nl_axis);
+    }
+  }
+}
+
+// Schedule 2, chessboard pass d >= 1 of a job without indices: a block is
+// kRgWinVox consecutive voxels of the item (kRgWinVox / kRgThreads per lane),
+// each computed by cheb_window from the previous pass's values.
+constexpr int kRgWinVox = 4 * kRgThreads;
+
+__global__ __launch_bounds__(kRgThreads) void rg_cheb_window_kernel(
+    RgArgs a, const RgBlk* __restrict__ blks, int d) {
+  const RgBlk bk = blks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const RgJob& jb = a.jobs[bk.job];
+  const Shape& sh = jb.sh;
+  const int nd = sh.ndim;
+  int32_t* out = (int32_t*)(a.dist + jb.out_off);
+  int32_t* buf = (int32_t*)(a.work + jb.ws_off);
+  int32_t* dst = ((nd - 1 - d) & 1) ? buf : out;
+  const int32_t* src = ((nd - d) & 1) ? buf : out;
+  const bool fin = d == nd - 1;
+  const int64_t step = sh.stride[d];
+  const int L = (int)sh.size[d];
+  for (int j = 0; j < kRgWinVox / kRgThreads; ++j) {
+    const int e = j * kRgThreads + (int)threadIdx.x;
+    if (e >= bk.nlines) break;
+    const int64_t v = bk.line0 + e;
+    const int64_t hi = v / step;
+    const int i = (int)(hi % L);
+    const int64_t base = v - (int64_t)i * step;
+    const int32_t r = cheb_window(src, base, step, L, i);
+    dst[v] = fin ? cdt_final_bits(r) : r;
+  }
+}
+
+// Schedule 2, row-staged passes along the last axis (d = ndim - 1): each
+// workgroup owns kCLanes consecutive rows of one job.
+template <int KIND>
+__global__ __launch_bounds__(kCLanes) void rg_rows_kernel(RgArgs a,
+                                                          const RgBlk* __restrict__ blks) {
+  extern __shared__ int32_t lds_rows[];
+  const RgBlk bk = blks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const RgJob& jb = a.jobs[bk.job];
+  const Shape& sh = jb.sh;
+  char* wk = a.work + jb.ws_off;
+  const int64_t numel = sh.numel;
+  const int nd = sh.ndim;
+  const int64_t nl_axis = numel / sh.size[nd - 1];
+  if (KIND == RG_EDT_ROWS2 || KIND == RG_EDT_ROWS3) {
+    const int nbuf = nd > 1 ? nd - 1 : 1;
+    int32_t* A = (int32_t*)wk;
+    int32_t* B = A + (int64_t)nbuf * numel;
+    const int32_t* src = ((nd - 1) & 1) ? A : B;
+    int32_t* gpos = B + (int64_t)nbuf * numel;
+    double* gh = (double*)(((uintptr_t)(gpos + numel) + 255) & ~(uintptr_t)255);
+    if (KIND == RG_EDT_ROWS2)
+      edt_rows_final<1, 32>(src, numel, sh, bk.line0, bk.nlines, jb.sp, gpos, gh, nl_axis,
+                            a.dist + jb.out_off, lds_rows);
+    else
+      edt_rows_final<2, 16>(src, numel, sh, bk.line0, bk.nlines, jb.sp, gpos, gh, nl_axis,
+                            a.dist + jb.out_off, lds_rows);
+  } else {
+    cdt_rows_taxicab<32>((int32_t*)(a.dist + jb.out_off), sh, bk.line0, bk.nlines, lds_rows);
+  }
+}
+
+// LDS bytes of rg_rows_kernel<KIND>.
+static size_t rows_lds(int kind) {
+  if (kind == RG_EDT_ROWS2) return (size_t)2 * kCLanes * 33 * sizeof(int32_t);
+  if (kind == RG_EDT_ROWS3) return (size_t)3 * kCLanes * 17 * sizeof(int32_t);
+  return (size_t)kCLanes * 33 * sizeof(int32_t);
+}
+
+// Schedule 2, chamfer with indices: one workgroup per job runs raster pass d
+// (every thread takes part: the pass has workgroup barriers).
+template <typename TIn>
+__global__ __launch_bounds__(kRgThreads) void rg_raster_kernel(const TIn* __restrict__ in_all,
+                                                               RgArgs a,
+                                                               const RgBlk* __restrict__ blks,
+                                                               int d) {
+  const RgJob& jb = a.jobs[blks[blockIdx.x].job];
+  const int64_t numel = jb.sh.numel;
+  CdtOut o;
+  o.dist = (a.dist && jb.out_off >= 0) ? a.dist + jb.out_off : nullptr;
+  o.idx = (a.idx && jb.idx_off >= 0) ? (char*)a.idx + jb.idx_off * (a.idx64 ? 8 : 4) : nullptr;
+  o.idx64 = a.idx64;
+  o.plane = numel;
+  int32_t* dt = (int32_t*)(a.work + jb.ws_off);
+  cdt_item_pass(in_all + jb.in_off, dt, dt + numel, jb.sh, jb.sh.ndim == 3 ? 1 : 0,
+                jb.op == TM_RG_CDT_TAXICAB ? 1 : 0, d, o);
+}
+
 // ------------------------------------------------------------------ host --
+
+// One schedule-2 launch: blocks [lo, hi) of the block table, all of one
+// kind of pass at one level; lds = EDT envelope with the stack in LDS.
+struct RgLaunch {
+  int level, kind, lds, lo, hi;
+};
+
+constexpr int kRgLdsDepth = 128;  // EDT stack in LDS for lines up to this long
+constexpr int kRgLdsBlock = 64;   // lines per block then (80 KB of LDS)
 
 struct RgPlan {
   std::vector<RgJob> jobs;
   std::vector<RgTask> tasks;
+  std::vector<RgBlk> blks;
+  std::vector<RgLaunch> launches;
   int level_lo[kRgLevels + 1];
   int nlevels;
-  size_t ctr_bytes, jobs_off, tasks_off, work_off, total;
+  size_t ctr_bytes, jobs_off, tasks_off, blks_off, work_off, total;
 };
 
 static size_t job_work_bytes(const RgJob& j) {
@@ -352,10 +551,66 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
   p.level_lo[p.nlevels] = (int)p.tasks.size();
   TM_CHECK_ARG(p.tasks.size() < (size_t)1 << 31, "tm_ragged_dt: too many tasks");
 
+  // schedule 2: per level, per kind, blocks of one line per lane
+  p.blks.clear();
+  p.launches.clear();
+  for (int lev = 0; lev < p.nlevels; ++lev) {
+    for (int g = 0; g < 10; ++g) {
+      // g: 0 EDT0, 1 EDT envelope in LDS, 2 EDT envelope in global memory,
+      //    3 CDT0, 4 taxicab, 5 chessboard (every axis >= 1, voxel blocks,
+      //    cheb_window), 6 raster; along the contiguous last axis,
+      //    row-staged: 7 EDT 2-D, 8 EDT 3-D, 9 taxicab
+      RgLaunch ln{lev, 0, 0, (int)p.blks.size(), 0};
+      for (int32_t ji : order) {
+        const RgJob& j = p.jobs[(size_t)ji];
+        if (j.raster) {
+          if (g != 6 || lev >= 2) continue;
+          p.blks.push_back(RgBlk{ji, 1, 0});
+          continue;
+        }
+        if (lev >= j.sh.ndim) continue;
+        const int64_t L = j.sh.size[lev];
+        const bool last = lev >= 1 && lev == j.sh.ndim - 1;
+        int grp;
+        if (j.op == TM_RG_EDT) {
+          if (lev == 0) grp = 0;
+          else if (last && j.idx_off < 0 && j.sh.ndim <= 3) grp = j.sh.ndim == 2 ? 7 : 8;
+          else grp = L <= kRgLdsDepth ? 1 : 2;
+        } else if (lev == 0) {
+          grp = 3;
+        } else if (j.op == TM_RG_CDT_TAXICAB) {
+          grp = last ? 9 : 4;
+        } else {
+          grp = 5;
+        }
+        if (grp != g) continue;
+        if (g == 5) {  // voxel blocks
+          for (int64_t v0 = 0; v0 < j.sh.numel; v0 += kRgWinVox)
+            p.blks.push_back(RgBlk{ji, (int32_t)std::min<int64_t>(kRgWinVox, j.sh.numel - v0), v0});
+          continue;
+        }
+        const int64_t nl = j.sh.numel / L;
+        const int64_t per = g == 1 ? kRgLdsBlock : kRgThreads;
+        for (int64_t l0 = 0; l0 < nl; l0 += per)
+          p.blks.push_back(RgBlk{ji, (int32_t)std::min(per, nl - l0), l0});
+      }
+      ln.hi = (int)p.blks.size();
+      static const int kinds[10] = {RG_EDT0,       RG_EDT_STACK,       RG_EDT_STACK,
+                                    RG_CDT0,       RG_CDT_TAXICAB,     RG_CDT_CHEB_WINDOW,
+                                    RG_CDT_RASTER, RG_EDT_ROWS2,       RG_EDT_ROWS3,
+                                    RG_CDT_TAXICAB_ROWS};
+      ln.kind = kinds[g];
+      ln.lds = g == 1 ? 1 : 0;
+      if (ln.hi > ln.lo) p.launches.push_back(ln);
+    }
+  }
+  TM_CHECK_ARG(p.blks.size() < (size_t)1 << 31, "tm_ragged_dt: too many line blocks");
+
   p.ctr_bytes = align256((size_t)(kRgJob0 + njobs) * sizeof(int));
   p.jobs_off = p.ctr_bytes;
   p.tasks_off = p.jobs_off + align256(p.jobs.size() * sizeof(RgJob));
-  p.work_off = p.tasks_off + align256(p.tasks.size() * sizeof(RgTask));
+  p.blks_off = p.tasks_off + align256(p.tasks.size() * sizeof(RgTask));
+  p.work_off = p.blks_off + align256(p.blks.size() * sizeof(RgBlk));
   p.total = p.work_off + work;
   return TM_OK;
 }
@@ -396,6 +651,56 @@ static int rg_launch(const TIn* in, const RgPlan& p, RgArgs a, int schedule, int
   const int nctr = kRgJob0 + (int)p.jobs.size();
   hipLaunchKernelGGL(rg_zero_kernel, dim3(grid_for(nctr, 256)), dim3(256), 0, st, a.ctr, nctr);
   TM_LAUNCH_CHECK();
+  if (schedule == 2) {
+    const RgBlk* blks = (const RgBlk*)(a.work - (p.work_off - p.blks_off));
+    for (const RgLaunch& ln : p.launches) {
+      const RgBlk* b = blks + ln.lo;
+      const dim3 g((unsigned)(ln.hi - ln.lo));
+      const int d = ln.level;
+      switch (ln.kind) {
+        case RG_EDT0:
+          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT0, kRgThreads, 0>), g, dim3(kRgThreads), 0,
+                             st, in, a, b, d);
+          break;
+        case RG_EDT_STACK:
+          if (ln.lds)
+            hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT_STACK, kRgLdsBlock, kRgLdsDepth>), g,
+                               dim3(kRgLdsBlock), 0, st, in, a, b, d);
+          else
+            hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT_STACK, kRgThreads, 0>), g,
+                               dim3(kRgThreads), 0, st, in, a, b, d);
+          break;
+        case RG_CDT0:
+          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_CDT0, kRgThreads, 0>), g, dim3(kRgThreads), 0,
+                             st, in, a, b, d);
+          break;
+        case RG_CDT_TAXICAB:
+          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_CDT_TAXICAB, kRgThreads, 0>), g,
+                             dim3(kRgThreads), 0, st, in, a, b, d);
+          break;
+        case RG_CDT_CHEB_WINDOW:
+          hipLaunchKernelGGL(rg_cheb_window_kernel, g, dim3(kRgThreads), 0, st, a, b, d);
+          break;
+        case RG_EDT_ROWS2:
+          hipLaunchKernelGGL((rg_rows_kernel<RG_EDT_ROWS2>), g, dim3(kCLanes),
+                             rows_lds(RG_EDT_ROWS2), st, a, b);
+          break;
+        case RG_EDT_ROWS3:
+          hipLaunchKernelGGL((rg_rows_kernel<RG_EDT_ROWS3>), g, dim3(kCLanes),
+                             rows_lds(RG_EDT_ROWS3), st, a, b);
+          break;
+        case RG_CDT_TAXICAB_ROWS:
+          hipLaunchKernelGGL((rg_rows_kernel<RG_CDT_TAXICAB_ROWS>), g, dim3(kCLanes),
+                             rows_lds(RG_CDT_TAXICAB_ROWS), st, a, b);
+          break;
+        default:
+          hipLaunchKernelGGL((rg_raster_kernel<TIn>), g, dim3(kRgThreads), 0, st, in, a, b, d);
+          break;
+      }
+      TM_LAUNCH_CHECK();
+    }
+    return TM_OK;
+  }
   const int cap = grid_limit(dslot, (const void*)rg_kernel<TIn>);
   if (schedule == 0) {
     a.qslot = 0;
@@ -443,8 +748,9 @@ extern "C" int tm_ragged_dt(const tm_ragged_job* jobs, int64_t njobs, const void
                "tm_ragged_dt: input dtype must be uint8/bool or float32");
   TM_CHECK_ARG(!indices || index_dtype == TM_I32 || index_dtype == TM_I64,
                "tm_ragged_dt: index dtype must be int32 or int64");
-  TM_CHECK_ARG(schedule == 0 || schedule == 1,
-               "tm_ragged_dt: schedule must be 0 (persistent) or 1 (one launch per level)");
+  TM_CHECK_ARG(schedule >= 0 && schedule <= 2,
+               "tm_ragged_dt: schedule must be 0 (persistent), 1 (one launch per level) or 2 "
+               "(one launch per level and kind of pass)");
   RgPlan p;
   const int rc = build_plan(jobs, njobs, distances != nullptr, indices != nullptr, p);
   if (rc != TM_OK) return rc;
@@ -467,6 +773,7 @@ extern "C" int tm_ragged_dt(const tm_ragged_job* jobs, int64_t njobs, const void
     char* h = (char*)s.host;
     memcpy(h, p.jobs.data(), jb);
     memcpy(h + (p.tasks_off - p.jobs_off), p.tasks.data(), tb);
+    memcpy(h + (p.blks_off - p.jobs_off), p.blks.data(), p.blks.size() * sizeof(RgBlk));
     rc2 = stage_upload(s, ws + p.jobs_off, need, st);
     if (rc2 != TM_OK) return rc2;
   }

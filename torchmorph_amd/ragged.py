@@ -59,7 +59,7 @@ _OPS = {
     "manhattan": TM_RG_CDT_TAXICAB,
 }
 
-_SCHEDULES = {"persistent": 0, "per_level": 1}
+_SCHEDULES = {"persistent": 0, "per_level": 1, "per_pass": 2}
 
 
 class TmRaggedJob(ctypes.Structure):
@@ -109,14 +109,16 @@ class RaggedDistanceTransform:
             or one number/sequence per mask.
         return_distances / return_indices: outputs to produce.
         index_dtype: torch.int32 or torch.int64.
-        schedule: ``"persistent"`` (one launch, device dependency counters) or
+        schedule: ``"per_pass"`` (default: one launch per pass level and kind
+            of pass, one line per lane, row-staged passes along the contiguous
+            axis), ``"persistent"`` (one launch, device dependency counters) or
             ``"per_level"`` (one launch per pass level).
         device: CUDA device of the masks and outputs.
     """
 
     def __init__(self, shapes: Sequence[Sequence[int]], ops: Sequence[str] = ("edt",),
                  sampling=None, return_distances: bool = True, return_indices: bool = False,
-                 index_dtype: torch.dtype = torch.int64, schedule: str = "persistent",
+                 index_dtype: torch.dtype = torch.int64, schedule: str = "per_pass",
                  device: torch.device | str | None = None):
         if not shapes:
             raise ValueError("at least one mask shape is required")
