@@ -145,7 +145,7 @@ def test_edtnd_every_variant_matches_oracle(tm, oracle_mod, shape, idx):
     x = torch.as_tensor(mask, device="cuda")
     key = _edtnd_key(shape, idx != "none")
     try:
-        for v in range(8):
+        for v in range(16):
             autotune.clear()
             autotune.load({key: v})
             if idx == "none":
@@ -156,5 +156,31 @@ def test_edtnd_every_variant_matches_oracle(tm, oracle_mod, shape, idx):
                 d = tm.euclidean_distance_transform(x, sampling=samp, indices=ind)
                 assert torch.equal(d.cpu(), want_d), v
                 assert np.array_equal(ind.cpu().numpy(), oi), v
+    finally:
+        autotune.clear()
+
+
+@pytest.mark.parametrize("shape", [(2, 1, 20, 33, 40), (1, 1, 7, 130, 300), (3, 1, 50, 64, 2),
+                                   (2, 1, 100, 150), (1, 1, 3, 4, 5, 6), (1, 1, 9, 1, 70),
+                                   (4, 1, 300), (1, 1, 600, 7)])
+@pytest.mark.parametrize("metric", ["chessboard", "taxicab"])
+def test_cdt_separable_every_variant_matches_oracle(tm, oracle_mod, shape, metric):
+    from torchmorph_amd import autotune
+
+    rng = np.random.default_rng(sum(shape) + 11)
+    mask = rng.random(shape) < 0.9
+    mask[0, 0] = True  # an item without background: -1 everywhere
+    mask[-1, 0].flat[0] = False
+    want = torch.from_numpy(oracle_mod.cdt(mask, metric)[0].astype(np.float32))
+    sp = shape[2:]
+    key = " ".join(["cdtsep", "0" if metric == "chessboard" else "1", str(len(sp)),
+                    str(_p2(shape[0] * shape[1]))] + [str(_p2(s)) for s in sp])
+    x = torch.as_tensor(mask, device="cuda")
+    try:
+        for v in range(2):
+            autotune.clear()
+            autotune.load({key: v})
+            d = tm.chamfer_distance_transform(x, metric=metric)
+            assert torch.equal(d.cpu(), want), v
     finally:
         autotune.clear()

@@ -24,12 +24,19 @@
 //    are coalesced along the contiguous axis.
 // Pass 2 runs the same code on reversed coordinates, in place, as SciPy does.
 //
-// Without indices the values are computed separably instead (one lane per
-// line, every line of an axis in parallel; tm_dt_lines.h shows why the result
-// is the same): the float output holds the int32 intermediates in place, the
-// chessboard envelope keeps its stack in the workspace.
+// Without indices the values are computed separably instead (every line of
+// an axis in parallel; tm_dt_lines.h shows why the result is the same).  Two
+// launch variants, chosen by the autotuner, both exact:
+//  * 0: one lane per line; the float output holds the int32 intermediates in
+//    place, the chessboard envelope keeps its stack in the workspace;
+//  * 1: chessboard axes >= 1 per voxel by expanding windows (cheb_window,
+//    alternating between the workspace and the output), and the taxicab
+//    pass along the contiguous last axis row-staged through LDS
+//    (tm_dt_chunk.h).
 #include "tm_cdt_dev.h"
+#include "tm_dt_chunk.h"
 #include "tm_dt_lines.h"
+#include "tm_tune.h"
 
 namespace tmh {
 
@@ -78,8 +85,90 @@ __global__ __launch_bounds__(256) void cdt_sepd_kernel(int32_t* __restrict__ w, 
 }
 
 template <typename TIn>
+static int cdt_separable_lanes(const TIn* in, int64_t batch, const Shape& sh, int metric,
+                               float* dist, void* ws, hipStream_t st);
+
+// Chessboard axis d >= 1, one voxel per thread (cheb_window), src -> dst.
+template <bool FINAL>
+__global__ __launch_bounds__(256) void cdt_window_kernel(const int32_t* __restrict__ src,
+                                                         int32_t* __restrict__ dst, Shape sh,
+                                                         int d, int64_t n) {
+  const int64_t v = (int64_t)xcd_swizzle(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (v >= n) return;
+  const int64_t step = sh.stride[d];
+  const int L = (int)sh.size[d];
+  const int i = (int)((v / step) % L);
+  const int32_t r = cheb_window(src, v - (int64_t)i * step, step, L, i);
+  dst[v] = FINAL ? cdt_final_bits(r) : r;
+}
+
+// Taxicab along the contiguous last axis (final pass), 256 rows per block.
+__global__ __launch_bounds__(kCLanes) void cdt_rows_kernel(int32_t* __restrict__ w, Shape sh,
+                                                           int64_t nrows_all) {
+  extern __shared__ int32_t lds_rows[];
+  const int64_t row0 = (int64_t)xcd_swizzle(blockIdx.x, gridDim.x) * kCLanes;
+  const int nrows = (int)(nrows_all - row0 < kCLanes ? nrows_all - row0 : kCLanes);
+  cdt_rows_taxicab<32>(w, sh, row0, nrows, lds_rows);
+}
+
+template <typename TIn>
 static int cdt_separable(const TIn* in, int64_t batch, const Shape& sh, int metric, float* dist,
                          void* ws, hipStream_t st) {
+  const int64_t N = batch * sh.numel;
+  auto run = [&](int v) -> int {
+    if (v == 1) {
+      // chessboard: passes alternate so that the last one lands in the output
+      int32_t* out = (int32_t*)dist;
+      int32_t* buf = (int32_t*)ws;
+      auto dst_of = [&](int d) { return ((sh.ndim - 1 - d) & 1) ? buf : out; };
+      int32_t* w0 = metric == 0 ? dst_of(0) : out;
+      const int64_t nlines = batch * (sh.numel / sh.size[0]);
+      if (sh.ndim == 1)
+        hipLaunchKernelGGL((cdt_sep0_kernel<TIn, true>), dim3(grid_for(nlines, 256)), dim3(256),
+                           0, st, in, w0, sh, nlines);
+      else
+        hipLaunchKernelGGL((cdt_sep0_kernel<TIn, false>), dim3(grid_for(nlines, 256)), dim3(256),
+                           0, st, in, w0, sh, nlines);
+      TM_LAUNCH_CHECK();
+      for (int d = 1; d < sh.ndim; ++d) {
+        const bool fin = d == sh.ndim - 1;
+        if (metric == 0) {
+          const dim3 g(grid_for(N, 256));
+          if (fin)
+            hipLaunchKernelGGL((cdt_window_kernel<true>), g, dim3(256), 0, st, dst_of(d - 1),
+                               dst_of(d), sh, d, N);
+          else
+            hipLaunchKernelGGL((cdt_window_kernel<false>), g, dim3(256), 0, st, dst_of(d - 1),
+                               dst_of(d), sh, d, N);
+        } else if (fin) {
+          const int64_t nrows = N / sh.size[d];
+          hipLaunchKernelGGL(cdt_rows_kernel, dim3(grid_for(nrows, kCLanes)), dim3(kCLanes),
+                             (size_t)kCLanes * 33 * sizeof(int32_t), st, out, sh, nrows);
+        } else {
+          const int64_t nlines = batch * (sh.numel / sh.size[d]);
+          hipLaunchKernelGGL((cdt_sepd_kernel<1, false>), dim3(grid_for(nlines, 256)), dim3(256),
+                             0, st, out, sh, d, nlines, nullptr, nullptr, nullptr);
+        }
+        TM_LAUNCH_CHECK();
+      }
+      return TM_OK;
+    }
+    return cdt_separable_lanes(in, batch, sh, metric, dist, ws, st);
+  };
+  auto p2 = [](int64_t x) { int64_t q = 1; while (q < x) q <<= 1; return q; };
+  int64_t key[kMaxNdim + 3];
+  key[0] = metric;
+  key[1] = sh.ndim;
+  key[2] = p2(batch);
+  for (int j = 0; j < sh.ndim; ++j) key[3 + j] = p2(sh.size[j]);
+  bool ran = false;
+  const int v = tune_pick("cdtsep", key, 3 + sh.ndim, 2, 1, run, st, &ran);
+  return ran ? TM_OK : run(v);
+}
+
+template <typename TIn>
+static int cdt_separable_lanes(const TIn* in, int64_t batch, const Shape& sh, int metric,
+                               float* dist, void* ws, hipStream_t st) {
   const int64_t N = batch * sh.numel;
   int32_t* w = (int32_t*)dist;
   uint16_t* S = (uint16_t*)ws;
@@ -91,8 +180,8 @@ static int cdt_separable(const TIn* in, int64_t batch, const Shape& sh, int metr
       hipLaunchKernelGGL((cdt_sep0_kernel<TIn, true>), dim3(grid_for(nlines, 256)), dim3(256), 0,
                          st, in, w, sh, nlines);
     else
-      hipLaunchKernelGGL((cdt_sep0_kernel<TIn, false>), dim3(grid_for(nlines, 256)), dim3(256), 0,
-                         st, in, w, sh, nlines);
+      hipLaunchKernelGGL((cdt_sep0_kernel<TIn, false>), dim3(grid_for(nlines, 256)), dim3(256),
+                         0, st, in, w, sh, nlines);
     TM_LAUNCH_CHECK();
   }
   for (int d = 1; d < sh.ndim; ++d) {

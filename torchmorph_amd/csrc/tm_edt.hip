@@ -26,6 +26,7 @@
 //
 // Build with -ffp-contract=off: every double multiply and add must round
 // separately, as SciPy's C code does.
+#include "tm_dt_chunk.h"
 #include "tm_dt_fast.h"
 #include "tm_dt_lines.h"
 #include "tm_tune.h"
@@ -322,6 +323,21 @@ __global__ __launch_bounds__(BLOCK) void edt_stack_kernel(
     edt_line_stack_fast<FINAL, PF>(fin, fout, plane, sh, d, t, sp, gpos + t, gh + t, nlines, o);
 }
 
+// Last pass (axis ndim - 1, the contiguous one) of 2-D / 3-D items without
+// indices, row-staged through LDS (tm_dt_chunk.h): 256 rows per block.
+template <int NP, int CH>
+__global__ __launch_bounds__(kCLanes) void edt_rows_kernel(const int32_t* __restrict__ fin,
+                                                           int64_t plane, Shape sh,
+                                                           int64_t nlines, Samp sp,
+                                                           int32_t* __restrict__ gpos,
+                                                           double* __restrict__ gh,
+                                                           float* __restrict__ dist) {
+  extern __shared__ int32_t lds_rows[];
+  const int64_t row0 = (int64_t)xcd_swizzle(blockIdx.x, gridDim.x) * kCLanes;
+  const int nrows = (int)(nlines - row0 < kCLanes ? nlines - row0 : kCLanes);
+  edt_rows_final<NP, CH>(fin, plane, sh, row0, nrows, sp, gpos, gh, nlines, dist, lds_rows);
+}
+
 static bool unit_sampling(int ndim, const double* s) {
   if (!s) return true;
   for (int j = 0; j < ndim; ++j)
@@ -426,10 +442,13 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   //           0: LDS (128 deep, 80 KB per 64 lanes) when the line fits, else
   //              global with 64 lanes; 1: global, 64 lanes; 2: global, 256
   //              lanes; 3: LDS 64 deep when the line fits, else global, 256
-  //   v >> 2  loads issued 8 (0) or 16 (1) voxels ahead
+  //   v >> 2 & 1  loads issued 8 (0) or 16 (1) voxels ahead
+  //   v >> 3  1: the last pass of a 2-D / 3-D item without indices runs
+  //           row-staged (edt_rows_kernel)
   auto run = [&](int v) -> int {
     const int w = v & 3;
-    const bool pf16 = (v >> 2) != 0;
+    const bool pf16 = ((v >> 2) & 1) != 0;
+    const bool rows = (v >> 3) != 0 && !o.idx && (ndim == 2 || ndim == 3);
     {
       const int64_t nlines = batch * sh.stride[0];
       const dim3 g(grid_for(nlines, 256)), blk(256);
@@ -456,6 +475,19 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
       } else if (w == 3 && L <= 64) {
         depth = 64;
         block = 64;
+      }
+      if (fin_pass && rows) {
+        const dim3 g(grid_for(nlines, kCLanes)), blk(kCLanes);
+        if (ndim == 2)
+          hipLaunchKernelGGL((edt_rows_kernel<1, 32>), g, blk,
+                             (size_t)2 * kCLanes * 33 * sizeof(int32_t), st, fin, N, sh, nlines,
+                             sp, gpos, gh, o.dist);
+        else
+          hipLaunchKernelGGL((edt_rows_kernel<2, 16>), g, blk,
+                             (size_t)3 * kCLanes * 17 * sizeof(int32_t), st, fin, N, sh, nlines,
+                             sp, gpos, gh, o.dist);
+        TM_LAUNCH_CHECK();
+        break;
       }
       const dim3 g(grid_for(nlines, block)), blk(block);
 #define TM_EDT_STACK(FINAL, PF, DEPTH, B)                                                        \
@@ -495,7 +527,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   key[2] = o.idx ? 1 : 0;
   for (int j = 0; j < ndim; ++j) key[3 + j] = p2(sh.size[j]);
   bool ran = false;
-  const int v = tune_pick("edtnd", key, 3 + ndim, 8, 0, run, st, &ran);
+  const int v = tune_pick("edtnd", key, 3 + ndim, 16, 8, run, st, &ran);
   if (ran) return TM_OK;
   return run(v);
 }
