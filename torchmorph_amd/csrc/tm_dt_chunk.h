@@ -70,13 +70,13 @@ __device__ __forceinline__ float edt_dist_of(int ndim, const int64_t* c, const i
 
 // EDT, last pass (d = ndim - 1, NP = d feature planes, 1 or 2), distances
 // only.  fin: the item's feature planes (plane stride `plane`); rows
-// row0 .. row0 + nrows - 1; stack of line t at gpos/gh + t, stride nl_axis.
-// lds: (NP + 1) * kCLanes * (CH + 1) ints.
+// row0 .. row0 + nrows - 1; the compact stack of edt_line_stack_c for line t
+// at E/F1 + t, stride nl_axis.  lds: (NP + 1) * kCLanes * (CH + 1) ints.
 template <int NP, int CH>
 __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, const Shape& sh,
-                               int64_t row0, int nrows, const Samp& sp, int32_t* __restrict__ gpos,
-                               double* __restrict__ gh, int64_t nl_axis, float* __restrict__ dist,
-                               int32_t* lds) {
+                               int64_t row0, int nrows, const Samp& sp, uint32_t* __restrict__ Eall,
+                               uint16_t* __restrict__ Fall, int64_t nl_axis,
+                               float* __restrict__ dist, int32_t* lds) {
   constexpr int RS = CH + 1;
   const int d = sh.ndim - 1;
   const int L = (int)sh.size[d];
@@ -92,15 +92,14 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
   int32_t* in0 = lds;
   int32_t* in1 = lds + kCLanes * RS;
   int32_t* ob = lds + NP * kCLanes * RS;
-  int32_t* pos = gpos + t;
-  double* hh = gh + t;
+  uint32_t* E = Eall + t;
+  uint16_t* F1 = Fall + t;
   const int64_t ss = nl_axis;
   const int nch = (L + CH - 1) / CH;
 
-  // build
+  // build: q1 = entry top, q0 = entry top - 1; memory holds 0 .. top - 2
   int top = -1;
-  int32_t p1 = 0, p0 = 0;
-  double h1 = 0.0, h0 = 0.0;
+  EdtSite q1{0, 0, 0, 0.0}, q0{0, 0, 0, 0.0};
   int32_t r0[CH], r1[CH];
   chunk_fetch<CH>(r0, fin, row0, L, nrows, 0);
   if (NP == 2) chunk_fetch<CH>(r1, fin + plane, row0, L, nrows, 0);
@@ -119,49 +118,38 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
         if (i >= L) break;
         const int32_t f0 = in0[lane * RS + k];
         if (f0 < 0) continue;
-        double wR = 0.0;
-        {
-          double tt = (double)(f0 - c0) * s0;
-          wR += tt * tt;
-          if (NP == 2) {
-            double u = (double)(in1[lane * RS + k] - c1) * s1;
-            wR += u * u;
-          }
-        }
+        const int32_t f1 = NP == 2 ? in1[lane * RS + k] : 0;
+        const double wR = edt_partial<NP>(f0, f1, c0, c1, s0, s1);
         while (top >= 1) {
-          const double a = (double)(p1 - p0) * sd;
-          const double bb = (double)(i - p1) * sd;
+          const double a = (double)(q1.p - q0.p) * sd;
+          const double bb = (double)(i - q1.p) * sd;
           const double cc = a + bb;
-          if (cc * h1 - bb * h0 - a * wR - a * bb * cc <= 0.0) break;
+          if (cc * q1.h - bb * q0.h - a * wR - a * bb * cc <= 0.0) break;
           --top;
-          p1 = p0;
-          h1 = h0;
-          if (top >= 1) {
-            p0 = pos[(top - 1) * ss];
-            h0 = hh[(top - 1) * ss];
-          }
+          q1 = q0;
+          if (top >= 1) q0 = edt_site_get<NP>(E, F1, top - 1, ss, c0, c1, s0, s1);
         }
+        if (top >= 1) edt_site_put<NP>(E, F1, top - 1, ss, q0);
         ++top;
-        p0 = p1;
-        h0 = h1;
-        p1 = i;
-        h1 = wR;
-        pos[top * ss] = i;
-        hh[top * ss] = wR;
+        q0 = q1;
+        q1 = EdtSite{i, f0, f1, wR};
       }
     }
   }
+  if (active && top >= 1) edt_site_put<NP>(E, F1, top - 1, ss, q0);
+  if (active && top >= 0) edt_site_put<NP>(E, F1, top, ss, q1);
 
-  // query
+  // query: cur = site l, nx = site l + 1, nn = raw entry l + 2 (loaded ahead)
   int l = 0;
-  int32_t pl = 0, pn = 0;
-  double hl = 0.0, hn = 0.0;
+  EdtSite cur{0, 0, 0, 0.0}, nx{0, 0, 0, 0.0};
+  uint32_t nnE = 0;
+  uint16_t nnF = 0;
   if (active && top >= 0) {
-    pl = pos[0];
-    hl = hh[0];
-    if (top >= 1) {
-      pn = pos[ss];
-      hn = hh[ss];
+    cur = edt_site_get<NP>(E, F1, 0, ss, c0, c1, s0, s1);
+    if (top >= 1) nx = edt_site_get<NP>(E, F1, 1, ss, c0, c1, s0, s1);
+    if (top >= 2) {
+      nnE = E[2 * ss];
+      if (NP == 2) nnF = F1[2 * ss];
     }
   }
   int32_t fnone[kMaxNdim];  // no site on the line: feature (-1, 0, ..., 0)
@@ -177,19 +165,24 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
           c[d] = i;
           out = edt_dist_of(sh.ndim, c, fnone, sp);
         } else {
-          double tt = (double)(pl - i) * sd;
-          double d1 = hl + tt * tt;
+          double tt = (double)(cur.p - i) * sd;
+          double d1 = cur.h + tt * tt;
           while (l < top) {
-            double t2 = (double)(pn - i) * sd;
-            double d2 = hn + t2 * t2;
+            double t2 = (double)(nx.p - i) * sd;
+            double d2 = nx.h + t2 * t2;
             if (d1 <= d2) break;
             d1 = d2;
             ++l;
-            pl = pn;
-            hl = hn;
+            cur = nx;
             if (l < top) {
-              pn = pos[(l + 1) * ss];
-              hn = hh[(l + 1) * ss];
+              nx.p = (int32_t)(nnE & 0xffffu);
+              nx.f0 = (int32_t)(nnE >> 16);
+              nx.f1 = (int32_t)nnF;
+              nx.h = edt_partial<NP>(nx.f0, nx.f1, c0, c1, s0, s1);
+              if (l + 2 <= top) {
+                nnE = E[(int64_t)(l + 2) * ss];
+                if (NP == 2) nnF = F1[(int64_t)(l + 2) * ss];
+              }
             }
           }
           out = (float)sqrt(d1);

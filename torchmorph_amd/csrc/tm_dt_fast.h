@@ -172,4 +172,166 @@ __device__ __forceinline__ void edt_line_stack_fast(const int32_t* __restrict__ 
   }
 }
 
+// Compact-stack form for d = 1, 2 (items of up to 3 axes).  A stack entry is
+// the site and its features on the earlier axes -- E[k * ss] = site |
+// f0 << 16, F1[k * ss] = f1 (d = 2) -- 4 or 6 bytes instead of the site and
+// its double partial sum: the partial sum is recomputed from f0 (and f1)
+// with the same operations as at the push, so it has the same bits.  Only
+// entries buried two deep are written: the top two live in registers, and a
+// site pushed and popped again never touches memory.  The query reads the
+// site's features from its entry (no gather of the feature planes).
+struct EdtSite {
+  int32_t p, f0, f1;
+  double h;
+};
+
+template <int D>
+__device__ __forceinline__ double edt_partial(int32_t f0, int32_t f1, int32_t c0, int32_t c1,
+                                              double s0, double s1) {
+  double wR = 0.0;
+  double tt = (double)(f0 - c0) * s0;
+  wR += tt * tt;
+  if (D == 2) {
+    double u = (double)(f1 - c1) * s1;
+    wR += u * u;
+  }
+  return wR;
+}
+
+template <int D>
+__device__ __forceinline__ void edt_site_put(uint32_t* E, uint16_t* F1, int64_t k, int64_t ss,
+                                             const EdtSite& e) {
+  E[k * ss] = (uint32_t)(uint16_t)e.p | ((uint32_t)(uint16_t)e.f0 << 16);
+  if (D == 2) F1[k * ss] = (uint16_t)e.f1;
+}
+
+template <int D>
+__device__ __forceinline__ EdtSite edt_site_get(const uint32_t* E, const uint16_t* F1, int64_t k,
+                                                int64_t ss, int32_t c0, int32_t c1, double s0,
+                                                double s1) {
+  EdtSite e;
+  const uint32_t w = E[k * ss];
+  e.p = (int32_t)(w & 0xffffu);
+  e.f0 = (int32_t)(w >> 16);
+  e.f1 = D == 2 ? (int32_t)F1[k * ss] : 0;
+  e.h = edt_partial<D>(e.f0, e.f1, c0, c1, s0, s1);
+  return e;
+}
+
+template <bool FINAL, int D, int PF>
+__device__ __forceinline__ void edt_line_stack_c(const int32_t* __restrict__ fin,
+                                                 int32_t* __restrict__ fout, int64_t plane,
+                                                 const Shape& sh, int64_t t, const Samp& sp,
+                                                 uint32_t* E, uint16_t* F1, int64_t ss,
+                                                 const EdtOut& o) {
+  const int d = D;
+  int64_t c[kMaxNdim], base, b;
+  decode_line(sh, d, t, c, &base, &b);
+  const int64_t step = sh.stride[d];
+  const int L = (int)sh.size[d];
+  const double sd = sp.s[d];
+  const double s0 = sp.s[0], s1 = sp.s[1];
+  const int32_t c0 = (int32_t)c[0], c1 = (int32_t)c[1];
+
+  // build: q1 = entry top, q0 = entry top - 1 (registers); memory holds
+  // entries 0 .. top - 2
+  int top = -1;
+  EdtSite q1{0, 0, 0, 0.0}, q0{0, 0, 0, 0.0};
+  LineChunk<PF> ch;
+  line_chunk_load(ch, fin, plane, base, step, 0, L, d);
+  for (int i0 = 0; i0 < L; i0 += PF) {
+    LineChunk<PF> cur = ch;
+    if (i0 + PF < L) line_chunk_load(ch, fin, plane, base, step, i0 + PF, L, d);
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+      const int32_t f0 = cur.f0[k];
+      if (f0 < 0) continue;  // also every voxel past the line's end
+      const int i = i0 + k;
+      const int32_t f1 = D == 2 ? cur.f1[k] : 0;
+      const double wR = edt_partial<D>(f0, f1, c0, c1, s0, s1);
+      while (top >= 1) {
+        const double a = (double)(q1.p - q0.p) * sd;
+        const double bb = (double)(i - q1.p) * sd;
+        const double cc = a + bb;
+        if (cc * q1.h - bb * q0.h - a * wR - a * bb * cc <= 0.0) break;
+        --top;
+        q1 = q0;
+        if (top >= 1) q0 = edt_site_get<D>(E, F1, top - 1, ss, c0, c1, s0, s1);
+      }
+      if (top >= 1) edt_site_put<D>(E, F1, top - 1, ss, q0);
+      ++top;
+      q0 = q1;
+      q1 = EdtSite{i, f0, f1, wR};
+    }
+  }
+  if (top >= 1) edt_site_put<D>(E, F1, top - 1, ss, q0);
+  if (top >= 0) edt_site_put<D>(E, F1, top, ss, q1);
+
+  int32_t f[kMaxNdim];
+  if (top < 0) {
+    // no site on this line: every voxel keeps the feature (-1, 0, ..., 0)
+    for (int j = 0; j < sh.ndim; ++j) f[j] = 0;
+    f[0] = -1;
+    for (int i = 0; i < L; ++i) {
+      const int64_t v = base + (int64_t)i * step;
+      if (FINAL) {
+        c[d] = i;
+        edt_store(o, sh.ndim, v, c, f, sp);
+      } else {
+        for (int j = 0; j <= d; ++j) fout[j * plane + v] = f[j];
+      }
+    }
+    return;
+  }
+
+  // query: cur = site l, nx = site l + 1, nn = raw entry l + 2 (loaded ahead)
+  const bool dist_only = FINAL && !o.idx;
+  int l = 0;
+  EdtSite cur = edt_site_get<D>(E, F1, 0, ss, c0, c1, s0, s1);
+  EdtSite nx = cur;
+  if (top >= 1) nx = edt_site_get<D>(E, F1, 1, ss, c0, c1, s0, s1);
+  uint32_t nnE = 0;
+  uint16_t nnF = 0;
+  if (top >= 2) {
+    nnE = E[2 * ss];
+    if (D == 2) nnF = F1[2 * ss];
+  }
+  for (int i = 0; i < L; ++i) {
+    double tt = (double)(cur.p - i) * sd;
+    double d1 = cur.h + tt * tt;
+    while (l < top) {
+      double t2 = (double)(nx.p - i) * sd;
+      double d2 = nx.h + t2 * t2;
+      if (d1 <= d2) break;
+      d1 = d2;
+      ++l;
+      cur = nx;
+      if (l < top) {
+        nx.p = (int32_t)(nnE & 0xffffu);
+        nx.f0 = (int32_t)(nnE >> 16);
+        nx.f1 = (int32_t)nnF;
+        nx.h = edt_partial<D>(nx.f0, nx.f1, c0, c1, s0, s1);
+        if (l + 2 <= top) {
+          nnE = E[(int64_t)(l + 2) * ss];
+          if (D == 2) nnF = F1[(int64_t)(l + 2) * ss];
+        }
+      }
+    }
+    const int64_t v = base + (int64_t)i * step;
+    if (dist_only) {
+      o.dist[v] = (float)sqrt(d1);
+      continue;
+    }
+    f[0] = cur.f0;
+    if (D == 2) f[1] = cur.f1;
+    f[d] = cur.p;
+    if (FINAL) {
+      c[d] = i;
+      edt_store(o, sh.ndim, v, c, f, sp);
+    } else {
+      for (int j = 0; j <= d; ++j) fout[j * plane + v] = f[j];
+    }
+  }
+}
+
 }  // namespace tmh

@@ -305,22 +305,33 @@ __global__ __launch_bounds__(256) void edt_pass0_kernel(const TIn* __restrict__ 
 }
 
 // Axis d >= 1, one lane per line (tm_dt_fast.h, loads PF voxels ahead).
-// LDS_DEPTH > 0: stack in LDS (requires L <= LDS_DEPTH); LDS_DEPTH == 0:
-// stack in the interleaved global scratch gpos/gh (entry k of line t at
-// k * nlines + t, coalesced across lanes).
-template <bool FINAL, int PF, int LDS_DEPTH, int BLOCK>
+// COMPACT (items of up to 3 axes): 4/6-byte stack entries
+// (edt_line_stack_c); otherwise site + double partial sum.  LDS_DEPTH > 0:
+// stack in LDS (requires L <= LDS_DEPTH); LDS_DEPTH == 0: stack in the
+// interleaved global scratch gpos/gh (entry k of line t at k * nlines + t,
+// coalesced across lanes).
+template <bool FINAL, int PF, int LDS_DEPTH, int BLOCK, bool COMPACT>
 __global__ __launch_bounds__(BLOCK) void edt_stack_kernel(
     const int32_t* __restrict__ fin, int32_t* __restrict__ fout, int64_t plane, Shape sh, int d,
     int64_t nlines, Samp sp, int32_t* __restrict__ gpos, double* __restrict__ gh, EdtOut o) {
-  __shared__ uint16_t lpos[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
-  __shared__ double lh[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+  constexpr int NE = LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1;
+  __shared__ uint32_t lA[COMPACT ? NE : 1];
+  __shared__ uint16_t lB[NE];
+  __shared__ double lh[COMPACT ? 1 : NE];
   const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (t >= nlines) return;
-  if (LDS_DEPTH > 0)
-    edt_line_stack_fast<FINAL, PF>(fin, fout, plane, sh, d, t, sp, lpos + threadIdx.x,
+  if (COMPACT) {
+    uint32_t* E = LDS_DEPTH > 0 ? lA + threadIdx.x : (uint32_t*)gpos + t;
+    uint16_t* F1 = LDS_DEPTH > 0 ? lB + threadIdx.x : (uint16_t*)gh + t;
+    const int64_t ss = LDS_DEPTH > 0 ? BLOCK : nlines;
+    if (d == 1) edt_line_stack_c<FINAL, 1, PF>(fin, fout, plane, sh, t, sp, E, F1, ss, o);
+    else edt_line_stack_c<FINAL, 2, PF>(fin, fout, plane, sh, t, sp, E, F1, ss, o);
+  } else if (LDS_DEPTH > 0) {
+    edt_line_stack_fast<FINAL, PF>(fin, fout, plane, sh, d, t, sp, lB + threadIdx.x,
                                    lh + threadIdx.x, BLOCK, o);
-  else
+  } else {
     edt_line_stack_fast<FINAL, PF>(fin, fout, plane, sh, d, t, sp, gpos + t, gh + t, nlines, o);
+  }
 }
 
 // Last pass (axis ndim - 1, the contiguous one) of 2-D / 3-D items without
@@ -335,7 +346,8 @@ __global__ __launch_bounds__(kCLanes) void edt_rows_kernel(const int32_t* __rest
   extern __shared__ int32_t lds_rows[];
   const int64_t row0 = (int64_t)xcd_swizzle(blockIdx.x, gridDim.x) * kCLanes;
   const int nrows = (int)(nlines - row0 < kCLanes ? nlines - row0 : kCLanes);
-  edt_rows_final<NP, CH>(fin, plane, sh, row0, nrows, sp, gpos, gh, nlines, dist, lds_rows);
+  edt_rows_final<NP, CH>(fin, plane, sh, row0, nrows, sp, (uint32_t*)gpos, (uint16_t*)gh,
+                         nlines, dist, lds_rows);
 }
 
 static bool unit_sampling(int ndim, const double* s) {
@@ -490,14 +502,23 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
         break;
       }
       const dim3 g(grid_for(nlines, block)), blk(block);
-#define TM_EDT_STACK(FINAL, PF, DEPTH, B)                                                        \
-  hipLaunchKernelGGL((edt_stack_kernel<FINAL, PF, DEPTH, B>), g, blk, 0, st, fin, fout, N, sh, d, \
-                     nlines, sp, gpos, gh, o)
-#define TM_EDT_STACK_ANY(FINAL, PF)                                   \
-  if (depth == 128) TM_EDT_STACK(FINAL, PF, 128, 64);                \
-  else if (depth == 64) TM_EDT_STACK(FINAL, PF, 64, 64);             \
-  else if (block == 64) TM_EDT_STACK(FINAL, PF, 0, 64);              \
-  else TM_EDT_STACK(FINAL, PF, 0, 256);
+#define TM_EDT_STACK(FINAL, PF, DEPTH, B)                                                      \
+  if (ndim <= 3)                                                                              \
+    hipLaunchKernelGGL((edt_stack_kernel<FINAL, PF, DEPTH, B, true>), g, blk, 0, st, fin, fout, \
+                       N, sh, d, nlines, sp, gpos, gh, o);                                    \
+  else                                                                                        \
+    hipLaunchKernelGGL((edt_stack_kernel<FINAL, PF, DEPTH, B, false>), g, blk, 0, st, fin,      \
+                       fout, N, sh, d, nlines, sp, gpos, gh, o)
+#define TM_EDT_STACK_ANY(FINAL, PF)                 \
+  if (depth == 128) {                              \
+    TM_EDT_STACK(FINAL, PF, 128, 64);              \
+  } else if (depth == 64) {                        \
+    TM_EDT_STACK(FINAL, PF, 64, 64);               \
+  } else if (block == 64) {                        \
+    TM_EDT_STACK(FINAL, PF, 0, 64);                \
+  } else {                                         \
+    TM_EDT_STACK(FINAL, PF, 0, 256);               \
+  }
       if (fin_pass) {
         if (pf16) {
           TM_EDT_STACK_ANY(true, 16)

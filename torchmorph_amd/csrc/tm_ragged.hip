@@ -254,8 +254,9 @@ template <typename TIn, int KIND, int BLOCK, int LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__ in_all,
                                                          RgArgs a, const RgBlk* __restrict__ blks,
                                                          int d) {
-  __shared__ uint16_t lpos[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
-  __shared__ double lh[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+  // LDS stack (jobs of up to 3 axes): compact entries of edt_line_stack_c
+  __shared__ uint32_t lE[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+  __shared__ uint16_t lF[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
   const RgBlk bk = blks[xcd_swizzle(blockIdx.x, gridDim.x)];
   if ((int)threadIdx.x >= bk.nlines) return;
   const RgJob& jb = a.jobs[bk.job];
@@ -282,14 +283,20 @@ __global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__
     } else {
       const int32_t* src = (d & 1) ? A : B;
       int32_t* dst = (d & 1) ? B : A;
-      if (LDS_DEPTH > 0) {
-        uint16_t* pp = lpos + threadIdx.x;
-        double* hp = lh + threadIdx.x;
-        if (fin) edt_line_stack_fast<true, 8>(src, dst, numel, sh, d, t, jb.sp, pp, hp, BLOCK, o);
-        else edt_line_stack_fast<false, 8>(src, dst, numel, sh, d, t, jb.sp, pp, hp, BLOCK, o);
-      } else {
-        int32_t* gpos = B + (int64_t)nbuf * numel;
-        double* gh = (double*)(((uintptr_t)(gpos + numel) + 255) & ~(uintptr_t)255);
+      int32_t* gpos = B + (int64_t)nbuf * numel;
+      double* gh = (double*)(((uintptr_t)(gpos + numel) + 255) & ~(uintptr_t)255);
+      if (nd <= 3) {
+        uint32_t* E = LDS_DEPTH > 0 ? lE + threadIdx.x : (uint32_t*)gpos + t;
+        uint16_t* F1 = LDS_DEPTH > 0 ? lF + threadIdx.x : (uint16_t*)gh + t;
+        const int64_t ss = LDS_DEPTH > 0 ? BLOCK : nl_axis;
+        if (d == 1) {
+          if (fin) edt_line_stack_c<true, 1, 8>(src, dst, numel, sh, t, jb.sp, E, F1, ss, o);
+          else edt_line_stack_c<false, 1, 8>(src, dst, numel, sh, t, jb.sp, E, F1, ss, o);
+        } else {
+          if (fin) edt_line_stack_c<true, 2, 8>(src, dst, numel, sh, t, jb.sp, E, F1, ss, o);
+          else edt_line_stack_c<false, 2, 8>(src, dst, numel, sh, t, jb.sp, E, F1, ss, o);
+        }
+      } else if (LDS_DEPTH == 0) {
         if (fin)
           edt_line_stack_fast<true, 8>(src, dst, numel, sh, d, t, jb.sp, gpos + t, gh + t,
                                        nl_axis, o);
@@ -373,10 +380,12 @@ __global__ __launch_bounds__(kCLanes) void rg_rows_kernel(RgArgs a,
     int32_t* gpos = B + (int64_t)nbuf * numel;
     double* gh = (double*)(((uintptr_t)(gpos + numel) + 255) & ~(uintptr_t)255);
     if (KIND == RG_EDT_ROWS2)
-      edt_rows_final<1, 32>(src, numel, sh, bk.line0, bk.nlines, jb.sp, gpos, gh, nl_axis,
+      edt_rows_final<1, 32>(src, numel, sh, bk.line0, bk.nlines, jb.sp, (uint32_t*)gpos,
+                            (uint16_t*)gh, nl_axis,
                             a.dist + jb.out_off, lds_rows);
     else
-      edt_rows_final<2, 16>(src, numel, sh, bk.line0, bk.nlines, jb.sp, gpos, gh, nl_axis,
+      edt_rows_final<2, 16>(src, numel, sh, bk.line0, bk.nlines, jb.sp, (uint32_t*)gpos,
+                            (uint16_t*)gh, nl_axis,
                             a.dist + jb.out_off, lds_rows);
   } else {
     cdt_rows_taxicab<32>((int32_t*)(a.dist + jb.out_off), sh, bk.line0, bk.nlines, lds_rows);
@@ -418,7 +427,7 @@ struct RgLaunch {
 };
 
 constexpr int kRgLdsDepth = 128;  // EDT stack in LDS for lines up to this long
-constexpr int kRgLdsBlock = 64;   // lines per block then (80 KB of LDS)
+constexpr int kRgLdsBlock = 64;   // lines per block then (48 KB of LDS)
 
 struct RgPlan {
   std::vector<RgJob> jobs;
@@ -575,7 +584,7 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
         if (j.op == TM_RG_EDT) {
           if (lev == 0) grp = 0;
           else if (last && j.idx_off < 0 && j.sh.ndim <= 3) grp = j.sh.ndim == 2 ? 7 : 8;
-          else grp = L <= kRgLdsDepth ? 1 : 2;
+          else grp = (L <= kRgLdsDepth && j.sh.ndim <= 3) ? 1 : 2;
         } else if (lev == 0) {
           grp = 3;
         } else if (j.op == TM_RG_CDT_TAXICAB) {
