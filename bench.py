@@ -318,6 +318,20 @@ def extra_morph3d():
                                 "GB_s": _gb(2 * n, ms),
                                 "parity": bool(np.array_equal(yb[:1, :1].cpu().numpy(),
                                                               want.astype(bool)))}
+    # composites, fused epilogue against the plain composition of the same kernels
+    composed = {
+        "morphological_gradient": lambda: tm.grey_dilation(x, size=3) - tm.grey_erosion(x, size=3),
+        "white_tophat": lambda: x - tm.grey_opening(x, size=3),
+        "morphological_laplace": lambda: (tm.grey_dilation(x, size=3) + tm.grey_erosion(x, size=3)
+                                          - 2 * x),
+    }
+    for name, plain in composed.items():
+        fn = getattr(tm, name)
+        y = fn(x, size=3)
+        ms = _timed(lambda: fn(x, size=3), 5, st)
+        ms_plain = _timed(plain, 5, st)
+        out[f"{name}_cube3"] = {"ms": round(ms, 3), "composed_ms": round(ms_plain, 3),
+                                "parity_with_composed": bool(torch.equal(y, plain()))}
     out["bytes_per_vox"] = "grey 4 in + 4 out; binary 1 in + 1 out (algorithmic)"
     # CPU oracle beside it: one 128^3 item, single thread
     t0 = time.perf_counter()

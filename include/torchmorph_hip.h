@@ -159,6 +159,34 @@ int tm_grey(const float* input, int64_t batch, int ndim, const int64_t* shape,
             const int64_t* origin, int mode, float cval, int dilation, float* output,
             void* workspace, size_t workspace_bytes, void* stream);
 
+/* 1 when items of `ndim` axes and this structuring element run on the fused
+ * streaming kernels (ndim <= 3; a flat box within the LDS budget, or at most
+ * 256 active elements with offsets in -128..127), which tm_grey_ex requires;
+ * 0 otherwise. */
+int tm_grey_fused_ok(int ndim, const float* structure, const uint8_t* footprint,
+                     const int64_t* se_shape, const int64_t* origin, int dilation);
+
+/* 1 when gradient / laplace of a flat box of this shape and origin run in one
+ * pass (tm_grey_ex epilogues 4, 5): odd sizes, zero origin, ndim <= 3. */
+int tm_grey_dual_ok(int ndim, const int64_t* se_shape, const int64_t* origin);
+
+/* tm_grey with the final arithmetic of a composite fused into the store
+ * (replaces the reference's separate tensor expressions of
+ * morphological_gradient / morphological_laplace / white_tophat /
+ * black_tophat, grey.py:200-289; same float32 operations in the same order):
+ *   epilogue 1: out = acc - a      gradient (a = erosion), black tophat (a = input)
+ *            2: out = a - acc      white tophat (a = input)
+ *            3: out = (acc + a) - 2 b   laplace (a = erosion, b = input)
+ * where acc is the erosion / dilation of `input`.  a, b: float32 arrays of
+ * the output's layout, not aliasing the output.  Needs tm_grey_fused_ok.
+ *   epilogue 4: out = dilation - erosion, 5: out = (dilation + erosion) -
+ *            2 input, both in one pass over the input (flat box, no structure
+ *            or footprint, tm_grey_dual_ok; `dilation`, a and b unused). */
+int tm_grey_ex(const float* input, int64_t batch, int ndim, const int64_t* shape,
+               const float* structure, const uint8_t* footprint, const int64_t* se_shape,
+               const int64_t* origin, int mode, float cval, int dilation, int epilogue,
+               const float* a, const float* b, float* output, void* stream);
+
 size_t tm_grey_backward_workspace_bytes(int64_t batch, int ndim, const int64_t* shape);
 
 /* No reference counterpart (the reference ops are not differentiable).

@@ -107,3 +107,35 @@ def test_ragged_grey_oversized_element_is_reported_on_the_host():
                             ctypes.c_void_p(0x100000), need, None)
     assert rc == -1
     assert "fits neither streaming kernel" in _lib.last_error()
+
+
+def test_grey_fused_query_and_epilogue_errors_need_no_gpu():
+    from torchmorph_amd import _lib
+
+    lib = _lib.lib()
+    i64 = _lib.i64_array
+    assert lib.tm_grey_fused_ok(3, None, None, i64([3, 3, 3]), i64([0, 0, 0]), 0) == 1
+    assert lib.tm_grey_fused_ok(4, None, None, i64([3, 3, 3, 3]), i64([0] * 4), 0) == 0
+    st = (ctypes.c_float * 300)(*([1.0] * 300))  # 300 non-flat elements: no streaming kernel
+    assert lib.tm_grey_fused_ok(2, st, None, i64([20, 15]), i64([0, 0]), 1) == 0
+    a, out = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    args = (ctypes.c_void_p(0x3000), 1, 2, i64([8, 8]), None, None, i64([3, 3]), i64([0, 0]),
+            1, 0.0, 1)
+    assert lib.tm_grey_ex(*args, 6, a, None, out, None) == -1
+    assert "epilogue must be" in _lib.last_error()
+    # one pass (4, 5): flat boxes of odd sizes with zero origin only
+    assert lib.tm_grey_dual_ok(2, i64([3, 5]), i64([0, 0])) == 1
+    assert lib.tm_grey_dual_ok(2, i64([3, 4]), i64([0, 0])) == 0
+    assert lib.tm_grey_dual_ok(2, i64([3, 3]), i64([1, 0])) == 0
+    assert lib.tm_grey_dual_ok(4, i64([3] * 4), i64([0] * 4)) == 0
+    st9 = (ctypes.c_float * 9)(*([0.0] * 9))
+    assert lib.tm_grey_ex(args[0], 1, 2, i64([8, 8]), st9, None, i64([3, 3]), i64([0, 0]), 1,
+                          0.0, 1, 4, None, None, out, None) == -1
+    assert "flat box" in _lib.last_error()
+    assert lib.tm_grey_ex(args[0], 1, 2, i64([8, 8]), None, None, i64([4, 3]), i64([0, 0]), 1,
+                          0.0, 1, 5, None, None, out, None) == -1
+    assert "odd sizes" in _lib.last_error()
+    assert lib.tm_grey_ex(*args, 1, out, None, out, None) == -1
+    assert "must not alias" in _lib.last_error()
+    assert lib.tm_grey_ex(*args, 3, a, None, out, None) == -1
+    assert "operand is NULL" in _lib.last_error()

@@ -179,13 +179,34 @@ __device__ __forceinline__ void grey_block(int ntx, int ntiles, int nz, int* tx,
   *item = rest / nz;
 }
 
+// Epilogue of the composite operators (reference grey.py:200-289), applied
+// to the min/max `acc` of output voxel p; a and b are item-relative arrays of
+// the output's layout (a may be the output itself: each voxel reads its own
+// value before writing it).  Same float32 operations, in the same order, as
+// the reference's tensor expressions:
+//   1: acc - a[p]            gradient (a = erosion), black tophat (a = input)
+//   2: a[p] - acc            white tophat (a = input)
+//   3: (acc + a[p]) - 2 b[p] laplace (a = erosion, b = input)
+struct GreyEpi {
+  const float* a;
+  const float* b;
+};
+
+template <int EPI>
+__device__ __forceinline__ float grey_epi(float acc, const GreyEpi& e, int64_t p) {
+  if (EPI == 1) return acc - e.a[p];
+  if (EPI == 2) return e.a[p] - acc;
+  if (EPI == 3) return (acc + e.a[p]) - 2.0f * e.b[p];
+  return acc;
+}
+
 // One (x, y) tile of one item over output planes [zb, ze).
-template <bool MAX, int NR>
+template <bool MAX, int NR, int EPI = 0>
 __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
                                                float* __restrict__ ovol, int D, int H, int W,
                                                int kz, int ky, int kx, int sz, int sy, int sx,
                                                int mode, float cval, int tx, int ty, int zb,
-                                               int ze, float* sm) {
+                                               int ze, float* sm, GreyEpi epi = GreyEpi{}) {
   const int SX = kTX + kx - 1, SY = kTY + ky - 1;
   float* stage = sm;
   float* xr = stage + SY * SX;
@@ -227,19 +248,20 @@ __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
         if (y < H && x < W) {
           float acc = ID;
           for (int i = 0; i < kz; ++i) acc = gop<MAX>(acc, ring[i * (kTY * kTX) + e]);
-          ovol[((int64_t)z * H + y) * W + x] = acc;
+          const int64_t p = ((int64_t)z * H + y) * W + x;
+          ovol[p] = grey_epi<EPI>(acc, epi, p);
         }
       }
     }
   }
 }
 
-template <bool MAX, int NR>
+template <bool MAX, int NR, int EPI = 0>
 __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict__ in,
                                                         float* __restrict__ out, int D, int H,
                                                         int W, int kz, int ky, int kx, int sz,
                                                         int sy, int sx, int mode, float cval,
-                                                        int zchunk) {
+                                                        int zchunk, GreyEpi epi) {
   extern __shared__ float sm[];
   const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
   const int nz = (D + zchunk - 1) / zchunk;
@@ -248,8 +270,99 @@ __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict_
   grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
   const int zb = zci * zchunk;
   const int64_t n = (int64_t)D * H * W;
-  grey_box3_tile<MAX, NR>(in + item * n, out + item * n, D, H, W, kz, ky, kx, sz, sy, sx, mode,
-                          cval, tx, ty, zb, min(D, zb + zchunk), sm);
+  const GreyEpi ei{epi.a ? epi.a + item * n : nullptr, epi.b ? epi.b + item * n : nullptr};
+  grey_box3_tile<MAX, NR, EPI>(in + item * n, out + item * n, D, H, W, kz, ky, kx, sz, sy, sx,
+                               mode, cval, tx, ty, zb, min(D, zb + zchunk), sm, ei);
+}
+
+// Gradient (dilation - erosion, COMB 0) or laplace ((dilation + erosion) -
+// 2 input, COMB 1) of a flat box whose erosion and dilation windows coincide
+// (odd sizes, zero origin), in one pass: each staged plane is reduced to its
+// min and its max together.  The stage holds the raw mode-extended values;
+// the max path adds the flat dilation's +0.0f (as grey_box3_tile's bias
+// does), the min path adds nothing (the erosion's -0.0f is an identity).
+template <int NR, int COMB>
+__device__ __forceinline__ void grey_box3_dual_tile(const float* __restrict__ vol,
+                                                    float* __restrict__ ovol, int D, int H,
+                                                    int W, int kz, int ky, int kx, int sz, int sy,
+                                                    int sx, int mode, float cval, int tx, int ty,
+                                                    int zb, int ze, float* sm) {
+  const int SX = kTX + kx - 1, SY = kTY + ky - 1;
+  float* stage = sm;
+  float* xmn = stage + SY * SX;
+  float* xmx = xmn + SY * kTX;
+  float* rmn = xmx + SY * kTX;
+  float* rmx = rmn + kz * kTY * kTX;
+  const int x0 = tx * kTX, y0 = ty * kTY;
+  const PlaneGeom g{D, H, W, SY, SX, mode, cval, -0.0f};
+  const int nplanes = (ze - zb) + kz - 1;
+  const PlaneMap<NR> pm = plane_map<NR>(g, y0 + sy, x0 + sx);
+  PlaneRegs<NR> R;
+  plane_fetch(R, vol, g, pm, zb + sz);
+  for (int n = 0; n < nplanes; ++n) {
+    const int zi = zb + sz + n;
+    plane_store(R, stage, vol, g, zi, y0 + sy, x0 + sx);
+    __syncthreads();
+    if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
+    for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
+      const int r = e >> 6, c = e & 63;
+      const float* s = stage + r * SX + c;
+      float lo = INFINITY, hi = -INFINITY;
+      for (int l = 0; l < kx; ++l) {
+        lo = fminf(lo, s[l]);
+        hi = fmaxf(hi, s[l] + 0.0f);
+      }
+      xmn[e] = lo;
+      xmx[e] = hi;
+    }
+    __syncthreads();
+    const int slot = (n % kz) * (kTY * kTX);
+    for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
+      const int r = e >> 6, c = e & 63;
+      float lo = INFINITY, hi = -INFINITY;
+      for (int j = 0; j < ky; ++j) {
+        lo = fminf(lo, xmn[(r + j) * kTX + c]);
+        hi = fmaxf(hi, xmx[(r + j) * kTX + c]);
+      }
+      rmn[slot + e] = lo;
+      rmx[slot + e] = hi;
+    }
+    __syncthreads();
+    const int z = zb + n - (kz - 1);
+    if (z >= zb) {
+      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        const int y = y0 + r, x = x0 + c;
+        if (y < H && x < W) {
+          float lo = INFINITY, hi = -INFINITY;
+          for (int i = 0; i < kz; ++i) {
+            lo = fminf(lo, rmn[i * (kTY * kTX) + e]);
+            hi = fmaxf(hi, rmx[i * (kTY * kTX) + e]);
+          }
+          const int64_t p = ((int64_t)z * H + y) * W + x;
+          ovol[p] = COMB == 0 ? hi - lo : (hi + lo) - 2.0f * vol[p];
+        }
+      }
+    }
+  }
+}
+
+template <int NR, int COMB>
+__global__ __launch_bounds__(256) void grey_box3_dual_kernel(const float* __restrict__ in,
+                                                             float* __restrict__ out, int D,
+                                                             int H, int W, int kz, int ky, int kx,
+                                                             int sz, int sy, int sx, int mode,
+                                                             float cval, int zchunk) {
+  extern __shared__ float sm[];
+  const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
+  const int nz = (D + zchunk - 1) / zchunk;
+  int tx, ty, zci;
+  int64_t item;
+  grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
+  const int zb = zci * zchunk;
+  const int64_t n = (int64_t)D * H * W;
+  grey_box3_dual_tile<NR, COMB>(in + item * n, out + item * n, D, H, W, kz, ky, kx, sz, sy, sx,
+                                mode, cval, tx, ty, zb, min(D, zb + zchunk), sm);
 }
 
 constexpr int kMaxTable = 256;
@@ -263,13 +376,13 @@ struct GreyTable {
 // offsets.  Per plane step the LDS offset of every table element (ring slot
 // of its dz, then dy, dx) is computed once, so the inner loop is one LDS
 // read, one add and one min/max per element.
-template <bool MAX, int NR>
+template <bool MAX, int NR, int EPI = 0>
 __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
                                                  float* __restrict__ ovol, int D, int H, int W,
                                                  const GreyTable& tab, int zlo, int zhi, int ylo,
                                                  int yhi, int xlo, int xhi, int mode, float cval,
                                                  int tx, int ty, int zb, int ze, float* sm,
-                                                 int* soff) {
+                                                 int* soff, GreyEpi epi = GreyEpi{}) {
   const int kz = zhi - zlo + 1;
   const int SX = kTX + xhi - xlo, SY = kTY + yhi - ylo;
   const int plane = SY * SX;
@@ -298,19 +411,23 @@ __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
         const float* base = sm + r * SX + c;
         float acc = ID;
         for (int t = 0; t < tab.n; ++t) acc = gop<MAX>(acc, base[soff[t]] + tab.v[t]);
-        if (y < H && x < W) ovol[((int64_t)z * H + y) * W + x] = acc;
+        if (y < H && x < W) {
+          const int64_t p = ((int64_t)z * H + y) * W + x;
+          ovol[p] = grey_epi<EPI>(acc, epi, p);
+        }
       }
     }
     __syncthreads();
   }
 }
 
-template <bool MAX, int NR>
+template <bool MAX, int NR, int EPI = 0>
 __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restrict__ in,
                                                           float* __restrict__ out, int D, int H,
                                                           int W, GreyTable tab, int zlo, int zhi,
                                                           int ylo, int yhi, int xlo, int xhi,
-                                                          int mode, float cval, int zchunk) {
+                                                          int mode, float cval, int zchunk,
+                                                          GreyEpi epi) {
   extern __shared__ float sm[];
   __shared__ int soff[kMaxTable];
   const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
@@ -320,8 +437,10 @@ __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restric
   grey_block(ntx, ntx * nty, nz, &tx, &ty, &zci, &item);
   const int zb = zci * zchunk;
   const int64_t n = (int64_t)D * H * W;
-  grey_table3_tile<MAX, NR>(in + item * n, out + item * n, D, H, W, tab, zlo, zhi, ylo, yhi, xlo,
-                            xhi, mode, cval, tx, ty, zb, min(D, zb + zchunk), sm, soff);
+  const GreyEpi ei{epi.a ? epi.a + item * n : nullptr, epi.b ? epi.b + item * n : nullptr};
+  grey_table3_tile<MAX, NR, EPI>(in + item * n, out + item * n, D, H, W, tab, zlo, zhi, ylo, yhi,
+                                 xlo, xhi, mode, cval, tx, ty, zb, min(D, zb + zchunk), sm, soff,
+                                 ei);
 }
 
 // Ragged lists (no reference counterpart): items of different (D, H, W) at
@@ -563,17 +682,52 @@ extern "C" size_t tm_grey_workspace_bytes(int64_t batch, int ndim, const int64_t
   return (size_t)n * sizeof(float) + 256 + (size_t)ne * (ndim + 1) * sizeof(int32_t);
 }
 
-extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_t* shape,
-                       const float* structure, const uint8_t* footprint, const int64_t* se_shape,
-                       const int64_t* origin, int mode, float cval, int dilation, float* output,
-                       void* workspace, size_t workspace_bytes, void* stream) {
-  using namespace tmh;
+namespace tmh {
+
+template <bool MX, int NR>
+static void launch_box3(int epi, unsigned G, size_t lds, hipStream_t st, const float* in,
+                        float* out, int D, int H, int W, int kz, int ky, int kx, int sz, int sy,
+                        int sx, int mode, float cval, int zchunk, GreyEpi e) {
+#define TM_BOX(EP)                                                                               \
+  hipLaunchKernelGGL((grey_box3_kernel<MX, NR, EP>), dim3(G), dim3(256), lds, st, in, out, D, H, W, \
+                     kz, ky, kx, sz, sy, sx, mode, cval, zchunk, e)
+  switch (epi) {
+    case 1: TM_BOX(1); break;
+    case 2: TM_BOX(2); break;
+    case 3: TM_BOX(3); break;
+    default: TM_BOX(0); break;
+  }
+#undef TM_BOX
+}
+
+template <bool MX, int NR>
+static void launch_table3(int epi, unsigned G, size_t lds, hipStream_t st, const float* in,
+                          float* out, int D, int H, int W, const GreyTable& tab, const Grey3& g3,
+                          int mode, float cval, int zchunk, GreyEpi e) {
+#define TM_TAB(EP)                                                                                 \
+  hipLaunchKernelGGL((grey_table3_kernel<MX, NR, EP>), dim3(G), dim3(256), lds, st, in, out, D, H, \
+                     W, tab, (int)g3.lo[0], (int)g3.hi[0], (int)g3.lo[1], (int)g3.hi[1],          \
+                     (int)g3.lo[2], (int)g3.hi[2], mode, cval, zchunk, e)
+  switch (epi) {
+    case 1: TM_TAB(1); break;
+    case 2: TM_TAB(2); break;
+    case 3: TM_TAB(3); break;
+    default: TM_TAB(0); break;
+  }
+#undef TM_TAB
+}
+
+static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* shape,
+                    const float* structure, const uint8_t* footprint, const int64_t* se_shape,
+                    const int64_t* origin, int mode, float cval, int dilation, int epi,
+                    const float* epi_a, const float* epi_b, float* output, void* workspace,
+                    size_t workspace_bytes, hipStream_t st) {
   TM_CHECK_ARG(input && output, "tm_grey: input/output is NULL");
   TM_CHECK_ARG(input != output, "tm_grey: output must not alias input");
   TM_CHECK_ARG(ndim >= 1 && ndim <= kMaxNdim, "tm_grey: ndim must be in 1..8, got %d", ndim);
   TM_CHECK_ARG(mode >= 0 && mode <= 4, "tm_grey: mode must be in 0..4");
   TM_CHECK_ARG(batch >= 1, "tm_grey: batch must be >= 1");
-  hipStream_t st = (hipStream_t)stream;
+  const GreyEpi ge{epi_a, epi_b};
   Shape sh = make_shape(ndim, shape);
   const int64_t total = batch * sh.numel;
   for (int j = 0; j < ndim; ++j) TM_CHECK_ARG(shape[j] >= 1, "tm_grey: input axis %d is empty", j);
@@ -618,10 +772,9 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
           return TM_ERR_ARG;
         }
         const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
-#define TM_BOX(MX, NR)                                                                          \
-  hipLaunchKernelGGL((grey_box3_kernel<MX, NR>), dim3((unsigned)G), dim3(256), lds, st, input,     \
-                     output, D, H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1], (int)lo3[2], mode,     \
-                     cval, zchunk)
+#define TM_BOX(MX, NR)                                                                         \
+  launch_box3<MX, NR>(epi, (unsigned)G, lds, st, input, output, D, H, W, kz, ky, kx, (int)lo3[0], \
+                      (int)lo3[1], (int)lo3[2], mode, cval, zchunk, ge)
         if (is_max) {
           if (small_se) TM_BOX(true, kPlaneRowsSmall);
           else TM_BOX(true, kPlaneRowsLarge);
@@ -649,10 +802,9 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
           return TM_ERR_ARG;
         }
         const bool small_se = kTY + (int)(hi3[1] - lo3[1]) <= 4 * kPlaneRowsSmall;
-#define TM_TAB(MX, NR)                                                                          \
-  hipLaunchKernelGGL((grey_table3_kernel<MX, NR>), dim3((unsigned)G), dim3(256), lds, st, input,   \
-                     output, D, H, W, tab, (int)lo3[0], (int)hi3[0], (int)lo3[1], (int)hi3[1],     \
-                     (int)lo3[2], (int)hi3[2], mode, cval, zchunk)
+#define TM_TAB(MX, NR)                                                                        \
+  launch_table3<MX, NR>(epi, (unsigned)G, lds, st, input, output, D, H, W, tab, g3, mode, cval, \
+                        zchunk, ge)
         if (is_max) {
           if (small_se) TM_TAB(true, kPlaneRowsSmall);
           else TM_TAB(true, kPlaneRowsLarge);
@@ -670,6 +822,9 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
       return ran ? TM_OK : run(v);
     }
   }
+  TM_CHECK_ARG(epi == 0,
+               "tm_grey_ex: a composite epilogue needs items of at most 3 axes and a "
+               "structuring element of the streaming kernels (tm_grey_fused_ok)");
   const size_t need = tm_grey_workspace_bytes(batch, ndim, shape, se_shape);
   if (!workspace || workspace_bytes < need) {
     set_error("tm_grey: workspace of %zu bytes is smaller than the %zu required",
@@ -711,6 +866,130 @@ extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_
   // the pageable copies above are staged synchronously, so the host table
   // may be reused once this call returns
   return TM_OK;
+}
+
+}  // namespace tmh
+
+extern "C" int tm_grey(const float* input, int64_t batch, int ndim, const int64_t* shape,
+                       const float* structure, const uint8_t* footprint, const int64_t* se_shape,
+                       const int64_t* origin, int mode, float cval, int dilation, float* output,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+  return tmh::grey_run(input, batch, ndim, shape, structure, footprint, se_shape, origin, mode,
+                       cval, dilation, 0, nullptr, nullptr, output, workspace, workspace_bytes,
+                       (hipStream_t)stream);
+}
+
+extern "C" int tm_grey_fused_ok(int ndim, const float* structure, const uint8_t* footprint,
+                                const int64_t* se_shape, const int64_t* origin, int dilation) {
+  using namespace tmh;
+  if (ndim < 1 || ndim > 3 || !se_shape || !origin) return 0;
+  GreyElems E;
+  if (grey_elements("tm_grey_fused_ok", ndim, structure, footprint, se_shape, origin, dilation,
+                    &E) != TM_OK)
+    return 0;
+  const Grey3 g3 = grey3_of(ndim, E);
+  return (grey_box3_fits(E, g3) || grey_table3_fits(E, g3)) ? 1 : 0;
+}
+
+namespace tmh {
+
+static size_t box3_dual_lds(int kz, int ky, int kx) {
+  const size_t SX = kTX + kx - 1, SY = kTY + ky - 1;
+  return (SY * SX + 2 * SY * kTX + 2 * (size_t)kz * kTY * kTX) * sizeof(float);
+}
+
+// Whether gradient / laplace of this element runs in one pass
+// (grey_box3_dual_kernel): a flat box of odd sizes with zero origin, within
+// the LDS budget.
+static bool grey_dual_fits(int ndim, const GreyElems& E, const Grey3& g3) {
+  if (!(E.flat && E.full) || ndim > 3) return false;
+  for (int j = 0; j < 3; ++j)
+    if (g3.lo[j] != -g3.hi[j]) return false;
+  return box3_dual_lds(g3.kz, g3.ky, g3.kx) <= kLdsBudget;
+}
+
+static int grey_dual_run(const float* input, int64_t batch, int ndim, const int64_t* shape,
+                         const int64_t* se_shape, const int64_t* origin, int mode, float cval,
+                         int comb, float* output, hipStream_t st) {
+  TM_CHECK_ARG(input && output && input != output, "tm_grey_ex: input/output invalid");
+  TM_CHECK_ARG(mode >= 0 && mode <= 4, "tm_grey_ex: mode must be in 0..4");
+  TM_CHECK_ARG(batch >= 1, "tm_grey_ex: batch must be >= 1");
+  for (int j = 0; j < ndim; ++j) TM_CHECK_ARG(shape[j] >= 1, "tm_grey_ex: input axis %d is empty", j);
+  GreyElems E;
+  int rc = grey_elements("tm_grey_ex", ndim, nullptr, nullptr, se_shape, origin, 1, &E);
+  if (rc != TM_OK) return rc;
+  const Grey3 g3 = grey3_of(ndim, E);
+  TM_CHECK_ARG(grey_dual_fits(ndim, E, g3),
+               "tm_grey_ex: one-pass gradient / laplace needs a flat box of odd sizes with zero "
+               "origin (tm_grey_dual_ok)");
+  int64_t s3[3] = {1, 1, 1};
+  for (int j = 0; j < ndim; ++j) s3[3 - ndim + j] = shape[j];
+  const int D = (int)s3[0], H = (int)s3[1], W = (int)s3[2];
+  const int kz = g3.kz, ky = g3.ky, kx = g3.kx;
+  const int64_t ntiles = ((W + kTX - 1) / kTX) * (int64_t)((H + kTY - 1) / kTY);
+  const size_t lds = box3_dual_lds(kz, ky, kx);
+  const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
+  auto run = [&](int v) -> int {
+    const int zchunk = std::min(D, 8 << v);
+    const int64_t G = ntiles * ((D + zchunk - 1) / zchunk) * batch;
+    if (G >= ((int64_t)1 << 31)) {
+      set_error("tm_grey_ex: batch too large for one launch");
+      return TM_ERR_ARG;
+    }
+#define TM_DUAL(NR, CB)                                                                          \
+  hipLaunchKernelGGL((grey_box3_dual_kernel<NR, CB>), dim3((unsigned)G), dim3(256), lds, st, input, \
+                     output, D, H, W, kz, ky, kx, (int)g3.lo[0], (int)g3.lo[1], (int)g3.lo[2],      \
+                     mode, cval, zchunk)
+    if (comb == 0) {
+      if (small_se) TM_DUAL(kPlaneRowsSmall, 0);
+      else TM_DUAL(kPlaneRowsLarge, 0);
+    } else {
+      if (small_se) TM_DUAL(kPlaneRowsSmall, 1);
+      else TM_DUAL(kPlaneRowsLarge, 1);
+    }
+#undef TM_DUAL
+    TM_LAUNCH_CHECK();
+    return TM_OK;
+  };
+  auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
+  const int64_t key[8] = {3, kz, ky, kx, p2(D), p2(H), p2(W), p2(batch)};
+  bool ran = false;
+  const int v = tune_pick("grey3", key, 8, 4, 1, run, st, &ran);
+  return ran ? TM_OK : run(v);
+}
+
+}  // namespace tmh
+
+extern "C" int tm_grey_dual_ok(int ndim, const int64_t* se_shape, const int64_t* origin) {
+  using namespace tmh;
+  if (ndim < 1 || ndim > 3 || !se_shape || !origin) return 0;
+  GreyElems E;
+  if (grey_elements("tm_grey_dual_ok", ndim, nullptr, nullptr, se_shape, origin, 1, &E) != TM_OK)
+    return 0;
+  return grey_dual_fits(ndim, E, grey3_of(ndim, E)) ? 1 : 0;
+}
+
+extern "C" int tm_grey_ex(const float* input, int64_t batch, int ndim, const int64_t* shape,
+                          const float* structure, const uint8_t* footprint,
+                          const int64_t* se_shape, const int64_t* origin, int mode, float cval,
+                          int dilation, int epilogue, const float* a, const float* b,
+                          float* output, void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(epilogue >= 1 && epilogue <= 5, "tm_grey_ex: epilogue must be in 1..5");
+  TM_CHECK_ARG(ndim >= 1 && ndim <= 3, "tm_grey_ex: items must have 1..3 axes");
+  if (epilogue >= 4) {
+    TM_CHECK_ARG(!structure && !footprint,
+                 "tm_grey_ex: one-pass gradient / laplace takes a flat box (no structure or "
+                 "footprint)");
+    return grey_dual_run(input, batch, ndim, shape, se_shape, origin, mode, cval, epilogue - 4,
+                         output, (hipStream_t)stream);
+  }
+  TM_CHECK_ARG(a != nullptr && (epilogue != 3 || b != nullptr),
+               "tm_grey_ex: epilogue operand is NULL");
+  TM_CHECK_ARG(a != output && b != output,
+               "tm_grey_ex: epilogue operands must not alias the output");
+  return grey_run(input, batch, ndim, shape, structure, footprint, se_shape, origin, mode, cval,
+                  dilation, epilogue, a, b, output, nullptr, 0, (hipStream_t)stream);
 }
 
 // ------------------------------------------------------- ragged lists --

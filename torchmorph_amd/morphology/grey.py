@@ -106,6 +106,71 @@ def _launch(x: Tensor, struct, fp, se_shape, origin, mode_int, cval, dilation: b
         _lib.check(status, "tm_grey")
 
 
+def _host_element(struct, fp):
+    st_np = None if struct is None else struct.numpy()
+    fp_np = None if fp is None else fp.to(torch.uint8).numpy()
+    return st_np, fp_np
+
+
+def _cptr(a):
+    return None if a is None else a.ctypes.data_as(_lib.ctypes.c_void_p)
+
+
+def _fused_ok(nd, struct, fp, se_shape, origin) -> bool:
+    st_np, fp_np = _host_element(struct, fp)
+    L = _lib.lib()
+    return all(L.tm_grey_fused_ok(nd, _cptr(st_np), _cptr(fp_np), _lib.i64_array(se_shape),
+                                  _lib.i64_array(origin), dil) == 1 for dil in (0, 1))
+
+
+def _launch_ex(x: Tensor, struct, fp, se_shape, origin, mode_int, cval, dilation: bool, epi: int,
+               a: Tensor, b: Tensor | None, out: Tensor) -> None:
+    """Erosion/dilation of x with a composite's final arithmetic fused into the
+    store (tm_grey_ex): epi 1 -> acc - a, 2 -> a - acc, 3 -> (acc + a) - 2 b."""
+    L = _lib.lib()
+    st_np, fp_np = _host_element(struct, fp)
+    with torch.cuda.device(x.device):
+        _lib.check(L.tm_grey_ex(_lib.ptr(x), x.shape[0] * x.shape[1], x.ndim - 2,
+                                _lib.i64_array(x.shape[2:]), _cptr(st_np), _cptr(fp_np),
+                                _lib.i64_array(se_shape), _lib.i64_array(origin), mode_int,
+                                float(cval), int(dilation), epi, _lib.ptr(a),
+                                None if b is None else _lib.ptr(b), _lib.ptr(out),
+                                _lib.stream_of(x.device)), "tm_grey_ex")
+
+
+def _composite(input: Tensor, output, kw, first: bool, second: bool, second_on_first: bool,
+               epi: int, a_is_first: bool, b_is_input: bool, fallback, one_pass: int = 0):
+    """A composite in two launches: pass `first` (False erosion, True
+    dilation) of the input into a temporary, then pass `second` -- of the
+    temporary when `second_on_first`, else of the input -- with the epilogue
+    fused into its store; epilogue operand a is the temporary or the input,
+    b the input.  `one_pass` (4 gradient, 5 laplace): a flat box of odd sizes
+    with zero origin runs in a single pass over the input instead.  Falls back
+    to the plain composition (the same kernels and tensor arithmetic) where
+    the fused kernels do not apply."""
+    validate_bcs_input(input)
+    validate_output(input, output)
+    struct, fp, origin_t, mode_int, se_shape = _resolve_element(
+        input, kw["size"], kw["footprint"], kw["structure"], kw["mode"], kw["origin"])
+    nd = input.ndim - 2
+    if not _fused_ok(nd, struct, fp, se_shape, origin_t):
+        return _finish(fallback(), output)
+    x = input.detach()
+    x = x.contiguous() if x.dtype == torch.float32 else x.float().contiguous()
+    if one_pass and struct is None and fp is None and _lib.lib().tm_grey_dual_ok(
+            nd, _lib.i64_array(se_shape), _lib.i64_array(origin_t)) == 1:
+        result = torch.empty_like(x)
+        _launch_ex(x, None, None, se_shape, origin_t, mode_int, kw["cval"], True, one_pass, x,
+                   None, result)
+        return _finish(result, output)
+    tmp = torch.empty_like(x)
+    _launch(x, struct, fp, se_shape, origin_t, mode_int, kw["cval"], first, tmp)
+    result = torch.empty_like(x)
+    _launch_ex(tmp if second_on_first else x, struct, fp, se_shape, origin_t, mode_int, kw["cval"],
+               second, epi, tmp if a_is_first else x, x if b_is_input else None, result)
+    return _finish(result, output)
+
+
 def _grey_morphology(input: Tensor, size=None, footprint=None, structure=None, output=None,
                      mode: str = "reflect", cval: float = 0.0, origin=0, *, operation: str,
                      differentiable: bool = False) -> Tensor:
@@ -169,36 +234,42 @@ def _finish(result, output):
 
 def morphological_gradient(input: Tensor, size=None, footprint=None, structure=None, output=None,
                            mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
-    """dilation - erosion (reference grey.py:200-221)."""
-    validate_bcs_input(input)
-    validate_output(input, output)
+    """dilation - erosion (reference grey.py:200-221): one pass for a flat box
+    of odd sizes, else the subtraction fused into the dilation's store."""
     kw = _element_kwargs(size, footprint, structure, mode, cval, origin)
-    return _finish(grey_dilation(input, **kw) - grey_erosion(input, **kw), output)
+    return _composite(input, output, kw, first=False, second=True, second_on_first=False, epi=1,
+                      a_is_first=True, b_is_input=False,
+                      fallback=lambda: grey_dilation(input, **kw) - grey_erosion(input, **kw),
+                      one_pass=4)
 
 
 def white_tophat(input: Tensor, size=None, footprint=None, structure=None, output=None,
                  mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
-    """input - opening (reference grey.py:224-243)."""
-    validate_bcs_input(input)
-    validate_output(input, output)
-    opened = grey_opening(input, **_element_kwargs(size, footprint, structure, mode, cval, origin))
-    return _finish(input.detach().float() - opened, output)
+    """input - opening (reference grey.py:224-243), the subtraction fused
+    into the opening's dilation."""
+    kw = _element_kwargs(size, footprint, structure, mode, cval, origin)
+    return _composite(input, output, kw, first=False, second=True, second_on_first=True, epi=2,
+                      a_is_first=False, b_is_input=False,
+                      fallback=lambda: input.detach().float() - grey_opening(input, **kw))
 
 
 def black_tophat(input: Tensor, size=None, footprint=None, structure=None, output=None,
                  mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
-    """closing - input (reference grey.py:246-265)."""
-    validate_bcs_input(input)
-    validate_output(input, output)
-    closed = grey_closing(input, **_element_kwargs(size, footprint, structure, mode, cval, origin))
-    return _finish(closed - input.detach().float(), output)
+    """closing - input (reference grey.py:246-265), the subtraction fused
+    into the closing's erosion."""
+    kw = _element_kwargs(size, footprint, structure, mode, cval, origin)
+    return _composite(input, output, kw, first=True, second=False, second_on_first=True, epi=1,
+                      a_is_first=False, b_is_input=False,
+                      fallback=lambda: grey_closing(input, **kw) - input.detach().float())
 
 
 def morphological_laplace(input: Tensor, size=None, footprint=None, structure=None, output=None,
                           mode: str = "reflect", cval: float = 0.0, origin=0) -> Tensor:
-    """dilation + erosion - 2 input (reference grey.py:268-289)."""
-    validate_bcs_input(input)
-    validate_output(input, output)
+    """dilation + erosion - 2 input (reference grey.py:268-289): one pass for
+    a flat box of odd sizes, else the sums fused into the dilation's store."""
     kw = _element_kwargs(size, footprint, structure, mode, cval, origin)
-    return _finish(grey_dilation(input, **kw) + grey_erosion(input, **kw)
-                   - 2 * input.detach().float(), output)
+    return _composite(input, output, kw, first=False, second=True, second_on_first=False, epi=3,
+                      a_is_first=True, b_is_input=True,
+                      fallback=lambda: (grey_dilation(input, **kw) + grey_erosion(input, **kw)
+                                        - 2 * input.detach().float()),
+                      one_pass=5)
