@@ -448,6 +448,41 @@ def extra_ragged():
     res["cpu_baseline"] = {"value": round(done_vox / dt / 1e6, 2), "unit": "Mvox-op/s", "cores": 1,
                            "kind": "port", "sample": f"{k} 2-D items x 3 ops, oracle/tm_oracle.c"}
     res["bytes_per_vox_op"] = "1 mask in + 4 f32 out (algorithmic)"
+    # ragged morphology on the same item shapes (one launch per list and step)
+    from torchmorph_amd.ragged import RaggedBinaryMorphology, RaggedGreyMorphology
+    morph = {}
+    for nd in (2, 3):
+        idx = [k for k, sh in enumerate(shapes) if len(sh) == nd]
+        sh_nd = [shapes[k] for k in idx]
+        vox = sum(math.prod(sh) for sh in sh_nd)
+        vals = [torch.floor(torch.rand(sh, generator=gen, device="cuda") * 256) / 256
+                for sh in sh_nd]
+        gplan = RaggedGreyMorphology(sh_nd, "dilation", size=3, device="cuda")
+        gflat = gplan.pack(vals)
+        gplan.run(gflat)
+        ms = _timed(lambda: gplan.run(gflat), 5, st)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dense = [tm.grey_dilation(v[None, None], size=3) for v in vals]
+        torch.cuda.synchronize()
+        dense_ms = (time.perf_counter() - t0) * 1e3
+        same = all(torch.equal(a, b[0, 0]) for a, b in zip(gplan.results(), dense))
+        bplan = RaggedBinaryMorphology(sh_nd, "erosion", iterations=4, device="cuda")
+        bmasks = [masks[k] for k in idx]
+        bflat = bplan.pack(bmasks)
+        bres = bplan(bflat)
+        bms = _timed(lambda: bplan.run(bflat), 5, st)
+        bsame = all(torch.equal(bres[j], tm.binary_erosion(bmasks[j][None, None],
+                                                          iterations=4)[0, 0])
+                    for j in range(0, len(idx), 7))
+        morph[f"{nd}d"] = {"items": len(idx), "voxels": vox,
+                           "grey_dilation_size3": {"ms": round(ms, 3),
+                                                   "GB_s": _gb(8 * vox, ms),
+                                                   "dense_per_item_ms": round(dense_ms, 2),
+                                                   "equal_to_dense": bool(same)},
+                           "binary_erosion_x4": {"ms": round(bms, 3),
+                                                 "equal_to_dense": bool(bsame)}}
+    res["morphology"] = morph
     return res
 
 

@@ -155,3 +155,37 @@ def test_ragged_grey_rejects_oversized_element(tm):
     plan = RaggedGreyMorphology([(4, 30, 30)], "erosion", structure=st, device="cuda")
     with pytest.raises(RuntimeError, match="fits neither streaming kernel"):
         plan([torch.zeros((4, 30, 30), device="cuda")])
+
+
+@pytest.mark.parametrize("operation", ["erosion", "dilation", "opening", "closing"])
+@pytest.mark.parametrize("nd", [1, 2, 3])
+@pytest.mark.parametrize("iterations", [1, 3])
+def test_ragged_binary_equals_dense_per_item(tm, nd, operation, iterations):
+    rng = np.random.default_rng(10 * nd + iterations)
+    items = [rng.random(sh) < 0.7 for sh in SHAPES[nd]]
+    full = torch.ones((3,) * nd, dtype=torch.bool)
+    for structure, border, origin in ((None, False, 0), (full, True, 0),
+                                      (full, False, (1, -1, 0)[-nd:])):
+        res = getattr(tm, f"binary_{operation}_ragged")(
+            [torch.as_tensor(x, device="cuda") for x in items], structure=structure,
+            iterations=iterations, border_value=border, origin=origin)
+        for k, x in enumerate(items):
+            dense = getattr(tm, f"binary_{operation}")(
+                torch.as_tensor(x[None, None], device="cuda"), structure=structure,
+                iterations=iterations, border_value=border, origin=origin)[0, 0]
+            assert res[k].dtype == torch.bool
+            assert torch.equal(res[k], dense), (k, x.shape, border, origin)
+
+
+def test_ragged_binary_empty_structure_and_limits(tm):
+    from torchmorph_amd.ragged import RaggedBinaryMorphology
+
+    items = [torch.rand((5, 7), device="cuda") < 0.5, torch.rand((3, 9), device="cuda") < 0.5]
+    empty = torch.zeros((3, 3), dtype=torch.bool)
+    for op in ("erosion", "dilation", "opening", "closing"):
+        res = getattr(tm, f"binary_{op}_ragged")(items, structure=empty)
+        for k, x in enumerate(items):
+            dense = getattr(tm, f"binary_{op}")(x[None, None], structure=empty)[0, 0]
+            assert torch.equal(res[k], dense), (op, k)
+    with pytest.raises(ValueError, match="iterations >= 1"):
+        RaggedBinaryMorphology([(4, 4)], iterations=0, device="cuda")

@@ -39,10 +39,15 @@ from .optimal_transport import (
     sinkhorn_points,
 )
 from .ragged import (
+    RaggedBinaryMorphology,
     RaggedDistanceTransform,
     RaggedGreyMorphology,
     chamfer_distance_transform_ragged,
     euclidean_distance_transform_ragged,
+    binary_closing_ragged,
+    binary_dilation_ragged,
+    binary_erosion_ragged,
+    binary_opening_ragged,
     grey_closing_ragged,
     grey_dilation_ragged,
     grey_erosion_ragged,
@@ -71,6 +76,11 @@ __all__ = [
     "grey_dilation_ragged",
     "grey_opening_ragged",
     "grey_closing_ragged",
+    "RaggedBinaryMorphology",
+    "binary_erosion_ragged",
+    "binary_dilation_ragged",
+    "binary_opening_ragged",
+    "binary_closing_ragged",
 ]
 
 __version__ = "0.1.0"

@@ -20,6 +20,12 @@ on that item (reference grey.py:29-197 takes one dense tensor per call).
 
     plan = RaggedGreyMorphology([v.shape for v in volumes], "opening", size=3)
     opened = plan(volumes)                 # one tensor per volume
+
+Binary morphology of a ragged list (``RaggedBinaryMorphology``) runs its steps
+on the same kernels: one step of binary erosion / dilation (reference
+binary_kernel.cu:56-127, binary.py:15-25) is grey erosion / dilation of the
+0/1 mask with the structure as a flat footprint, constant mode and cval =
+border_value (AND = min and OR = max on {0, 1}, the same element offsets).
 """
 from __future__ import annotations
 
@@ -42,6 +48,11 @@ __all__ = [
     "grey_dilation_ragged",
     "grey_opening_ragged",
     "grey_closing_ragged",
+    "RaggedBinaryMorphology",
+    "binary_erosion_ragged",
+    "binary_dilation_ragged",
+    "binary_opening_ragged",
+    "binary_closing_ragged",
 ]
 
 TM_RG_EDT, TM_RG_CDT_CHESSBOARD, TM_RG_CDT_TAXICAB = 0, 1, 2
@@ -443,3 +454,119 @@ def grey_closing_ragged(items: Sequence[Tensor], size=None, footprint=None, stru
                         mode: str = "reflect", cval: float = 0.0, origin=0) -> list[Tensor]:
     """Grey closing (erosion of the dilation) of every item, two launches."""
     return _grey_ragged(items, "closing", size, footprint, structure, mode, cval, origin)
+
+
+# operation -> dilation flag of each block of `iterations` steps (reference
+# binary.py:91-136, 231-290)
+_BINARY_BLOCKS = {"erosion": (0,), "dilation": (1,), "opening": (0, 1), "closing": (1, 0)}
+
+
+class RaggedBinaryMorphology:
+    """Binary erosion / dilation / opening / closing of a fixed list of item
+    shapes, every step of every item in one launch.
+
+    Args:
+        shapes: spatial shape of every item (1..3 axes, the same number for
+            every item).
+        operation: ``"erosion"``, ``"dilation"``, ``"opening"`` or ``"closing"``.
+        structure: boolean structuring element (default: the cross of
+            ``generate_binary_structure(ndim, 1)``).
+        iterations: steps per erosion / dilation, >= 1 (repeating until stable
+            and ``mask`` are not offered on ragged lists).
+        border_value, origin: as ``binary_erosion``.
+    """
+
+    def __init__(self, shapes: Sequence[Sequence[int]], operation: str = "erosion",
+                 structure: Tensor | None = None, iterations: int = 1,
+                 border_value: bool = False, origin=0,
+                 device: torch.device | str | None = None):
+        from .morphology.binary import _normalize_structure
+        from .morphology.structure import generate_binary_structure
+
+        if operation not in _BINARY_BLOCKS:
+            raise ValueError(f"operation must be one of {sorted(_BINARY_BLOCKS)}")
+        if int(iterations) < 1:
+            raise ValueError("ragged binary morphology needs iterations >= 1")
+        if not shapes:
+            raise ValueError("at least one item shape is required")
+        nd = len(shapes[0])
+        if structure is None:
+            structure = generate_binary_structure(nd, 1)
+        structure = _normalize_structure(structure, nd)
+        self.operation = operation
+        self.iterations = int(iterations)
+        self.empty = not bool(structure.any().item())
+        footprint = structure if not self.empty else torch.ones((1,) * nd, dtype=torch.bool)
+        self._plan = RaggedGreyMorphology(shapes, "erosion", footprint=footprint,
+                                          mode="constant", cval=1.0 if border_value else 0.0,
+                                          origin=origin if not self.empty else 0, device=device)
+        self.shapes = self._plan.shapes
+        self.device = self._plan.device
+        self._buf = torch.empty_like(self._plan.output)
+
+    def pack(self, items: Sequence[Tensor]) -> Tensor:
+        """Flat float32 0/1 buffer of the items in list order."""
+        return self._plan.pack([(x != 0) for x in items])
+
+    def run(self, flat: Tensor) -> Tensor:
+        """Runs the operation on a packed 0/1 input; returns the flat float32
+        0/1 result."""
+        steps = [dil for dil in _BINARY_BLOCKS[self.operation] for _ in range(self.iterations)]
+        if self.empty:
+            # every step yields a constant (reference binary.py:54-58)
+            self._plan.output.fill_(0.0 if steps[-1] else 1.0)
+            return self._plan.output
+        # step k writes bufs[(n - 1 - k) % 2]: the last step lands in the output
+        bufs = (self._plan.output, self._buf)
+        n = len(steps)
+        src = flat
+        with torch.cuda.device(self.device):
+            st = _lib.stream_of(self.device)
+            for k, dil in enumerate(steps):
+                dst = bufs[(n - 1 - k) % 2]
+                self._plan._pass(src, dst, dil, st)
+                src = dst
+        return src
+
+    def __call__(self, items: Sequence[Tensor] | Tensor) -> list[Tensor]:
+        flat = items if isinstance(items, Tensor) else self.pack(items)
+        out = self.run(flat)
+        return [out[self._plan.offsets[k]:self._plan.offsets[k + 1]].view(sh) != 0
+                for k, sh in enumerate(self.shapes)]
+
+
+def _binary_ragged(items, operation, structure, iterations, border_value, origin):
+    if not items:
+        raise ValueError("at least one item is required")
+    plan = RaggedBinaryMorphology([x.shape for x in items], operation, structure, iterations,
+                                  border_value, origin, device=items[0].device)
+    return plan(items)
+
+
+def binary_erosion_ragged(items: Sequence[Tensor], structure: Tensor | None = None,
+                          iterations: int = 1, border_value: bool = False,
+                          origin=0) -> list[Tensor]:
+    """Binary erosion of every item of a ragged list; one bool result per item,
+    as ``binary_erosion`` returns it for that item."""
+    return _binary_ragged(items, "erosion", structure, iterations, border_value, origin)
+
+
+def binary_dilation_ragged(items: Sequence[Tensor], structure: Tensor | None = None,
+                           iterations: int = 1, border_value: bool = False,
+                           origin=0) -> list[Tensor]:
+    """Binary dilation of every item of a ragged list."""
+    return _binary_ragged(items, "dilation", structure, iterations, border_value, origin)
+
+
+def binary_opening_ragged(items: Sequence[Tensor], structure: Tensor | None = None,
+                          iterations: int = 1, border_value: bool = False,
+                          origin=0) -> list[Tensor]:
+    """Binary opening (erosions, then dilations) of every item of a ragged list."""
+    return _binary_ragged(items, "opening", structure, iterations, border_value, origin)
+
+
+def binary_closing_ragged(items: Sequence[Tensor], structure: Tensor | None = None,
+                          iterations: int = 1, border_value: bool = False,
+                          origin=0) -> list[Tensor]:
+    """Binary closing (dilations, then erosions) of every item of a ragged list."""
+    return _binary_ragged(items, "closing", structure, iterations, border_value, origin)
