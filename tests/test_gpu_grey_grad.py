@@ -99,3 +99,24 @@ def test_grey_backward_config4_shape(tm, oracle_mod):
                                rtol=1e-6, atol=1e-6)
     # total routed gradient equals the total upstream gradient (no cval reads)
     assert abs(float(xt.grad.double().sum()) - float(dy.double().sum())) < 1e-2
+
+
+@pytest.mark.parametrize("mode", ["reflect", "mirror", "nearest", "constant", "wrap"])
+@pytest.mark.parametrize("dilation", [False, True])
+@pytest.mark.parametrize("shape", [(1, 1, 3, 2), (2, 1, 1, 5), (1, 1, 5, 1), (1, 1, 70, 130)])
+def test_grey_backward_tiny_items_and_large_elements(tm, oracle_mod, mode, dilation, shape):
+    """Items smaller than the element (many reflections of a border voxel)
+    and a 7 x 5 non-flat element with an origin, against the oracle."""
+    rng = np.random.default_rng(sum(shape))
+    x = (np.floor(rng.random(shape) * 6) / 6).astype(np.float32)
+    dy = rng.normal(size=shape).astype(np.float32)
+    fp = rng.random((7, 5)) < 0.7
+    fp[3, 2] = True
+    st = rng.integers(0, 3, size=(7, 5)).astype(np.float32)
+    kw = dict(footprint=torch.tensor(fp), structure=torch.tensor(st), mode=mode, cval=0.5,
+              origin=(1, -2))
+    _, dx = _grad(tm, x, dy, dilation, **kw)
+    for b in range(shape[0]):
+        want = oracle_mod.grey_backward(x[b, 0], st, fp, (7, 5), (1, -2), mode, 0.5, dilation,
+                                        dy[b, 0])
+        np.testing.assert_allclose(dx[b, 0], want, rtol=1e-6, atol=1e-6)

@@ -1286,6 +1286,138 @@ __global__ __launch_bounds__(256) void grey_grad_gather_kernel(
   dx[g] = acc;
 }
 
+// Fused backward for 2-D items (modes other than wrap): a workgroup owns a
+// 64 x 16 tile of inputs q.  The outputs p that can route into the tile lie
+// within e' = ext + 1 of it (an element reads p + o; a mode maps a read
+// outside the item back within ext + 1 of the border); their winning
+// elements are recomputed from x staged with a further ext halo (the
+// forward's arithmetic and first-element tie rule), and their dy staged
+// beside them.  Input q then sums, in element order t, dy[p] over the
+// outputs p = q' - o_t whose winning element is t, for every q' that the
+// mode maps to q (q itself, plus reflections of border positions); an
+// interior q has only q' = q, so its sum is grey_grad_gather_kernel's, term
+// for term.
+constexpr int kBwTX = 64, kBwTY = 16;
+
+// Candidate i of the positions the mode may map to c: i = 0 is c itself,
+// i = 1 .. e are -e .. -1, i = e + 1 .. 2e are n .. n - 1 + e.
+__device__ __forceinline__ int grey_pre_cand(int i, int c, int n, int e) {
+  return i == 0 ? c : (i <= e ? i - 1 - e : n + i - e - 1);
+}
+
+// Bit i set when candidate i maps to c (c itself always; constant mode maps
+// nothing from outside the item).
+__device__ __forceinline__ uint64_t grey_preimages(int c, int n, int e, int mode) {
+  uint64_t m = 1;
+  if (mode == 0 || (c >= e && c <= n - 1 - e)) return m;
+  for (int i = 1; i <= 2 * e; ++i) {
+    bool uc;
+    if (gmap(grey_pre_cand(i, c, n, e), n, mode, &uc) == c && !uc) m |= (uint64_t)1 << i;
+  }
+  return m;
+}
+
+template <bool MAX>
+__global__ __launch_bounds__(256) void grey_bw2d_kernel(const float* __restrict__ in,
+                                                        const float* __restrict__ dy,
+                                                        float* __restrict__ dx, int H, int W,
+                                                        GreyTabN tab, int mode, float cval) {
+  extern __shared__ float smem[];
+  const int ey = tab.ext[0], ex = tab.ext[1];
+  const int py = ey + 1, px = ex + 1;  // output halo
+  const int OR = kBwTY + 2 * py, OC = kBwTX + 2 * px;
+  const int XR = OR + 2 * ey, XC = OC + 2 * ex;
+  float* xs = smem;                     // [XR][XC] mode-extended input
+  float* ds = xs + XR * XC;             // [OR][OC] dy of in-item outputs
+  int32_t* kx = (int32_t*)(ds + OR * OC);  // [n] element offset in xs
+  int32_t* ko = kx + kMaxTable;            // [n] element offset in as / ds
+  float* kv = (float*)(ko + kMaxTable);    // [n] element value
+  int16_t* as = (int16_t*)(kv + kMaxTable);  // [OR][OC] winning element, -1 = not an output
+  for (int k = threadIdx.x; k < tab.n; k += blockDim.x) {
+    kx[k] = tab.o[k][0] * XC + tab.o[k][1];
+    ko[k] = tab.o[k][0] * OC + tab.o[k][1];
+    kv[k] = tab.v[k];
+  }
+  const int ntx = (W + kBwTX - 1) / kBwTX, nty = (H + kBwTY - 1) / kBwTY;
+  const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = lb % (ntx * nty);
+  const int64_t item = lb / (ntx * nty);
+  const int x0 = (t % ntx) * kBwTX, y0 = (t / ntx) * kBwTY;
+  const int64_t n = (int64_t)H * W;
+  const float* vol = in + item * n;
+  const float* dyv = dy + item * n;
+  // input region starts at (y0 - py - ey, x0 - px - ex), output region at
+  // (y0 - py, x0 - px)
+  for (int e = threadIdx.x; e < XR * XC; e += blockDim.x) {
+    const int r = e / XC, c = e - r * XC;
+    bool uy, ux;
+    const int64_t ym = gmap(y0 - py - ey + r, H, mode, &uy);
+    const int64_t xm = gmap(x0 - px - ex + c, W, mode, &ux);
+    xs[e] = (uy || ux) ? cval : vol[ym * W + xm];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < OR * OC; e += blockDim.x) {
+    const int r = e / OC, c = e - r * OC;
+    const int y = y0 - py + r, x = x0 - px + c;
+    if (y < 0 || y >= H || x < 0 || x >= W) {
+      as[e] = -1;
+      continue;
+    }
+    const float* base = xs + (r + ey) * XC + (c + ex);
+    float best = 0.f;
+    int bi = 0;
+    for (int k = 0; k < tab.n; ++k) {
+      const float v = base[kx[k]] + kv[k];
+      if (k == 0 || (MAX ? v > best : v < best)) {
+        best = v;
+        bi = k;
+      }
+    }
+    as[e] = (int16_t)bi;
+    ds[e] = dyv[(int64_t)y * W + x];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kBwTY * kBwTX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    const int qy = y0 + r, qx = x0 + c;
+    if (qy >= H || qx >= W) continue;
+    if (qy >= py && qy <= H - 1 - py && qx >= px && qx <= W - 1 - px) {
+      // interior: q is its own only preimage
+      const int s0 = (r + py) * OC + (c + px);
+      float acc = 0.f;
+      for (int k = 0; k < tab.n; ++k) {
+        const int s = s0 - ko[k];
+        if (as[s] == k) acc += ds[s];
+      }
+      dx[item * n + (int64_t)qy * W + qx] = acc;
+      continue;
+    }
+    const uint64_t my = grey_preimages(qy, H, py, mode);
+    const uint64_t mx = grey_preimages(qx, W, px, mode);
+    float acc = 0.f;
+    for (int k = 0; k < tab.n; ++k) {
+      for (uint64_t wy = my; wy; wy &= wy - 1) {
+        const int pr = grey_pre_cand(__builtin_ctzll(wy), qy, H, py) - tab.o[k][0] - (y0 - py);
+        for (uint64_t wx = mx; wx; wx &= wx - 1) {
+          const int pc =
+              grey_pre_cand(__builtin_ctzll(wx), qx, W, px) - tab.o[k][1] - (x0 - px);
+          if (pr < 0 || pr >= OR || pc < 0 || pc >= OC) continue;
+          const int s = pr * OC + pc;
+          if (as[s] == k) acc += ds[s];
+        }
+      }
+    }
+    dx[item * n + (int64_t)qy * W + qx] = acc;
+  }
+}
+
+static size_t grey_bw2d_lds(const GreyTabN& tab) {
+  const int py = tab.ext[0] + 1, px = tab.ext[1] + 1;
+  const size_t OR = kBwTY + 2 * py, OC = kBwTX + 2 * px;
+  const size_t XR = OR + 2 * tab.ext[0], XC = OC + 2 * tab.ext[1];
+  return (XR * XC + OR * OC + 3 * (size_t)kMaxTable) * sizeof(float) + OR * OC * sizeof(int16_t);
+}
+
 }  // namespace tmh
 
 extern "C" size_t tm_grey_backward_workspace_bytes(int64_t batch, int ndim, const int64_t* shape) {
@@ -1343,6 +1475,21 @@ extern "C" int tm_grey_backward(const float* input, int64_t batch, int ndim, con
   }
   int16_t* arg = (int16_t*)workspace;
   for (int j = 0; j < kMaxNdim; ++j) tab.ext[j] = j < ndim ? ext[j] : 0;
+  if (ndim == 2 && mode != 4 && ext[0] <= 16 && ext[1] <= 16) {
+    const int H = (int)shape[0], W = (int)shape[1];
+    const int64_t G = (int64_t)((W + kBwTX - 1) / kBwTX) * ((H + kBwTY - 1) / kBwTY) * batch;
+    if (G < ((int64_t)1 << 31)) {
+      const size_t lds = grey_bw2d_lds(tab);
+      if (dilation)
+        hipLaunchKernelGGL(grey_bw2d_kernel<true>, dim3((unsigned)G), dim3(256), lds, st, input,
+                           grad_out, grad_in, H, W, tab, mode, cval);
+      else
+        hipLaunchKernelGGL(grey_bw2d_kernel<false>, dim3((unsigned)G), dim3(256), lds, st, input,
+                           grad_out, grad_in, H, W, tab, mode, cval);
+      TM_LAUNCH_CHECK();
+      return TM_OK;
+    }
+  }
   for (int t = 0; t < tab.n; ++t) {
     int64_t f = 0;
     for (int j = 0; j < ndim; ++j) f += (int64_t)tab.o[t][j] * sh.stride[j];
