@@ -1317,26 +1317,54 @@ __device__ __forceinline__ uint64_t grey_preimages(int c, int n, int e, int mode
   return m;
 }
 
-template <bool MAX>
+// gmap for 32-bit coordinates.
+__device__ __forceinline__ int gmap32(int c, int size, int mode, bool* use_cval) {
+  *use_cval = false;
+  if ((unsigned)c < (unsigned)size) return c;
+  bool uc;
+  const int64_t m = gmap(c, size, mode, &uc);
+  *use_cval = uc;
+  return (int)m;
+}
+
+// E > 0: both element extents equal E, known at compile time (tile extents
+// are constants and the element table lives in registers); E == 0: any
+// extents up to 16, the table in LDS.
+template <bool MAX, int E>
 __global__ __launch_bounds__(256) void grey_bw2d_kernel(const float* __restrict__ in,
                                                         const float* __restrict__ dy,
                                                         float* __restrict__ dx, int H, int W,
                                                         GreyTabN tab, int mode, float cval) {
   extern __shared__ float smem[];
-  const int ey = tab.ext[0], ex = tab.ext[1];
+  constexpr bool CT = E > 0;
+  constexpr int NK = CT ? (2 * E + 1) * (2 * E + 1) : 1;
+  const int ey = CT ? E : tab.ext[0], ex = CT ? E : tab.ext[1];
   const int py = ey + 1, px = ex + 1;  // output halo
   const int OR = kBwTY + 2 * py, OC = kBwTX + 2 * px;
   const int XR = OR + 2 * ey, XC = OC + 2 * ex;
   float* xs = smem;                     // [XR][XC] mode-extended input
   float* ds = xs + XR * XC;             // [OR][OC] dy of in-item outputs
-  int32_t* kx = (int32_t*)(ds + OR * OC);  // [n] element offset in xs
+  int32_t* kx = (int32_t*)(ds + OR * OC);  // [n] element offset in xs (E == 0)
   int32_t* ko = kx + kMaxTable;            // [n] element offset in as / ds
   float* kv = (float*)(ko + kMaxTable);    // [n] element value
   int16_t* as = (int16_t*)(kv + kMaxTable);  // [OR][OC] winning element, -1 = not an output
-  for (int k = threadIdx.x; k < tab.n; k += blockDim.x) {
-    kx[k] = tab.o[k][0] * XC + tab.o[k][1];
-    ko[k] = tab.o[k][0] * OC + tab.o[k][1];
-    kv[k] = tab.v[k];
+  const int nk = tab.n;
+  int rkx[NK], rko[NK];
+  float rkv[NK];
+  if (CT) {
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      const int k2 = k < nk ? k : 0;
+      rkx[k] = tab.o[k2][0] * XC + tab.o[k2][1];
+      rko[k] = tab.o[k2][0] * OC + tab.o[k2][1];
+      rkv[k] = tab.v[k2];
+    }
+  } else {
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+      kx[k] = tab.o[k][0] * XC + tab.o[k][1];
+      ko[k] = tab.o[k][0] * OC + tab.o[k][1];
+      kv[k] = tab.v[k];
+    }
   }
   const int ntx = (W + kBwTX - 1) / kBwTX, nty = (H + kBwTY - 1) / kBwTY;
   const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -1348,33 +1376,88 @@ __global__ __launch_bounds__(256) void grey_bw2d_kernel(const float* __restrict_
   const float* dyv = dy + item * n;
   // input region starts at (y0 - py - ey, x0 - px - ex), output region at
   // (y0 - py, x0 - px)
-  for (int e = threadIdx.x; e < XR * XC; e += blockDim.x) {
-    const int r = e / XC, c = e - r * XC;
-    bool uy, ux;
-    const int64_t ym = gmap(y0 - py - ey + r, H, mode, &uy);
-    const int64_t xm = gmap(x0 - px - ex + c, W, mode, &ux);
-    xs[e] = (uy || ux) ? cval : vol[ym * W + xm];
+  if (CT) {
+    // every load of the thread is issued before the first LDS store: one
+    // memory round trip for x and dy together
+    constexpr int CXR = kBwTY + 4 * E + 2, CXC = kBwTX + 4 * E + 2;
+    constexpr int COR = kBwTY + 2 * E + 2, COC = kBwTX + 2 * E + 2;
+    constexpr int NX = (CXR * CXC + 255) / 256, NO = (COR * COC + 255) / 256;
+    float rx[NX], rd[NO];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      const int e = (int)threadIdx.x + j * 256;
+      rx[j] = 0.f;
+      if (e < CXR * CXC) {
+        const int r = e / CXC, c = e - r * CXC;
+        bool uy, ux;
+        const int ym = gmap32(y0 - py - ey + r, H, mode, &uy);
+        const int xm = gmap32(x0 - px - ex + c, W, mode, &ux);
+        rx[j] = (uy || ux) ? cval : vol[(int64_t)ym * W + xm];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+      const int e = (int)threadIdx.x + j * 256;
+      rd[j] = 0.f;
+      if (e < COR * COC) {
+        const int r = e / COC, c = e - r * COC;
+        const int y = y0 - py + r, x = x0 - px + c;
+        if ((unsigned)y < (unsigned)H && (unsigned)x < (unsigned)W)
+          rd[j] = dyv[(int64_t)y * W + x];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      const int e = (int)threadIdx.x + j * 256;
+      if (e < CXR * CXC) xs[e] = rx[j];
+    }
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+      const int e = (int)threadIdx.x + j * 256;
+      if (e < COR * COC) ds[e] = rd[j];
+    }
+  } else {
+    for (int e = threadIdx.x; e < XR * XC; e += blockDim.x) {
+      const int r = e / XC, c = e - r * XC;
+      bool uy, ux;
+      const int ym = gmap32(y0 - py - ey + r, H, mode, &uy);
+      const int xm = gmap32(x0 - px - ex + c, W, mode, &ux);
+      xs[e] = (uy || ux) ? cval : vol[(int64_t)ym * W + xm];
+    }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < OR * OC; e += blockDim.x) {
     const int r = e / OC, c = e - r * OC;
     const int y = y0 - py + r, x = x0 - px + c;
-    if (y < 0 || y >= H || x < 0 || x >= W) {
+    if ((unsigned)y >= (unsigned)H || (unsigned)x >= (unsigned)W) {
       as[e] = -1;
       continue;
     }
     const float* base = xs + (r + ey) * XC + (c + ex);
     float best = 0.f;
     int bi = 0;
-    for (int k = 0; k < tab.n; ++k) {
-      const float v = base[kx[k]] + kv[k];
-      if (k == 0 || (MAX ? v > best : v < best)) {
-        best = v;
-        bi = k;
+    if (CT) {
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        if (k < nk) {
+          const float v = base[rkx[k]] + rkv[k];
+          if (k == 0 || (MAX ? v > best : v < best)) {
+            best = v;
+            bi = k;
+          }
+        }
+      }
+    } else {
+      for (int k = 0; k < nk; ++k) {
+        const float v = base[kx[k]] + kv[k];
+        if (k == 0 || (MAX ? v > best : v < best)) {
+          best = v;
+          bi = k;
+        }
       }
     }
     as[e] = (int16_t)bi;
-    ds[e] = dyv[(int64_t)y * W + x];
+    if (!CT) ds[e] = dyv[(int64_t)y * W + x];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < kBwTY * kBwTX; e += blockDim.x) {
@@ -1385,9 +1468,19 @@ __global__ __launch_bounds__(256) void grey_bw2d_kernel(const float* __restrict_
       // interior: q is its own only preimage
       const int s0 = (r + py) * OC + (c + px);
       float acc = 0.f;
-      for (int k = 0; k < tab.n; ++k) {
-        const int s = s0 - ko[k];
-        if (as[s] == k) acc += ds[s];
+      if (CT) {
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+          if (k < nk) {
+            const int s = s0 - rko[k];
+            if (as[s] == k) acc += ds[s];
+          }
+        }
+      } else {
+        for (int k = 0; k < nk; ++k) {
+          const int s = s0 - ko[k];
+          if (as[s] == k) acc += ds[s];
+        }
       }
       dx[item * n + (int64_t)qy * W + qx] = acc;
       continue;
@@ -1395,7 +1488,7 @@ __global__ __launch_bounds__(256) void grey_bw2d_kernel(const float* __restrict_
     const uint64_t my = grey_preimages(qy, H, py, mode);
     const uint64_t mx = grey_preimages(qx, W, px, mode);
     float acc = 0.f;
-    for (int k = 0; k < tab.n; ++k) {
+    for (int k = 0; k < nk; ++k) {
       for (uint64_t wy = my; wy; wy &= wy - 1) {
         const int pr = grey_pre_cand(__builtin_ctzll(wy), qy, H, py) - tab.o[k][0] - (y0 - py);
         for (uint64_t wx = mx; wx; wx &= wx - 1) {
@@ -1480,12 +1573,18 @@ extern "C" int tm_grey_backward(const float* input, int64_t batch, int ndim, con
     const int64_t G = (int64_t)((W + kBwTX - 1) / kBwTX) * ((H + kBwTY - 1) / kBwTY) * batch;
     if (G < ((int64_t)1 << 31)) {
       const size_t lds = grey_bw2d_lds(tab);
-      if (dilation)
-        hipLaunchKernelGGL(grey_bw2d_kernel<true>, dim3((unsigned)G), dim3(256), lds, st, input,
-                           grad_out, grad_in, H, W, tab, mode, cval);
-      else
-        hipLaunchKernelGGL(grey_bw2d_kernel<false>, dim3((unsigned)G), dim3(256), lds, st, input,
-                           grad_out, grad_in, H, W, tab, mode, cval);
+      const bool e1 = ext[0] == 1 && ext[1] == 1;
+#define TM_BW(MX, EE)                                                                           \
+  hipLaunchKernelGGL((grey_bw2d_kernel<MX, EE>), dim3((unsigned)G), dim3(256), lds, st, input,     \
+                     grad_out, grad_in, H, W, tab, mode, cval)
+      if (dilation) {
+        if (e1) TM_BW(true, 1);
+        else TM_BW(true, 0);
+      } else {
+        if (e1) TM_BW(false, 1);
+        else TM_BW(false, 0);
+      }
+#undef TM_BW
       TM_LAUNCH_CHECK();
       return TM_OK;
     }
