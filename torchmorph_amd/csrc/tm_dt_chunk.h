@@ -23,36 +23,53 @@ namespace tmh {
 constexpr int kCLanes = 256;
 
 // Thread tid's share of one chunk: element q of its registers is row
-// (q * kCLanes + tid) / CH, column (q * kCLanes + tid) % CH of the chunk.
+// q * (kCLanes / CH) + tid / CH, column tid % CH of the chunk.  Addresses are
+// a workgroup-uniform base plus 32-bit lane byte offsets (a block of rows
+// spans at most kCLanes * 65535 elements), recomputed per chunk rather than
+// held in CH registers; slots outside the rows or the row length load the
+// chunk's first element and keep 0.
+__device__ __forceinline__ uint32_t chunk_lane_off(int L, int CH) {
+  uint32_t o = (uint32_t)(((int)threadIdx.x / CH) * L + (int)threadIdx.x % CH) * 4u;
+  asm volatile("" : "+v"(o));  // keeps the per-slot offsets out of loop-invariant registers
+  return o;
+}
+
 template <int CH>
 __device__ __forceinline__ void chunk_fetch(int32_t (&r)[CH], const int32_t* __restrict__ src,
                                             int64_t row0, int L, int nrows, int i0) {
+  constexpr int RPQ = kCLanes / CH;
+  const int tr = (int)threadIdx.x / CH, tk = (int)threadIdx.x % CH;
+  const char* blk = (const char*)(src + row0 * L + i0);
+  const bool colok = i0 + tk < L;
+  const uint32_t off0 = chunk_lane_off(L, CH), step = (uint32_t)(RPQ * L) * 4u;
 #pragma unroll
   for (int q = 0; q < CH; ++q) {
-    const int e = q * kCLanes + (int)threadIdx.x;
-    const int row = e / CH, k = e % CH;
-    r[q] = (row < nrows && i0 + k < L) ? src[(row0 + row) * L + i0 + k] : 0;
+    const bool ok = colok && q * RPQ + tr < nrows;
+    const int32_t v = *(const int32_t*)(blk + (ok ? off0 + (uint32_t)q * step : 0u));
+    r[q] = ok ? v : 0;
   }
 }
 
 template <int CH>
 __device__ __forceinline__ void chunk_put(const int32_t (&r)[CH], int32_t* lds) {
+  constexpr int RPQ = kCLanes / CH;
+  const int tr = (int)threadIdx.x / CH, tk = (int)threadIdx.x % CH;
 #pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    const int e = q * kCLanes + (int)threadIdx.x;
-    lds[(e / CH) * (CH + 1) + e % CH] = r[q];
-  }
+  for (int q = 0; q < CH; ++q) lds[(q * RPQ + tr) * (CH + 1) + tk] = r[q];
 }
 
 template <int CH>
 __device__ __forceinline__ void chunk_flush(const int32_t* lds, int32_t* __restrict__ dst,
                                             int64_t row0, int L, int nrows, int i0) {
+  constexpr int RPQ = kCLanes / CH;
+  const int tr = (int)threadIdx.x / CH, tk = (int)threadIdx.x % CH;
+  char* blk = (char*)(dst + row0 * L + i0);
+  const bool colok = i0 + tk < L;
+  const uint32_t off0 = chunk_lane_off(L, CH), step = (uint32_t)(RPQ * L) * 4u;
 #pragma unroll
-  for (int q = 0; q < CH; ++q) {
-    const int e = q * kCLanes + (int)threadIdx.x;
-    const int row = e / CH, k = e % CH;
-    if (row < nrows && i0 + k < L) dst[(row0 + row) * L + i0 + k] = lds[row * (CH + 1) + k];
-  }
+  for (int q = 0; q < CH; ++q)
+    if (colok && q * RPQ + tr < nrows)
+      *(int32_t*)(blk + off0 + (uint32_t)q * step) = lds[(q * RPQ + tr) * (CH + 1) + tk];
 }
 
 // The float distance edt_store writes for features f at coordinates c.
@@ -68,16 +85,70 @@ __device__ __forceinline__ float edt_dist_of(int ndim, const int64_t* c, const i
   return (float)sqrt(acc);
 }
 
+// Envelope stacks of the row passes, blocked per row: entry k of row t sits
+// at slot ((k >> 3) * nl + t) * 8 + (k & 7), so eight consecutive entries of
+// one row share 32 bytes of E (p | f0 << 16) and 16 bytes of F1 (f1, two
+// feature planes only), and the query fetches them eight at a time.  E holds
+// rows_stack_slots(L, nl) slots from a 256-byte boundary, F1 as many from the
+// next 256-byte boundary after E.  Slot numbers are 32-bit (rows_stack_fits).
+__host__ __device__ inline int64_t rows_stack_slots(int64_t L, int64_t nl) {
+  return (L + 7) / 8 * 8 * nl;
+}
+__host__ __device__ inline int64_t rows_stack_bytes(int64_t L, int64_t nl) {
+  return 256 + (rows_stack_slots(L, nl) * 4 + 255) / 256 * 256 + rows_stack_slots(L, nl) * 2;
+}
+__host__ __device__ inline bool rows_stack_fits(int64_t L, int64_t nl) {
+  return rows_stack_slots(L, nl) < ((int64_t)1 << 31);
+}
+// E and F1 for a stack area starting at `area` (any 4-byte boundary).
+__host__ __device__ inline uint32_t* rows_stack_e(void* area) {
+  return (uint32_t*)(((uintptr_t)area + 255) & ~(uintptr_t)255);
+}
+__host__ __device__ inline uint16_t* rows_stack_f1(void* area, int64_t L, int64_t nl) {
+  const uintptr_t e = (uintptr_t)rows_stack_e(area) + (uintptr_t)rows_stack_slots(L, nl) * 4;
+  return (uint16_t*)((e + 255) & ~(uintptr_t)255);
+}
+// LDS bytes of edt_rows_final<NP, CH>: the chunk staging, NP planes in, then
+// the outputs in the same space.
+template <int NP, int CH>
+constexpr size_t edt_rows_lds() {
+  return (size_t)NP * kCLanes * (CH + 1) * sizeof(int32_t);
+}
+
+// Entry j (0..7) of a fetched block of eight, chosen with a select tree so
+// that the block stays in registers.
+__device__ __forceinline__ uint32_t pick8(const uint32_t (&w)[8], int j) {
+  const bool b0 = (j & 1) != 0, b1 = (j & 2) != 0, b2 = (j & 4) != 0;
+  const uint32_t x0 = b0 ? w[1] : w[0], x1 = b0 ? w[3] : w[2];
+  const uint32_t x2 = b0 ? w[5] : w[4], x3 = b0 ? w[7] : w[6];
+  const uint32_t y0 = b1 ? x1 : x0, y1 = b1 ? x3 : x2;
+  return b2 ? y1 : y0;
+}
+__device__ __forceinline__ uint32_t pick8h(const uint32_t (&f)[4], int j) {
+  const bool b1 = (j & 2) != 0, b2 = (j & 4) != 0;
+  const uint32_t y0 = b1 ? f[1] : f[0], y1 = b1 ? f[3] : f[2];
+  const uint32_t v = b2 ? y1 : y0;
+  return (j & 1) ? v >> 16 : v & 0xffffu;
+}
+
 // EDT, last pass (d = ndim - 1, NP = d feature planes, 1 or 2), distances
 // only.  fin: the item's feature planes (plane stride `plane`); rows
-// row0 .. row0 + nrows - 1; the compact stack of edt_line_stack_c for line t
-// at E/F1 + t, stride nl_axis.  lds: (NP + 1) * kCLanes * (CH + 1) ints.
+// row0 .. row0 + nrows - 1 of the nl rows whose stacks E / F1 hold (blocked
+// layout above).  lds: edt_rows_lds<NP, CH>() bytes.
+//
+// The envelope is the one of edt_line_stack_c, entry for entry: the top two
+// entries live in registers, every deeper entry is written once when it is
+// buried, and the R most recently buried ones also stay in registers, so a
+// pop (rarely more than two deep) reads memory only past those.  The query
+// walks the stack with the block of eight holding its next entry in
+// registers and the block after it already in flight.
 template <int NP, int CH>
 __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, const Shape& sh,
-                               int64_t row0, int nrows, const Samp& sp, uint32_t* __restrict__ Eall,
-                               uint16_t* __restrict__ Fall, int64_t nl_axis,
+                               int64_t row0, int nrows, const Samp& sp, uint32_t* __restrict__ E,
+                               uint16_t* __restrict__ F1, int64_t nl,
                                float* __restrict__ dist, int32_t* lds) {
   constexpr int RS = CH + 1;
+  constexpr int R = 4;
   const int d = sh.ndim - 1;
   const int L = (int)sh.size[d];
   const int lane = threadIdx.x;
@@ -91,15 +162,32 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
   const int32_t c0 = (int32_t)c[0], c1 = (int32_t)c[1];
   int32_t* in0 = lds;
   int32_t* in1 = lds + kCLanes * RS;
-  int32_t* ob = lds + NP * kCLanes * RS;
-  uint32_t* E = Eall + t;
-  uint16_t* F1 = Fall + t;
-  const int64_t ss = nl_axis;
+  const uint32_t nl8 = (uint32_t)nl * 8u, t8 = (uint32_t)t * 8u;
   const int nch = (L + CH - 1) / CH;
+  auto slot = [&](int k) -> uint32_t {
+    return (uint32_t)(k >> 3) * nl8 + t8 + (uint32_t)(k & 7);
+  };
+  auto site_of = [&](uint32_t w, uint32_t f) -> EdtSite {
+    EdtSite e;
+    e.p = (int32_t)(w & 0xffffu);
+    e.f0 = (int32_t)(w >> 16);
+    e.f1 = NP == 2 ? (int32_t)f : 0;
+    e.h = edt_partial<NP>(e.f0, e.f1, c0, c1, s0, s1);
+    return e;
+  };
+  auto put = [&](int k, const EdtSite& e) {
+    const uint32_t s = slot(k);
+    E[s] = (uint32_t)(uint16_t)e.p | ((uint32_t)(uint16_t)e.f0 << 16);
+    if (NP == 2) F1[s] = (uint16_t)e.f1;
+  };
 
-  // build: q1 = entry top, q0 = entry top - 1; memory holds 0 .. top - 2
-  int top = -1;
+  // build: q1 = entry top, q0 = entry top - 1; memory holds 0 .. top - 2,
+  // g[0 .. nv - 1] = entries top - 2, top - 3, ... (raw)
+  int top = -1, nv = 0;
   EdtSite q1{0, 0, 0, 0.0}, q0{0, 0, 0, 0.0};
+  uint32_t g[R], gf[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) g[j] = gf[j] = 0;
   int32_t r0[CH], r1[CH];
   chunk_fetch<CH>(r0, fin, row0, L, nrows, 0);
   if (NP == 2) chunk_fetch<CH>(r1, fin + plane, row0, L, nrows, 0);
@@ -127,30 +215,73 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
           if (cc * q1.h - bb * q0.h - a * wR - a * bb * cc <= 0.0) break;
           --top;
           q1 = q0;
-          if (top >= 1) q0 = edt_site_get<NP>(E, F1, top - 1, ss, c0, c1, s0, s1);
+          if (top >= 1) {
+            uint32_t w, wf = 0;
+            if (nv > 0) {
+              w = g[0];
+              wf = gf[0];
+#pragma unroll
+              for (int j = 0; j + 1 < R; ++j) {
+                g[j] = g[j + 1];
+                gf[j] = gf[j + 1];
+              }
+              --nv;
+            } else {
+              const uint32_t s = slot(top - 1);
+              w = E[s];
+              if (NP == 2) wf = F1[s];
+            }
+            q0 = site_of(w, wf);
+          }
         }
-        if (top >= 1) edt_site_put<NP>(E, F1, top - 1, ss, q0);
+        if (top >= 1) {
+          put(top - 1, q0);
+#pragma unroll
+          for (int j = R - 1; j > 0; --j) {
+            g[j] = g[j - 1];
+            gf[j] = gf[j - 1];
+          }
+          g[0] = (uint32_t)(uint16_t)q0.p | ((uint32_t)(uint16_t)q0.f0 << 16);
+          gf[0] = (uint32_t)(uint16_t)q0.f1;
+          nv = nv < R ? nv + 1 : R;
+        }
         ++top;
         q0 = q1;
         q1 = EdtSite{i, f0, f1, wR};
       }
     }
   }
-  if (active && top >= 1) edt_site_put<NP>(E, F1, top - 1, ss, q0);
-  if (active && top >= 0) edt_site_put<NP>(E, F1, top, ss, q1);
+  if (active && top >= 1) put(top - 1, q0);
+  if (active && top >= 0) put(top, q1);
 
-  // query: cur = site l, nx = site l + 1, nn = raw entry l + 2 (loaded ahead)
-  int l = 0;
-  EdtSite cur{0, 0, 0, 0.0}, nx{0, 0, 0, 0.0};
-  uint32_t nnE = 0;
-  uint16_t nnF = 0;
-  if (active && top >= 0) {
-    cur = edt_site_get<NP>(E, F1, 0, ss, c0, c1, s0, s1);
-    if (top >= 1) nx = edt_site_get<NP>(E, F1, 1, ss, c0, c1, s0, s1);
-    if (top >= 2) {
-      nnE = E[2 * ss];
-      if (NP == 2) nnF = F1[2 * ss];
+  // query: cur = entry l, nx = entry l + 1, n = l + 2 the next entry to
+  // decode; wa = block n >> 3, wb = the block after it (loaded when it holds
+  // entries)
+  __syncthreads();  // the staging area holds outputs from here on
+  int32_t* ob = lds;
+  auto fetch = [&](int blk, uint32_t (&w)[8], uint32_t (&f)[4]) {
+    const uint32_t s = (uint32_t)blk * nl8 + t8;
+    const uint4 x = *(const uint4*)(E + s);
+    const uint4 y = *(const uint4*)(E + s + 4);
+    w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
+    w[4] = y.x; w[5] = y.y; w[6] = y.z; w[7] = y.w;
+    if (NP == 2) {
+      const uint4 z = *(const uint4*)(F1 + s);
+      f[0] = z.x; f[1] = z.y; f[2] = z.z; f[3] = z.w;
     }
+  };
+  int l = 0, n = 2;
+  EdtSite cur{0, 0, 0, 0.0}, nx{0, 0, 0, 0.0};
+  uint32_t wa[8], wb[8], fa[4], fb[4];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) wa[j] = wb[j] = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) fa[j] = fb[j] = 0;
+  if (active && top >= 0) {
+    fetch(0, wa, fa);
+    if (top >= 8) fetch(1, wb, fb);
+    cur = site_of(wa[0], fa[0] & 0xffffu);
+    if (top >= 1) nx = site_of(wa[1], fa[0] >> 16);
   }
   int32_t fnone[kMaxNdim];  // no site on the line: feature (-1, 0, ..., 0)
   for (int j = 0; j < kMaxNdim; ++j) fnone[j] = 0;
@@ -175,13 +306,15 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
             ++l;
             cur = nx;
             if (l < top) {
-              nx.p = (int32_t)(nnE & 0xffffu);
-              nx.f0 = (int32_t)(nnE >> 16);
-              nx.f1 = (int32_t)nnF;
-              nx.h = edt_partial<NP>(nx.f0, nx.f1, c0, c1, s0, s1);
-              if (l + 2 <= top) {
-                nnE = E[(int64_t)(l + 2) * ss];
-                if (NP == 2) nnF = F1[(int64_t)(l + 2) * ss];
+              const int j = n & 7;
+              nx = site_of(pick8(wa, j), NP == 2 ? pick8h(fa, j) : 0u);
+              ++n;
+              if ((n & 7) == 0) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) wa[e] = wb[e];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) fa[e] = fb[e];
+                if ((n >> 3) + 1 <= (top >> 3)) fetch((n >> 3) + 1, wb, fb);
               }
             }
           }

@@ -340,14 +340,22 @@ template <int NP, int CH>
 __global__ __launch_bounds__(kCLanes) void edt_rows_kernel(const int32_t* __restrict__ fin,
                                                            int64_t plane, Shape sh,
                                                            int64_t nlines, Samp sp,
-                                                           int32_t* __restrict__ gpos,
-                                                           double* __restrict__ gh,
+                                                           uint32_t* __restrict__ E,
+                                                           uint16_t* __restrict__ F1,
                                                            float* __restrict__ dist) {
   extern __shared__ int32_t lds_rows[];
   const int64_t row0 = (int64_t)xcd_swizzle(blockIdx.x, gridDim.x) * kCLanes;
   const int nrows = (int)(nlines - row0 < kCLanes ? nlines - row0 : kCLanes);
-  edt_rows_final<NP, CH>(fin, plane, sh, row0, nrows, sp, (uint32_t*)gpos, (uint16_t*)gh,
-                         nlines, dist, lds_rows);
+  edt_rows_final<NP, CH>(fin, plane, sh, row0, nrows, sp, E, F1, nlines, dist, lds_rows);
+}
+
+// Whether the last pass of a distances-only call may run row-staged: 2 or 3
+// axes and blocked stacks with 32-bit slot numbers.
+static bool rows_path_fits(int64_t batch, int ndim, const int64_t* shape) {
+  if (ndim != 2 && ndim != 3) return false;
+  int64_t nl = batch;
+  for (int j = 0; j + 1 < ndim; ++j) nl *= shape[j];
+  return rows_stack_fits(shape[ndim - 1], nl);
 }
 
 static bool unit_sampling(int ndim, const double* s) {
@@ -449,6 +457,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   int32_t* gpos = bufB + (size_t)nbuf * N;
   double* gh = (double*)(((uintptr_t)(gpos + N) + 255) & ~(uintptr_t)255);
 
+  const bool rows_ok = !o.idx && rows_path_fits(batch, ndim, shape);
   // launch variant v (one choice for the whole call):
   //   v & 3   where each envelope stack lives and the workgroup size:
   //           0: LDS (128 deep, 80 KB per 64 lanes) when the line fits, else
@@ -460,7 +469,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   auto run = [&](int v) -> int {
     const int w = v & 3;
     const bool pf16 = ((v >> 2) & 1) != 0;
-    const bool rows = (v >> 3) != 0 && !o.idx && (ndim == 2 || ndim == 3);
+    const bool rows = (v >> 3) != 0 && rows_ok;
     {
       const int64_t nlines = batch * sh.stride[0];
       const dim3 g(grid_for(nlines, 256)), blk(256);
@@ -490,14 +499,14 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
       }
       if (fin_pass && rows) {
         const dim3 g(grid_for(nlines, kCLanes)), blk(kCLanes);
+        uint32_t* E = rows_stack_e(gpos);
+        uint16_t* F1 = rows_stack_f1(gpos, L, nlines);
         if (ndim == 2)
-          hipLaunchKernelGGL((edt_rows_kernel<1, 32>), g, blk,
-                             (size_t)2 * kCLanes * 33 * sizeof(int32_t), st, fin, N, sh, nlines,
-                             sp, gpos, gh, o.dist);
+          hipLaunchKernelGGL((edt_rows_kernel<1, 32>), g, blk, (edt_rows_lds<1, 32>()), st, fin,
+                             N, sh, nlines, sp, E, F1, o.dist);
         else
-          hipLaunchKernelGGL((edt_rows_kernel<2, 16>), g, blk,
-                             (size_t)3 * kCLanes * 17 * sizeof(int32_t), st, fin, N, sh, nlines,
-                             sp, gpos, gh, o.dist);
+          hipLaunchKernelGGL((edt_rows_kernel<2, 16>), g, blk, (edt_rows_lds<2, 16>()), st, fin,
+                             N, sh, nlines, sp, E, F1, o.dist);
         TM_LAUNCH_CHECK();
         break;
       }
@@ -562,7 +571,12 @@ extern "C" size_t tm_edt_workspace_bytes(int64_t batch, int ndim, const int64_t*
   for (int j = 0; j < ndim; ++j) n *= shape[j];
   if (tmh::use_2d_path(ndim, shape, sampling)) return (size_t)n * sizeof(int16_t);
   const int nbuf = ndim > 1 ? ndim - 1 : 1;
-  return (size_t)n * 4 * (2 * nbuf + 1) + 256 + (size_t)n * 8;
+  size_t b = (size_t)n * 4 * (2 * nbuf + 1) + 256 + (size_t)n * 8;
+  if (tmh::rows_path_fits(batch, ndim, shape)) {  // blocked stacks of the row-staged last pass
+    const int64_t L = shape[ndim - 1];
+    b = std::max(b, (size_t)n * 8 * nbuf + (size_t)tmh::rows_stack_bytes(L, n / L));
+  }
+  return b;
 }
 
 extern "C" int tm_edt(const void* input, int input_dtype, int64_t batch, int ndim,

@@ -249,7 +249,8 @@ __global__ __launch_bounds__(kRgThreads) void rg_kernel(const TIn* __restrict__ 
   }
 }
 
-// Schedule 2: one launch per (level d, kind); lane = line.
+// Schedule 2: one launch per (level d, kind); lane = line.  The block table
+// of a launch lists the longest lines first and is read in block id order.
 template <typename TIn, int KIND, int BLOCK, int LDS_DEPTH>
 __global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__ in_all,
                                                          RgArgs a, const RgBlk* __restrict__ blks,
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__
   // LDS stack (jobs of up to 3 axes): compact entries of edt_line_stack_c
   __shared__ uint32_t lE[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
   __shared__ uint16_t lF[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
-  const RgBlk bk = blks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const RgBlk bk = blks[blockIdx.x];
   if ((int)threadIdx.x >= bk.nlines) return;
   const RgJob& jb = a.jobs[bk.job];
   const Shape& sh = jb.sh;
@@ -365,7 +366,7 @@ template <int KIND>
 __global__ __launch_bounds__(kCLanes) void rg_rows_kernel(RgArgs a,
                                                           const RgBlk* __restrict__ blks) {
   extern __shared__ int32_t lds_rows[];
-  const RgBlk bk = blks[xcd_swizzle(blockIdx.x, gridDim.x)];
+  const RgBlk bk = blks[blockIdx.x];  // longest rows first
   const RgJob& jb = a.jobs[bk.job];
   const Shape& sh = jb.sh;
   char* wk = a.work + jb.ws_off;
@@ -377,15 +378,15 @@ __global__ __launch_bounds__(kCLanes) void rg_rows_kernel(RgArgs a,
     int32_t* A = (int32_t*)wk;
     int32_t* B = A + (int64_t)nbuf * numel;
     const int32_t* src = ((nd - 1) & 1) ? A : B;
-    int32_t* gpos = B + (int64_t)nbuf * numel;
-    double* gh = (double*)(((uintptr_t)(gpos + numel) + 255) & ~(uintptr_t)255);
+    int32_t* area = B + (int64_t)nbuf * numel;
+    const int64_t L = sh.size[nd - 1];
+    uint32_t* E = rows_stack_e(area);
+    uint16_t* F1 = rows_stack_f1(area, L, nl_axis);
     if (KIND == RG_EDT_ROWS2)
-      edt_rows_final<1, 32>(src, numel, sh, bk.line0, bk.nlines, jb.sp, (uint32_t*)gpos,
-                            (uint16_t*)gh, nl_axis,
+      edt_rows_final<1, 32>(src, numel, sh, bk.line0, bk.nlines, jb.sp, E, F1, nl_axis,
                             a.dist + jb.out_off, lds_rows);
     else
-      edt_rows_final<2, 16>(src, numel, sh, bk.line0, bk.nlines, jb.sp, (uint32_t*)gpos,
-                            (uint16_t*)gh, nl_axis,
+      edt_rows_final<2, 16>(src, numel, sh, bk.line0, bk.nlines, jb.sp, E, F1, nl_axis,
                             a.dist + jb.out_off, lds_rows);
   } else {
     cdt_rows_taxicab<32>((int32_t*)(a.dist + jb.out_off), sh, bk.line0, bk.nlines, lds_rows);
@@ -394,8 +395,8 @@ __global__ __launch_bounds__(kCLanes) void rg_rows_kernel(RgArgs a,
 
 // LDS bytes of rg_rows_kernel<KIND>.
 static size_t rows_lds(int kind) {
-  if (kind == RG_EDT_ROWS2) return (size_t)2 * kCLanes * 33 * sizeof(int32_t);
-  if (kind == RG_EDT_ROWS3) return (size_t)3 * kCLanes * 17 * sizeof(int32_t);
+  if (kind == RG_EDT_ROWS2) return edt_rows_lds<1, 32>();
+  if (kind == RG_EDT_ROWS3) return edt_rows_lds<2, 16>();
   return (size_t)kCLanes * 33 * sizeof(int32_t);
 }
 
@@ -439,13 +440,26 @@ struct RgPlan {
   size_t ctr_bytes, jobs_off, tasks_off, blks_off, work_off, total;
 };
 
+// Whether the job's last EDT pass runs row-staged (rg_rows_kernel).
+static bool rows_final(const RgJob& j) {
+  const int nd = j.sh.ndim;
+  if (j.op != TM_RG_EDT || j.raster || j.idx_off >= 0 || nd < 2 || nd > 3) return false;
+  const int64_t L = j.sh.size[nd - 1];
+  return rows_stack_fits(L, j.sh.numel / L);
+}
+
 static size_t job_work_bytes(const RgJob& j) {
   const int64_t n = j.sh.numel;
   const int nd = j.sh.ndim;
   if (j.op == TM_RG_EDT) {
     // feature planes A and B (ndim-1 each, one for 1-D), stack positions, stack values
     const int nbuf = nd > 1 ? nd - 1 : 1;
-    return align256((size_t)n * 4 * (2 * nbuf + 1)) + (size_t)n * 8;
+    size_t b = align256((size_t)n * 4 * (2 * nbuf + 1)) + (size_t)n * 8;
+    if (rows_final(j)) {  // the blocked stacks of edt_rows_final after the planes
+      const int64_t L = j.sh.size[nd - 1];
+      b = std::max(b, (size_t)n * 8 * nbuf + (size_t)rows_stack_bytes(L, n / L));
+    }
+    return b;
   }
   if (j.raster) return (size_t)n * 8;
   return j.op == TM_RG_CDT_CHESSBOARD ? (size_t)n * 8 : 0;
@@ -583,7 +597,7 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
         int grp;
         if (j.op == TM_RG_EDT) {
           if (lev == 0) grp = 0;
-          else if (last && j.idx_off < 0 && j.sh.ndim <= 3) grp = j.sh.ndim == 2 ? 7 : 8;
+          else if (last && rows_final(j)) grp = j.sh.ndim == 2 ? 7 : 8;
           else grp = (L <= kRgLdsDepth && j.sh.ndim <= 3) ? 1 : 2;
         } else if (lev == 0) {
           grp = 3;
@@ -604,6 +618,16 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
           p.blks.push_back(RgBlk{ji, (int32_t)std::min(per, nl - l0), l0});
       }
       ln.hi = (int)p.blks.size();
+      if (g != 5 && g != 6) {
+        // a lane walks its whole line: longest lines first, so that the
+        // longest blocks start in the first round of workgroups (launched
+        // without the XCD swizzle, block id order = dispatch order)
+        std::stable_sort(p.blks.begin() + ln.lo, p.blks.begin() + ln.hi,
+                         [&](const RgBlk& x, const RgBlk& y) {
+                           return p.jobs[(size_t)x.job].sh.size[lev] >
+                                  p.jobs[(size_t)y.job].sh.size[lev];
+                         });
+      }
       static const int kinds[10] = {RG_EDT0,       RG_EDT_STACK,       RG_EDT_STACK,
                                     RG_CDT0,       RG_CDT_TAXICAB,     RG_CDT_CHEB_WINDOW,
                                     RG_CDT_RASTER, RG_EDT_ROWS2,       RG_EDT_ROWS3,
