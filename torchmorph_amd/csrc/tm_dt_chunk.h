@@ -26,8 +26,9 @@ constexpr int kCLanes = 256;
 // q * (kCLanes / CH) + tid / CH, column tid % CH of the chunk.  Addresses are
 // a workgroup-uniform base plus 32-bit lane byte offsets (a block of rows
 // spans at most kCLanes * 65535 elements), recomputed per chunk rather than
-// held in CH registers; slots outside the rows or the row length load the
-// chunk's first element and keep 0.
+// held in CH registers.  Slots outside the rows or the row length load the
+// chunk's first element, which no lane reads: the loaded registers are left
+// untouched until chunk_put, so the loads stay in flight meanwhile.
 __device__ __forceinline__ uint32_t chunk_lane_off(int L, int CH) {
   uint32_t o = (uint32_t)(((int)threadIdx.x / CH) * L + (int)threadIdx.x % CH) * 4u;
   asm volatile("" : "+v"(o));  // keeps the per-slot offsets out of loop-invariant registers
@@ -45,8 +46,7 @@ __device__ __forceinline__ void chunk_fetch(int32_t (&r)[CH], const int32_t* __r
 #pragma unroll
   for (int q = 0; q < CH; ++q) {
     const bool ok = colok && q * RPQ + tr < nrows;
-    const int32_t v = *(const int32_t*)(blk + (ok ? off0 + (uint32_t)q * step : 0u));
-    r[q] = ok ? v : 0;
+    r[q] = *(const int32_t*)(blk + (ok ? off0 + (uint32_t)q * step : 0u));
   }
 }
 
@@ -100,13 +100,14 @@ __host__ __device__ inline int64_t rows_stack_bytes(int64_t L, int64_t nl) {
 __host__ __device__ inline bool rows_stack_fits(int64_t L, int64_t nl) {
   return rows_stack_slots(L, nl) < ((int64_t)1 << 31);
 }
-// E and F1 for a stack area starting at `area` (any 4-byte boundary).
+// E and F1 for a stack area starting at `area` (any 4-byte boundary); plain
+// pointer offsets, so the global address space carries over.
 __host__ __device__ inline uint32_t* rows_stack_e(void* area) {
-  return (uint32_t*)(((uintptr_t)area + 255) & ~(uintptr_t)255);
+  return (uint32_t*)((char*)area + ((256 - ((uintptr_t)area & 255)) & 255));
 }
 __host__ __device__ inline uint16_t* rows_stack_f1(void* area, int64_t L, int64_t nl) {
-  const uintptr_t e = (uintptr_t)rows_stack_e(area) + (uintptr_t)rows_stack_slots(L, nl) * 4;
-  return (uint16_t*)((e + 255) & ~(uintptr_t)255);
+  char* e = (char*)rows_stack_e(area) + rows_stack_slots(L, nl) * 4;
+  return (uint16_t*)(e + ((256 - ((uintptr_t)e & 255)) & 255));
 }
 // LDS bytes of edt_rows_final<NP, CH>: the chunk staging, NP planes in, then
 // the outputs in the same space.
@@ -115,19 +116,38 @@ constexpr size_t edt_rows_lds() {
   return (size_t)NP * kCLanes * (CH + 1) * sizeof(int32_t);
 }
 
-// Entry j (0..7) of a fetched block of eight, chosen with a select tree so
-// that the block stays in registers.
-__device__ __forceinline__ uint32_t pick8(const uint32_t (&w)[8], int j) {
-  const bool b0 = (j & 1) != 0, b1 = (j & 2) != 0, b2 = (j & 4) != 0;
-  const uint32_t x0 = b0 ? w[1] : w[0], x1 = b0 ? w[3] : w[2];
-  const uint32_t x2 = b0 ? w[5] : w[4], x3 = b0 ? w[7] : w[6];
-  const uint32_t y0 = b1 ? x1 : x0, y1 = b1 ? x3 : x2;
-  return b2 ? y1 : y0;
+// One block of eight stack entries of a row in registers: e* from E, f*
+// from F1 (two entries per word).  Named scalars rather than arrays, so that
+// picking entry j stays a select tree on registers.
+struct StkBlk {
+  uint32_t e0, e1, e2, e3, e4, e5, e6, e7, f0, f1, f2, f3;
+};
+
+template <int NP>
+__device__ __forceinline__ void stk_fetch(StkBlk& b, const uint32_t* __restrict__ E,
+                                          const uint16_t* __restrict__ F1, uint32_t s) {
+  const uint4 x = *(const uint4*)(E + s);
+  const uint4 y = *(const uint4*)(E + s + 4);
+  b.e0 = x.x; b.e1 = x.y; b.e2 = x.z; b.e3 = x.w;
+  b.e4 = y.x; b.e5 = y.y; b.e6 = y.z; b.e7 = y.w;
+  if (NP == 2) {
+    const uint4 z = *(const uint4*)(F1 + s);
+    b.f0 = z.x; b.f1 = z.y; b.f2 = z.z; b.f3 = z.w;
+  }
 }
-__device__ __forceinline__ uint32_t pick8h(const uint32_t (&f)[4], int j) {
-  const bool b1 = (j & 2) != 0, b2 = (j & 4) != 0;
-  const uint32_t y0 = b1 ? f[1] : f[0], y1 = b1 ? f[3] : f[2];
-  const uint32_t v = b2 ? y1 : y0;
+
+__device__ __forceinline__ uint32_t stk_pick_e(const StkBlk& b, int j) {
+  const bool s0 = (j & 1) != 0, s1 = (j & 2) != 0, s2 = (j & 4) != 0;
+  const uint32_t x0 = s0 ? b.e1 : b.e0, x1 = s0 ? b.e3 : b.e2;
+  const uint32_t x2 = s0 ? b.e5 : b.e4, x3 = s0 ? b.e7 : b.e6;
+  const uint32_t y0 = s1 ? x1 : x0, y1 = s1 ? x3 : x2;
+  return s2 ? y1 : y0;
+}
+
+__device__ __forceinline__ uint32_t stk_pick_f(const StkBlk& b, int j) {
+  const bool s1 = (j & 2) != 0, s2 = (j & 4) != 0;
+  const uint32_t y0 = s1 ? b.f1 : b.f0, y1 = s1 ? b.f3 : b.f2;
+  const uint32_t v = s2 ? y1 : y0;
   return (j & 1) ? v >> 16 : v & 0xffffu;
 }
 
@@ -182,12 +202,10 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
   };
 
   // build: q1 = entry top, q0 = entry top - 1; memory holds 0 .. top - 2,
-  // g[0 .. nv - 1] = entries top - 2, top - 3, ... (raw)
+  // g0 .. g{nv - 1} = entries top - 2, top - 3, ... (raw, f1 in h*)
   int top = -1, nv = 0;
   EdtSite q1{0, 0, 0, 0.0}, q0{0, 0, 0, 0.0};
-  uint32_t g[R], gf[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) g[j] = gf[j] = 0;
+  uint32_t g0 = 0, g1 = 0, g2 = 0, g3 = 0, h0 = 0, h1 = 0, h2 = 0, h3 = 0;
   int32_t r0[CH], r1[CH];
   chunk_fetch<CH>(r0, fin, row0, L, nrows, 0);
   if (NP == 2) chunk_fetch<CH>(r1, fin + plane, row0, L, nrows, 0);
@@ -218,31 +236,27 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
           if (top >= 1) {
             uint32_t w, wf = 0;
             if (nv > 0) {
-              w = g[0];
-              wf = gf[0];
-#pragma unroll
-              for (int j = 0; j + 1 < R; ++j) {
-                g[j] = g[j + 1];
-                gf[j] = gf[j + 1];
-              }
+              w = g0;
+              wf = h0;
+              g0 = g1; g1 = g2; g2 = g3;
+              h0 = h1; h1 = h2; h2 = h3;
               --nv;
             } else {
               const uint32_t s = slot(top - 1);
               w = E[s];
               if (NP == 2) wf = F1[s];
+              // wait for these loads here, not wherever w is next written
+              asm volatile("" : "+v"(w), "+v"(wf));
             }
             q0 = site_of(w, wf);
           }
         }
         if (top >= 1) {
           put(top - 1, q0);
-#pragma unroll
-          for (int j = R - 1; j > 0; --j) {
-            g[j] = g[j - 1];
-            gf[j] = gf[j - 1];
-          }
-          g[0] = (uint32_t)(uint16_t)q0.p | ((uint32_t)(uint16_t)q0.f0 << 16);
-          gf[0] = (uint32_t)(uint16_t)q0.f1;
+          g3 = g2; g2 = g1; g1 = g0;
+          h3 = h2; h2 = h1; h1 = h0;
+          g0 = (uint32_t)(uint16_t)q0.p | ((uint32_t)(uint16_t)q0.f0 << 16);
+          h0 = (uint32_t)(uint16_t)q0.f1;
           nv = nv < R ? nv + 1 : R;
         }
         ++top;
@@ -259,29 +273,14 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
   // entries)
   __syncthreads();  // the staging area holds outputs from here on
   int32_t* ob = lds;
-  auto fetch = [&](int blk, uint32_t (&w)[8], uint32_t (&f)[4]) {
-    const uint32_t s = (uint32_t)blk * nl8 + t8;
-    const uint4 x = *(const uint4*)(E + s);
-    const uint4 y = *(const uint4*)(E + s + 4);
-    w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
-    w[4] = y.x; w[5] = y.y; w[6] = y.z; w[7] = y.w;
-    if (NP == 2) {
-      const uint4 z = *(const uint4*)(F1 + s);
-      f[0] = z.x; f[1] = z.y; f[2] = z.z; f[3] = z.w;
-    }
-  };
   int l = 0, n = 2;
   EdtSite cur{0, 0, 0, 0.0}, nx{0, 0, 0, 0.0};
-  uint32_t wa[8], wb[8], fa[4], fb[4];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) wa[j] = wb[j] = 0;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) fa[j] = fb[j] = 0;
+  StkBlk wa{}, wb{};
   if (active && top >= 0) {
-    fetch(0, wa, fa);
-    if (top >= 8) fetch(1, wb, fb);
-    cur = site_of(wa[0], fa[0] & 0xffffu);
-    if (top >= 1) nx = site_of(wa[1], fa[0] >> 16);
+    stk_fetch<NP>(wa, E, F1, t8);
+    if (top >= 8) stk_fetch<NP>(wb, E, F1, nl8 + t8);
+    cur = site_of(wa.e0, wa.f0 & 0xffffu);
+    if (top >= 1) nx = site_of(wa.e1, wa.f0 >> 16);
   }
   int32_t fnone[kMaxNdim];  // no site on the line: feature (-1, 0, ..., 0)
   for (int j = 0; j < kMaxNdim; ++j) fnone[j] = 0;
@@ -307,14 +306,12 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
             cur = nx;
             if (l < top) {
               const int j = n & 7;
-              nx = site_of(pick8(wa, j), NP == 2 ? pick8h(fa, j) : 0u);
+              nx = site_of(stk_pick_e(wa, j), NP == 2 ? stk_pick_f(wa, j) : 0u);
               ++n;
               if ((n & 7) == 0) {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) wa[e] = wb[e];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) fa[e] = fb[e];
-                if ((n >> 3) + 1 <= (top >> 3)) fetch((n >> 3) + 1, wb, fb);
+                wa = wb;
+                if ((n >> 3) + 1 <= (top >> 3))
+                  stk_fetch<NP>(wb, E, F1, (uint32_t)((n >> 3) + 1) * nl8 + t8);
               }
             }
           }
