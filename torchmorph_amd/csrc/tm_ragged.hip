@@ -50,6 +50,7 @@
 #include <vector>
 
 #include "tm_cdt_dev.h"
+#include "tm_dt_axis0.h"
 #include "tm_dt_chunk.h"
 #include "tm_dt_fast.h"
 #include "tm_dt_lines.h"
@@ -83,6 +84,7 @@ struct RgJob {
   int64_t out_off;  // elements of the distances, -1 = none
   int64_t idx_off;  // elements of the indices (ndim planes of numel), -1 = none
   int64_t ws_off;   // bytes into the work area
+  int64_t w0_off;   // bytes into the job's work: axis-0 chunk words (schedule 2)
   int32_t op;       // TM_RG_*
   int32_t raster;   // chamfer with indices: raster passes
 };
@@ -279,8 +281,9 @@ __global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__
     int32_t* A = (int32_t*)wk;
     int32_t* B = A + (int64_t)nbuf * numel;
     if (KIND == RG_EDT0) {
-      if (fin) edt_line_axis0<TIn, true>(in, A, sh, t, jb.sp, o);
-      else edt_line_axis0<TIn, false>(in, A, sh, t, jb.sp, o);
+      uint32_t* words = (uint32_t*)(wk + jb.w0_off);
+      if (fin) edt_axis0_words<TIn, true>(in, A, sh, t, jb.sp, o, words);
+      else edt_axis0_words<TIn, false>(in, A, sh, t, jb.sp, o, words);
     } else {
       const int32_t* src = (d & 1) ? A : B;
       int32_t* dst = (d & 1) ? B : A;
@@ -315,8 +318,9 @@ __global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__
     decode_line(sh, d, t, c, &base, &b);
     const int64_t step = sh.stride[d], L = sh.size[d];
     if (KIND == RG_CDT0) {
-      if (fin) cdt_line_axis0<TIn, true>(in, w, base, step, L);
-      else cdt_line_axis0<TIn, false>(in, w, base, step, L);
+      uint32_t* words = (uint32_t*)(wk + jb.w0_off);
+      if (fin) cdt_axis0_words<TIn, true>(in, w, base, step, (int)L, t, nl_axis, words);
+      else cdt_axis0_words<TIn, false>(in, w, base, step, (int)L, t, nl_axis, words);
     } else if (KIND == RG_CDT_TAXICAB) {
       if (fin) cdt_line_taxicab<true>(w, base, step, L);
       else cdt_line_taxicab<false>(w, base, step, L);
@@ -448,7 +452,7 @@ static bool rows_final(const RgJob& j) {
   return rows_stack_fits(L, j.sh.numel / L);
 }
 
-static size_t job_work_bytes(const RgJob& j) {
+static size_t job_pass_bytes(const RgJob& j) {
   const int64_t n = j.sh.numel;
   const int nd = j.sh.ndim;
   if (j.op == TM_RG_EDT) {
@@ -463,6 +467,13 @@ static size_t job_work_bytes(const RgJob& j) {
   }
   if (j.raster) return (size_t)n * 8;
   return j.op == TM_RG_CDT_CHESSBOARD ? (size_t)n * 8 : 0;
+}
+
+// The pass buffers, then the chunk words of the first pass (tm_dt_axis0.h).
+static size_t job_work_bytes(const RgJob& j) {
+  if (j.raster) return job_pass_bytes(j);
+  return align256(job_pass_bytes(j)) +
+         (size_t)axis0_words_bytes(j.sh.size[0], j.sh.numel / j.sh.size[0]);
 }
 
 static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bool want_idx,
@@ -507,6 +518,7 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
     j.op = q.op;
     j.raster = (q.op != TM_RG_EDT && x) ? 1 : 0;
     j.ws_off = (int64_t)work;
+    j.w0_off = (int64_t)align256(job_pass_bytes(j));
     work += align256(job_work_bytes(j));
   }
 
