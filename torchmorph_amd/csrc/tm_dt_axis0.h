@@ -1,5 +1,7 @@
-// First pass (axis 0) of the EDT and of the chamfer transforms, lane = line:
-// for every voxel the nearest background at or before it and at or after it.
+// Lane-per-line passes along strided axes with batched loads.  First pass
+// (axis 0) of the EDT and of the chamfer transforms: for every voxel the
+// nearest background at or before it and at or after it; and the taxicab
+// pass along later strided axes (cdt_taxicab_chunked, at the end).
 //
 // Two sweeps over chunks of 32 voxels.  The forward sweep reads the mask once
 // (a chunk's 32 loads in flight together) and keeps, per chunk, its background
@@ -117,6 +119,49 @@ __device__ __forceinline__ void cdt_axis0_words(const TIn* __restrict__ in,
                   if (nx >= 0 && nx - i < a) a = nx - i;
                   w[v] = FINAL ? cdt_final_bits(a) : a;
                 });
+}
+
+// Taxicab pass along a strided axis d >= 1, in place (cdt_line_taxicab's
+// forward and backward sweeps, same values), in chunks of 16 voxels whose
+// loads are issued together: a lane waits once per chunk instead of once per
+// voxel.
+template <bool FINAL>
+__device__ __forceinline__ void cdt_taxicab_chunked(int32_t* __restrict__ w, int64_t base,
+                                                    int64_t step, int L) {
+  constexpr int K = 16;
+  int32_t* p = w + base;
+  int32_t prev = kCdtInf;
+  for (int i0 = 0; i0 < L; i0 += K) {
+    const int n = L - i0 < K ? L - i0 : K;
+    int32_t a[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) a[k] = k < n ? p[(int64_t)(i0 + k) * step] : 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (k >= n) break;
+      if (prev + 1 < a[k]) a[k] = prev + 1;
+      prev = a[k];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (k < n) p[(int64_t)(i0 + k) * step] = a[k];
+  }
+  int32_t nxt = kCdtInf;
+  for (int i1 = L; i1 > 0; i1 -= K) {
+    const int lo = i1 > K ? i1 - K : 0, n = i1 - lo;
+    int32_t a[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) a[k] = k < n ? p[(int64_t)(lo + k) * step] : 0;
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+      if (k >= n) continue;
+      if (nxt + 1 < a[k]) a[k] = nxt + 1;
+      nxt = a[k];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (k < n) p[(int64_t)(lo + k) * step] = FINAL ? cdt_final_bits(a[k]) : a[k];
+  }
 }
 
 }  // namespace tmh

@@ -385,22 +385,32 @@ __device__ void cdt_rows_taxicab(int32_t* __restrict__ w, const Shape& sh, int64
 // with r, so max(r, m(r)) is smallest where m(r) first drops to r or below),
 // or the minimum of the whole line when no such r reaches the line's ends
 // (kCdtInf: no background on the line).  Equal to cdt_line_chessboard's
-// values; the cost per voxel is its distance.
+// values; the cost per voxel is its distance.  The window grows four steps
+// per round: the eight loads of a round are issued together (indices
+// clamped into the line, the out-of-line ones then ignored), so a voxel at
+// distance r waits for ceil(r / 4) rounds of loads rather than r.
 __device__ __forceinline__ int32_t cheb_window(const int32_t* __restrict__ g, int64_t base,
                                                int64_t step, int L, int i) {
-  int32_t m = g[base + (int64_t)i * step];
+  const int32_t* p = g + base;
+  int32_t m = p[(int64_t)i * step];
   if (m <= 0) return m;
   const int rfull = i > L - 1 - i ? i : L - 1 - i;
-  for (int r = 1; r <= rfull; ++r) {
-    if (i - r >= 0) {
-      const int32_t v = g[base + (int64_t)(i - r) * step];
-      m = v < m ? v : m;
+  for (int r0 = 1; r0 <= rfull; r0 += 4) {
+    int32_t lo[4], hi[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int a = i - r0 - q, b = i + r0 + q;
+      lo[q] = p[(int64_t)(a >= 0 ? a : 0) * step];
+      hi[q] = p[(int64_t)(b < L ? b : L - 1) * step];
     }
-    if (i + r < L) {
-      const int32_t v = g[base + (int64_t)(i + r) * step];
-      m = v < m ? v : m;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + q;
+      if (r > rfull) break;
+      if (i - r >= 0 && lo[q] < m) m = lo[q];
+      if (i + r < L && hi[q] < m) m = hi[q];
+      if (m <= r) return r;
     }
-    if (m <= r) return r;
   }
   return m;
 }

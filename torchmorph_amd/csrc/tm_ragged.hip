@@ -322,8 +322,8 @@ __global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__
       if (fin) cdt_axis0_words<TIn, true>(in, w, base, step, (int)L, t, nl_axis, words);
       else cdt_axis0_words<TIn, false>(in, w, base, step, (int)L, t, nl_axis, words);
     } else if (KIND == RG_CDT_TAXICAB) {
-      if (fin) cdt_line_taxicab<true>(w, base, step, L);
-      else cdt_line_taxicab<false>(w, base, step, L);
+      if (fin) cdt_taxicab_chunked<true>(w, base, step, (int)L);
+      else cdt_taxicab_chunked<false>(w, base, step, (int)L);
     } else {
       uint16_t* S = (uint16_t*)wk;
       uint16_t* T = S + numel;
