@@ -85,30 +85,6 @@ __device__ __forceinline__ float edt_dist_of(int ndim, const int64_t* c, const i
   return (float)sqrt(acc);
 }
 
-// Envelope stacks of the row passes, blocked per row: entry k of row t sits
-// at slot ((k >> 3) * nl + t) * 8 + (k & 7), so eight consecutive entries of
-// one row share 32 bytes of E (p | f0 << 16) and 16 bytes of F1 (f1, two
-// feature planes only), and the query fetches them eight at a time.  E holds
-// rows_stack_slots(L, nl) slots from a 256-byte boundary, F1 as many from the
-// next 256-byte boundary after E.  Slot numbers are 32-bit (rows_stack_fits).
-__host__ __device__ inline int64_t rows_stack_slots(int64_t L, int64_t nl) {
-  return (L + 7) / 8 * 8 * nl;
-}
-__host__ __device__ inline int64_t rows_stack_bytes(int64_t L, int64_t nl) {
-  return 256 + (rows_stack_slots(L, nl) * 4 + 255) / 256 * 256 + rows_stack_slots(L, nl) * 2;
-}
-__host__ __device__ inline bool rows_stack_fits(int64_t L, int64_t nl) {
-  return rows_stack_slots(L, nl) < ((int64_t)1 << 31);
-}
-// E and F1 for a stack area starting at `area` (any 4-byte boundary); plain
-// pointer offsets, so the global address space carries over.
-__host__ __device__ inline uint32_t* rows_stack_e(void* area) {
-  return (uint32_t*)((char*)area + ((256 - ((uintptr_t)area & 255)) & 255));
-}
-__host__ __device__ inline uint16_t* rows_stack_f1(void* area, int64_t L, int64_t nl) {
-  char* e = (char*)rows_stack_e(area) + rows_stack_slots(L, nl) * 4;
-  return (uint16_t*)(e + ((256 - ((uintptr_t)e & 255)) & 255));
-}
 // LDS bytes of edt_rows_final<NP, CH>: the chunk staging, NP planes in, then
 // the outputs in the same space.
 template <int NP, int CH>
@@ -116,59 +92,17 @@ constexpr size_t edt_rows_lds() {
   return (size_t)NP * kCLanes * (CH + 1) * sizeof(int32_t);
 }
 
-// One block of eight stack entries of a row in registers: e* from E, f*
-// from F1 (two entries per word).  Named scalars rather than arrays, so that
-// picking entry j stays a select tree on registers.
-struct StkBlk {
-  uint32_t e0, e1, e2, e3, e4, e5, e6, e7, f0, f1, f2, f3;
-};
-
-template <int NP>
-__device__ __forceinline__ void stk_fetch(StkBlk& b, const uint32_t* __restrict__ E,
-                                          const uint16_t* __restrict__ F1, uint32_t s) {
-  const uint4 x = *(const uint4*)(E + s);
-  const uint4 y = *(const uint4*)(E + s + 4);
-  b.e0 = x.x; b.e1 = x.y; b.e2 = x.z; b.e3 = x.w;
-  b.e4 = y.x; b.e5 = y.y; b.e6 = y.z; b.e7 = y.w;
-  if (NP == 2) {
-    const uint4 z = *(const uint4*)(F1 + s);
-    b.f0 = z.x; b.f1 = z.y; b.f2 = z.z; b.f3 = z.w;
-  }
-}
-
-__device__ __forceinline__ uint32_t stk_pick_e(const StkBlk& b, int j) {
-  const bool s0 = (j & 1) != 0, s1 = (j & 2) != 0, s2 = (j & 4) != 0;
-  const uint32_t x0 = s0 ? b.e1 : b.e0, x1 = s0 ? b.e3 : b.e2;
-  const uint32_t x2 = s0 ? b.e5 : b.e4, x3 = s0 ? b.e7 : b.e6;
-  const uint32_t y0 = s1 ? x1 : x0, y1 = s1 ? x3 : x2;
-  return s2 ? y1 : y0;
-}
-
-__device__ __forceinline__ uint32_t stk_pick_f(const StkBlk& b, int j) {
-  const bool s1 = (j & 2) != 0, s2 = (j & 4) != 0;
-  const uint32_t y0 = s1 ? b.f1 : b.f0, y1 = s1 ? b.f3 : b.f2;
-  const uint32_t v = s2 ? y1 : y0;
-  return (j & 1) ? v >> 16 : v & 0xffffu;
-}
-
 // EDT, last pass (d = ndim - 1, NP = d feature planes, 1 or 2), distances
 // only.  fin: the item's feature planes (plane stride `plane`); rows
 // row0 .. row0 + nrows - 1 of the nl rows whose stacks E / F1 hold (blocked
-// layout above).  lds: edt_rows_lds<NP, CH>() bytes.
-//
-// The envelope is the one of edt_line_stack_c, entry for entry: the top two
-// entries live in registers, every deeper entry is written once when it is
-// buried, and the R most recently buried ones also stay in registers, so a
-// pop (rarely more than two deep) reads memory only past those.  The query
-// walks the stack with the block of eight holding its next entry in
-// registers and the block after it already in flight.
+// layout, rows_stack_bytes).  lds: edt_rows_lds<NP, CH>() bytes.  The
+// envelope and its walk are those of edt_line_stack_c (tm_dt_fast.h).
 template <int NP, int CH>
 __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, const Shape& sh,
                                int64_t row0, int nrows, const Samp& sp, uint32_t* __restrict__ E,
                                uint16_t* __restrict__ F1, int64_t nl,
                                float* __restrict__ dist, int32_t* lds) {
   constexpr int RS = CH + 1;
-  constexpr int R = 4;
   const int d = sh.ndim - 1;
   const int L = (int)sh.size[d];
   const int lane = threadIdx.x;
@@ -177,35 +111,21 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
   int64_t c[kMaxNdim], base = 0, b = 0;
   for (int j = 0; j < kMaxNdim; ++j) c[j] = 0;
   if (active) decode_line(sh, d, t, c, &base, &b);
-  const double sd = sp.s[d];
-  const double s0 = sp.s[0], s1 = sp.s[1];
-  const int32_t c0 = (int32_t)c[0], c1 = (int32_t)c[1];
+  Envelope<NP, true> env;
+  env.A.E = E;
+  env.A.F1 = F1;
+  env.A.ss = 0;
+  env.A.nl8 = nl * 8;
+  env.A.t8 = t * 8;
+  env.sd = sp.s[d];
+  env.s0 = sp.s[0];
+  env.s1 = sp.s[1];
+  env.c0 = (int32_t)c[0];
+  env.c1 = (int32_t)c[1];
   int32_t* in0 = lds;
   int32_t* in1 = lds + kCLanes * RS;
-  const uint32_t nl8 = (uint32_t)nl * 8u, t8 = (uint32_t)t * 8u;
   const int nch = (L + CH - 1) / CH;
-  auto slot = [&](int k) -> uint32_t {
-    return (uint32_t)(k >> 3) * nl8 + t8 + (uint32_t)(k & 7);
-  };
-  auto site_of = [&](uint32_t w, uint32_t f) -> EdtSite {
-    EdtSite e;
-    e.p = (int32_t)(w & 0xffffu);
-    e.f0 = (int32_t)(w >> 16);
-    e.f1 = NP == 2 ? (int32_t)f : 0;
-    e.h = edt_partial<NP>(e.f0, e.f1, c0, c1, s0, s1);
-    return e;
-  };
-  auto put = [&](int k, const EdtSite& e) {
-    const uint32_t s = slot(k);
-    E[s] = (uint32_t)(uint16_t)e.p | ((uint32_t)(uint16_t)e.f0 << 16);
-    if (NP == 2) F1[s] = (uint16_t)e.f1;
-  };
 
-  // build: q1 = entry top, q0 = entry top - 1; memory holds 0 .. top - 2,
-  // g0 .. g{nv - 1} = entries top - 2, top - 3, ... (raw, f1 in h*)
-  int top = -1, nv = 0;
-  EdtSite q1{0, 0, 0, 0.0}, q0{0, 0, 0, 0.0};
-  uint32_t g0 = 0, g1 = 0, g2 = 0, g3 = 0, h0 = 0, h1 = 0, h2 = 0, h3 = 0;
   int32_t r0[CH], r1[CH];
   chunk_fetch<CH>(r0, fin, row0, L, nrows, 0);
   if (NP == 2) chunk_fetch<CH>(r1, fin + plane, row0, L, nrows, 0);
@@ -225,63 +145,16 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
         const int32_t f0 = in0[lane * RS + k];
         if (f0 < 0) continue;
         const int32_t f1 = NP == 2 ? in1[lane * RS + k] : 0;
-        const double wR = edt_partial<NP>(f0, f1, c0, c1, s0, s1);
-        while (top >= 1) {
-          const double a = (double)(q1.p - q0.p) * sd;
-          const double bb = (double)(i - q1.p) * sd;
-          const double cc = a + bb;
-          if (cc * q1.h - bb * q0.h - a * wR - a * bb * cc <= 0.0) break;
-          --top;
-          q1 = q0;
-          if (top >= 1) {
-            uint32_t w, wf = 0;
-            if (nv > 0) {
-              w = g0;
-              wf = h0;
-              g0 = g1; g1 = g2; g2 = g3;
-              h0 = h1; h1 = h2; h2 = h3;
-              --nv;
-            } else {
-              const uint32_t s = slot(top - 1);
-              w = E[s];
-              if (NP == 2) wf = F1[s];
-              // wait for these loads here, not wherever w is next written
-              asm volatile("" : "+v"(w), "+v"(wf));
-            }
-            q0 = site_of(w, wf);
-          }
-        }
-        if (top >= 1) {
-          put(top - 1, q0);
-          g3 = g2; g2 = g1; g1 = g0;
-          h3 = h2; h2 = h1; h1 = h0;
-          g0 = (uint32_t)(uint16_t)q0.p | ((uint32_t)(uint16_t)q0.f0 << 16);
-          h0 = (uint32_t)(uint16_t)q0.f1;
-          nv = nv < R ? nv + 1 : R;
-        }
-        ++top;
-        q0 = q1;
-        q1 = EdtSite{i, f0, f1, wR};
+        env.add(i, f0, f1, edt_partial<NP>(f0, f1, env.c0, env.c1, env.s0, env.s1));
       }
     }
   }
-  if (active && top >= 1) put(top - 1, q0);
-  if (active && top >= 0) put(top, q1);
+  if (active) env.close();
 
-  // query: cur = entry l, nx = entry l + 1, n = l + 2 the next entry to
-  // decode; wa = block n >> 3, wb = the block after it (loaded when it holds
-  // entries)
   __syncthreads();  // the staging area holds outputs from here on
   int32_t* ob = lds;
-  int l = 0, n = 2;
-  EdtSite cur{0, 0, 0, 0.0}, nx{0, 0, 0, 0.0};
-  StkBlk wa{}, wb{};
-  if (active && top >= 0) {
-    stk_fetch<NP>(wa, E, F1, t8);
-    if (top >= 8) stk_fetch<NP>(wb, E, F1, nl8 + t8);
-    cur = site_of(wa.e0, wa.f0 & 0xffffu);
-    if (top >= 1) nx = site_of(wa.e1, wa.f0 >> 16);
-  }
+  EnvWalk<NP, true> walk;
+  if (active && env.top >= 0) walk.start(env);
   int32_t fnone[kMaxNdim];  // no site on the line: feature (-1, 0, ..., 0)
   for (int j = 0; j < kMaxNdim; ++j) fnone[j] = 0;
   fnone[0] = -1;
@@ -291,31 +164,11 @@ __device__ void edt_rows_final(const int32_t* __restrict__ fin, int64_t plane, c
         const int i = q * CH + k;
         if (i >= L) break;
         float out;
-        if (top < 0) {
+        if (env.top < 0) {
           c[d] = i;
           out = edt_dist_of(sh.ndim, c, fnone, sp);
         } else {
-          double tt = (double)(cur.p - i) * sd;
-          double d1 = cur.h + tt * tt;
-          while (l < top) {
-            double t2 = (double)(nx.p - i) * sd;
-            double d2 = nx.h + t2 * t2;
-            if (d1 <= d2) break;
-            d1 = d2;
-            ++l;
-            cur = nx;
-            if (l < top) {
-              const int j = n & 7;
-              nx = site_of(stk_pick_e(wa, j), NP == 2 ? stk_pick_f(wa, j) : 0u);
-              ++n;
-              if ((n & 7) == 0) {
-                wa = wb;
-                if ((n >> 3) + 1 <= (top >> 3))
-                  stk_fetch<NP>(wb, E, F1, (uint32_t)((n >> 3) + 1) * nl8 + t8);
-              }
-            }
-          }
-          out = (float)sqrt(d1);
+          out = (float)sqrt(walk.next(env, i));
         }
         ob[lane * RS + k] = __float_as_int(out);
       }

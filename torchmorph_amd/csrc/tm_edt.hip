@@ -306,10 +306,10 @@ __global__ __launch_bounds__(256) void edt_pass0_kernel(const TIn* __restrict__ 
 
 // Axis d >= 1, one lane per line (tm_dt_fast.h, loads PF voxels ahead).
 // COMPACT (items of up to 3 axes): 4/6-byte stack entries
-// (edt_line_stack_c); otherwise site + double partial sum.  LDS_DEPTH > 0:
-// stack in LDS (requires L <= LDS_DEPTH); LDS_DEPTH == 0: stack in the
-// interleaved global scratch gpos/gh (entry k of line t at k * nlines + t,
-// coalesced across lanes).
+// (edt_line_stack_c), in LDS (LDS_DEPTH > 0, requires L <= LDS_DEPTH) or in
+// the blocked global stacks from gpos on; otherwise site + double partial
+// sum, in LDS or in the interleaved global scratch gpos/gh (entry k of line
+// t at k * nlines + t, coalesced across lanes).
 template <bool FINAL, int PF, int LDS_DEPTH, int BLOCK, bool COMPACT>
 __global__ __launch_bounds__(BLOCK) void edt_stack_kernel(
     const int32_t* __restrict__ fin, int32_t* __restrict__ fout, int64_t plane, Shape sh, int d,
@@ -321,11 +321,23 @@ __global__ __launch_bounds__(BLOCK) void edt_stack_kernel(
   const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (t >= nlines) return;
   if (COMPACT) {
-    uint32_t* E = LDS_DEPTH > 0 ? lA + threadIdx.x : (uint32_t*)gpos + t;
-    uint16_t* F1 = LDS_DEPTH > 0 ? lB + threadIdx.x : (uint16_t*)gh + t;
-    const int64_t ss = LDS_DEPTH > 0 ? BLOCK : nlines;
-    if (d == 1) edt_line_stack_c<FINAL, 1, PF>(fin, fout, plane, sh, t, sp, E, F1, ss, o);
-    else edt_line_stack_c<FINAL, 2, PF>(fin, fout, plane, sh, t, sp, E, F1, ss, o);
+    constexpr bool BLK = LDS_DEPTH == 0;
+    StkAddr<BLK> A;
+    if (BLK) {
+      const int64_t L = sh.size[d];
+      A.E = rows_stack_e(gpos);
+      A.F1 = rows_stack_f1(gpos, L, nlines);
+      A.ss = 0;
+      A.nl8 = nlines * 8;
+      A.t8 = t * 8;
+    } else {
+      A.E = lA + threadIdx.x;
+      A.F1 = lB + threadIdx.x;
+      A.ss = BLOCK;
+      A.nl8 = A.t8 = 0;
+    }
+    if (d == 1) edt_line_stack_c<FINAL, 1, PF, BLK>(fin, fout, plane, sh, t, sp, A, o);
+    else edt_line_stack_c<FINAL, 2, PF, BLK>(fin, fout, plane, sh, t, sp, A, o);
   } else if (LDS_DEPTH > 0) {
     edt_line_stack_fast<FINAL, PF>(fin, fout, plane, sh, d, t, sp, lB + threadIdx.x,
                                    lh + threadIdx.x, BLOCK, o);
@@ -349,13 +361,17 @@ __global__ __launch_bounds__(kCLanes) void edt_rows_kernel(const int32_t* __rest
   edt_rows_final<NP, CH>(fin, plane, sh, row0, nrows, sp, E, F1, nlines, dist, lds_rows);
 }
 
-// Whether the last pass of a distances-only call may run row-staged: 2 or 3
-// axes and blocked stacks with 32-bit slot numbers.
-static bool rows_path_fits(int64_t batch, int ndim, const int64_t* shape) {
-  if (ndim != 2 && ndim != 3) return false;
-  int64_t nl = batch;
-  for (int j = 0; j + 1 < ndim; ++j) nl *= shape[j];
-  return rows_stack_fits(shape[ndim - 1], nl);
+// Bytes of the blocked envelope stacks (tm_dt_fast.h) of the passes d >= 1
+// of items of 2 or 3 axes, the largest over the passes; 0 otherwise.
+static int64_t compact_stack_bytes(int64_t batch, int ndim, const int64_t* shape) {
+  if (ndim != 2 && ndim != 3) return 0;
+  int64_t n = batch, m = 0;
+  for (int j = 0; j < ndim; ++j) n *= shape[j];
+  for (int d = 1; d < ndim; ++d) {
+    const int64_t b = rows_stack_bytes(shape[d], n / shape[d]);
+    m = b > m ? b : m;
+  }
+  return m;
 }
 
 static bool unit_sampling(int ndim, const double* s) {
@@ -457,7 +473,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   int32_t* gpos = bufB + (size_t)nbuf * N;
   double* gh = (double*)(((uintptr_t)(gpos + N) + 255) & ~(uintptr_t)255);
 
-  const bool rows_ok = !o.idx && rows_path_fits(batch, ndim, shape);
+  const bool rows_ok = !o.idx && (ndim == 2 || ndim == 3);
   // launch variant v (one choice for the whole call):
   //   v & 3   where each envelope stack lives and the workgroup size:
   //           0: LDS (128 deep, 80 KB per 64 lanes) when the line fits, else
@@ -571,12 +587,10 @@ extern "C" size_t tm_edt_workspace_bytes(int64_t batch, int ndim, const int64_t*
   for (int j = 0; j < ndim; ++j) n *= shape[j];
   if (tmh::use_2d_path(ndim, shape, sampling)) return (size_t)n * sizeof(int16_t);
   const int nbuf = ndim > 1 ? ndim - 1 : 1;
-  size_t b = (size_t)n * 4 * (2 * nbuf + 1) + 256 + (size_t)n * 8;
-  if (tmh::rows_path_fits(batch, ndim, shape)) {  // blocked stacks of the row-staged last pass
-    const int64_t L = shape[ndim - 1];
-    b = std::max(b, (size_t)n * 8 * nbuf + (size_t)tmh::rows_stack_bytes(L, n / L));
-  }
-  return b;
+  // feature planes A and B, then the envelope stacks: interleaved positions
+  // and partial sums (4 or more axes) or the blocked compact stacks
+  const size_t b = (size_t)n * 4 * (2 * nbuf + 1) + 256 + (size_t)n * 8;
+  return std::max(b, (size_t)n * 8 * nbuf + (size_t)tmh::compact_stack_bytes(batch, ndim, shape));
 }
 
 extern "C" int tm_edt(const void* input, int input_dtype, int64_t batch, int ndim,

@@ -30,10 +30,11 @@
 //
 // schedule = 2 has no work-list: one ordinary grid launch per (level, kind
 // of pass), one line per lane, and a per-block table naming each block's job
-// and first line.  Each launch is specialised to its kind of pass (EDT axis
-// 0, EDT envelope with the stack in LDS or in global memory, chamfer axis 0,
-// taxicab, chessboard, raster), and the EDT envelope uses the prefetching
-// form of tm_dt_fast.h.  Passes along an item's contiguous last axis
+// and first line, longest lines first.  Each launch is specialised to its
+// kind of pass (EDT axis 0 and chamfer axis 0 from per-chunk background
+// bitmasks (tm_dt_axis0.h), EDT envelope, taxicab, chessboard, raster); the
+// EDT envelope is the prefetching form of tm_dt_fast.h with its stacks in
+// the blocked global layout.  Passes along an item's contiguous last axis
 // (distances-only EDT of 2-D and 3-D items, taxicab chamfer) run row-staged
 // instead (tm_dt_chunk.h): a workgroup owns 256 consecutive rows and moves
 // along them in chunks staged through LDS.  Chessboard passes along every
@@ -253,13 +254,11 @@ __global__ __launch_bounds__(kRgThreads) void rg_kernel(const TIn* __restrict__ 
 
 // Schedule 2: one launch per (level d, kind); lane = line.  The block table
 // of a launch lists the longest lines first and is read in block id order.
-template <typename TIn, int KIND, int BLOCK, int LDS_DEPTH>
-__global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__ in_all,
-                                                         RgArgs a, const RgBlk* __restrict__ blks,
-                                                         int d) {
-  // LDS stack (jobs of up to 3 axes): compact entries of edt_line_stack_c
-  __shared__ uint32_t lE[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
-  __shared__ uint16_t lF[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+template <typename TIn, int KIND>
+__global__ __launch_bounds__(kRgThreads) void rg_level_kernel(const TIn* __restrict__ in_all,
+                                                              RgArgs a,
+                                                              const RgBlk* __restrict__ blks,
+                                                              int d) {
   const RgBlk bk = blks[blockIdx.x];
   if ((int)threadIdx.x >= bk.nlines) return;
   const RgJob& jb = a.jobs[bk.job];
@@ -289,18 +288,21 @@ __global__ __launch_bounds__(BLOCK) void rg_level_kernel(const TIn* __restrict__
       int32_t* dst = (d & 1) ? B : A;
       int32_t* gpos = B + (int64_t)nbuf * numel;
       double* gh = (double*)(((uintptr_t)(gpos + numel) + 255) & ~(uintptr_t)255);
-      if (nd <= 3) {
-        uint32_t* E = LDS_DEPTH > 0 ? lE + threadIdx.x : (uint32_t*)gpos + t;
-        uint16_t* F1 = LDS_DEPTH > 0 ? lF + threadIdx.x : (uint16_t*)gh + t;
-        const int64_t ss = LDS_DEPTH > 0 ? BLOCK : nl_axis;
+      if (nd <= 3) {  // compact entries, blocked global stacks (tm_dt_fast.h)
+        StkAddr<true> A;
+        A.E = rows_stack_e(gpos);
+        A.F1 = rows_stack_f1(gpos, sh.size[d], nl_axis);
+        A.ss = 0;
+        A.nl8 = nl_axis * 8;
+        A.t8 = t * 8;
         if (d == 1) {
-          if (fin) edt_line_stack_c<true, 1, 8>(src, dst, numel, sh, t, jb.sp, E, F1, ss, o);
-          else edt_line_stack_c<false, 1, 8>(src, dst, numel, sh, t, jb.sp, E, F1, ss, o);
+          if (fin) edt_line_stack_c<true, 1, 8, true>(src, dst, numel, sh, t, jb.sp, A, o);
+          else edt_line_stack_c<false, 1, 8, true>(src, dst, numel, sh, t, jb.sp, A, o);
         } else {
-          if (fin) edt_line_stack_c<true, 2, 8>(src, dst, numel, sh, t, jb.sp, E, F1, ss, o);
-          else edt_line_stack_c<false, 2, 8>(src, dst, numel, sh, t, jb.sp, E, F1, ss, o);
+          if (fin) edt_line_stack_c<true, 2, 8, true>(src, dst, numel, sh, t, jb.sp, A, o);
+          else edt_line_stack_c<false, 2, 8, true>(src, dst, numel, sh, t, jb.sp, A, o);
         }
-      } else if (LDS_DEPTH == 0) {
+      } else {
         if (fin)
           edt_line_stack_fast<true, 8>(src, dst, numel, sh, d, t, jb.sp, gpos + t, gh + t,
                                        nl_axis, o);
@@ -426,13 +428,10 @@ __global__ __launch_bounds__(kRgThreads) void rg_raster_kernel(const TIn* __rest
 // ------------------------------------------------------------------ host --
 
 // One schedule-2 launch: blocks [lo, hi) of the block table, all of one
-// kind of pass at one level; lds = EDT envelope with the stack in LDS.
+// kind of pass at one level.
 struct RgLaunch {
-  int level, kind, lds, lo, hi;
+  int level, kind, lo, hi;
 };
-
-constexpr int kRgLdsDepth = 128;  // EDT stack in LDS for lines up to this long
-constexpr int kRgLdsBlock = 64;   // lines per block then (48 KB of LDS)
 
 struct RgPlan {
   std::vector<RgJob> jobs;
@@ -447,9 +446,7 @@ struct RgPlan {
 // Whether the job's last EDT pass runs row-staged (rg_rows_kernel).
 static bool rows_final(const RgJob& j) {
   const int nd = j.sh.ndim;
-  if (j.op != TM_RG_EDT || j.raster || j.idx_off >= 0 || nd < 2 || nd > 3) return false;
-  const int64_t L = j.sh.size[nd - 1];
-  return rows_stack_fits(L, j.sh.numel / L);
+  return j.op == TM_RG_EDT && !j.raster && j.idx_off < 0 && nd >= 2 && nd <= 3;
 }
 
 static size_t job_pass_bytes(const RgJob& j) {
@@ -459,10 +456,10 @@ static size_t job_pass_bytes(const RgJob& j) {
     // feature planes A and B (ndim-1 each, one for 1-D), stack positions, stack values
     const int nbuf = nd > 1 ? nd - 1 : 1;
     size_t b = align256((size_t)n * 4 * (2 * nbuf + 1)) + (size_t)n * 8;
-    if (rows_final(j)) {  // the blocked stacks of edt_rows_final after the planes
-      const int64_t L = j.sh.size[nd - 1];
-      b = std::max(b, (size_t)n * 8 * nbuf + (size_t)rows_stack_bytes(L, n / L));
-    }
+    if (nd <= 3)  // the blocked compact stacks (tm_dt_fast.h) after the planes, per pass
+      for (int d = 1; d < nd; ++d)
+        b = std::max(b, (size_t)n * 8 * nbuf + (size_t)rows_stack_bytes(j.sh.size[d],
+                                                                          n / j.sh.size[d]));
     return b;
   }
   if (j.raster) return (size_t)n * 8;
@@ -590,16 +587,15 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
   p.blks.clear();
   p.launches.clear();
   for (int lev = 0; lev < p.nlevels; ++lev) {
-    for (int g = 0; g < 10; ++g) {
-      // g: 0 EDT0, 1 EDT envelope in LDS, 2 EDT envelope in global memory,
-      //    3 CDT0, 4 taxicab, 5 chessboard (every axis >= 1, voxel blocks,
-      //    cheb_window), 6 raster; along the contiguous last axis,
-      //    row-staged: 7 EDT 2-D, 8 EDT 3-D, 9 taxicab
-      RgLaunch ln{lev, 0, 0, (int)p.blks.size(), 0};
+    for (int g = 0; g < 9; ++g) {
+      // g: 0 EDT0, 1 EDT envelope, 2 CDT0, 3 taxicab, 4 chessboard (every
+      //    axis >= 1, voxel blocks, cheb_window), 5 raster; along the
+      //    contiguous last axis, row-staged: 6 EDT 2-D, 7 EDT 3-D, 8 taxicab
+      RgLaunch ln{lev, 0, (int)p.blks.size(), 0};
       for (int32_t ji : order) {
         const RgJob& j = p.jobs[(size_t)ji];
         if (j.raster) {
-          if (g != 6 || lev >= 2) continue;
+          if (g != 5 || lev >= 2) continue;
           p.blks.push_back(RgBlk{ji, 1, 0});
           continue;
         }
@@ -609,28 +605,27 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
         int grp;
         if (j.op == TM_RG_EDT) {
           if (lev == 0) grp = 0;
-          else if (last && rows_final(j)) grp = j.sh.ndim == 2 ? 7 : 8;
-          else grp = (L <= kRgLdsDepth && j.sh.ndim <= 3) ? 1 : 2;
+          else if (last && rows_final(j)) grp = j.sh.ndim == 2 ? 6 : 7;
+          else grp = 1;
         } else if (lev == 0) {
-          grp = 3;
+          grp = 2;
         } else if (j.op == TM_RG_CDT_TAXICAB) {
-          grp = last ? 9 : 4;
+          grp = last ? 8 : 3;
         } else {
-          grp = 5;
+          grp = 4;
         }
         if (grp != g) continue;
-        if (g == 5) {  // voxel blocks
+        if (g == 4) {  // voxel blocks
           for (int64_t v0 = 0; v0 < j.sh.numel; v0 += kRgWinVox)
             p.blks.push_back(RgBlk{ji, (int32_t)std::min<int64_t>(kRgWinVox, j.sh.numel - v0), v0});
           continue;
         }
         const int64_t nl = j.sh.numel / L;
-        const int64_t per = g == 1 ? kRgLdsBlock : kRgThreads;
-        for (int64_t l0 = 0; l0 < nl; l0 += per)
-          p.blks.push_back(RgBlk{ji, (int32_t)std::min(per, nl - l0), l0});
+        for (int64_t l0 = 0; l0 < nl; l0 += kRgThreads)
+          p.blks.push_back(RgBlk{ji, (int32_t)std::min<int64_t>(kRgThreads, nl - l0), l0});
       }
       ln.hi = (int)p.blks.size();
-      if (g != 5 && g != 6) {
+      if (g != 4 && g != 5) {
         // a lane walks its whole line: longest lines first, so that the
         // longest blocks start in the first round of workgroups (launched
         // without the XCD swizzle, block id order = dispatch order)
@@ -640,12 +635,10 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
                                   p.jobs[(size_t)y.job].sh.size[lev];
                          });
       }
-      static const int kinds[10] = {RG_EDT0,       RG_EDT_STACK,       RG_EDT_STACK,
-                                    RG_CDT0,       RG_CDT_TAXICAB,     RG_CDT_CHEB_WINDOW,
-                                    RG_CDT_RASTER, RG_EDT_ROWS2,       RG_EDT_ROWS3,
-                                    RG_CDT_TAXICAB_ROWS};
+      static const int kinds[9] = {RG_EDT0,       RG_EDT_STACK,  RG_CDT0,
+                                   RG_CDT_TAXICAB, RG_CDT_CHEB_WINDOW, RG_CDT_RASTER,
+                                   RG_EDT_ROWS2,  RG_EDT_ROWS3,  RG_CDT_TAXICAB_ROWS};
       ln.kind = kinds[g];
-      ln.lds = g == 1 ? 1 : 0;
       if (ln.hi > ln.lo) p.launches.push_back(ln);
     }
   }
@@ -704,23 +697,19 @@ static int rg_launch(const TIn* in, const RgPlan& p, RgArgs a, int schedule, int
       const int d = ln.level;
       switch (ln.kind) {
         case RG_EDT0:
-          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT0, kRgThreads, 0>), g, dim3(kRgThreads), 0,
-                             st, in, a, b, d);
+          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT0>), g, dim3(kRgThreads), 0, st, in, a,
+                             b, d);
           break;
         case RG_EDT_STACK:
-          if (ln.lds)
-            hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT_STACK, kRgLdsBlock, kRgLdsDepth>), g,
-                               dim3(kRgLdsBlock), 0, st, in, a, b, d);
-          else
-            hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT_STACK, kRgThreads, 0>), g,
-                               dim3(kRgThreads), 0, st, in, a, b, d);
+          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT_STACK>), g, dim3(kRgThreads), 0, st, in,
+                             a, b, d);
           break;
         case RG_CDT0:
-          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_CDT0, kRgThreads, 0>), g, dim3(kRgThreads), 0,
-                             st, in, a, b, d);
+          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_CDT0>), g, dim3(kRgThreads), 0, st, in, a,
+                             b, d);
           break;
         case RG_CDT_TAXICAB:
-          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_CDT_TAXICAB, kRgThreads, 0>), g,
+          hipLaunchKernelGGL((rg_level_kernel<TIn, RG_CDT_TAXICAB>), g,
                              dim3(kRgThreads), 0, st, in, a, b, d);
           break;
         case RG_CDT_CHEB_WINDOW:
