@@ -92,6 +92,26 @@ int tm_cdt(const void* input, int input_dtype, int64_t batch, int ndim, const in
            int metric, float* distances, void* indices, int index_dtype, void* workspace,
            size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------- brute-force DT --- */
+
+/* Brute-force distance transform: every foreground voxel scans every
+ * background voxel of its item.  Replaces torchmorph._C.bfdt_cuda (reference
+ * torchmorph.cpp:49-55, bfdt_kernel.cu:170-296).  O(voxels^2) per item: a
+ * checker for larger fixtures, not a fast path.  Values are computed in FP64
+ * and rounded to float32; the nearest background is the first in row-major
+ * order at the smallest distance.
+ *   metric      0 = euclidean, 1 = taxicab, 2 = chessboard (sampling applies
+ *               to all three, as in the reference)
+ *   sampling    ndim spacings (NULL = unit)
+ *   distances   float32 (batch, spatial...) or NULL; background 0; an item
+ *               without background gets +inf (euclidean) or 4294967295
+ *   indices     (ndim, batch, spatial...) of index_dtype or NULL; background
+ *               voxels index themselves, an item without background gets 0
+ * batch <= 65535; no workspace. */
+int tm_bfdt(const void* input, int input_dtype, int64_t batch, int ndim, const int64_t* shape,
+            const double* sampling, int metric, float* distances, void* indices,
+            int index_dtype, void* stream);
+
 /* ------------------------------------------------------------- ragged --- */
 
 /* No reference counterpart: the reference takes dense (B, C, spatial...)

@@ -6,8 +6,8 @@ CPU baseline.  The shipped package (``torchmorph_amd``) never imports it.
 
 Contents:
   * ctypes bindings to ``libtm_oracle.so`` (built from ``tm_oracle.c`` by
-    ``make -C oracle``): EDT, CDT, grey and binary morphology, one item at a
-    time, looped over the leading (B, C) axes here;
+    ``make -C oracle``): EDT, CDT, brute-force DT, grey and binary
+    morphology, one item at a time, looped over the leading (B, C) axes here;
   * numpy FP64 restatements of the Sinkhorn iterations
     (reference ``torchmorph/optimal_transport.py:246-300``) and of the
     point-cloud log-domain solver the build adds.
@@ -49,7 +49,8 @@ def _load():
                                  ctypes.c_float, ctypes.c_int, P]
         lib.tmo_binary.argtypes = [P, ctypes.c_int, i64p, P, i64p, i64p, ctypes.c_int,
                                    ctypes.c_int, P]
-        for f in (lib.tmo_edt, lib.tmo_cdt, lib.tmo_grey, lib.tmo_binary):
+        lib.tmo_bfdt.argtypes = [P, ctypes.c_int, i64p, P, ctypes.c_int, P, P]
+        for f in (lib.tmo_edt, lib.tmo_cdt, lib.tmo_grey, lib.tmo_binary, lib.tmo_bfdt):
             f.restype = ctypes.c_int
         _lib = lib
     return _lib
@@ -90,6 +91,35 @@ def edt(mask: np.ndarray, sampling=None, lead: int = 2):
         di = np.empty(spatial, np.float64)
         rc = lib.tmo_edt(_ptr(it), nd, shp, _ptr(samp) if samp is not None else None,
                          _ptr(fi), _ptr(di))
+        assert rc == 0
+        dist[i] = di
+        feat[i] = fi
+    dist = dist.reshape(lead_shape + spatial)
+    feat = np.moveaxis(feat, 1, 0).reshape((nd,) + lead_shape + spatial)
+    return dist, feat
+
+
+def bfdt(mask: np.ndarray, metric: str = "euclidean", sampling=None, lead: int = 2):
+    """Brute-force DT of every item (reference bfdt semantics).  Returns
+    (float64 distances, int32 indices with shape (ndim, *mask.shape))."""
+    lib = _load()
+    code = {"euclidean": 0, "taxicab": 1, "chessboard": 2}[metric]
+    mask = np.ascontiguousarray(mask != 0, dtype=np.uint8)
+    lead_shape, items = _items(mask, lead)
+    spatial = items.shape[1:]
+    nd = len(spatial)
+    samp = None
+    if sampling is not None:
+        samp = np.ascontiguousarray(np.broadcast_to(np.asarray(sampling, np.float64), (nd,)))
+    dist = np.empty(items.shape, np.float64)
+    feat = np.empty((items.shape[0], nd) + spatial, np.int32)
+    shp = _i64(spatial)
+    for i in range(items.shape[0]):
+        it = np.ascontiguousarray(items[i])
+        fi = np.empty((nd,) + spatial, np.int32)
+        di = np.empty(spatial, np.float64)
+        rc = lib.tmo_bfdt(_ptr(it), nd, shp, _ptr(samp) if samp is not None else None, code,
+                          _ptr(di), _ptr(fi))
         assert rc == 0
         dist[i] = di
         feat[i] = fi

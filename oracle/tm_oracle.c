@@ -41,6 +41,15 @@
  *  - binary: the reference gather formulation (binary_kernel.cu:56-127,
  *          offsets :199-222): erosion ANDs in[p + delta_i], dilation ORs
  *          in[p - delta_i], out-of-range reads give border_value.
+ *  - brute-force DT: the reference's bfdt (bfdt_kernel.cu:23-129, host
+ *          :170-296): every foreground voxel takes the background voxel at
+ *          the smallest distance, the first one in flat order on ties
+ *          (strict minimum), euclidean / taxicab / chessboard with sampling
+ *          applied to every metric; background voxels 0 and their own index;
+ *          no background: +inf (euclidean) or 4294967295, index 0.  In double
+ *          (the reference's float arithmetic is not reproducible); pinned
+ *          against scipy.ndimage.distance_transform_bf and against the EDT
+ *          oracle (tests/test_oracle.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -390,5 +399,57 @@ int tmo_binary(const uint8_t* in, int ndim, const int64_t* shape, const uint8_t*
         }
     }
     free(delta);
+    return 0;
+}
+
+/* Brute-force distance transform of one item (metric 0 euclidean, 1
+ * taxicab, 2 chessboard).  dist: distances (sqrt taken for euclidean);
+ * features: ndim planes of the chosen background's coordinates. */
+int tmo_bfdt(const uint8_t* mask, int ndim, const int64_t* shape, const double* sampling,
+             int metric, double* dist, int32_t* features) {
+    if (ndim < 1 || ndim > TMO_MAX_NDIM || metric < 0 || metric > 2) return 1;
+    const int64_t n = prod(shape, 0, ndim);
+    double s[TMO_MAX_NDIM];
+    for (int j = 0; j < ndim; ++j) s[j] = sampling ? sampling[j] : 1.0;
+    int32_t* coords = (int32_t*)malloc(sizeof(int32_t) * n * ndim);
+    if (!coords) return 2;
+    for (int64_t p = 0; p < n; ++p) {
+        int64_t rem = p;
+        for (int j = ndim - 1; j >= 0; --j) {
+            coords[p * ndim + j] = (int32_t)(rem % shape[j]);
+            rem /= shape[j];
+        }
+    }
+    for (int64_t p = 0; p < n; ++p) {
+        const int32_t* c = coords + p * ndim;
+        int64_t best_q = -1;
+        double best = 0.0;
+        if (mask[p]) {
+            for (int64_t q = 0; q < n; ++q) {
+                if (mask[q]) continue;
+                const int32_t* y = coords + q * ndim;
+                double acc = 0.0;
+                for (int j = 0; j < ndim; ++j) {
+                    double t = (double)(c[j] - y[j]) * s[j];
+                    if (metric == 0) t = t * t;
+                    else if (t < 0) t = -t;
+                    if (j == 0) acc = t;
+                    else if (metric == 2) acc = t > acc ? t : acc;
+                    else acc = acc + t;
+                }
+                if (best_q < 0 || acc < best) { best = acc; best_q = q; }
+            }
+        }
+        double out;
+        const int32_t* f;
+        int32_t zero[TMO_MAX_NDIM] = {0};
+        if (!mask[p]) { out = 0.0; f = c; }
+        else if (best_q < 0) { out = metric == 0 ? INFINITY : 4294967295.0; f = zero; }
+        else { out = metric == 0 ? sqrt(best) : best; f = coords + best_q * ndim; }
+        if (dist) dist[p] = out;
+        if (features)
+            for (int j = 0; j < ndim; ++j) features[j * n + p] = f[j];
+    }
+    free(coords);
     return 0;
 }

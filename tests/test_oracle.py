@@ -334,3 +334,79 @@ def test_deepsets_oracle_gradient_matches_central_differences():
     # loss is invariant to a constant shift of the targets (dual gauge)
     l2, _ = ds.loss_and_grad(params, pairs, [t + 3.0 for t in targets])
     assert abs(l2 - loss) < 1e-12
+
+
+# ----------------------------------------------------------- brute-force DT --
+BF_METRICS = ["euclidean", "taxicab", "chessboard"]
+
+
+@pytest.mark.parametrize("metric", BF_METRICS)
+@pytest.mark.parametrize("seed", range(12))
+def test_bfdt_oracle_matches_scipy_bf(oracle_mod, metric, seed):
+    # scipy.ndimage.distance_transform_bf is the reference test's comparison
+    # (reference test/test_distance_transform.py:46-66); it applies sampling
+    # to the euclidean metric only, so the other metrics run unit-sampled
+    ndi = pytest.importorskip("scipy.ndimage")
+    rng = np.random.default_rng(300 + seed)
+    nd = int(rng.integers(1, 4))
+    shape = tuple(int(v) for v in rng.integers(1, 8, size=nd))
+    mask = rng.random(shape) < rng.uniform(0.3, 0.95)
+    mask.flat[int(rng.integers(mask.size))] = False  # at least one background voxel
+    samp = rng.uniform(0.5, 2.0, size=nd) if (metric == "euclidean" and seed % 2) else None
+    dist, idx = oracle_mod.bfdt(mask[None, None], metric, samp)
+    want = ndi.distance_transform_bf(mask, metric=metric, sampling=samp)
+    assert np.array_equal(dist[0, 0], want.astype(np.float64))
+    # every index is a background voxel at the reported distance, and the
+    # first such voxel in row-major order (np.argmin keeps the first minimum)
+    coords = np.stack(np.meshgrid(*[np.arange(n) for n in shape], indexing="ij"), -1)
+    s = np.ones(nd) if samp is None else samp
+    bg = coords[~mask]
+    delta = np.abs(coords[..., None, :] - bg) * s
+    if metric == "euclidean":
+        all_d = np.sqrt((delta ** 2).sum(-1))
+    elif metric == "taxicab":
+        all_d = delta.sum(-1)
+    else:
+        all_d = delta.max(-1)
+    first = bg[np.argmin(all_d, axis=-1)]
+    for p in itertools.product(*[range(n) for n in shape]):
+        f = tuple(idx[(slice(None), 0, 0) + p])
+        if mask[p]:
+            assert f == tuple(first[p]), (p, f)
+        else:
+            assert f == p
+
+
+def test_bfdt_oracle_special_items(oracle_mod):
+    # all background: zeros and own indices; all foreground: +inf /
+    # 4294967295 and index 0 (reference bfdt_kernel.cu:203-220); a single
+    # background voxel: |x - x0| * s per metric, in closed form
+    zeros = np.zeros((1, 1, 3, 4), bool)
+    d, i = oracle_mod.bfdt(zeros, "taxicab")
+    assert (d == 0).all()
+    assert np.array_equal(i[:, 0, 0], np.indices((3, 4)))
+    ones = np.ones((1, 1, 3, 4), bool)
+    assert np.isinf(oracle_mod.bfdt(ones, "euclidean")[0]).all()
+    d, i = oracle_mod.bfdt(ones, "chessboard")
+    assert (d == 4294967295.0).all() and (i == 0).all()
+    m = np.ones((1, 1, 5, 7), bool)
+    m[0, 0, 1, 5] = False
+    s = np.array([0.75, 1.5])
+    yy, xx = np.indices((5, 7))
+    dy, dx = np.abs(yy - 1) * s[0], np.abs(xx - 5) * s[1]
+    assert np.array_equal(oracle_mod.bfdt(m, "euclidean", s)[0][0, 0], np.sqrt(dy ** 2 + dx ** 2))
+    assert np.array_equal(oracle_mod.bfdt(m, "taxicab", s)[0][0, 0], dy + dx)
+    assert np.array_equal(oracle_mod.bfdt(m, "chessboard", s)[0][0, 0], np.maximum(dy, dx))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_bfdt_oracle_equals_edt_oracle_distances(oracle_mod, seed):
+    # unit sampling: squared distances are exact integers, so the envelope
+    # and the brute-force minimum give the same double
+    rng = np.random.default_rng(400 + seed)
+    shape = tuple(int(v) for v in rng.integers(2, 12, size=2 + seed % 2))
+    mask = rng.random(shape) < 0.85
+    mask.flat[0] = False
+    bf, _ = oracle_mod.bfdt(mask[None, None])
+    edt, _ = oracle_mod.edt(mask[None, None])
+    assert np.array_equal(bf, edt)

@@ -85,9 +85,15 @@ def cdt_cuda(input, metric, return_distances, return_indices):
 
 
 def bfdt_cuda(input, metric, sampling, return_distances, return_indices):
-    """Reference torchmorph.cpp:49-55 -- the brute-force DT is outside this
-    build's operator path."""
-    raise RuntimeError("bfdt_cuda is not provided by torchmorph_amd")
+    """Reference torchmorph.cpp:49-55."""
+    from .distance_transform import brute_force_distance_transform
+
+    out = brute_force_distance_transform(input, metric=metric, sampling=list(sampling),
+                                         return_distances=return_distances,
+                                         return_indices=return_indices)
+    if return_distances and return_indices:
+        return out
+    return (out, None) if return_distances else (None, out)
 
 
 def _hist(a, b, M, M_T, u, v, n_iter, epsilon, log_space):

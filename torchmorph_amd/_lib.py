@@ -38,6 +38,7 @@ SIGNATURES: dict[str, tuple] = {
     "tm_edt": (_I, [_P, _I, _I64, _I, _I64P, _DP, _P, _P, _I, _P, _SZ, _P]),
     "tm_cdt_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
     "tm_cdt": (_I, [_P, _I, _I64, _I, _I64P, _I, _P, _P, _I, _P, _SZ, _P]),
+    "tm_bfdt": (_I, [_P, _I, _I64, _I, _I64P, _DP, _I, _P, _P, _I, _P]),
     "tm_grey_workspace_bytes": (_SZ, [_I64, _I, _I64P, _I64P]),
     "tm_grey": (_I, [_P, _I64, _I, _I64P, _P, _P, _I64P, _I64P, _I, _F, _I, _P, _P, _SZ, _P]),
     "tm_sinkhorn_hist": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _I64, _I, _I, _D,

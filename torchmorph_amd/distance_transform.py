@@ -1,6 +1,6 @@
-"""Distance transforms: exact Euclidean (EDT) and chamfer (CDT).
+"""Distance transforms: exact Euclidean (EDT), chamfer (CDT) and brute force.
 
-Drop-in for reference torchmorph/distance_transform.py:10-138: same names,
+Drop-in for reference torchmorph/distance_transform.py:10-175: same names,
 arguments, validation messages and output semantics.  Differences by design:
 
 * the mask is handed to the kernels as it is (bool/uint8 bytes or float32),
@@ -177,13 +177,36 @@ def brute_force_distance_transform(
     return_indices: bool = False,
     distances: Tensor | None = None,
     indices: Tensor | None = None,
-):
-    """Not part of this build's operator path (reference bfdt_kernel.cu is an
-    O(foreground x background) test oracle).  Use
-    :func:`euclidean_distance_transform` / :func:`chamfer_distance_transform`."""
-    raise NotImplementedError(
-        "brute_force_distance_transform is not provided by torchmorph_amd; "
-        "use euclidean_distance_transform or chamfer_distance_transform")
+) -> Tensor | tuple[Tensor, Tensor] | None:
+    """Brute-force distance transform (reference distance_transform.py:141-175).
+
+    Every foreground voxel scans every background voxel of its (B, C) item:
+    O(voxels^2), a checker rather than a fast path.  float32 distances
+    (computed in FP64: ``scipy.ndimage.distance_transform_bf``'s values
+    rounded to float32), sampling applied to every metric as the reference
+    does; indices point at the first background voxel in row-major order at
+    the smallest distance.  An item without background gets +inf (euclidean)
+    or 4294967295 (taxicab, chessboard) and indices 0, as in the reference.
+    """
+    nd, return_distances, return_indices = _prepare_distance_transform(
+        input, return_distances, return_indices, distances, indices)
+    if metric not in {"euclidean", "taxicab", "chessboard"}:
+        raise ValueError("metric must be 'euclidean', 'taxicab', or 'chessboard'.")
+    samp = _normalize_sampling(sampling, nd)
+    L = _lib.lib()
+    x, in_dtype = _lib.mask_operand(input.detach())
+    batch = input.shape[0] * input.shape[1]
+    shape = list(input.shape[2:])
+    raw_d, raw_i, idx_dtype = _out_buffers(input, nd, return_distances, return_indices,
+                                           distances, indices)
+    code = {"euclidean": 0, "taxicab": 1, "chessboard": 2}[metric]
+    with torch.cuda.device(input.device):
+        shp = _lib.i64_array(shape)
+        sp = None if all(s == 1.0 for s in samp) else _lib.f64_array(samp)
+        _lib.check(L.tm_bfdt(_lib.ptr(x), in_dtype, batch, nd, shp, sp, code, _lib.ptr(raw_d),
+                             _lib.ptr(raw_i), idx_dtype, _lib.stream_of(input.device)),
+                   "tm_bfdt")
+    return _finish(raw_d, raw_i, return_distances, return_indices, distances, indices)
 
 
 # SciPy-style aliases named in the operator contract.
