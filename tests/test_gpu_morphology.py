@@ -186,3 +186,23 @@ def test_binary_empty_structure(tm):
     z = torch.zeros((3, 3), dtype=torch.bool)
     assert tm.binary_erosion(x, structure=z).all()
     assert not tm.binary_dilation(x, structure=z).any()
+
+
+@pytest.mark.parametrize("mode", ["reflect", "constant", "nearest", "mirror", "wrap"])
+@pytest.mark.parametrize("size,origin", [((5, 6, 7), (0, 1, -2)), ((8, 5, 11), (-1, 0, 3)),
+                                         ((13, 4, 6), (2, -1, 0)), ((6, 9, 3), (0, 0, 1))])
+def test_grey_box_block_scans_match_oracle(tm, oracle_mod, mode, size, origin):
+    # axes of 5 or more reduce by block prefix / suffix scans; sizes, origins
+    # and z depths that cut the scan blocks unevenly, with ties, infinities,
+    # signed zeros and NaN in the input
+    rng = np.random.default_rng(sum(size) + len(mode))
+    x = (np.floor(rng.random((1, 2, 19, 21, 70)) * 8) / 4 - 1).astype(np.float32)
+    flat = x.reshape(-1)
+    idx = rng.choice(flat.size, size=40, replace=False)
+    flat[idx[:10]] = -0.0
+    flat[idx[10:20]] = np.inf
+    flat[idx[20:30]] = -np.inf
+    flat[idx[30:]] = np.nan
+    for dilation in (False, True):
+        _grey_vs_oracle(tm, oracle_mod, x, dilation, size=size, origin=origin, mode=mode,
+                        cval=0.5)

@@ -117,9 +117,15 @@ __device__ __forceinline__ PlaneMap<NR> plane_map(const PlaneGeom& g, int y0, in
   return m;
 }
 
+// Raw loads of one plane's halo tile, in flight until plane_store: every
+// load has an in-range address (border positions read a clamped one), and
+// the border substitution and bias are applied at the store, so nothing
+// consumes the loaded registers early and the loads overlap the reduction
+// of the previous plane.
 template <int NR>
 struct PlaneRegs {
   float v[NR][2];
+  bool zc;  // the plane lies in the constant border
 };
 
 template <int NR>
@@ -127,7 +133,20 @@ __device__ __forceinline__ void plane_fetch(PlaneRegs<NR>& R, const float* __res
                                             const PlaneGeom& g, const PlaneMap<NR>& m, int zi) {
   bool zc;
   const int64_t zm = gmap(zi, g.D, g.mode, &zc);
+  R.zc = zc;
   const float* pl = vol + zm * g.H * g.W;
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int64_t row = (int64_t)(m.ym[k] < 0 ? 0 : m.ym[k]) * g.W;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) R.v[k][h] = pl[row + (m.xm[h] < 0 ? 0 : m.xm[h])];
+  }
+}
+
+template <int NR>
+__device__ __forceinline__ void plane_store(const PlaneRegs<NR>& R, const PlaneMap<NR>& m,
+                                            float* stage, const float* __restrict__ vol,
+                                            const PlaneGeom& g, int zi, int y0, int x0) {
   const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
@@ -136,25 +155,9 @@ __device__ __forceinline__ void plane_fetch(PlaneRegs<NR>& R, const float* __res
     for (int h = 0; h < 2; ++h) {
       const int c = cc + 64 * h;
       if (r < g.SY && c < g.SX) {
-        const bool border = zc || m.ym[k] < 0 || m.xm[h] < 0;
-        R.v[k][h] = (border ? g.cval : pl[(int64_t)m.ym[k] * g.W + m.xm[h]]) + g.bias;
+        const bool border = R.zc || m.ym[k] < 0 || m.xm[h] < 0;
+        stage[r * g.SX + c] = (border ? g.cval : R.v[k][h]) + g.bias;
       }
-    }
-  }
-}
-
-template <int NR>
-__device__ __forceinline__ void plane_store(const PlaneRegs<NR>& R, float* stage,
-                                            const float* __restrict__ vol, const PlaneGeom& g,
-                                            int zi, int y0, int x0) {
-  const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int r = rr + 4 * k;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int c = cc + 64 * h;
-      if (r < g.SY && c < g.SX) stage[r * g.SX + c] = R.v[k][h];
     }
   }
   if (g.SY > 4 * NR) {
@@ -200,6 +203,40 @@ __device__ __forceinline__ float grey_epi(float acc, const GreyEpi& e, int64_t p
   return acc;
 }
 
+// Windows of at least kVhgw along an axis are reduced by van Herk /
+// Gil-Werman: the axis is cut into blocks of k, each block gets its suffix
+// and prefix min/max by one serial scan, and the window [c, c + k - 1] is
+// suffix[c] combined with prefix[c + k - 1] (just suffix[c] when c starts a
+// block): three operations per value instead of k.  min/max are exact, so
+// any grouping gives the same values.
+constexpr int kVhgw = 5;
+
+// Suffixes and prefixes of the blocks of `len` values of `lines` lines: line
+// l, value j at a[l * ls + j * es].  Suffix j goes to sfx[l * ss + j * es2]
+// for j < nout (the outputs' window starts); the prefix overwrites a in
+// place (only values at j >= k - 1 are read afterwards).
+template <bool MAX>
+__device__ __forceinline__ void vhgw_scan(float* a, int ls, int es, float* sfx, int ss, int es2,
+                                          int lines, int len, int nout, int k) {
+  const int nb = (len + k - 1) / k;
+  const float ID = gid<MAX>();
+  for (int task = threadIdx.x; task < lines * nb; task += blockDim.x) {
+    const int l = task / nb, b = task - l * nb;
+    const int j0 = b * k, j1 = min(len, j0 + k);
+    float* al = a + l * ls;
+    float acc = ID;
+    for (int j = j1 - 1; j >= j0; --j) {
+      acc = gop<MAX>(al[j * es], acc);
+      if (j < nout) sfx[l * ss + j * es2] = acc;
+    }
+    acc = ID;
+    for (int j = j0; j < j1; ++j) {
+      acc = gop<MAX>(acc, al[j * es]);
+      al[j * es] = acc;
+    }
+  }
+}
+
 // One (x, y) tile of one item over output planes [zb, ze).
 template <bool MAX, int NR, int EPI = 0>
 __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
@@ -216,41 +253,79 @@ __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, MAX ? 0.0f : -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
   const PlaneMap<NR> pm = plane_map<NR>(g, y0 + sy, x0 + sx);
+  const bool vx = kx >= kVhgw, vy = ky >= kVhgw, vz = kz >= kVhgw;
+  // z by blocks of kz planes: a running prefix per output voxel (this
+  // thread's four), and the ring holding the previous block's suffixes
+  float zpre[kTY * kTX / 256];
   PlaneRegs<NR> R;
   plane_fetch(R, vol, g, pm, zb + sz);
   for (int n = 0; n < nplanes; ++n) {
     const int zi = zb + sz + n;
-    plane_store(R, stage, vol, g, zi, y0 + sy, x0 + sx);
+    plane_store(R, pm, stage, vol, g, zi, y0 + sy, x0 + sx);
     __syncthreads();
     // the next plane's loads are in flight while this one is reduced
     if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
-    for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
-      const int r = e >> 6, c = e & 63;
-      const float* s = stage + r * SX + c;
-      float acc = ID;
-      for (int l = 0; l < kx; ++l) acc = gop<MAX>(acc, s[l]);
-      xr[e] = acc;
+    if (vx) {
+      vhgw_scan<MAX>(stage, SX, 1, xr, kTX, 1, SY, SX, kTX, kx);
+      __syncthreads();
+      for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        if (c % kx) xr[e] = gop<MAX>(xr[e], stage[r * SX + c + kx - 1]);
+      }
+    } else {
+      for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        const float* s = stage + r * SX + c;
+        float acc = ID;
+        for (int l = 0; l < kx; ++l) acc = gop<MAX>(acc, s[l]);
+        xr[e] = acc;
+      }
     }
     __syncthreads();
     float* slot = ring + (n % kz) * (kTY * kTX);
-    for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
-      const int r = e >> 6, c = e & 63;
-      float acc = ID;
-      for (int j = 0; j < ky; ++j) acc = gop<MAX>(acc, xr[(r + j) * kTX + c]);
-      slot[e] = acc;
+    if (vy) {
+      vhgw_scan<MAX>(xr, 1, kTX, slot, 1, kTX, kTX, SY, kTY, ky);
+      __syncthreads();
+      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        if (r % ky) slot[e] = gop<MAX>(slot[e], xr[(r + ky - 1) * kTX + c]);
+      }
+    } else {
+      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        float acc = ID;
+        for (int j = 0; j < ky; ++j) acc = gop<MAX>(acc, xr[(r + j) * kTX + c]);
+        slot[e] = acc;
+      }
     }
     __syncthreads();
     const int z = zb + n - (kz - 1);
-    if (z >= zb) {
-      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
-        const int y = y0 + r, x = x0 + c;
-        if (y < H && x < W) {
-          float acc = ID;
-          for (int i = 0; i < kz; ++i) acc = gop<MAX>(acc, ring[i * (kTY * kTX) + e]);
-          const int64_t p = ((int64_t)z * H + y) * W + x;
-          ovol[p] = grey_epi<EPI>(acc, epi, p);
+    const int pos = n % kz;  // place of plane n in its block of kz
+#pragma unroll
+    for (int q = 0; q < kTY * kTX / 256; ++q) {
+      const int e = q * 256 + (int)threadIdx.x;
+      float acc;
+      if (vz) {
+        const float v = ring[pos * (kTY * kTX) + e];
+        zpre[q] = pos == 0 ? v : gop<MAX>(zpre[q], v);
+        // window planes n - kz + 1 .. n: the previous block's suffix from
+        // n - kz + 1 (ring slot pos + 1) and this block's prefix
+        acc = pos == kz - 1 ? zpre[q] : gop<MAX>(ring[(pos + 1) * (kTY * kTX) + e], zpre[q]);
+        if (pos == kz - 1) {  // block complete: its suffixes, in place
+          for (int i = kz - 2; i >= 0; --i)
+            ring[i * (kTY * kTX) + e] =
+                gop<MAX>(ring[i * (kTY * kTX) + e], ring[(i + 1) * (kTY * kTX) + e]);
         }
+      } else {
+        acc = ID;
+        if (z >= zb)
+          for (int i = 0; i < kz; ++i) acc = gop<MAX>(acc, ring[i * (kTY * kTX) + e]);
+      }
+      const int r = e >> 6, c = e & 63;
+      const int y = y0 + r, x = x0 + c;
+      if (z >= zb && y < H && x < W) {
+        const int64_t p = ((int64_t)z * H + y) * W + x;
+        ovol[p] = grey_epi<EPI>(acc, epi, p);
       }
     }
   }
@@ -301,7 +376,7 @@ __device__ __forceinline__ void grey_box3_dual_tile(const float* __restrict__ vo
   plane_fetch(R, vol, g, pm, zb + sz);
   for (int n = 0; n < nplanes; ++n) {
     const int zi = zb + sz + n;
-    plane_store(R, stage, vol, g, zi, y0 + sy, x0 + sx);
+    plane_store(R, pm, stage, vol, g, zi, y0 + sy, x0 + sx);
     __syncthreads();
     if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
     for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
@@ -395,7 +470,7 @@ __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
   plane_fetch(R, vol, g, pm, zb + zlo);
   for (int n = 0; n < nplanes; ++n) {
     const int zi = zb + zlo + n;
-    plane_store(R, sm + (n % kz) * plane, vol, g, zi, y0 + ylo, x0 + xlo);
+    plane_store(R, pm, sm + (n % kz) * plane, vol, g, zi, y0 + ylo, x0 + xlo);
     // plane of offset dz sits at ring index (n - (zhi - dz)) mod kz
     for (int t = threadIdx.x; t < tab.n; t += blockDim.x) {
       const int ridx = ((n - (zhi - tab.dz[t])) % kz + kz) % kz;
