@@ -42,12 +42,14 @@ __device__ __forceinline__ void axis0_nearest(const TIn* __restrict__ in, int64_
     const int i0 = c * kA0Chunk;
     const int n = L - i0 < kA0Chunk ? L - i0 : kA0Chunk;
     const TIn* p = in + base + (int64_t)i0 * step;
+    // every load in range (past the line's end: its last voxel), issued
+    // together, the bits formed afterwards
     TIn m[kA0Chunk];
 #pragma unroll
-    for (int k = 0; k < kA0Chunk; ++k) m[k] = k < n ? p[(int64_t)k * step] : TIn(1);
+    for (int k = 0; k < kA0Chunk; ++k) m[k] = p[(int64_t)(k < n ? k : n - 1) * step];
     uint32_t bm = 0;
 #pragma unroll
-    for (int k = 0; k < kA0Chunk; ++k) bm |= (m[k] == TIn(0) ? 1u : 0u) << k;
+    for (int k = 0; k < kA0Chunk; ++k) bm |= (k < n && m[k] == TIn(0) ? 1u : 0u) << k;
     bmw[(int64_t)c * nl] = bm;
     pvw[(int64_t)c * nl] = last;
     if (bm) last = i0 + 31 - __clz(bm);
@@ -135,7 +137,7 @@ __device__ __forceinline__ void cdt_taxicab_chunked(int32_t* __restrict__ w, int
     const int n = L - i0 < K ? L - i0 : K;
     int32_t a[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) a[k] = k < n ? p[(int64_t)(i0 + k) * step] : 0;
+    for (int k = 0; k < K; ++k) a[k] = p[(int64_t)(i0 + (k < n ? k : n - 1)) * step];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (k >= n) break;
@@ -151,7 +153,7 @@ __device__ __forceinline__ void cdt_taxicab_chunked(int32_t* __restrict__ w, int
     const int lo = i1 > K ? i1 - K : 0, n = i1 - lo;
     int32_t a[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) a[k] = k < n ? p[(int64_t)(lo + k) * step] : 0;
+    for (int k = 0; k < K; ++k) a[k] = p[(int64_t)(lo + (k < n ? k : n - 1)) * step];
 #pragma unroll
     for (int k = K - 1; k >= 0; --k) {
       if (k >= n) continue;

@@ -19,7 +19,10 @@
 namespace tmh {
 
 // Features of planes 0 and 1 of the next PF voxels of a line (planes above 1
-// are read where they are used: items of four or more axes).
+// are read where they are used: items of four or more axes).  The loads are
+// unconditional (past the line's end they read its last voxel, which the
+// consumer skips) and nothing touches the registers until the chunk is
+// consumed, so they stay in flight meanwhile.
 template <int PF>
 struct LineChunk {
   int32_t f0[PF];
@@ -33,11 +36,11 @@ __device__ __forceinline__ void line_chunk_load(LineChunk<PF>& ch,
                                                 int d) {
 #pragma unroll
   for (int k = 0; k < PF; ++k)
-    ch.f0[k] = (i0 + k < L) ? fin[base + (int64_t)(i0 + k) * step] : -1;
+    ch.f0[k] = fin[base + (int64_t)(i0 + k < L ? i0 + k : L - 1) * step];
   if (d >= 2) {
 #pragma unroll
     for (int k = 0; k < PF; ++k)
-      ch.f1[k] = (i0 + k < L) ? fin[plane + base + (int64_t)(i0 + k) * step] : 0;
+      ch.f1[k] = fin[plane + base + (int64_t)(i0 + k < L ? i0 + k : L - 1) * step];
   }
 }
 
@@ -67,7 +70,7 @@ __device__ __forceinline__ void edt_line_stack_fast(const int32_t* __restrict__ 
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
       const int32_t f0 = cur.f0[k];
-      if (f0 < 0) continue;  // also every voxel past the line's end
+      if (i0 + k >= L || f0 < 0) continue;
       const int i = i0 + k;
       double wR = 0.0;
       {
@@ -449,7 +452,7 @@ __device__ __forceinline__ void edt_line_stack_c(const int32_t* __restrict__ fin
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
       const int32_t f0 = cur.f0[k];
-      if (f0 < 0) continue;  // also every voxel past the line's end
+      if (i0 + k >= L || f0 < 0) continue;
       const int32_t f1 = D == 2 ? cur.f1[k] : 0;
       env.add(i0 + k, f0, f1, edt_partial<D>(f0, f1, env.c0, env.c1, env.s0, env.s1));
     }

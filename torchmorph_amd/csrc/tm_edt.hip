@@ -65,11 +65,29 @@ __device__ __forceinline__ uint32_t bgbits(const TIn* __restrict__ p, int x, int
   return r;
 }
 
+// Background bits of a loaded word: RAWW words are the raw data (one
+// element for CPT = 1, four bytes for CPT = 4), turned into bits only after
+// every load of the thread has been issued; other words are bits already.
+template <typename TIn, int CPT, bool RAWW>
+__device__ __forceinline__ uint32_t word_bits(uint32_t w) {
+  if (!RAWW) return w;
+  if (CPT == 1) {
+    if constexpr (sizeof(TIn) == 4) return __uint_as_float(w) == 0.0f ? 1u : 0u;
+    else return w == 0 ? 1u : 0u;
+  }
+  return (uint32_t)((w & 0xffu) == 0) | ((uint32_t)((w & 0xff00u) == 0) << 1) |
+         ((uint32_t)((w & 0xff0000u) == 0) << 2) | ((uint32_t)((w & 0xff000000u) == 0) << 3);
+}
+
 // CPT adjacent columns per thread (1 or 4), 16 column groups x 16 row
-// segments per workgroup, NCH chunks of 16 rows per segment.
-template <typename TIn, int NCH, int CPT>
+// segments per workgroup, NCH chunks of 16 rows per segment.  VEC: 4-byte
+// aligned rows of a byte mask (W % 4 == 0).
+template <typename TIn, int NCH, int CPT, bool VEC>
 __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__ in, int H, int W,
                                                          int16_t* __restrict__ f0, int aligned) {
+  // raw words: one element per row (CPT = 1) or four mask bytes (CPT = 4,
+  // VEC), loaded unconditionally so that all of them are in flight at once
+  constexpr bool RAWW = CPT == 1 || (VEC && sizeof(TIn) == 1);
   constexpr int WGC = 16 * CPT;  // columns per workgroup
   __shared__ int16_t sfirst[kColSegs][WGC];
   __shared__ int16_t slast[kColSegs][WGC];
@@ -95,7 +113,15 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
 #pragma unroll
       for (int b = 0; b < kChunkRows; ++b) {
         const int y = min(ys + k * kChunkRows + b, ye - 1);
-        raw[k][b] = bgbits<TIn, CPT>(im + (int64_t)y * W, xb, W, vec);
+        const TIn* row = im + (int64_t)y * W;
+        if constexpr (RAWW && CPT == 1) {
+          if constexpr (sizeof(TIn) == 4) raw[k][b] = __float_as_uint((float)row[xb]);
+          else raw[k][b] = (uint32_t)row[xb];
+        } else if constexpr (RAWW) {
+          raw[k][b] = *(const uint32_t*)(row + xb);
+        } else {
+          raw[k][b] = bgbits<TIn, CPT>(row, xb, W, vec);
+        }
       }
   }
   uint32_t m[CPT][NCH];
@@ -107,7 +133,7 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
 #pragma unroll
     for (int b = 0; b < kChunkRows; ++b) {
       const bool rowok = live && (ys + k * kChunkRows + b < ye);
-      const uint32_t v = rowok ? raw[k][b] : 0u;
+      const uint32_t v = rowok ? word_bits<TIn, CPT, RAWW>(raw[k][b]) : 0u;
 #pragma unroll
       for (int c = 0; c < CPT; ++c) mk[c] |= ((v >> c) & 1u) << b;
     }
@@ -409,6 +435,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     int16_t* f0 = (int16_t*)ws;
     const int seglen = (H + kColSegs - 1) / kColSegs;
     const int aligned = ((uintptr_t)in % 16) == 0;
+    const bool vecw = sizeof(TIn) == 1 && aligned && W % 4 == 0;
     // variant v: columns per thread = (v & 1) ? 4 : 1, rows per band = 1 << (v >> 1)
     auto run = [&](int v) -> int {
       const int cpt = (v & 1) ? 4 : 1;
@@ -423,9 +450,13 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
         return TM_ERR_ARG;
       }
       const dim3 cgrid((unsigned)(ncb * batch));
-#define TM_COLS(NCH, CPT)                                                                         \
-  hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, CPT>), cgrid, dim3(256), 0, st, in, H, W, f0, \
-                     aligned)
+#define TM_COLS(NCH, CPT)                                                                   \
+  if (vecw)                                                                                 \
+    hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, CPT, true>), cgrid, dim3(256), 0, st, in, \
+                       H, W, f0, aligned);                                                  \
+  else                                                                                      \
+    hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, CPT, false>), cgrid, dim3(256), 0, st, in, \
+                       H, W, f0, aligned)
       if (cpt == 4) {
         if (seglen <= 16) TM_COLS(1, 4);
         else if (seglen <= 32) TM_COLS(2, 4);
