@@ -33,6 +33,9 @@ struct Packed {
   int64_t W;                // row length in voxels
 };
 
+// One packed word per lane.  The 64 loads are unconditional (past the row's
+// end they read its last element) and the bits are formed after all of them
+// were issued.
 template <typename TIn>
 __global__ __launch_bounds__(256) void bin_pack_kernel(const TIn* __restrict__ in,
                                                        uint64_t* __restrict__ out, Packed pk,
@@ -42,10 +45,41 @@ __global__ __launch_bounds__(256) void bin_pack_kernel(const TIn* __restrict__ i
   const int64_t row = g / pk.ww;
   const int64_t w = g - row * pk.ww;
   const TIn* src = in + row * pk.W + w * 64;
-  const int64_t nb = min((int64_t)64, pk.W - w * 64);
+  const int nb = (int)min((int64_t)64, pk.W - w * 64);
+  TIn v[64];
+#pragma unroll
+  for (int b = 0; b < 64; ++b) v[b] = src[b < nb ? b : nb - 1];
   uint64_t bits = 0;
-  for (int b = 0; b < nb; ++b)
-    if (src[b] != TIn(0)) bits |= 1ull << b;
+#pragma unroll
+  for (int b = 0; b < 64; ++b)
+    if (b < nb && v[b] != TIn(0)) bits |= 1ull << b;
+  out[g] = bits;
+}
+
+// Nonzero bytes of a 32-bit word as 4 bits (byte i -> bit i).
+__device__ __forceinline__ uint32_t nz4(uint32_t v) {
+  const uint32_t hi = (((v & 0x7f7f7f7fu) + 0x7f7f7f7fu) | v) & 0x80808080u;
+  return (((hi >> 7) * 0x01020408u) >> 24) & 0xfu;
+}
+
+// Byte masks whose rows are whole 64-byte words (W % 64 == 0, 16-byte
+// aligned input): four 16-byte loads per word.
+__global__ __launch_bounds__(256) void bin_pack_bytes_kernel(const uint8_t* __restrict__ in,
+                                                             uint64_t* __restrict__ out,
+                                                             int64_t nwords) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nwords) return;
+  const uint4* src = (const uint4*)(in + g * 64);
+  uint4 q[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = src[k];
+  uint64_t bits = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t b16 = nz4(q[k].x) | (nz4(q[k].y) << 4) | (nz4(q[k].z) << 8) |
+                         (nz4(q[k].w) << 12);
+    bits |= (uint64_t)b16 << (16 * k);
+  }
   out[g] = bits;
 }
 
@@ -59,12 +93,35 @@ __global__ __launch_bounds__(256) void bin_unpack_kernel(const uint64_t* __restr
   out[g] = (uint8_t)((in[row * pk.ww + (x >> 6)] >> (x & 63)) & 1ull);
 }
 
-// word q of row `row` of one item with everything outside [0, W) reading
-// `border` (bits past W in the last word are masked).
-__device__ __forceinline__ uint64_t bin_word(const uint64_t* __restrict__ item, const Packed& pk,
-                                             int64_t row, int64_t q, uint64_t border) {
+// 4 bits as 4 bytes of 0 / 1 (bit i -> byte i).
+__device__ __forceinline__ uint32_t bytes4(uint32_t b) {
+  return (b * 0x00204081u) & 0x01010101u;
+}
+
+// Rows of whole 16-byte groups (W % 16 == 0, 16-byte aligned output): 16
+// output bytes per lane, one 16-byte store.
+__global__ __launch_bounds__(256) void bin_unpack_bytes_kernel(const uint64_t* __restrict__ in,
+                                                               uint8_t* __restrict__ out,
+                                                               Packed pk, int64_t ngroups) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const int64_t gpr = pk.W / 16;  // groups per row
+  const int64_t row = g / gpr;
+  const int64_t x = (g - row * gpr) * 16;
+  const uint32_t b = (uint32_t)((in[row * pk.ww + (x >> 6)] >> (x & 63)) & 0xffffull);
+  uint4 v;
+  v.x = bytes4(b & 0xfu);
+  v.y = bytes4((b >> 4) & 0xfu);
+  v.z = bytes4((b >> 8) & 0xfu);
+  v.w = bytes4((b >> 12) & 0xfu);
+  *(uint4*)(out + g * 16) = v;
+}
+
+// Loaded word v of word index q of a row, with everything outside [0, W)
+// reading `border` (q outside the row, bits past W in the last word).
+__device__ __forceinline__ uint64_t bin_fix(uint64_t v, const Packed& pk, int64_t q,
+                                            uint64_t border) {
   if (q < 0 || q >= pk.ww) return border;
-  uint64_t v = item[row * pk.ww + q];
   const int64_t valid = pk.W - q * 64;
   if (valid < 64) {
     const uint64_t m = (1ull << valid) - 1ull;
@@ -102,32 +159,55 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
       r /= pk.size[j];
     }
   }
+  // elements in groups of 8: every address of a group is formed first
+  // (words outside the item read word 0 and are replaced by the border after
+  // the load), the 16 loads are issued together, then the terms combine
   uint64_t acc = dilation ? 0ull : ~0ull;
-  for (int t = 0; t < tab.n; ++t) {
-    bool inb = true;
-    int64_t r2 = row;
-    for (int j = 0; j < lead; ++j) {
-      const int64_t cj = c[j] + tab.d[t][j];
-      if (cj < 0 || cj >= pk.size[j]) { inb = false; break; }
-      r2 += (int64_t)tab.d[t][j] * pk.rstride[j];
-    }
-    uint64_t term;
-    if (!inb) {
-      term = border;
-    } else {
+  for (int t0 = 0; t0 < tab.n; t0 += 8) {
+    int64_t a_lo[8], q_lo[8];
+    int sh[8];
+    bool inb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u < tab.n ? t0 + u : tab.n - 1;
+      bool ok = true;
+      int64_t r2 = row;
+      for (int j = 0; j < lead; ++j) {
+        const int64_t cj = c[j] + tab.d[t][j];
+        ok = ok && cj >= 0 && cj < pk.size[j];
+        r2 += (int64_t)tab.d[t][j] * pk.rstride[j];
+      }
       const int64_t tb = w * 64 + tab.d[t][lead];
       const int64_t q = tb >= 0 ? tb / 64 : -((-tb + 63) / 64);
-      const int s = (int)(tb - q * 64);
-      const uint64_t lo = bin_word(src, pk, r2, q, border);
-      if (s == 0) {
-        term = lo;
-      } else {
-        const uint64_t hi = bin_word(src, pk, r2, q + 1, border);
-        term = (lo >> s) | (hi << (64 - s));
-      }
+      sh[u] = (int)(tb - q * 64);
+      inb[u] = ok;
+      q_lo[u] = q;
+      a_lo[u] = ok ? r2 * pk.ww + q : 0;
     }
-    if (dilation) acc |= term;
-    else acc &= term;
+    uint64_t lo[8], hi[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const bool l_in = inb[u] && q_lo[u] >= 0 && q_lo[u] < pk.ww;
+      const bool h_in = inb[u] && q_lo[u] + 1 >= 0 && q_lo[u] + 1 < pk.ww;
+      lo[u] = src[l_in ? a_lo[u] : 0];
+      hi[u] = src[h_in ? a_lo[u] + 1 : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (t0 + u >= tab.n) break;
+      uint64_t term = border;
+      if (inb[u]) {
+        const uint64_t l = bin_fix(lo[u], pk, q_lo[u], border);
+        if (sh[u] == 0) {
+          term = l;
+        } else {
+          const uint64_t h = bin_fix(hi[u], pk, q_lo[u] + 1, border);
+          term = (l >> sh[u]) | (h << (64 - sh[u]));
+        }
+      }
+      if (dilation) acc |= term;
+      else acc &= term;
+    }
   }
   if (mask) {
     const uint64_t m = mask[g];
@@ -193,10 +273,19 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
   uint64_t* mpk = orig + nwords;
   int* flags = (int*)(mpk + nwords);
   const dim3 wgrid(grid_for(nwords, 256)), blk(256);
-  hipLaunchKernelGGL((bin_pack_kernel<T>), wgrid, blk, 0, st, input, bufA, pk, nwords);
+  auto bytes_ok = [&](const void* p) { return pk.W % 64 == 0 && ((uintptr_t)p % 16) == 0; };
+  if (sizeof(T) == 1 && bytes_ok(input))
+    hipLaunchKernelGGL(bin_pack_bytes_kernel, wgrid, blk, 0, st, (const uint8_t*)input, bufA,
+                       nwords);
+  else
+    hipLaunchKernelGGL((bin_pack_kernel<T>), wgrid, blk, 0, st, input, bufA, pk, nwords);
   TM_LAUNCH_CHECK();
   if (mask_bytes) {
-    hipLaunchKernelGGL((bin_pack_kernel<uint8_t>), wgrid, blk, 0, st, mask_bytes, mpk, pk, nwords);
+    if (bytes_ok(mask_bytes))
+      hipLaunchKernelGGL(bin_pack_bytes_kernel, wgrid, blk, 0, st, mask_bytes, mpk, nwords);
+    else
+      hipLaunchKernelGGL((bin_pack_kernel<uint8_t>), wgrid, blk, 0, st, mask_bytes, mpk, pk,
+                         nwords);
     TM_LAUNCH_CHECK();
   }
   uint64_t* cur = bufA;
@@ -244,8 +333,12 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
       }
     }
   }
-  hipLaunchKernelGGL(bin_unpack_kernel, dim3(grid_for(total, 256)), blk, 0, st, cur, output, pk,
-                     total);
+  if (pk.W % 16 == 0 && ((uintptr_t)output % 16) == 0)
+    hipLaunchKernelGGL(bin_unpack_bytes_kernel, dim3(grid_for(total / 16, 256)), blk, 0, st, cur,
+                       output, pk, total / 16);
+  else
+    hipLaunchKernelGGL(bin_unpack_kernel, dim3(grid_for(total, 256)), blk, 0, st, cur, output,
+                       pk, total);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
