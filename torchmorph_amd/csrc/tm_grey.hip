@@ -457,7 +457,8 @@ __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
                                                  const GreyTable& tab, int zlo, int zhi, int ylo,
                                                  int yhi, int xlo, int xhi, int mode, float cval,
                                                  int tx, int ty, int zb, int ze, float* sm,
-                                                 int* soff, GreyEpi epi = GreyEpi{}) {
+                                                 int* soff, float* sval,
+                                                 GreyEpi epi = GreyEpi{}) {
   const int kz = zhi - zlo + 1;
   const int SX = kTX + xhi - xlo, SY = kTY + yhi - ylo;
   const int plane = SY * SX;
@@ -471,24 +472,51 @@ __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
   for (int n = 0; n < nplanes; ++n) {
     const int zi = zb + zlo + n;
     plane_store(R, pm, sm + (n % kz) * plane, vol, g, zi, y0 + ylo, x0 + xlo);
-    // plane of offset dz sits at ring index (n - (zhi - dz)) mod kz
-    for (int t = threadIdx.x; t < tab.n; t += blockDim.x) {
-      const int ridx = ((n - (zhi - tab.dz[t])) % kz + kz) % kz;
-      soff[t] = ridx * plane + (tab.dy[t] - ylo) * SX + (tab.dx[t] - xlo);
+    // plane of offset dz sits at ring index (n - (zhi - dz)) mod kz; the
+    // table is padded to a multiple of 4 with elements that read offset 0
+    // and add the identity (they change no min / max)
+    const int npad = (tab.n + 3) & ~3;
+    for (int t = threadIdx.x; t < npad; t += blockDim.x) {
+      if (t < tab.n) {
+        const int ridx = ((n - (zhi - tab.dz[t])) % kz + kz) % kz;
+        soff[t] = ridx * plane + (tab.dy[t] - ylo) * SX + (tab.dx[t] - xlo);
+        sval[t] = tab.v[t];
+      } else {
+        soff[t] = 0;
+        sval[t] = ID;
+      }
     }
     __syncthreads();
     if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
     const int z = zb + n - (kz - 1);
     if (z >= zb) {
-      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
-        const int y = y0 + r, x = x0 + c;
-        const float* base = sm + r * SX + c;
-        float acc = ID;
-        for (int t = 0; t < tab.n; ++t) acc = gop<MAX>(acc, base[soff[t]] + tab.v[t]);
+      // four adjacent outputs of one row per lane: each element's offset and
+      // value are read once for the four, four independent chains
+      const int r = threadIdx.x >> 4, c0 = (threadIdx.x & 15) * 4;
+      const float* base = sm + r * SX + c0;
+      float a0 = ID, a1 = ID, a2 = ID, a3 = ID;
+      for (int t = 0; t < npad; t += 4) {
+        const int4 o = *(const int4*)(soff + t);
+        const float4 v = *(const float4*)(sval + t);
+        const int oo[4] = {o.x, o.y, o.z, o.w};
+        const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float* b = base + oo[u];
+          a0 = gop<MAX>(a0, b[0] + vv[u]);
+          a1 = gop<MAX>(a1, b[1] + vv[u]);
+          a2 = gop<MAX>(a2, b[2] + vv[u]);
+          a3 = gop<MAX>(a3, b[3] + vv[u]);
+        }
+      }
+      const int y = y0 + r;
+      const float acc[4] = {a0, a1, a2, a3};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int x = x0 + c0 + u;
         if (y < H && x < W) {
           const int64_t p = ((int64_t)z * H + y) * W + x;
-          ovol[p] = grey_epi<EPI>(acc, epi, p);
+          ovol[p] = grey_epi<EPI>(acc[u], epi, p);
         }
       }
     }
@@ -504,7 +532,8 @@ __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restric
                                                           int mode, float cval, int zchunk,
                                                           GreyEpi epi) {
   extern __shared__ float sm[];
-  __shared__ int soff[kMaxTable];
+  __shared__ __align__(16) int soff[kMaxTable];
+  __shared__ __align__(16) float sval[kMaxTable];
   const int ntx = (W + kTX - 1) / kTX, nty = (H + kTY - 1) / kTY;
   const int nz = (D + zchunk - 1) / zchunk;
   int tx, ty, zci;
@@ -515,7 +544,7 @@ __global__ __launch_bounds__(256) void grey_table3_kernel(const float* __restric
   const GreyEpi ei{epi.a ? epi.a + item * n : nullptr, epi.b ? epi.b + item * n : nullptr};
   grey_table3_tile<MAX, NR, EPI>(in + item * n, out + item * n, D, H, W, tab, zlo, zhi, ylo, yhi,
                                  xlo, xhi, mode, cval, tx, ty, zb, min(D, zb + zchunk), sm, soff,
-                                 ei);
+                                 sval, ei);
 }
 
 // Ragged lists (no reference counterpart): items of different (D, H, W) at
@@ -562,12 +591,13 @@ __global__ __launch_bounds__(256) void grey_table3_ragged_kernel(
     const int32_t* __restrict__ blk_item, GreyTable tab, int zlo, int zhi, int ylo, int yhi,
     int xlo, int xhi, int mode, float cval) {
   extern __shared__ float sm[];
-  __shared__ int soff[kMaxTable];
+  __shared__ __align__(16) int soff[kMaxTable];
+  __shared__ __align__(16) float sval[kMaxTable];
   int tx, ty, zb;
   const GreyRgItem it = grey_rg_block(items, blk_item, &tx, &ty, &zb);
   grey_table3_tile<MAX, NR>(in + it.in_off, out + it.out_off, it.D, it.H, it.W, tab, zlo, zhi,
                             ylo, yhi, xlo, xhi, mode, cval, tx, ty, zb,
-                            min(it.D, zb + kRgZChunk), sm, soff);
+                            min(it.D, zb + kRgZChunk), sm, soff, sval);
 }
 
 // One 1-D window pass along `axis` (flat SE): out = op over in[p + (start+l)e_axis].
