@@ -189,10 +189,10 @@ def test_binary_empty_structure(tm):
 
 
 @pytest.mark.parametrize("mode", ["reflect", "constant", "nearest", "mirror", "wrap"])
-@pytest.mark.parametrize("size,origin", [((5, 6, 7), (0, 1, -2)), ((8, 5, 11), (-1, 0, 3)),
-                                         ((13, 4, 6), (2, -1, 0)), ((6, 9, 3), (0, 0, 1))])
+@pytest.mark.parametrize("size,origin", [((11, 6, 13), (0, 1, -2)), ((8, 12, 10), (-1, 0, 3)),
+                                         ((14, 4, 6), (2, -1, 0)), ((6, 9, 3), (0, 0, 1))])
 def test_grey_box_block_scans_match_oracle(tm, oracle_mod, mode, size, origin):
-    # axes of 5 or more reduce by block prefix / suffix scans; sizes, origins
+    # axes of 10 or more reduce by block prefix / suffix scans; sizes, origins
     # and z depths that cut the scan blocks unevenly, with ties, infinities,
     # signed zeros and NaN in the input
     rng = np.random.default_rng(sum(size) + len(mode))
