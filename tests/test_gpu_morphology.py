@@ -161,6 +161,29 @@ def test_binary_vs_oracle(tm, oracle_mod, shape, iterations, dilation):
     np.testing.assert_array_equal(got.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("shape,iterations", [((2, 1, 37, 21, 130), 4), ((1, 2, 19, 40, 64), 9),
+                                              ((1, 1, 70, 131), 20), ((3, 1, 300), 3),
+                                              ((1, 1, 5, 6, 7, 70), 2)])
+@pytest.mark.parametrize("border_value", [False, True])
+def test_binary_fused_steps_vs_oracle(tm, oracle_mod, shape, iterations, border_value):
+    # several steps per launch in LDS tiles: tile and item edges, rows of
+    # more than one word, reach 2 along every axis, origins, step counts above
+    # one launch's share, and (4 axes) the per-step kernel
+    rng = np.random.default_rng(sum(shape) + iterations)
+    nd = len(shape) - 2
+    x = rng.random(shape) < 0.75
+    st = np.asarray(tm.iterate_structure(tm.generate_binary_structure(nd, 1), 2))
+    st = st & (rng.random(st.shape) < 0.8)
+    st.flat[st.size // 2] = True
+    origin = tuple(int(v) for v in rng.integers(-1, 2, size=nd))
+    for dilation in (False, True):
+        fn = tm.binary_dilation if dilation else tm.binary_erosion
+        got = fn(torch.as_tensor(x, device="cuda"), structure=torch.as_tensor(st),
+                 iterations=iterations, border_value=border_value, origin=origin)
+        want = oracle_mod.binary(x, st, origin, border_value, dilation, iterations=iterations)
+        np.testing.assert_array_equal(got.cpu().numpy(), want, err_msg=f"dilation={dilation}")
+
+
 def test_binary_full_structure_border_and_mask(tm, oracle_mod):
     rng = np.random.default_rng(5)
     x = rng.random((2, 1, 20, 25, 70)) < 0.6

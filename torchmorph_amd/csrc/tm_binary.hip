@@ -221,6 +221,244 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
   }
 }
 
+// Several steps of one direction fused in LDS, for items of up to three axes
+// (z, y, x; missing leading axes have size 1) without a mask.  A workgroup
+// owns a tile of ty x tz rows of an item and loads it with a halo of hy / hz
+// rows (hy = steps * max |d_y|, likewise z) into LDS; every step recomputes
+// the rows that the remaining steps still need (the region shrinks by the
+// structure's reach per step) and the last step leaves the tile exact.  Rows
+// outside the item hold `border` in every step, as the step kernel reads
+// them.  A row sits in LDS as ww words between two pad words of `border`, so
+// the x offsets (|d_x| <= 63) read words w - 1 .. w + 1 without checks; bits
+// past W in the last word are reset to `border` after every step.
+struct BinTileGeom {
+  int Z, Y, ww, W;      // item rows along z, y; words per row; voxels per row
+  float inv_pitch, inv_by, inv_ww, inv_ty;  // 1 / divisor for small_div
+  int n;                 // structure elements
+  // element e reads LDS word (row + eo[e] + w) and the one after it, shifted
+  // right by es[e]: eo holds the element's row offset and the word of bit
+  // 64 w + d_x relative to w, plus the pad word; entries past n repeat
+  // element 0 (plain words, read with scalar loads)
+  int eo[kMaxBinTable], es[kMaxBinTable];
+  int ty, tz, hy, hz;   // tile rows and halo rows
+  int by, bz, pitch;    // LDS rows along y, z (tile + 2 halo); words per LDS row
+  int tiles_y, tiles_z;
+  int ry, rz;           // reach of the structure along y, z
+};
+
+// a / d for 0 <= a < 65536 and 1 <= d <= 1024, with inv = 1.0f / d rounded
+// to float: (a + 0.5) * inv lies within 2^-24 relative of (a + 0.5) / d,
+// closer than 0.5 / d to its integer part, so truncation gives the quotient
+// (checked exhaustively over that range).
+__device__ __forceinline__ int small_div(int a, float inv) {
+  return (int)(((float)a + 0.5f) * inv);
+}
+
+// Bits sh .. sh + 63 of the 128-bit pair (hi:lo), 0 <= sh <= 63, without a
+// branch: hi << (64 - sh) taken in two shifts is 0 for sh = 0.
+__device__ __forceinline__ uint64_t bin_shift(uint64_t lo, uint64_t hi, int sh) {
+  return (lo >> sh) | ((hi << 1) << (63 - sh));
+}
+
+// NE > 0: at most NE elements, their LDS offsets in registers and the
+// element loop unrolled (every load of a word issued together); NE == 0: any
+// count, offsets in LDS.
+template <int NE>
+__global__ __launch_bounds__(256) void bin_tile_kernel(const uint64_t* __restrict__ in,
+                                                       uint64_t* __restrict__ out,
+                                                       BinTileGeom g, int dilation,
+                                                       int border_value, int steps) {
+  extern __shared__ uint64_t bin_lds[];
+  __shared__ int eo_s[NE > 0 ? 1 : kMaxBinTable];
+  __shared__ int es_s[NE > 0 ? 1 : kMaxBinTable];
+  const int per = g.by * g.bz * g.pitch;
+  uint64_t* A = bin_lds;
+  uint64_t* B = bin_lds + per;
+  const uint64_t border = border_value ? ~0ull : 0ull;
+  const int tile = blockIdx.x;
+  const int y0 = (tile % g.tiles_y) * g.ty - g.hy;
+  const int z0 = (tile / g.tiles_y) * g.tz - g.hz;
+  const int64_t item_rows = (int64_t)g.Z * g.Y;
+  const uint64_t* src = in + (int64_t)blockIdx.y * item_rows * g.ww;
+  uint64_t* dst = out + (int64_t)blockIdx.y * item_rows * g.ww;
+  const int last_valid = g.W - (g.ww - 1) * 64;  // voxels in the last word, 1..64
+  const uint64_t last_m = last_valid >= 64 ? ~0ull : ((1ull << last_valid) - 1ull);
+  const int n = g.n;
+  int eo_r[NE > 0 ? NE : 1], es_r[NE > 0 ? NE : 1];
+  if (NE > 0) {
+#pragma unroll
+    for (int k = 0; k < (NE > 0 ? NE : 1); ++k) {
+      eo_r[k] = g.eo[k];
+      es_r[k] = g.es[k];
+    }
+  } else {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      eo_s[k] = g.eo[k];
+      es_s[k] = g.es[k];
+    }
+  }
+  // load (and write the pads of both buffers): 8 words per lane per round,
+  // their loads issued together from addresses clamped into the item, the
+  // border and the pads applied afterwards
+  for (int i0 = 0; i0 < per; i0 += 8 * 256) {
+    uint64_t v[8];
+    int wpv[8];
+    bool okv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * 256 + (int)threadIdx.x;
+      const int ic = i < per ? i : per - 1;
+      const int r = small_div(ic, g.inv_pitch), wp = ic - r * g.pitch;
+      const int rz = small_div(r, g.inv_by);
+      const int y = y0 + (r - rz * g.by), z = z0 + rz;
+      okv[u] = wp >= 1 && wp <= g.ww && y >= 0 && y < g.Y && z >= 0 && z < g.Z;
+      wpv[u] = wp;
+      const int yc = y < 0 ? 0 : (y >= g.Y ? g.Y - 1 : y);
+      const int zc = z < 0 ? 0 : (z >= g.Z ? g.Z - 1 : z);
+      const int wc = wp < 1 ? 0 : (wp > g.ww ? g.ww - 1 : wp - 1);
+      v[u] = src[((int64_t)zc * g.Y + yc) * g.ww + wc];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * 256 + (int)threadIdx.x;
+      if (i >= per) break;
+      uint64_t x = okv[u] ? v[u] : border;
+      if (okv[u] && wpv[u] == g.ww) x = (x & last_m) | (border & ~last_m);
+      A[i] = x;
+      if (wpv[u] == 0 || wpv[u] >= g.ww + 1) B[i] = border;
+    }
+  }
+  __syncthreads();
+  for (int s = 1; s <= steps; ++s) {
+    // rows still needed after this step: [s * r, b - s * r) along each axis;
+    // one row per lane, all its words
+    const int ly = s * g.ry, lz = s * g.rz;
+    const int ny = g.by - 2 * ly, nz = g.bz - 2 * lz;
+    const float inv_ny = 1.0f / (float)ny;
+    for (int r = threadIdx.x; r < ny * nz; r += blockDim.x) {
+      const int rz = small_div(r, inv_ny);
+      const int byi = ly + (r - rz * ny), bzi = lz + rz;
+      const int y = y0 + byi, z = z0 + bzi;
+      const int row = (bzi * g.by + byi) * g.pitch;
+      const bool inside = y >= 0 && y < g.Y && z >= 0 && z < g.Z;
+      for (int w = 0; w < g.ww; ++w) {
+        uint64_t acc = border;
+        if (inside) {
+          acc = dilation ? 0ull : ~0ull;
+          if (NE > 0) {
+            uint64_t lo[NE > 0 ? NE : 1], hi[NE > 0 ? NE : 1];
+#pragma unroll
+            for (int k = 0; k < (NE > 0 ? NE : 1); ++k) {
+              const uint64_t* p = A + row + w + eo_r[k];
+              lo[k] = p[0];
+              hi[k] = p[1];
+            }
+            // entries past n repeat element 0: AND / OR absorb them
+#pragma unroll
+            for (int k = 0; k < (NE > 0 ? NE : 1); ++k) {
+              const uint64_t term = bin_shift(lo[k], hi[k], es_r[k]);
+              if (dilation) acc |= term;
+              else acc &= term;
+            }
+          } else {
+            for (int t = 0; t < n; ++t) {
+              const uint64_t* p = A + row + w + eo_s[t];
+              const uint64_t term = bin_shift(p[0], p[1], es_s[t]);
+              if (dilation) acc |= term;
+              else acc &= term;
+            }
+          }
+          if (w == g.ww - 1) acc = (acc & last_m) | (border & ~last_m);
+        }
+        B[row + w + 1] = acc;
+      }
+    }
+    __syncthreads();
+    uint64_t* t = A;
+    A = B;
+    B = t;
+  }
+  // the tile's rows inside the item
+  const int tw = g.ty * g.tz * g.ww;
+  for (int i = threadIdx.x; i < tw; i += blockDim.x) {
+    const int r = small_div(i, g.inv_ww), w = i - r * g.ww;
+    const int rz = small_div(r, g.inv_ty);
+    const int byi = g.hy + (r - rz * g.ty), bzi = g.hz + rz;
+    const int y = y0 + byi, z = z0 + bzi;
+    if (y < g.Y && z < g.Z)
+      dst[((int64_t)z * g.Y + y) * g.ww + w] = A[(bzi * g.by + byi) * g.pitch + w + 1];
+  }
+}
+
+// LDS bytes of the fused steps (two buffers).
+inline size_t bin_tile_lds(const BinTileGeom& g) {
+  return (size_t)2 * g.by * g.bz * g.pitch * 8;
+}
+
+constexpr size_t kBinTileLdsMax = 60 * 1024;
+
+// Geometry of the fused steps for `steps` steps of `tab`, or false when the
+// item, the structure or the halo do not fit (the step kernel runs then).
+inline bool bin_tile_plan(const Packed& pk, const BinTable& tab, int steps, BinTileGeom* g) {
+  if (pk.nd > 3 || steps < 1) return false;
+  int rx = 0, ry = 0, rz = 0;
+  for (int t = 0; t < tab.n; ++t) {
+    const int dx = tab.d[t][pk.nd - 1];
+    const int dy = pk.nd >= 2 ? tab.d[t][pk.nd - 2] : 0;
+    const int dz = pk.nd == 3 ? tab.d[t][0] : 0;
+    rx = max(rx, dx < 0 ? -dx : dx);
+    ry = max(ry, dy < 0 ? -dy : dy);
+    rz = max(rz, dz < 0 ? -dz : dz);
+  }
+  if (rx > 63) return false;
+  const int64_t Z = pk.nd == 3 ? pk.size[0] : 1, Y = pk.nd >= 2 ? pk.size[pk.nd - 2] : 1;
+  if (Z > 65535 * 64 || Y > 65535 * 64 || pk.ww > 64) return false;
+  BinTileGeom q{};
+  q.Z = (int)Z;
+  q.Y = (int)Y;
+  q.ww = (int)pk.ww;
+  q.W = (int)pk.W;
+  q.pitch = (q.ww + 2) | 1;  // odd: neighbouring rows start in different banks
+  q.ry = ry;
+  q.rz = rz;
+  q.hy = steps * ry;
+  q.hz = steps * rz;
+  // square-ish tiles of at most 16 x 16 rows (3-D) or 64 rows (2-D), halved
+  // until the two buffers fit
+  q.ty = (int)(Y < (pk.nd == 3 ? 16 : 64) ? Y : (pk.nd == 3 ? 16 : 64));
+  q.tz = (int)(Z < 16 ? Z : 16);
+  for (;;) {
+    q.by = q.ty + 2 * q.hy;
+    q.bz = q.tz + 2 * q.hz;
+    if (q.hy <= 64 && q.hz <= 64 && bin_tile_lds(q) <= kBinTileLdsMax) break;
+    if (q.ty >= q.tz && q.ty > 1) q.ty = (q.ty + 1) / 2;
+    else if (q.tz > 1) q.tz = (q.tz + 1) / 2;
+    else return false;
+  }
+  // halo rows should not outnumber the tile's own rows by much
+  if ((int64_t)q.by * q.bz > 8 * (int64_t)q.ty * q.tz) return false;
+  if (q.pitch > 1024 || q.by > 1024 || q.bz > 1024 || q.by * q.bz * q.pitch >= 65536) return false;
+  q.inv_pitch = 1.0f / (float)q.pitch;
+  q.inv_by = 1.0f / (float)q.by;
+  q.inv_ww = 1.0f / (float)q.ww;
+  q.inv_ty = 1.0f / (float)q.ty;
+  q.n = tab.n;
+  for (int t = 0; t < kMaxBinTable; ++t) {
+    const int u = t < tab.n ? t : 0;
+    const int dz = pk.nd == 3 ? tab.d[u][0] : 0;
+    const int dy = pk.nd >= 2 ? tab.d[u][pk.nd - 2] : 0;
+    const int dx = tab.d[u][pk.nd - 1];
+    const int qd = dx < 0 ? -1 : 0;  // word of bit 64 w + dx, relative to w
+    q.eo[t] = (dz * q.by + dy) * q.pitch + qd + 1;
+    q.es[t] = dx - 64 * qd;
+  }
+  q.tiles_y = (int)((Y + q.ty - 1) / q.ty);
+  q.tiles_z = (int)((Z + q.tz - 1) / q.tz);
+  if ((int64_t)q.tiles_y * q.tiles_z > 2147483647) return false;
+  *g = q;
+  return true;
+}
+
 __global__ void bin_clear_flags(int* flags, int n) {
   if ((int)threadIdx.x < n) flags[threadIdx.x] = 0;
 }
@@ -297,10 +535,35 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
       TM_CHECK_HIP(hipMemcpyAsync(orig, cur, nwords * 8, hipMemcpyDeviceToDevice, st));
     }
     if (iters >= 1) {
-      for (int k = 0; k < iters; ++k) {
-        hipLaunchKernelGGL(bin_step_kernel, wgrid, blk, 0, st, cur, nxt, pk, nwords, tab, dil,
-                           border_value, mask_bytes ? orig : nullptr,
-                           mask_bytes ? mpk : nullptr, nullptr);
+      // without a mask, up to 8 steps per launch in LDS tiles (fewer when the
+      // halo would not fit); otherwise one step per launch
+      for (int done = 0; done < iters;) {
+        int k = iters - done < 8 ? iters - done : 8;
+        BinTileGeom g{};
+        bool tiled = !mask_bytes && batch <= 65535;
+        while (tiled && !bin_tile_plan(pk, tab, k, &g)) {
+          if (k == 1) tiled = false;
+          else k /= 2;
+        }
+        if (tiled) {
+          const dim3 tg(g.tiles_y * g.tiles_z, (unsigned)batch);
+          const size_t lds = bin_tile_lds(g);
+          if (tab.n <= 8)
+            hipLaunchKernelGGL(bin_tile_kernel<8>, tg, blk, lds, st, cur, nxt, g, dil, border_value,
+                               k);
+          else if (tab.n <= 32)
+            hipLaunchKernelGGL(bin_tile_kernel<32>, tg, blk, lds, st, cur, nxt, g, dil, border_value,
+                               k);
+          else
+            hipLaunchKernelGGL(bin_tile_kernel<0>, tg, blk, lds, st, cur, nxt, g, dil, border_value,
+                               k);
+          done += k;
+        } else {
+          hipLaunchKernelGGL(bin_step_kernel, wgrid, blk, 0, st, cur, nxt, pk, nwords, tab, dil,
+                             border_value, mask_bytes ? orig : nullptr,
+                             mask_bytes ? mpk : nullptr, nullptr);
+          done += 1;
+        }
         TM_LAUNCH_CHECK();
         uint64_t* t = cur;
         cur = nxt;

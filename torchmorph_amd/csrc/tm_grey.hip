@@ -30,26 +30,31 @@
 
 namespace tmh {
 
+// Border remap of coordinate c of an axis of `size` voxels (reference
+// grey_kernel.cu:63-97): constant -> cval, nearest -> clamp, reflect
+// (half-sample symmetric, period 2 size), mirror (whole-sample symmetric,
+// period 2 (size - 1)), wrap (period size).  Outside the axis, reflections /
+// shifts by one period repeat until c lands inside: the same index as the
+// modulo form, in a few instructions per call site (a 64-bit modulo inlined
+// at every load of an unrolled tile load made the kernels several times
+// larger than the instruction cache).  Each round moves c at least one voxel
+// closer to the axis, so the loop ends; callers pass c within a halo of the
+// axis, where it runs once or twice unless the axis is shorter than the halo.
 __device__ __forceinline__ int64_t gmap(int64_t c, int64_t size, int mode, bool* use_cval) {
   *use_cval = false;
   if (c >= 0 && c < size) return c;
-  switch (mode) {
-    case 0: *use_cval = true; return 0;  // constant
-    case 2: return c < 0 ? 0 : size - 1;  // nearest
-    case 1: {                               // reflect (half-sample symmetric)
-      const int64_t period = 2 * size;
-      int64_t m = ((c % period) + period) % period;
-      return m >= size ? period - 1 - m : m;
-    }
-    case 3: {  // mirror (whole-sample symmetric)
-      const int64_t period = 2 * (size - 1);
-      if (period <= 0) return 0;
-      int64_t m = ((c % period) + period) % period;
-      return m >= size ? period - m : m;
-    }
-    case 4: return ((c % size) + size) % size;  // wrap
-    default: *use_cval = true; return 0;
+  if (mode == 2) return c < 0 ? 0 : size - 1;  // nearest
+  if (mode == 3 && size == 1) return 0;        // mirror of a single voxel
+  if (mode != 1 && mode != 3 && mode != 4) {   // constant
+    *use_cval = true;
+    return 0;
   }
+  while (c < 0 || c >= size) {
+    if (mode == 1) c = c < 0 ? -c - 1 : 2 * size - 1 - c;  // reflect
+    else if (mode == 3) c = c < 0 ? -c : 2 * (size - 1) - c;  // mirror
+    else c = c < 0 ? c + size : c - size;  // wrap
+  }
+  return c;
 }
 
 template <bool MAX>

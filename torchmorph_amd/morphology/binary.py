@@ -9,6 +9,8 @@ pass per iteration (binary.py:79-82).
 """
 from __future__ import annotations
 
+import functools
+
 import torch
 from torch import Tensor
 
@@ -57,11 +59,18 @@ def _binary_morphology_cuda_step(input: Tensor, structure: Tensor, border_value:
     return _run(input, structure, origin, border_value, [(mode == "dilation", 1)], None)
 
 
+@functools.lru_cache(maxsize=None)
+def _default_structure(rank: int) -> Tensor:
+    """The cross of connectivity 1 (reference binary.py default), built once
+    per rank; only read, never handed to the caller."""
+    return generate_binary_structure(rank, 1)
+
+
 def _prepare(input, structure, origin, mask, output):
     spatial_ndim = validate_bcs_input(input)
     validate_output(input, output)
     if structure is None:
-        structure = generate_binary_structure(spatial_ndim, 1)
+        structure = _default_structure(spatial_ndim)
     structure = _normalize_structure(structure, spatial_ndim)
     origin = _normalize_origin(origin, spatial_ndim)
     _validate_origin(origin, structure)
