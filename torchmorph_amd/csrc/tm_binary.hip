@@ -19,10 +19,11 @@
 namespace tmh {
 
 constexpr int kMaxBinTable = 128;
-struct BinTable {
+struct BinTable {  // 4-byte multiple: copied to LDS as words
   int n;
   int8_t d[kMaxBinTable][kMaxNdim];  // per element: offsets for every axis
 };
+static_assert(sizeof(BinTable) % 4 == 0, "BinTable is copied as 32-bit words");
 
 struct Packed {
   int nd;
@@ -140,6 +141,13 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
                                                        const uint64_t* __restrict__ orig,
                                                        const uint64_t* __restrict__ mask,
                                                        int* __restrict__ changed) {
+  // the element table is copied to LDS once per workgroup: its int8 offsets
+  // at a loop index would otherwise be byte loads from the kernel arguments,
+  // each waited for before the next
+  __shared__ BinTable stab;
+  for (int i = threadIdx.x; i < (int)(sizeof(BinTable) / 4); i += blockDim.x)
+    ((uint32_t*)&stab)[i] = ((const uint32_t*)&tab)[i];
+  __syncthreads();
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nwords) return;
   const int64_t per_item = pk.rows * pk.ww;
@@ -163,21 +171,21 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
   // (words outside the item read word 0 and are replaced by the border after
   // the load), the 16 loads are issued together, then the terms combine
   uint64_t acc = dilation ? 0ull : ~0ull;
-  for (int t0 = 0; t0 < tab.n; t0 += 8) {
+  for (int t0 = 0; t0 < stab.n; t0 += 8) {
     int64_t a_lo[8], q_lo[8];
     int sh[8];
     bool inb[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const int t = t0 + u < tab.n ? t0 + u : tab.n - 1;
+      const int t = t0 + u < stab.n ? t0 + u : stab.n - 1;
       bool ok = true;
       int64_t r2 = row;
       for (int j = 0; j < lead; ++j) {
-        const int64_t cj = c[j] + tab.d[t][j];
+        const int64_t cj = c[j] + stab.d[t][j];
         ok = ok && cj >= 0 && cj < pk.size[j];
-        r2 += (int64_t)tab.d[t][j] * pk.rstride[j];
+        r2 += (int64_t)stab.d[t][j] * pk.rstride[j];
       }
-      const int64_t tb = w * 64 + tab.d[t][lead];
+      const int64_t tb = w * 64 + stab.d[t][lead];
       const int64_t q = tb >= 0 ? tb / 64 : -((-tb + 63) / 64);
       sh[u] = (int)(tb - q * 64);
       inb[u] = ok;
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      if (t0 + u >= tab.n) break;
+      if (t0 + u >= stab.n) break;
       uint64_t term = border;
       if (inb[u]) {
         const uint64_t l = bin_fix(lo[u], pk, q_lo[u], border);
