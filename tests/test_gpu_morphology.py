@@ -184,6 +184,25 @@ def test_binary_fused_steps_vs_oracle(tm, oracle_mod, shape, iterations, border_
         np.testing.assert_array_equal(got.cpu().numpy(), want, err_msg=f"dilation={dilation}")
 
 
+@pytest.mark.parametrize("shape,se_len,iterations", [((1, 1, 3, 4000), 3, 3), ((2, 1, 2, 4100), 3, 2),
+                                                     ((1, 1, 4000), 127, 2), ((1, 1, 4000), 131, 2)])
+def test_binary_long_rows_and_wide_structures(tm, oracle_mod, shape, se_len, iterations):
+    # rows of 63 and 65 words (the fused steps take up to 64) and x reaches of
+    # 63 and 65 voxels (the fused steps take up to 63): both sides of each limit
+    rng = np.random.default_rng(se_len + shape[-1])
+    nd = len(shape) - 2
+    x = rng.random(shape) < 0.97
+    st = np.zeros((1,) * (nd - 1) + (se_len,), bool)
+    st[..., rng.random(se_len) < 0.3] = True
+    st[..., 0] = st[..., -1] = st[..., se_len // 2] = True
+    for dilation in (False, True):
+        fn = tm.binary_dilation if dilation else tm.binary_erosion
+        got = fn(torch.as_tensor(x, device="cuda"), structure=torch.as_tensor(st),
+                 iterations=iterations)
+        want = oracle_mod.binary(x, st, (0,) * nd, False, dilation, iterations=iterations)
+        np.testing.assert_array_equal(got.cpu().numpy(), want, err_msg=f"dilation={dilation}")
+
+
 def test_binary_full_structure_border_and_mask(tm, oracle_mod):
     rng = np.random.default_rng(5)
     x = rng.random((2, 1, 20, 25, 70)) < 0.6
