@@ -184,6 +184,31 @@ def test_binary_fused_steps_vs_oracle(tm, oracle_mod, shape, iterations, border_
         np.testing.assert_array_equal(got.cpu().numpy(), want, err_msg=f"dilation={dilation}")
 
 
+@pytest.mark.parametrize("shape,iterations", [((2, 1, 20, 18, 128), 4), ((1, 1, 40, 192), 12),
+                                              ((3, 1, 64), 1)])
+@pytest.mark.parametrize("dtype", ["bool", "float32"])
+def test_binary_fused_byte_io_vs_oracle(tm, oracle_mod, shape, iterations, dtype):
+    # rows of whole 64-voxel words: the first tiled launch packs the bool
+    # input itself and the last writes the bool output (12 steps: two
+    # launches, one of each); float input keeps the pack pass
+    rng = np.random.default_rng(shape[-1] + iterations)
+    nd = len(shape) - 2
+    x = rng.random(shape) < 0.8
+    st = np.asarray(tm.generate_binary_structure(nd, nd))
+    xt = torch.as_tensor(x, device="cuda")
+    if dtype == "float32":
+        xt = xt.float() * 2.5
+    for dilation in (False, True):
+        for border_value in (False, True):
+            fn = tm.binary_dilation if dilation else tm.binary_erosion
+            got = fn(xt, structure=torch.as_tensor(st), iterations=iterations,
+                     border_value=border_value)
+            want = oracle_mod.binary(x, st, (0,) * nd, border_value, dilation,
+                                     iterations=iterations)
+            np.testing.assert_array_equal(got.cpu().numpy(), want,
+                                          err_msg=f"dilation={dilation} border={border_value}")
+
+
 @pytest.mark.parametrize("shape,se_len,iterations", [((1, 1, 3, 4000), 3, 3), ((2, 1, 2, 4100), 3, 2),
                                                      ((1, 1, 4000), 127, 2), ((1, 1, 4000), 131, 2)])
 def test_binary_long_rows_and_wide_structures(tm, oracle_mod, shape, se_len, iterations):

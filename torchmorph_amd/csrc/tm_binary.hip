@@ -63,17 +63,13 @@ __device__ __forceinline__ uint32_t nz4(uint32_t v) {
   return (((hi >> 7) * 0x01020408u) >> 24) & 0xfu;
 }
 
-// Byte masks whose rows are whole 64-byte words (W % 64 == 0, 16-byte
-// aligned input): four 16-byte loads per word.
-__global__ __launch_bounds__(256) void bin_pack_bytes_kernel(const uint8_t* __restrict__ in,
-                                                             uint64_t* __restrict__ out,
-                                                             int64_t nwords) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nwords) return;
-  const uint4* src = (const uint4*)(in + g * 64);
+// 64 bytes (16-byte aligned) as 64 bits, nonzero -> 1; four loads issued
+// together.
+__device__ __forceinline__ uint64_t bin_pack64(const uint8_t* p) {
+  const uint4* q4 = (const uint4*)p;
   uint4 q[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) q[k] = src[k];
+  for (int k = 0; k < 4; ++k) q[k] = q4[k];
   uint64_t bits = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -81,7 +77,17 @@ __global__ __launch_bounds__(256) void bin_pack_bytes_kernel(const uint8_t* __re
                          (nz4(q[k].w) << 12);
     bits |= (uint64_t)b16 << (16 * k);
   }
-  out[g] = bits;
+  return bits;
+}
+
+// Byte masks whose rows are whole 64-byte words (W % 64 == 0, 16-byte
+// aligned input): four 16-byte loads per word.
+__global__ __launch_bounds__(256) void bin_pack_bytes_kernel(const uint8_t* __restrict__ in,
+                                                             uint64_t* __restrict__ out,
+                                                             int64_t nwords) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nwords) return;
+  out[g] = bin_pack64(in + g * 64);
 }
 
 __global__ __launch_bounds__(256) void bin_unpack_kernel(const uint64_t* __restrict__ in,
@@ -97,6 +103,20 @@ __global__ __launch_bounds__(256) void bin_unpack_kernel(const uint64_t* __restr
 // 4 bits as 4 bytes of 0 / 1 (bit i -> byte i).
 __device__ __forceinline__ uint32_t bytes4(uint32_t b) {
   return (b * 0x00204081u) & 0x01010101u;
+}
+
+// 64 bits as 64 bytes of 0 / 1 (16-byte aligned), four 16-byte stores.
+__device__ __forceinline__ void bin_unpack64(uint64_t bits, uint8_t* p) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t b = (uint32_t)(bits >> (16 * k)) & 0xffffu;
+    uint4 v;
+    v.x = bytes4(b & 0xfu);
+    v.y = bytes4((b >> 4) & 0xfu);
+    v.z = bytes4((b >> 8) & 0xfu);
+    v.w = bytes4((b >> 12) & 0xfu);
+    ((uint4*)p)[k] = v;
+  }
 }
 
 // Rows of whole 16-byte groups (W % 16 == 0, 16-byte aligned output): 16
@@ -271,11 +291,18 @@ __device__ __forceinline__ uint64_t bin_shift(uint64_t lo, uint64_t hi, int sh) 
 // NE > 0: at most NE elements, their LDS offsets in registers and the
 // element loop unrolled (every load of a word issued together); NE == 0: any
 // count, offsets in LDS.
-template <int NE>
+// IO bit 0: read the bool bytes `in_b` (a row of whole 64-byte words) instead
+// of packed words; bit 1: write bool bytes to `out_b` instead of packed words.
+// Either replaces a pack / unpack pass over the volume.
+template <int NE, int IO>
 __global__ __launch_bounds__(256) void bin_tile_kernel(const uint64_t* __restrict__ in,
                                                        uint64_t* __restrict__ out,
+                                                       const uint8_t* __restrict__ in_b,
+                                                       uint8_t* __restrict__ out_b,
                                                        BinTileGeom g, int dilation,
                                                        int border_value, int steps) {
+  constexpr bool IN_B = (IO & 1) != 0, OUT_B = (IO & 2) != 0;
+  constexpr int RU = IN_B ? 4 : 8;  // words loaded per lane per round
   extern __shared__ uint64_t bin_lds[];
   __shared__ int eo_s[NE > 0 ? 1 : kMaxBinTable];
   __shared__ int es_s[NE > 0 ? 1 : kMaxBinTable];
@@ -305,15 +332,16 @@ __global__ __launch_bounds__(256) void bin_tile_kernel(const uint64_t* __restric
       es_s[k] = g.es[k];
     }
   }
-  // load (and write the pads of both buffers): 8 words per lane per round,
+  // load (and write the pads of both buffers): RU words per lane per round,
   // their loads issued together from addresses clamped into the item, the
   // border and the pads applied afterwards
-  for (int i0 = 0; i0 < per; i0 += 8 * 256) {
-    uint64_t v[8];
-    int wpv[8];
-    bool okv[8];
+  const int64_t item_w = (int64_t)blockIdx.y * item_rows * g.ww;  // first word of the item
+  for (int i0 = 0; i0 < per; i0 += RU * 256) {
+    uint64_t v[RU];
+    int wpv[RU];
+    bool okv[RU];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < RU; ++u) {
       const int i = i0 + u * 256 + (int)threadIdx.x;
       const int ic = i < per ? i : per - 1;
       const int r = small_div(ic, g.inv_pitch), wp = ic - r * g.pitch;
@@ -324,10 +352,12 @@ __global__ __launch_bounds__(256) void bin_tile_kernel(const uint64_t* __restric
       const int yc = y < 0 ? 0 : (y >= g.Y ? g.Y - 1 : y);
       const int zc = z < 0 ? 0 : (z >= g.Z ? g.Z - 1 : z);
       const int wc = wp < 1 ? 0 : (wp > g.ww ? g.ww - 1 : wp - 1);
-      v[u] = src[((int64_t)zc * g.Y + yc) * g.ww + wc];
+      const int64_t wi = ((int64_t)zc * g.Y + yc) * g.ww + wc;
+      if (IN_B) v[u] = bin_pack64(in_b + (item_w + wi) * 64);
+      else v[u] = src[wi];
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < RU; ++u) {
       const int i = i0 + u * 256 + (int)threadIdx.x;
       if (i >= per) break;
       uint64_t x = okv[u] ? v[u] : border;
@@ -393,8 +423,12 @@ __global__ __launch_bounds__(256) void bin_tile_kernel(const uint64_t* __restric
     const int rz = small_div(r, g.inv_ty);
     const int byi = g.hy + (r - rz * g.ty), bzi = g.hz + rz;
     const int y = y0 + byi, z = z0 + bzi;
-    if (y < g.Y && z < g.Z)
-      dst[((int64_t)z * g.Y + y) * g.ww + w] = A[(bzi * g.by + byi) * g.pitch + w + 1];
+    if (y < g.Y && z < g.Z) {
+      const uint64_t v = A[(bzi * g.by + byi) * g.pitch + w + 1];
+      const int64_t wi = ((int64_t)z * g.Y + y) * g.ww + w;
+      if (OUT_B) bin_unpack64(v, out_b + (item_w + wi) * 64);
+      else dst[wi] = v;
+    }
   }
 }
 
@@ -467,6 +501,23 @@ inline bool bin_tile_plan(const Packed& pk, const BinTable& tab, int steps, BinT
   return true;
 }
 
+template <int NE>
+static void bin_tile_launch(int io, dim3 grid, size_t lds, hipStream_t st, const uint64_t* in,
+                            uint64_t* out, const uint8_t* in_b, uint8_t* out_b,
+                            const BinTileGeom& g, int dil, int border_value, int steps) {
+  const dim3 blk(256);
+  switch (io) {
+    case 0: hipLaunchKernelGGL((bin_tile_kernel<NE, 0>), grid, blk, lds, st, in, out, in_b, out_b,
+                               g, dil, border_value, steps); break;
+    case 1: hipLaunchKernelGGL((bin_tile_kernel<NE, 1>), grid, blk, lds, st, in, out, in_b, out_b,
+                               g, dil, border_value, steps); break;
+    case 2: hipLaunchKernelGGL((bin_tile_kernel<NE, 2>), grid, blk, lds, st, in, out, in_b, out_b,
+                               g, dil, border_value, steps); break;
+    default: hipLaunchKernelGGL((bin_tile_kernel<NE, 3>), grid, blk, lds, st, in, out, in_b,
+                                out_b, g, dil, border_value, steps); break;
+  }
+}
+
 __global__ void bin_clear_flags(int* flags, int n) {
   if ((int)threadIdx.x < n) flags[threadIdx.x] = 0;
 }
@@ -520,7 +571,30 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
   int* flags = (int*)(mpk + nwords);
   const dim3 wgrid(grid_for(nwords, 256)), blk(256);
   auto bytes_ok = [&](const void* p) { return pk.W % 64 == 0 && ((uintptr_t)p % 16) == 0; };
-  if (sizeof(T) == 1 && bytes_ok(input))
+  // one launch of up to 8 steps in LDS tiles (g filled; fewer steps when the
+  // halo would not fit), or one step per launch (with a mask, or when no
+  // tile fits): returns the steps, sets *tiled
+  auto chunk_plan = [&](int left, BinTileGeom* g, bool* tiled) {
+    int k = left < 8 ? left : 8;
+    *tiled = !mask_bytes && batch <= 65535;
+    while (*tiled && !bin_tile_plan(pk, tab, k, g)) {
+      if (k == 1) *tiled = false;
+      else k /= 2;
+    }
+    return *tiled ? k : 1;
+  };
+  // the first tiled launch packs the bool input itself, the last one writes
+  // the bool output itself (rows of whole 64-byte words, 16-byte aligned)
+  bool fuse_in = false;
+  if (sizeof(T) == 1 && bytes_ok(input) && op_iterations[0] >= 1) {
+    BinTileGeom g0{};
+    chunk_plan(op_iterations[0], &g0, &fuse_in);
+  }
+  const bool out_bytes_ok = bytes_ok(output);
+  bool fused_out = false;
+  if (fuse_in) {
+    // no pack pass
+  } else if (sizeof(T) == 1 && bytes_ok(input))
     hipLaunchKernelGGL(bin_pack_bytes_kernel, wgrid, blk, 0, st, (const uint8_t*)input, bufA,
                        nwords);
   else
@@ -543,35 +617,33 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
       TM_CHECK_HIP(hipMemcpyAsync(orig, cur, nwords * 8, hipMemcpyDeviceToDevice, st));
     }
     if (iters >= 1) {
-      // without a mask, up to 8 steps per launch in LDS tiles (fewer when the
-      // halo would not fit); otherwise one step per launch
       for (int done = 0; done < iters;) {
-        int k = iters - done < 8 ? iters - done : 8;
         BinTileGeom g{};
-        bool tiled = !mask_bytes && batch <= 65535;
-        while (tiled && !bin_tile_plan(pk, tab, k, &g)) {
-          if (k == 1) tiled = false;
-          else k /= 2;
-        }
+        bool tiled = false;
+        const int k = chunk_plan(iters - done, &g, &tiled);
         if (tiled) {
+          const bool first = op == 0 && done == 0;
+          const bool last = op == nops - 1 && done + k == iters;
+          const int io = (first && fuse_in ? 1 : 0) | (last && out_bytes_ok ? 2 : 0);
           const dim3 tg(g.tiles_y * g.tiles_z, (unsigned)batch);
-          const size_t lds = bin_tile_lds(g);
+          const uint8_t* in_b = (io & 1) ? (const uint8_t*)input : nullptr;
+          uint8_t* out_b = (io & 2) ? output : nullptr;
           if (tab.n <= 8)
-            hipLaunchKernelGGL(bin_tile_kernel<8>, tg, blk, lds, st, cur, nxt, g, dil, border_value,
-                               k);
+            bin_tile_launch<8>(io, tg, bin_tile_lds(g), st, cur, nxt, in_b, out_b, g, dil,
+                               border_value, k);
           else if (tab.n <= 32)
-            hipLaunchKernelGGL(bin_tile_kernel<32>, tg, blk, lds, st, cur, nxt, g, dil, border_value,
-                               k);
+            bin_tile_launch<32>(io, tg, bin_tile_lds(g), st, cur, nxt, in_b, out_b, g, dil,
+                                border_value, k);
           else
-            hipLaunchKernelGGL(bin_tile_kernel<0>, tg, blk, lds, st, cur, nxt, g, dil, border_value,
-                               k);
-          done += k;
+            bin_tile_launch<0>(io, tg, bin_tile_lds(g), st, cur, nxt, in_b, out_b, g, dil,
+                               border_value, k);
+          fused_out = (io & 2) != 0;
         } else {
           hipLaunchKernelGGL(bin_step_kernel, wgrid, blk, 0, st, cur, nxt, pk, nwords, tab, dil,
                              border_value, mask_bytes ? orig : nullptr,
                              mask_bytes ? mpk : nullptr, nullptr);
-          done += 1;
         }
+        done += k;
         TM_LAUNCH_CHECK();
         uint64_t* t = cur;
         cur = nxt;
@@ -604,6 +676,7 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
       }
     }
   }
+  if (fused_out) return TM_OK;  // the last tiled launch wrote the bytes
   if (pk.W % 16 == 0 && ((uintptr_t)output % 16) == 0)
     hipLaunchKernelGGL(bin_unpack_bytes_kernel, dim3(grid_for(total / 16, 256)), blk, 0, st, cur,
                        output, pk, total / 16);
