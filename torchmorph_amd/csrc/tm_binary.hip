@@ -9,11 +9,14 @@
 // repeats until an iteration changes nothing.
 //
 // Layout: every row along the contiguous axis is bit-packed into 64-bit words
-// (bit b of word w = voxel 64w + b).  A step computes one output word per lane
-// from the shifted neighbour words of the element offsets, so the iterated
-// pipeline (pack -> steps -> unpack) touches HBM once for the bool input and
-// once for the bool output; intermediate steps run on the 1/8-size packed
-// volume, which stays in L2 / Infinity Cache.
+// (bit b of word w = voxel 64w + b).  A step computes an output word from the
+// shifted neighbour words of the element offsets.  Without a mask, up to 8
+// steps run per launch on tiles of rows held in LDS (bin_tile_kernel); the
+// first such launch can pack the bool input and the last unpack the bool
+// output, so the pipeline touches HBM about once for the input and once for
+// the output.  With a mask, or until stable, one step runs per launch
+// (bin_step_kernel) on the 1/8-size packed volume, which stays in L2 /
+// Infinity Cache.
 #include "tm_common.h"
 
 namespace tmh {
