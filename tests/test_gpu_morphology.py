@@ -209,6 +209,22 @@ def test_binary_fused_byte_io_vs_oracle(tm, oracle_mod, shape, iterations, dtype
                                           err_msg=f"dilation={dilation} border={border_value}")
 
 
+@pytest.mark.parametrize("shape,iterations", [((2, 1, 24, 30, 128), 3), ((1, 2, 70, 64), 10)])
+@pytest.mark.parametrize("op", ["opening", "closing"])
+def test_binary_opening_closing_whole_words_vs_oracle(tm, oracle_mod, shape, iterations, op):
+    # the second pass reads the first pass's output buffer: with rows of whole
+    # 64-voxel words it must still pack that input before writing over it
+    rng = np.random.default_rng(shape[-2] + iterations)
+    nd = len(shape) - 2
+    x = rng.random(shape) < 0.6
+    st = np.asarray(tm.generate_binary_structure(nd, 1))
+    first = op == "closing"  # closing dilates first
+    got = getattr(tm, f"binary_{op}")(torch.as_tensor(x, device="cuda"), iterations=iterations)
+    mid = oracle_mod.binary(x, st, (0,) * nd, False, first, iterations=iterations)
+    want = oracle_mod.binary(mid, st, (0,) * nd, False, not first, iterations=iterations)
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
 @pytest.mark.parametrize("shape,se_len,iterations", [((1, 1, 3, 4000), 3, 3), ((2, 1, 2, 4100), 3, 2),
                                                      ((1, 1, 4000), 127, 2), ((1, 1, 4000), 131, 2)])
 def test_binary_long_rows_and_wide_structures(tm, oracle_mod, shape, se_len, iterations):

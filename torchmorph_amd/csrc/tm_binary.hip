@@ -587,9 +587,14 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
     return *tiled ? k : 1;
   };
   // the first tiled launch packs the bool input itself, the last one writes
-  // the bool output itself (rows of whole 64-byte words, 16-byte aligned)
+  // the bool output itself (rows of whole 64-byte words, 16-byte aligned).
+  // Not when input and output overlap (the second pass of an opening or
+  // closing reads the output of the first): a launch that both reads and
+  // writes the bytes would let one tile overwrite another tile's halo.
+  const uintptr_t ib = (uintptr_t)input, ob = (uintptr_t)output;
+  const bool overlap = ib < ob + (uintptr_t)total && ob < ib + (uintptr_t)total * sizeof(T);
   bool fuse_in = false;
-  if (sizeof(T) == 1 && bytes_ok(input) && op_iterations[0] >= 1) {
+  if (sizeof(T) == 1 && !overlap && bytes_ok(input) && op_iterations[0] >= 1) {
     BinTileGeom g0{};
     chunk_plan(op_iterations[0], &g0, &fuse_in);
   }
