@@ -174,13 +174,11 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nwords) return;
   const int64_t per_item = pk.rows * pk.ww;
+  const int lead = pk.nd - 1;
   const int64_t item = g / per_item;
   const int64_t rem = g - item * per_item;
   const int64_t row = rem / pk.ww;
   const int64_t w = rem - row * pk.ww;
-  const uint64_t* src = in + item * per_item;
-  const uint64_t border = border_value ? ~0ull : 0ull;
-  const int lead = pk.nd - 1;
   // leading coordinates of this row
   int64_t c[kMaxNdim];
   {
@@ -190,6 +188,8 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
       r /= pk.size[j];
     }
   }
+  const uint64_t* src = in + item * per_item;
+  const uint64_t border = border_value ? ~0ull : 0ull;
   // elements in groups of 8: every address of a group is formed first
   // (words outside the item read word 0 and are replaced by the border after
   // the load), the 16 loads are issued together, then the terms combine
@@ -248,7 +248,9 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
   if (changed) {
     const int64_t valid = pk.W - w * 64;
     const uint64_t vm = valid >= 64 ? ~0ull : ((1ull << valid) - 1ull);
-    if ((acc ^ in[g]) & vm) atomicOr(changed, 1);
+    // one atomic per wave with a changed word, not one per lane
+    const uint64_t b = __ballot(((acc ^ in[g]) & vm) != 0);
+    if (b != 0 && (int)(threadIdx.x & 63) == __ffsll((long long)b) - 1) atomicOr(changed, 1);
   }
 }
 
@@ -539,7 +541,7 @@ extern "C" size_t tm_binary_workspace_bytes(int64_t batch, int ndim, const int64
   for (int j = 0; j < ndim - 1; ++j) rows *= shape[j];
   const int64_t ww = (shape[ndim - 1] + 63) / 64;
   // ping, pong, orig, mask (packed) + flags
-  return (size_t)rows * ww * 8 * 4 + 256;
+  return (size_t)rows * ww * 8 * 4 + 64 * sizeof(int);  // + 64 change flags
 }
 
 namespace tmh {
@@ -660,11 +662,16 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
     } else {
       // until an iteration changes nothing (at least two iterations, as the
       // reference compares consecutive results); extra iterations past the
-      // fixed point are exact no-ops, so the check runs every `chunk` steps.
-      constexpr int chunk = 8;
+      // fixed point are exact no-ops, so the host checks the last step's
+      // change flag after every chunk of steps, the chunk doubling from 8 to
+      // 64 (the flag words of the workspace): long propagations wait on the
+      // host about 7x less often; the steps run past the fixed point are at
+      // most one chunk.
+      constexpr int kMaxChunk = 64;
+      int chunk = 8;
       int64_t done = 0;
       for (;;) {
-        hipLaunchKernelGGL(bin_clear_flags, dim3(1), dim3(64), 0, st, flags, chunk);
+        hipLaunchKernelGGL(bin_clear_flags, dim3(1), dim3(kMaxChunk), 0, st, flags, chunk);
         TM_LAUNCH_CHECK();
         for (int k = 0; k < chunk; ++k) {
           hipLaunchKernelGGL(bin_step_kernel, wgrid, blk, 0, st, cur, nxt, pk, nwords, tab, dil,
@@ -676,11 +683,13 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
           nxt = t;
         }
         done += chunk;
-        int h[chunk];
-        TM_CHECK_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, st));
+        int last = 0;
+        TM_CHECK_HIP(hipMemcpyAsync(&last, flags + chunk - 1, sizeof(int), hipMemcpyDeviceToHost,
+                                    st));
         TM_CHECK_HIP(hipStreamSynchronize(st));
-        if (h[chunk - 1] == 0) break;
+        if (last == 0) break;
         if (done > (int64_t)1 << 40) break;
+        chunk = chunk * 2 < kMaxChunk ? chunk * 2 : kMaxChunk;
       }
     }
   }
