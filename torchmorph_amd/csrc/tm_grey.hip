@@ -36,10 +36,10 @@ namespace tmh {
 // period 2 (size - 1)), wrap (period size).  Outside the axis, reflections /
 // shifts by one period repeat until c lands inside: the same index as the
 // modulo form, in a few instructions per call site (a 64-bit modulo inlined
-// at every load of an unrolled tile load made the kernels several times
-// larger than the instruction cache).  Each round moves c at least one voxel
-// closer to the axis, so the loop ends; callers pass c within a halo of the
-// axis, where it runs once or twice unless the axis is shorter than the halo.
+// at every load of an unrolled tile load made each kernel larger than the
+// 64 KB instruction cache).  Each pass moves c at least one voxel closer to
+// the axis, so the loop ends; callers pass c within a halo of the axis, where
+// it runs once or twice unless the axis is shorter than the halo.
 __device__ __forceinline__ int64_t gmap(int64_t c, int64_t size, int mode, bool* use_cval) {
   *use_cval = false;
   if (c >= 0 && c < size) return c;
