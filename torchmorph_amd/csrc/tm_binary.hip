@@ -201,14 +201,24 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int t = t0 + u < stab.n ? t0 + u : stab.n - 1;
+      // the element's 8 offsets as two 32-bit LDS words, bytes taken by shifts
+      const uint32_t* dw = (const uint32_t*)&stab.d[t][0];
+      const uint32_t d03 = dw[0], d47 = dw[1];
+      auto off = [&](int j) {
+        const uint32_t v = j < 4 ? d03 >> (8 * j) : d47 >> (8 * (j - 4));
+        return (int)(int8_t)(v & 0xffu);
+      };
       bool ok = true;
       int64_t r2 = row;
-      for (int j = 0; j < lead; ++j) {
-        const int64_t cj = c[j] + stab.d[t][j];
-        ok = ok && cj >= 0 && cj < pk.size[j];
-        r2 += (int64_t)stab.d[t][j] * pk.rstride[j];
+#pragma unroll
+      for (int j = 0; j < kMaxNdim - 1; ++j) {
+        if (j < lead) {
+          const int64_t cj = c[j] + off(j);
+          ok = ok && cj >= 0 && cj < pk.size[j];
+          r2 += (int64_t)off(j) * pk.rstride[j];
+        }
       }
-      const int64_t tb = w * 64 + stab.d[t][lead];
+      const int64_t tb = w * 64 + off(lead);
       const int64_t q = tb >= 0 ? tb / 64 : -((-tb + 63) / 64);
       sh[u] = (int)(tb - q * 64);
       inb[u] = ok;
