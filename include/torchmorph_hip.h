@@ -262,6 +262,36 @@ int tm_binary(const void* input, int input_dtype, int64_t batch, int ndim, const
               const uint8_t* mask, uint8_t* output, void* workspace, size_t workspace_bytes,
               void* stream);
 
+/* Replaces the reference's binary_hit_or_miss composition of two
+ * binary_erosion calls and a logical_and (reference morphology/binary.py:
+ * 188-228): output = erosion(x, structure1, origin1) AND erosion(NOT x,
+ * structure2, origin2), border 0 for both, in one packed pipeline (pack x
+ * once, the miss step ANDs into the hit words, unpack once).  An all-zero
+ * structure1 hits everywhere; an all-zero structure2 misses nowhere.  Shapes
+ * and dtypes as tm_binary (any structure size); workspace as
+ * tm_binary_workspace_bytes. */
+int tm_binary_hit_or_miss(const void* input, int input_dtype, int64_t batch, int ndim,
+                          const int64_t* shape, const uint8_t* structure1,
+                          const int64_t* se1_shape, const int64_t* origin1,
+                          const uint8_t* structure2, const int64_t* se2_shape,
+                          const int64_t* origin2, uint8_t* output, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
+/* Replaces the reference's binary_fill_holes composition (reference
+ * morphology/binary.py:160-185: a propagation of an empty seed into the
+ * background with border_value 1, then a logical_not): the background
+ * reachable from outside the item by dilation with `structure` (origin),
+ * restricted to the background, is removed; everything else (objects and
+ * the holes in them) is set.  One packed pipeline: the background mask is
+ * packed and inverted once, the propagation runs until a step changes
+ * nothing (the host reads a device change flag after chunks of 8, 16, 32,
+ * then 64 steps), the inverted result is unpacked once.  Shapes, dtypes and
+ * workspace as tm_binary. */
+int tm_binary_fill_holes(const void* input, int input_dtype, int64_t batch, int ndim,
+                         const int64_t* shape, const uint8_t* structure, const int64_t* se_shape,
+                         const int64_t* origin, uint8_t* output, void* workspace,
+                         size_t workspace_bytes, void* stream);
+
 /* ----------------------------------------------------------- Sinkhorn --- */
 
 /* Histogram Sinkhorn, n_iter iterations on (nb, d) rows sharing one (d, d)
@@ -270,11 +300,43 @@ int tm_binary(const void* input, int input_dtype, int64_t batch, int ndim, const
  * (log_space=0: num = a / b, M = K = exp(-C/eps), u/v = scalings)
  * (reference torchmorph.cpp:58-77, ot_kernel.cu:188-283).  u and v are
  * updated in place.  dtype TM_F32 or TM_F64.  n_iter >= 100 replays a
- * captured 25-iteration hipGraph (reference optimal_transport.py:213-243);
+ * captured 25-iteration hipGraph (reference optimal_transport.py:34-68);
  * *used_fallback is set when capture failed and plain launches ran. */
 int tm_sinkhorn_hist(int dtype, int log_space, const void* num_a, const void* num_b,
                      const void* M, const void* M_T, void* u, void* v, int64_t nb, int d,
                      int n_iter, double epsilon, int* used_fallback, void* stream);
+
+/* Workspace bytes for tm_sinkhorn_hist_solve (a stop word, plus nb x d
+ * elements for the threshold check when threshold > 0 and nb x d for the
+ * value shares when want_value). */
+size_t tm_sinkhorn_hist_workspace_bytes(int dtype, int64_t nb, int d, double threshold,
+                                        int want_value);
+
+/* Replaces the reference's SinkhornSolver solve path for GPU tensors:
+ * _log_iterations / _scaling_iterations with their fused kernels and
+ * _run_fused (reference optimal_transport.py:34-125), and the value and
+ * centred potentials of _SinkhornDistance.forward / potentials()
+ * (optimal_transport.py:128-150,235-245) without the n x d x d plan.
+ * Inputs as tm_sinkhorn_hist (u, v start as 0 (log) or 1 (scaling) and hold
+ * the final state on return), plus C / C_T (cost and its transpose, needed
+ * when dist is wanted; in the log form they may equal M / M_T).  threshold
+ * > 0 stops after iteration i (i % 10 == 0) once max_b sum_i |u_new - u_old|
+ * <= threshold, checked on the device (the reference's torch iterations,
+ * optimal_transport.py:86-92,118-124); 0 runs max_iter iterations.  Outputs,
+ * each optional (NULL): f, g (nb, d) centred potentials eps log u - mean,
+ * eps log v - mean; dist (nb,) values <P, C>; *iters_run the iterations run. */
+int tm_sinkhorn_hist_solve(int dtype, int log_space, const void* num_a, const void* num_b,
+                           const void* M, const void* M_T, const void* C, const void* C_T,
+                           void* u, void* v, int64_t nb, int d, int max_iter, double epsilon,
+                           double threshold, void* f, void* g, void* dist, int* iters_run,
+                           int* used_fallback, void* workspace, size_t workspace_bytes,
+                           void* stream);
+
+/* Replaces the plan reconstruction of SinkhornSolver.plan (reference
+ * optimal_transport.py:228-233): plan (nb, d, d) = exp(log u_i - C_ij / eps +
+ * log v_j) from a solved state (u, v as left by tm_sinkhorn_hist_solve). */
+int tm_sinkhorn_hist_plan(int dtype, int log_space, const void* u, const void* v, const void* C,
+                          int64_t nb, int d, double epsilon, void* plan, void* stream);
 
 /* Workspace bytes for tm_sinkhorn_points. */
 size_t tm_sinkhorn_points_workspace_bytes(int npairs, const int32_t* n, const int32_t* m);
@@ -297,7 +359,7 @@ int tm_sinkhorn_points(const float* x, const float* y, const float* a, const flo
 size_t tm_sinkhorn_points_grad_workspace_bytes(int npairs, const int32_t* n, const int32_t* m);
 
 /* No reference counterpart (the reference's backward only returns the
- * potentials as marginal gradients, optimal_transport.py:321-325).  Gradient
+ * potentials as marginal gradients, optimal_transport.py:146-150).  Gradient
  * of every pair's entropic OT value <f,a> + <g,b> w.r.t. its points from the
  * converged potentials (envelope theorem at the fixed point):
  * gx_i = 2 sum_j P_ij (x_i - y_j), gy_j = 2 sum_i P_ij (y_j - x_i), with the

@@ -9,7 +9,7 @@ Contents:
     ``make -C oracle``): EDT, CDT, brute-force DT, grey and binary
     morphology, one item at a time, looped over the leading (B, C) axes here;
   * numpy FP64 restatements of the Sinkhorn iterations
-    (reference ``torchmorph/optimal_transport.py:246-300``) and of the
+    (reference ``torchmorph/optimal_transport.py:71-125``) and of the
     point-cloud log-domain solver the build adds.
 
 Parity pins (what each function is checked against) are listed in
@@ -230,7 +230,7 @@ def _lse(z: np.ndarray, axis: int) -> np.ndarray:
 
 def sinkhorn_log_hist(a, b, cost, eps, n_iter):
     """Log-domain Sinkhorn on (n, d) histograms sharing one (d, d) cost,
-    restating reference optimal_transport.py:290-300 (threshold 0) in FP64.
+    restating reference optimal_transport.py:96-125 (threshold 0) in FP64.
     Returns (log_u, log_v)."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
@@ -245,7 +245,7 @@ def sinkhorn_log_hist(a, b, cost, eps, n_iter):
 
 
 def sinkhorn_scaling_hist(a, b, cost, eps, n_iter):
-    """Scaling-form Sinkhorn (reference optimal_transport.py:259-268) in FP64."""
+    """Scaling-form Sinkhorn (reference optimal_transport.py:71-93) in FP64."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     k = np.exp(-np.asarray(cost, np.float64) / eps)

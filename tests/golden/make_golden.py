@@ -224,11 +224,166 @@ def binary_cases():
     np.savez_compressed(os.path.join(HERE, "binary_golden.npz"), **out)
 
 
+def composite_cases():
+    """binary_propagation / binary_fill_holes / binary_hit_or_miss
+    (reference test/test_binary.py:190-367: the CASE_* inputs, masks,
+    structures and origins there, plus seeded random volumes)."""
+    rng = np.random.default_rng(20260105)
+    case2d = np.array([[[[0, 1, 0], [1, 1, 1], [0, 1, 0]]]], dtype=bool)
+    case3d = np.zeros((2, 1, 5, 5, 5), dtype=bool)
+    case3d[0, 0, 2:3, 2:4, 1:2] = True
+    case4d = np.zeros((2, 1, 4, 4, 4, 4), dtype=bool)
+    case4d[0, 0, 1:3, 1:2, 2:3] = True
+    holes2 = np.zeros((1, 1, 5, 5), dtype=bool)
+    holes2[0, 0, 1:4, 1:4] = True
+    holes2[0, 0, 2, 2] = False
+    holes3 = np.zeros((1, 1, 5, 5, 5), dtype=bool)
+    holes3[0, 0, 1:4, 1:4, 1:4] = True
+    holes3[0, 0, 2, 2, 2] = False
+    hit3 = ndi.generate_binary_structure(3, 1)[None, None]
+    mask2 = np.array([[1, 1, 0], [1, 1, 1], [1, 1, 0]], dtype=bool)
+    mask3 = np.zeros((5, 5, 5), dtype=bool)
+    mask3[0:2, 0:1, 0:3] = True
+    mask4 = np.zeros((4, 4, 4, 4), dtype=bool)
+    mask4[1:2, 0:1, 1:3] = True
+    s2 = ndi.generate_binary_structure(2, 1)
+    s2f = ndi.generate_binary_structure(2, 2)
+    s3_2 = ndi.generate_binary_structure(3, 2)
+    s3_3 = ndi.generate_binary_structure(3, 3)
+    s4 = ndi.generate_binary_structure(4, 4)
+    # random blobs with enclosed holes: a thresholded smooth field
+    field2 = ndi.gaussian_filter(rng.normal(size=(2, 1, 48, 70)), (0, 0, 2.0, 2.0))
+    blobs2 = field2 > 0.02
+    field3 = ndi.gaussian_filter(rng.normal(size=(1, 2, 20, 24, 30)), (0, 0, 1.5, 1.5, 1.5))
+    blobs3 = field3 > 0.0
+    rnd2 = rng.random((2, 1, 33, 65)) < 0.55
+    rnd3 = rng.random((1, 2, 9, 14, 70)) < 0.6
+    seeds2 = rng.random((1, 1, 40, 40)) < 0.01
+    pmask2 = rng.random((40, 40)) < 0.62
+    seeds3 = rng.random((1, 1, 12, 16, 20)) < 0.01
+    pmask3 = rng.random((12, 16, 20)) < 0.7
+    hm_s1 = np.array([[0, 1, 0], [0, 1, 1], [0, 0, 0]], dtype=bool)
+    hm_s2 = np.array([[1, 0, 0], [0, 0, 0], [0, 0, 1]], dtype=bool)
+    cases = [
+        # propagation (reference :170-236)
+        ("prop_2d_basic", "propagation", case2d, dict()),
+        ("prop_2d_origin_border", "propagation", case2d, dict(structure=s2, origin=1,
+                                                              border_value=True)),
+        ("prop_3d_structure_1", "propagation", case3d, dict(structure=s3_2)),
+        ("prop_3d_origin_border", "propagation", case3d, dict(structure=s3_3, origin=1,
+                                                              border_value=True)),
+        ("prop_4d_structure", "propagation", case4d, dict(structure=s4)),
+        ("prop_2d_mask", "propagation", case2d, dict(mask=mask2)),
+        ("prop_3d_mask", "propagation", case3d, dict(mask=mask3)),
+        ("prop_4d_mask", "propagation", case4d, dict(mask=mask4)),
+        ("prop_seeds_2d_mask", "propagation", seeds2, dict(mask=pmask2)),
+        ("prop_seeds_2d_mask_full", "propagation", seeds2, dict(mask=pmask2, structure=s2f)),
+        ("prop_seeds_3d_mask", "propagation", seeds3, dict(mask=pmask3)),
+        ("prop_seeds_3d_mask_border", "propagation", seeds3, dict(mask=pmask3,
+                                                                  border_value=True)),
+        # fill holes (reference :265-300)
+        ("fill_2d_default", "fill_holes", holes2, dict()),
+        ("fill_2d_structure", "fill_holes", holes2, dict(structure=s2)),
+        ("fill_3d_default", "fill_holes", holes3, dict()),
+        ("fill_blobs_2d", "fill_holes", blobs2, dict()),
+        ("fill_blobs_2d_full", "fill_holes", blobs2, dict(structure=s2f)),
+        ("fill_blobs_2d_origin", "fill_holes", blobs2, dict(structure=s2f, origin=(1, -1))),
+        ("fill_blobs_3d", "fill_holes", blobs3, dict()),
+        ("fill_blobs_3d_26", "fill_holes", blobs3, dict(structure=s3_3)),
+        ("fill_rnd_2d", "fill_holes", rnd2, dict()),
+        # hit or miss (reference :305-367)
+        ("hm_2d_default", "hit_or_miss", case2d, dict()),
+        ("hm_2d_structures", "hit_or_miss", case2d, dict(structure1=s2,
+                                                         structure2=np.logical_not(s2))),
+        ("hm_3d_default", "hit_or_miss", hit3, dict()),
+        ("hm_rnd_2d_default", "hit_or_miss", rnd2, dict()),
+        ("hm_rnd_2d_pair", "hit_or_miss", rnd2, dict(structure1=hm_s1, structure2=hm_s2)),
+        ("hm_rnd_2d_origins", "hit_or_miss", rnd2, dict(structure1=hm_s1, structure2=hm_s2,
+                                                        origin1=(1, 0), origin2=(0, -1))),
+        ("hm_rnd_2d_origin1_only", "hit_or_miss", rnd2, dict(structure1=hm_s1, origin1=-1)),
+        ("hm_rnd_2d_empty_miss", "hit_or_miss", rnd2, dict(structure1=np.ones((3, 3), bool),
+                                                           structure2=np.zeros((3, 3), bool))),
+        ("hm_rnd_2d_empty_hit", "hit_or_miss", rnd2, dict(structure1=np.zeros((3, 3), bool),
+                                                          structure2=s2)),
+        ("hm_rnd_3d_default", "hit_or_miss", rnd3, dict()),
+        ("hm_rnd_3d_26", "hit_or_miss", rnd3, dict(structure1=s3_2,
+                                                   structure2=np.logical_not(s3_3))),
+    ]
+    out = {}
+    meta = []
+    for i, (name, op, x, kw) in enumerate(cases):
+        fn = getattr(ndi, f"binary_{op}")
+        res, _ = per_item(x, lambda it: fn(it, **kw))
+        out[f"{i}_input"] = x
+        out[f"{i}_expected"] = np.stack(res).reshape(x.shape).astype(bool)
+        m = {"name": name, "op": op}
+        for key, val in kw.items():
+            if isinstance(val, np.ndarray):
+                out[f"{i}_{key}"] = val
+                m[key] = f"{i}_{key}"
+            else:
+                m[key] = list(val) if isinstance(val, tuple) else val
+        meta.append(m)
+    out["meta"] = np.array(json.dumps(meta))
+    np.savez_compressed(os.path.join(HERE, "composite_golden.npz"), **out)
+
+
+def structure_cases():
+    """generate_binary_structure / iterate_structure, with origins
+    (reference test/test_structure.py:10-77, plus larger ranks and
+    iteration counts)."""
+    cases = []
+    for rank, conn in [(1, 1), (2, 1), (2, 2), (3, 1), (3, 2), (3, 3), (4, 2), (4, 4), (5, 3),
+                       (7, 2)]:
+        cases.append(({"name": f"gen_{rank}d{conn}c", "kind": "generate", "rank": rank,
+                       "connectivity": conn}, ndi.generate_binary_structure(rank, conn)))
+    rng = np.random.default_rng(20260106)
+    iters = [
+        ("1d_cross_3", ndi.generate_binary_structure(1, 1), 3, None),
+        ("2d_cross_1", ndi.generate_binary_structure(2, 1), 1, None),
+        ("2d_cross_2", ndi.generate_binary_structure(2, 1), 2, None),
+        ("2d_full_3", ndi.generate_binary_structure(2, 2), 3, None),
+        ("3d_cross_2", ndi.generate_binary_structure(3, 1), 2, None),
+        ("3d_18_3", ndi.generate_binary_structure(3, 2), 3, None),
+        ("3d_full_2", ndi.generate_binary_structure(3, 3), 2, None),
+        ("4d_c2_2", ndi.generate_binary_structure(4, 2), 2, None),
+        ("asymmetric", np.array([[False, True], [True, True]]), 3, None),
+        ("empty", np.zeros((3, 3), dtype=bool), 2, None),
+        ("random_2x4x3", rng.random((2, 4, 3)) < 0.5, 3, None),
+        ("origin_scalar", ndi.generate_binary_structure(2, 1), 3, 0),
+        ("origin_positive", ndi.generate_binary_structure(2, 1), 3, (1, 0)),
+        ("origin_mixed", ndi.generate_binary_structure(2, 1), 3, (-1, 1)),
+        ("origin_3d", ndi.generate_binary_structure(3, 1), 2, (1, -1, 0)),
+    ]
+    for name, st, k, org in iters:
+        if org is None:
+            res = ndi.iterate_structure(st, k)
+            m = {"name": f"iter_{name}", "kind": "iterate", "iterations": k}
+        else:
+            res, o = ndi.iterate_structure(st, k, origin=org)
+            m = {"name": f"iter_{name}", "kind": "iterate", "iterations": k,
+                 "origin": list(org) if isinstance(org, tuple) else org,
+                 "expected_origin": [int(v) for v in o]}
+        cases.append(((m, st), res))
+    out = {}
+    meta = []
+    for i, (m, res) in enumerate(cases):
+        if isinstance(m, tuple):
+            m, st = m
+            out[f"{i}_structure"] = st.astype(bool)
+        out[f"{i}_expected"] = np.asarray(res, dtype=bool)
+        meta.append(m)
+    out["meta"] = np.array(json.dumps(meta))
+    np.savez_compressed(os.path.join(HERE, "structure_golden.npz"), **out)
+
+
 if __name__ == "__main__":
     edt_cases()
     cdt_cases()
     grey_cases()
     binary_cases()
+    composite_cases()
+    structure_cases()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

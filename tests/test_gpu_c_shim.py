@@ -1,6 +1,6 @@
 """The reference-compatible ``_C`` shim: the reference wrappers' exact calling
 patterns (reference grey.py:92-100, binary.py:23-25, distance_transform.py:89-94,
-:125-130, optimal_transport.py:252-287) through torchmorph_amd._C."""
+:125-130, optimal_transport.py:75-113) through torchmorph_amd._C."""
 import numpy as np
 import pytest
 import torch

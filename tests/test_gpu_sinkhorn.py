@@ -86,6 +86,133 @@ def test_hist_small_epsilon_log_space(tm):
     assert torch.allclose(plan.sum(dim=1), a[0], atol=5e-3)
 
 
+def _asym_cost(d, seed, dtype=torch.float64):
+    """A non-symmetric cost: a grid distance plus a random skew, so a
+    transposed operand anywhere changes the answer."""
+    g = torch.Generator().manual_seed(seed)
+    base = torch.cdist(torch.arange(float(d))[:, None], torch.arange(float(d))[:, None])
+    return (base + 2.0 * torch.rand((d, d), generator=g, dtype=torch.float64).float()).to(dtype)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("log_space", [False, True])
+def test_hist_asymmetric_cost_vs_fp64_oracle(tm, oracle_mod, dtype, log_space):
+    """Non-symmetric cost against the FP64 oracle iterations directly:
+    potentials, the fused value <P, C> and the plan."""
+    n, d, eps, iters = 5, 13, 0.7, 300
+    s = _make_positive((n, d), 71).to(dtype)
+    t = _make_positive((n, d), 73).to(dtype)
+    cost = _asym_cost(d, 5, dtype)
+    assert not torch.equal(cost, cost.mT)
+    solver = tm.SinkhornSolver(epsilon=eps, max_iter=iters, log_space=log_space)
+    a, b, c = solver.data_preprocess(s, t, cost)
+    form = oracle_mod.sinkhorn_log_hist if log_space else oracle_mod.sinkhorn_scaling_hist
+    olu, olv = form(a.numpy(), b.numpy(), c.numpy(), eps, iters)
+    oplan = np.exp(olu[:, :, None] - c.double().numpy()[None] / eps + olv[:, None, :])
+    oval = (oplan * c.double().numpy()[None]).sum((1, 2))
+    of = eps * olu - (eps * olu).mean(1, keepdims=True)
+    og = eps * olv - (eps * olv).mean(1, keepdims=True)
+    tol = 2e-4 if dtype == torch.float32 else 1e-9
+    val = solver(s.cuda(), t.cuda(), cost.cuda())
+    f, g = solver.potentials(s.cuda(), t.cuda(), cost.cuda())
+    plan = solver.plan(s.cuda(), t.cuda(), cost.cuda())
+    assert val.dtype == dtype and plan.shape == (n, d, d)
+    np.testing.assert_allclose(val.double().cpu().numpy(), oval, rtol=tol, atol=tol)
+    np.testing.assert_allclose(f.double().cpu().numpy(), of, rtol=tol, atol=tol)
+    np.testing.assert_allclose(g.double().cpu().numpy(), og, rtol=tol, atol=tol)
+    np.testing.assert_allclose(plan.double().cpu().numpy(), oplan, rtol=tol, atol=tol)
+    # the transposed problem is a different one: the answer must move
+    val_t = solver(s.cuda(), t.cuda(), cost.mT.contiguous().cuda())
+    assert (val_t.double().cpu().numpy() - oval).__abs__().max() > 1e3 * tol
+
+
+@pytest.mark.parametrize("log_space", [False, True])
+def test_hist_grid_d1024_vs_fp64_oracle(tm, oracle_mod, log_space):
+    """A 32 x 32 grid cost (d = 1024, the reference benchmark's shape) with
+    several batch rows: float64 against the FP64 oracle, float32 against the
+    CPU path at the reference's rtol / atol 1e-4."""
+    cost = tm.build_cost_matrix((32, 32))
+    eps, iters = 4.0, 120
+    s = _make_positive((3, 1024), 91)
+    t = _make_positive((3, 1024), 97)
+    solver = tm.SinkhornSolver(epsilon=eps, max_iter=iters, log_space=log_space)
+    a, b, c = solver.data_preprocess(s.double(), t.double(), cost.double())
+    form = oracle_mod.sinkhorn_log_hist if log_space else oracle_mod.sinkhorn_scaling_hist
+    olu, _ = form(a.numpy(), b.numpy(), c.numpy(), eps, iters)
+    of = eps * olu - (eps * olu).mean(1, keepdims=True)
+    f64, _ = solver.potentials(s.double().cuda(), t.double().cuda(), cost.double().cuda())
+    np.testing.assert_allclose(f64.cpu().numpy(), of, rtol=1e-8, atol=1e-8)
+    g32 = solver(s.cuda(), t.cuda(), cost.cuda()).cpu()
+    c32 = solver(s, t, cost)
+    assert torch.allclose(g32, c32, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("log_space", [False, True])
+def test_hist_fp64_threshold_stops_like_cpu(tm, log_space):
+    """float64 CUDA honours threshold (the reference runs its torch
+    iterations there): same stopping iteration and values as the CPU path."""
+    from torchmorph_amd.optimal_transport import _hist_solve
+
+    solver = tm.SinkhornSolver(epsilon=1.5, max_iter=5000, threshold=1e-9, log_space=log_space)
+    s = _make_positive((4, 10), 211).double()
+    t = _make_positive((4, 10), 223).double()
+    a, b, c = solver.data_preprocess(s, t)
+    cpu = _hist_solve(a, b, c, 1.5, 5000, 1e-9, log_space, ("potentials", "value"))
+    gpu = _hist_solve(a.cuda(), b.cuda(), c.cuda(), 1.5, 5000, 1e-9, log_space,
+                      ("potentials", "value"))
+    assert 1 < cpu.iterations < 5000 and cpu.iterations % 10 == 1
+    assert gpu.iterations == cpu.iterations
+    assert torch.allclose(gpu.value.cpu(), cpu.value, rtol=1e-9, atol=1e-12)
+    assert torch.allclose(gpu.f.cpu(), cpu.f, rtol=1e-9, atol=1e-12)
+    assert torch.allclose(solver(s.cuda(), t.cuda()).cpu(), solver(s, t), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16])
+def test_hist_half_precision_matches_cpu(tm, dtype):
+    """float16 / bfloat16 CUDA inputs run (solved in float32, cast back)
+    and agree with the CPU path on the same inputs at half precision."""
+    solver = tm.SinkhornSolver(epsilon=1.5, max_iter=400, threshold=1e-7, log_space=True)
+    s = _make_positive((3, 8), 301).to(dtype)
+    t = _make_positive((3, 8), 307).to(dtype)
+    got = solver(s.cuda(), t.cuda())
+    f, g = solver.potentials(s.cuda(), t.cuda())
+    assert got.dtype == dtype and f.dtype == dtype
+    ref = solver(s.float(), t.float())
+    rf, rg = solver.potentials(s.float(), t.float())
+    tol = 1e-2 if dtype == torch.float16 else 4e-2
+    assert torch.allclose(got.float().cpu(), ref, rtol=tol, atol=tol)
+    assert torch.allclose(f.float().cpu(), rf, rtol=tol, atol=tol)
+    assert torch.allclose(g.float().cpu(), rg, rtol=tol, atol=tol)
+
+
+def test_hist_float32_ignores_threshold(tm):
+    """float32 CUDA runs exactly max_iter iterations, as the reference's
+    fused kernels do, whatever the threshold."""
+    from torchmorph_amd.optimal_transport import _hist_solve
+
+    s = _make_positive((2, 6), 401).cuda()
+    t = _make_positive((2, 6), 409).cuda()
+    sol = _hist_solve(s / s.sum(1, keepdim=True), t / t.sum(1, keepdim=True),
+                      tm.build_cost_matrix((6,), device="cuda"), 1.5, 137, 1e-3, True)
+    assert sol.iterations == 137
+
+
+def test_hist_misaligned_views_and_value_consistency(tm):
+    """Rows that do not start on a 16-byte boundary take the scalar loads;
+    the fused value equals <plan, C> from the plan kernel."""
+    solver = tm.SinkhornSolver(epsilon=0.9, max_iter=150, log_space=True)
+    big = _make_positive((1, 3 * 16 + 1), 501).cuda()
+    s = big[0, 1:].reshape(3, 16)
+    t = _make_positive((3, 16), 503).cuda()
+    cost = _asym_cost(16, 9, torch.float32).cuda()
+    assert s.data_ptr() % 16 != 0
+    val = solver(s, t, cost)
+    plan = solver.plan(s, t, cost)
+    assert torch.allclose(val, (plan * cost).sum((1, 2)), rtol=1e-5, atol=1e-6)
+    ref = solver(s.cpu(), t.cpu(), cost.cpu())
+    assert torch.allclose(val.cpu(), ref, rtol=1e-4, atol=1e-5)
+
+
 # ------------------------------------------------------------- point clouds
 
 def _gmm_pair(rng, n, m, d=3):

@@ -246,7 +246,7 @@ def test_sinkhorn_oracle_small_eps_approaches_exact_ot(oracle_mod):
 
 def test_sinkhorn_hist_forms_agree(oracle_mod):
     """Scaling and log-domain histogram iterations (reference
-    optimal_transport.py:259-300) reach the same potentials."""
+    optimal_transport.py:71-125) reach the same potentials."""
     rng = np.random.default_rng(1)
     a = rng.random((3, 6)) + 0.1
     a /= a.sum(1, keepdims=True)

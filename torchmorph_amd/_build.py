@@ -32,8 +32,19 @@ COMMON = [
 ]
 
 
-def _compile(src: str) -> str:
+def _stale(src: str, obj: str) -> bool:
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src, *glob.glob(os.path.join(CSRC, "*.h")),
+            os.path.join(HERE, "..", "include", "torchmorph_hip.h")]
+    return any(os.path.getmtime(p) > t for p in deps if os.path.exists(p))
+
+
+def _compile(src: str, force: bool = True) -> str:
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    if not force and not _stale(src, obj):
+        return obj
     cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -41,12 +52,14 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
+def build(verbose: bool = True, force: bool = True) -> str:
+    """Compiles every source (``force=False``: only objects older than their
+    source or a header) and links the library."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     jobs = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, force), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -59,4 +72,4 @@ def build(verbose: bool = True) -> str:
 
 
 if __name__ == "__main__":
-    build()
+    build(force="--changed" not in sys.argv)

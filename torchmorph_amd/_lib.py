@@ -43,6 +43,11 @@ SIGNATURES: dict[str, tuple] = {
     "tm_grey": (_I, [_P, _I64, _I, _I64P, _P, _P, _I64P, _I64P, _I, _F, _I, _P, _P, _SZ, _P]),
     "tm_sinkhorn_hist": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _I64, _I, _I, _D,
                               ctypes.POINTER(ctypes.c_int), _P]),
+    "tm_sinkhorn_hist_workspace_bytes": (_SZ, [_I, _I64, _I, _D, _I]),
+    "tm_sinkhorn_hist_solve": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _I, _I, _D, _D,
+                                    _P, _P, _P, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int), _P, _SZ, _P]),
+    "tm_sinkhorn_hist_plan": (_I, [_I, _I, _P, _P, _P, _I64, _I, _D, _P, _P]),
     "tm_sinkhorn_points_workspace_bytes": (_SZ, [_I, ctypes.POINTER(ctypes.c_int32),
                                                  ctypes.POINTER(ctypes.c_int32)]),
     "tm_sinkhorn_points": (_I, [_P, _P, _P, _P, _I, _I, ctypes.POINTER(ctypes.c_int32),
@@ -79,6 +84,9 @@ SIGNATURES: dict[str, tuple] = {
     "tm_ragged_grey_workspace_bytes": (_SZ, [_P, _I64]),
     "tm_ragged_grey": (_I, [_P, _I64, _P, _P, _P, _I64P, _I64P, _I, _F, _I, _P, _I, _P, _SZ, _P]),
     "tm_binary_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
+    "tm_binary_fill_holes": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _P, _P, _SZ, _P]),
+    "tm_binary_hit_or_miss": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _P, _I64P, _I64P,
+                                   _P, _P, _SZ, _P]),
     "tm_binary": (_I, [_P, _I, _I64, _I, _I64P, _P, _I64P, _I64P, _I, _I,
                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _P, _P, _P,
                        _SZ, _P]),

@@ -17,6 +17,8 @@
 // the output.  With a mask, or until stable, one step runs per launch
 // (bin_step_kernel) on the 1/8-size packed volume, which stays in L2 /
 // Infinity Cache.
+#include <vector>
+
 #include "tm_common.h"
 
 namespace tmh {
@@ -154,21 +156,51 @@ __device__ __forceinline__ uint64_t bin_fix(uint64_t v, const Packed& pk, int64_
   return v;
 }
 
-// One step.  `orig`/`mask` (optional) implement the reference's
-// torch.where(mask, step(x), x_input).  `changed` (optional) is set when any
-// output word differs from its input word (valid bits only).
+// Element tables of structures beyond BinTable (more than kMaxBinTable
+// active elements, or offsets beyond +-127): a step then runs as one launch
+// per chunk of kWideChunk elements, each chunk ANDing (erosion) or ORing
+// (dilation) into the previous chunk's partial result, with int16 offsets.
+constexpr int kWideChunk = 64;
+struct BinTableWide {  // 4-byte multiple: copied to LDS as words
+  int n;
+  int16_t d[kWideChunk][kMaxNdim];
+};
+static_assert(sizeof(BinTableWide) % 4 == 0, "BinTableWide is copied as 32-bit words");
+
+// Offset of element t along axis j, read from the LDS copy as 32-bit words
+// (8 int8 offsets in two words, or 8 int16 offsets in four).
+__device__ __forceinline__ int tab_off(const BinTable& tb, int t, int j) {
+  const uint32_t* dw = (const uint32_t*)&tb.d[t][0];
+  const uint32_t v = dw[j >> 2] >> (8 * (j & 3));
+  return (int)(int8_t)(v & 0xffu);
+}
+__device__ __forceinline__ int tab_off(const BinTableWide& tb, int t, int j) {
+  const uint32_t* dw = (const uint32_t*)&tb.d[t][0];
+  const uint32_t v = dw[j >> 1] >> (16 * (j & 1));
+  return (int)(int16_t)(v & 0xffffu);
+}
+
+// One step (or, for a wide table, one chunk of a step: `partial` then holds
+// the previous chunks' result, combined in place, and `last` is set on the
+// final chunk).  `invert` reads the complement of `in` (border stays
+// border_value): the miss half of a hit-or-miss transform.  `orig`/`mask` (optional, applied on the last chunk)
+// implement the reference's torch.where(mask, step(x), x_input).  `changed`
+// (optional, last chunk) is set when any output word differs from its input
+// word (valid bits only).
+template <typename Tab>
 __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restrict__ in,
                                                        uint64_t* __restrict__ out, Packed pk,
-                                                       int64_t nwords, BinTable tab, int dilation,
+                                                       int64_t nwords, Tab tab, int dilation,
                                                        int border_value,
                                                        const uint64_t* __restrict__ orig,
                                                        const uint64_t* __restrict__ mask,
-                                                       int* __restrict__ changed) {
-  // the element table is copied to LDS once per workgroup: its int8 offsets
-  // at a loop index would otherwise be byte loads from the kernel arguments,
-  // each waited for before the next
-  __shared__ BinTable stab;
-  for (int i = threadIdx.x; i < (int)(sizeof(BinTable) / 4); i += blockDim.x)
+                                                       int* __restrict__ changed, int partial,
+                                                       int last, int invert) {
+  // the element table is copied to LDS once per workgroup: its offsets at a
+  // loop index would otherwise be byte loads from the kernel arguments, each
+  // waited for before the next
+  __shared__ Tab stab;
+  for (int i = threadIdx.x; i < (int)(sizeof(Tab) / 4); i += blockDim.x)
     ((uint32_t*)&stab)[i] = ((const uint32_t*)&tab)[i];
   __syncthreads();
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,7 +225,7 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
   // elements in groups of 8: every address of a group is formed first
   // (words outside the item read word 0 and are replaced by the border after
   // the load), the 16 loads are issued together, then the terms combine
-  uint64_t acc = dilation ? 0ull : ~0ull;
+  uint64_t acc = partial ? out[g] : (dilation ? 0ull : ~0ull);
   for (int t0 = 0; t0 < stab.n; t0 += 8) {
     int64_t a_lo[8], q_lo[8];
     int sh[8];
@@ -201,13 +233,7 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int t = t0 + u < stab.n ? t0 + u : stab.n - 1;
-      // the element's 8 offsets as two 32-bit LDS words, bytes taken by shifts
-      const uint32_t* dw = (const uint32_t*)&stab.d[t][0];
-      const uint32_t d03 = dw[0], d47 = dw[1];
-      auto off = [&](int j) {
-        const uint32_t v = j < 4 ? d03 >> (8 * j) : d47 >> (8 * (j - 4));
-        return (int)(int8_t)(v & 0xffu);
-      };
+      auto off = [&](int j) { return tab_off(stab, t, j); };
       bool ok = true;
       int64_t r2 = row;
 #pragma unroll
@@ -238,11 +264,12 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
       if (t0 + u >= stab.n) break;
       uint64_t term = border;
       if (inb[u]) {
-        const uint64_t l = bin_fix(lo[u], pk, q_lo[u], border);
+        const uint64_t inv = invert ? ~0ull : 0ull;
+        const uint64_t l = bin_fix(lo[u] ^ inv, pk, q_lo[u], border);
         if (sh[u] == 0) {
           term = l;
         } else {
-          const uint64_t h = bin_fix(hi[u], pk, q_lo[u] + 1, border);
+          const uint64_t h = bin_fix(hi[u] ^ inv, pk, q_lo[u] + 1, border);
           term = (l >> sh[u]) | (h << (64 - sh[u]));
         }
       }
@@ -250,12 +277,12 @@ __global__ __launch_bounds__(256) void bin_step_kernel(const uint64_t* __restric
       else acc &= term;
     }
   }
-  if (mask) {
+  if (last && mask) {
     const uint64_t m = mask[g];
     acc = (acc & m) | (orig[g] & ~m);
   }
   out[g] = acc;
-  if (changed) {
+  if (last && changed) {
     const int64_t valid = pk.W - w * 64;
     const uint64_t vm = valid >= 64 ? ~0ull : ((1ull << valid) - 1ull);
     // one atomic per wave with a changed word, not one per lane
@@ -539,9 +566,10 @@ __global__ void bin_clear_flags(int* flags, int n) {
 
 template <typename T>
 static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape,
-                   const BinTable& tab, int border_value, int nops, const int* op_dilation,
-                   const int* op_iterations, const uint8_t* mask_bytes, uint8_t* output, void* ws,
-                   size_t ws_bytes, hipStream_t st);
+                   const BinTable& tab, const BinTableWide* wide, int nwide, int border_value,
+                   int nops, const int* op_dilation, const int* op_iterations,
+                   const uint8_t* mask_bytes, uint8_t* output, void* ws, size_t ws_bytes,
+                   hipStream_t st);
 
 }  // namespace tmh
 
@@ -558,9 +586,10 @@ namespace tmh {
 
 template <typename T>
 static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape,
-                   const BinTable& tab, int border_value, int nops, const int* op_dilation,
-                   const int* op_iterations, const uint8_t* mask_bytes, uint8_t* output, void* ws,
-                   size_t ws_bytes, hipStream_t st) {
+                   const BinTable& tab, const BinTableWide* wide, int nwide, int border_value,
+                   int nops, const int* op_dilation, const int* op_iterations,
+                   const uint8_t* mask_bytes, uint8_t* output, void* ws, size_t ws_bytes,
+                   hipStream_t st) {
   Packed pk{};
   pk.nd = ndim;
   for (int j = 0; j < ndim; ++j) pk.size[j] = shape[j];
@@ -585,13 +614,25 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
   uint64_t* mpk = orig + nwords;
   int* flags = (int*)(mpk + nwords);
   const dim3 wgrid(grid_for(nwords, 256)), blk(256);
+  // one step from src into dst: one launch, or one per chunk of a wide table
+  auto step = [&](const uint64_t* src, uint64_t* dst, int dil, const uint64_t* org, int* chg) {
+    if (!wide) {
+      hipLaunchKernelGGL(bin_step_kernel<BinTable>, wgrid, blk, 0, st, src, dst, pk, nwords, tab,
+                         dil, border_value, org, mask_bytes ? mpk : nullptr, chg, 0, 1, 0);
+      return;
+    }
+    for (int c = 0; c < nwide; ++c)
+      hipLaunchKernelGGL(bin_step_kernel<BinTableWide>, wgrid, blk, 0, st, src, dst, pk, nwords,
+                         wide[c], dil, border_value, org, mask_bytes ? mpk : nullptr, chg,
+                         c > 0 ? 1 : 0, c == nwide - 1 ? 1 : 0, 0);
+  };
   auto bytes_ok = [&](const void* p) { return pk.W % 64 == 0 && ((uintptr_t)p % 16) == 0; };
   // one launch of up to 8 steps in LDS tiles (g filled; fewer steps when the
   // halo would not fit), or one step per launch (with a mask, or when no
   // tile fits): returns the steps, sets *tiled
   auto chunk_plan = [&](int left, BinTileGeom* g, bool* tiled) {
     int k = left < 8 ? left : 8;
-    *tiled = !mask_bytes && batch <= 65535;
+    *tiled = !mask_bytes && batch <= 65535 && !wide;
     while (*tiled && !bin_tile_plan(pk, tab, k, g)) {
       if (k == 1) *tiled = false;
       else k /= 2;
@@ -659,9 +700,7 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
                                border_value, k);
           fused_out = (io & 2) != 0;
         } else {
-          hipLaunchKernelGGL(bin_step_kernel, wgrid, blk, 0, st, cur, nxt, pk, nwords, tab, dil,
-                             border_value, mask_bytes ? orig : nullptr,
-                             mask_bytes ? mpk : nullptr, nullptr);
+          step(cur, nxt, dil, mask_bytes ? orig : nullptr, nullptr);
         }
         done += k;
         TM_LAUNCH_CHECK();
@@ -684,9 +723,7 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
         hipLaunchKernelGGL(bin_clear_flags, dim3(1), dim3(kMaxChunk), 0, st, flags, chunk);
         TM_LAUNCH_CHECK();
         for (int k = 0; k < chunk; ++k) {
-          hipLaunchKernelGGL(bin_step_kernel, wgrid, blk, 0, st, cur, nxt, pk, nwords, tab, dil,
-                             border_value, mask_bytes ? orig : nullptr,
-                             mask_bytes ? mpk : nullptr, flags + k);
+          step(cur, nxt, dil, mask_bytes ? orig : nullptr, flags + k);
           TM_LAUNCH_CHECK();
           uint64_t* t = cur;
           cur = nxt;
@@ -714,6 +751,55 @@ static int bin_run(const T* input, int64_t batch, int ndim, const int64_t* shape
   return TM_OK;
 }
 
+// Element offsets of the active positions of `structure` (C order over
+// se_shape, centre se_shape / 2 + origin): erosion reads p + d_e, dilation
+// (`negate`) p - d_e (reference binary_kernel.cu:199-229).  Tables that fit
+// BinTable (at most kMaxBinTable elements, offsets within +-127) fill *tab
+// and set *narrow; larger ones fill *wide in chunks of kWideChunk.
+static int bin_tables(const uint8_t* structure, const int64_t* se_shape, const int64_t* origin,
+                      int ndim, bool negate, BinTable* tab, std::vector<BinTableWide>* wide,
+                      bool* narrow, int* count) {
+  int64_t ne = 1;
+  for (int j = 0; j < ndim; ++j) ne *= se_shape[j];
+  std::vector<int> offs;  // element-major, kMaxNdim per element
+  int n = 0;
+  bool fits = true;
+  for (int64_t e = 0; e < ne; ++e) {
+    if (!structure[e]) continue;
+    int64_t rem = e;
+    int64_t r[kMaxNdim];
+    for (int j = ndim - 1; j >= 0; --j) {
+      r[j] = rem % se_shape[j];
+      rem /= se_shape[j];
+    }
+    for (int j = 0; j < kMaxNdim; ++j) {
+      int64_t dd = j < ndim ? r[j] - (se_shape[j] / 2 + origin[j]) : 0;
+      if (negate) dd = -dd;
+      TM_CHECK_ARG(dd >= -32767 && dd <= 32767, "tm_binary: structure offsets beyond +-32767");
+      fits = fits && dd >= -127 && dd <= 127;
+      offs.push_back((int)dd);
+    }
+    ++n;
+  }
+  *count = n;
+  *narrow = fits && n <= kMaxBinTable;
+  *tab = BinTable{};
+  wide->clear();
+  if (*narrow) {
+    tab->n = n;
+    for (int e = 0; e < n; ++e)
+      for (int j = 0; j < kMaxNdim; ++j) tab->d[e][j] = (int8_t)offs[e * kMaxNdim + j];
+    return TM_OK;
+  }
+  wide->assign((n + kWideChunk - 1) / kWideChunk, BinTableWide{});
+  for (int e = 0; e < n; ++e) {
+    BinTableWide& w = (*wide)[e / kWideChunk];
+    for (int j = 0; j < kMaxNdim; ++j) w.d[e % kWideChunk][j] = (int16_t)offs[e * kMaxNdim + j];
+    w.n = e % kWideChunk + 1;
+  }
+  return TM_OK;
+}
+
 }  // namespace tmh
 
 extern "C" int tm_binary(const void* input, int input_dtype, int64_t batch, int ndim,
@@ -726,46 +812,22 @@ extern "C" int tm_binary(const void* input, int input_dtype, int64_t batch, int 
   TM_CHECK_ARG(ndim >= 1 && ndim <= kMaxNdim, "tm_binary: ndim must be in 1..8, got %d", ndim);
   TM_CHECK_ARG(batch >= 1, "tm_binary: batch must be >= 1");
   TM_CHECK_ARG(nops >= 1 && op_dilation && op_iterations, "tm_binary: no operation given");
-  BinTable tab{};
-  int64_t ne = 1;
-  for (int j = 0; j < ndim; ++j) {
+  for (int j = 0; j < ndim; ++j)
     TM_CHECK_ARG(shape[j] >= 1 && se_shape[j] >= 1, "tm_binary: empty axis");
-    ne *= se_shape[j];
-  }
-  for (int op = 0; op < nops; ++op) {
-    // element offsets per operation direction are built below per op; the
-    // table itself is direction-neutral (erosion offsets), dilation negates
-  }
-  // Build one table per distinct direction and run op by op.
   const hipStream_t st = (hipStream_t)stream;
   int status = TM_OK;
-  // ops are split into runs sharing a direction; each run reuses the packed
-  // pipeline by calling bin_run on the previous run's output.
   BinTable tabs[2]{};
+  std::vector<BinTableWide> wides[2];
+  bool narrow = true;
+  int nwide = 0;
   for (int dir = 0; dir < 2; ++dir) {
     int n = 0;
-    for (int64_t e = 0; e < ne; ++e) {
-      if (!structure[e]) continue;
-      TM_CHECK_ARG(n < kMaxBinTable, "tm_binary: more than %d active structure elements",
-                   kMaxBinTable);
-      int64_t rem = e;
-      int64_t r[kMaxNdim];
-      for (int j = ndim - 1; j >= 0; --j) {
-        r[j] = rem % se_shape[j];
-        rem /= se_shape[j];
-      }
-      for (int j = 0; j < ndim; ++j) {
-        int64_t d = r[j] - (se_shape[j] / 2 + origin[j]);
-        if (dir) d = -d;
-        TM_CHECK_ARG(d >= -127 && d <= 127, "tm_binary: structure offsets beyond +-127");
-        tabs[dir].d[n][j] = (int8_t)d;
-      }
-      ++n;
-    }
+    status = bin_tables(structure, se_shape, origin, ndim, dir != 0, &tabs[dir], &wides[dir],
+                        &narrow, &n);
+    if (status != TM_OK) return status;
     TM_CHECK_ARG(n > 0, "tm_binary: structure must contain at least one active position");
-    tabs[dir].n = n;
+    nwide = (int)wides[dir].size();
   }
-  (void)tab;
   // Single-direction sequences run in one packed pipeline; mixed sequences
   // (opening/closing) chain pipelines through the output buffer.
   int start = 0;
@@ -774,15 +836,17 @@ extern "C" int tm_binary(const void* input, int input_dtype, int64_t batch, int 
   while (start < nops) {
     int end = start + 1;
     while (end < nops && op_dilation[end] == op_dilation[start]) ++end;
-    const BinTable& t = tabs[op_dilation[start] ? 1 : 0];
+    const int dir = op_dilation[start] ? 1 : 0;
+    const BinTable& t = tabs[dir];
+    const BinTableWide* w = narrow ? nullptr : wides[dir].data();
     if (cur_dtype == TM_U8)
-      status = bin_run((const uint8_t*)cur_in, batch, ndim, shape, t, border_value, end - start,
-                       op_dilation + start, op_iterations + start, mask, output, workspace,
-                       workspace_bytes, st);
+      status = bin_run((const uint8_t*)cur_in, batch, ndim, shape, t, w, nwide, border_value,
+                       end - start, op_dilation + start, op_iterations + start, mask, output,
+                       workspace, workspace_bytes, st);
     else if (cur_dtype == TM_F32)
-      status = bin_run((const float*)cur_in, batch, ndim, shape, t, border_value, end - start,
-                       op_dilation + start, op_iterations + start, mask, output, workspace,
-                       workspace_bytes, st);
+      status = bin_run((const float*)cur_in, batch, ndim, shape, t, w, nwide, border_value,
+                       end - start, op_dilation + start, op_iterations + start, mask, output,
+                       workspace, workspace_bytes, st);
     else {
       set_error("tm_binary: input dtype must be uint8/bool or float32");
       return TM_ERR_ARG;
@@ -792,5 +856,215 @@ extern "C" int tm_binary(const void* input, int input_dtype, int64_t batch, int 
     cur_dtype = TM_U8;
     start = end;
   }
+  return TM_OK;
+}
+
+namespace tmh {
+
+__global__ void bin_fill_kernel(uint64_t* out, int64_t n, uint64_t v) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) out[g] = v;
+}
+
+// One step of the packed `src` into `dst` with a narrow or a chunked wide
+// table: `invert` reads the complement of src, `partial` combines into dst,
+// `orig`/`mask`/`changed` as bin_step_kernel.
+static int bin_step_into(const uint64_t* src, uint64_t* dst, const Packed& pk, int64_t nwords,
+                         bool narrow, const BinTable& tab, const std::vector<BinTableWide>& wide,
+                         int dil, int border, const uint64_t* orig, const uint64_t* mask,
+                         int* changed, int invert, int partial, hipStream_t st) {
+  const dim3 grid(grid_for(nwords, 256)), blk(256);
+  if (narrow) {
+    hipLaunchKernelGGL(bin_step_kernel<BinTable>, grid, blk, 0, st, src, dst, pk, nwords, tab,
+                       dil, border, orig, mask, changed, partial, 1, invert);
+  } else {
+    for (size_t c = 0; c < wide.size(); ++c)
+      hipLaunchKernelGGL(bin_step_kernel<BinTableWide>, grid, blk, 0, st, src, dst, pk, nwords,
+                         wide[c], dil, border, orig, mask, changed, (partial || c > 0) ? 1 : 0,
+                         c + 1 == wide.size() ? 1 : 0, invert);
+  }
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+static int bin_erode_into(const uint64_t* src, uint64_t* dst, const Packed& pk, int64_t nwords,
+                          bool narrow, const BinTable& tab, const std::vector<BinTableWide>& wide,
+                          int invert, int partial, hipStream_t st) {
+  return bin_step_into(src, dst, pk, nwords, narrow, tab, wide, 0, 0, nullptr, nullptr, nullptr,
+                       invert, partial, st);
+}
+
+__global__ void bin_not_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                               int64_t n) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) out[g] = ~in[g];
+}
+
+static Packed bin_packed_shape(int ndim, const int64_t* shape) {
+  Packed pk{};
+  pk.nd = ndim;
+  for (int j = 0; j < ndim; ++j) pk.size[j] = shape[j];
+  pk.W = shape[ndim - 1];
+  pk.ww = (pk.W + 63) / 64;
+  pk.rows = 1;
+  for (int j = ndim - 2; j >= 0; --j) {
+    pk.rstride[j] = pk.rows;
+    pk.rows *= shape[j];
+  }
+  return pk;
+}
+
+static void bin_pack_any(const void* input, int dtype, uint64_t* out, const Packed& pk,
+                         int64_t nwords, hipStream_t st) {
+  const dim3 grid(grid_for(nwords, 256)), blk(256);
+  if (dtype == TM_U8 && pk.W % 64 == 0 && ((uintptr_t)input % 16) == 0)
+    hipLaunchKernelGGL(bin_pack_bytes_kernel, grid, blk, 0, st, (const uint8_t*)input, out,
+                       nwords);
+  else if (dtype == TM_U8)
+    hipLaunchKernelGGL((bin_pack_kernel<uint8_t>), grid, blk, 0, st, (const uint8_t*)input, out,
+                       pk, nwords);
+  else
+    hipLaunchKernelGGL((bin_pack_kernel<float>), grid, blk, 0, st, (const float*)input, out, pk,
+                       nwords);
+}
+
+static void bin_unpack_any(const uint64_t* in, uint8_t* output, const Packed& pk, int64_t total,
+                           hipStream_t st) {
+  const dim3 blk(256);
+  if (pk.W % 16 == 0 && ((uintptr_t)output % 16) == 0)
+    hipLaunchKernelGGL(bin_unpack_bytes_kernel, dim3(grid_for(total / 16, 256)), blk, 0, st, in,
+                       output, pk, total / 16);
+  else
+    hipLaunchKernelGGL(bin_unpack_kernel, dim3(grid_for(total, 256)), blk, 0, st, in, output, pk,
+                       total);
+}
+
+}  // namespace tmh
+
+extern "C" int tm_binary_hit_or_miss(const void* input, int input_dtype, int64_t batch, int ndim,
+                                     const int64_t* shape, const uint8_t* structure1,
+                                     const int64_t* se1_shape, const int64_t* origin1,
+                                     const uint8_t* structure2, const int64_t* se2_shape,
+                                     const int64_t* origin2, uint8_t* output, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(input && output && structure1 && structure2,
+               "tm_binary_hit_or_miss: NULL input/output/structure");
+  TM_CHECK_ARG(ndim >= 1 && ndim <= kMaxNdim, "tm_binary_hit_or_miss: ndim must be in 1..8");
+  TM_CHECK_ARG(batch >= 1, "tm_binary_hit_or_miss: batch must be >= 1");
+  TM_CHECK_ARG(input_dtype == TM_U8 || input_dtype == TM_F32,
+               "tm_binary_hit_or_miss: input dtype must be uint8/bool or float32");
+  for (int j = 0; j < ndim; ++j)
+    TM_CHECK_ARG(shape[j] >= 1 && se1_shape[j] >= 1 && se2_shape[j] >= 1,
+                 "tm_binary_hit_or_miss: empty axis");
+  const size_t need = tm_binary_workspace_bytes(batch, ndim, shape);
+  if (!workspace || workspace_bytes < need) {
+    set_error("tm_binary_hit_or_miss: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  BinTable t1{}, t2{};
+  std::vector<BinTableWide> w1, w2;
+  bool nar1 = true, nar2 = true;
+  int n1 = 0, n2 = 0;
+  int status = bin_tables(structure1, se1_shape, origin1, ndim, false, &t1, &w1, &nar1, &n1);
+  if (status != TM_OK) return status;
+  status = bin_tables(structure2, se2_shape, origin2, ndim, false, &t2, &w2, &nar2, &n2);
+  if (status != TM_OK) return status;
+  const Packed pk = bin_packed_shape(ndim, shape);
+  const int64_t nwords = batch * pk.rows * pk.ww;
+  const int64_t total = batch * pk.rows * pk.W;
+  uint64_t* bufA = (uint64_t*)workspace;
+  uint64_t* bufB = bufA + nwords;
+  const hipStream_t st = (hipStream_t)stream;
+  const dim3 wgrid(grid_for(nwords, 256)), blk(256);
+  bin_pack_any(input, input_dtype, bufA, pk, nwords, st);
+  TM_LAUNCH_CHECK();
+  // hit: erosion of x by structure1 (border 0); an empty structure1 hits
+  // everywhere.  miss: erosion of not-x by structure2 (border 0), ANDed in.
+  if (n1 > 0) {
+    status = bin_erode_into(bufA, bufB, pk, nwords, nar1, t1, w1, 0, 0, st);
+  } else {
+    hipLaunchKernelGGL(bin_fill_kernel, wgrid, blk, 0, st, bufB, nwords, ~0ull);
+    TM_LAUNCH_CHECK();
+  }
+  if (status != TM_OK) return status;
+  if (n2 > 0) status = bin_erode_into(bufA, bufB, pk, nwords, nar2, t2, w2, 1, 1, st);
+  if (status != TM_OK) return status;
+  bin_unpack_any(bufB, output, pk, total, st);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+extern "C" int tm_binary_fill_holes(const void* input, int input_dtype, int64_t batch, int ndim,
+                                    const int64_t* shape, const uint8_t* structure,
+                                    const int64_t* se_shape, const int64_t* origin,
+                                    uint8_t* output, void* workspace, size_t workspace_bytes,
+                                    void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(input && output && structure, "tm_binary_fill_holes: NULL input/output/structure");
+  TM_CHECK_ARG(ndim >= 1 && ndim <= kMaxNdim, "tm_binary_fill_holes: ndim must be in 1..8");
+  TM_CHECK_ARG(batch >= 1, "tm_binary_fill_holes: batch must be >= 1");
+  TM_CHECK_ARG(input_dtype == TM_U8 || input_dtype == TM_F32,
+               "tm_binary_fill_holes: input dtype must be uint8/bool or float32");
+  for (int j = 0; j < ndim; ++j)
+    TM_CHECK_ARG(shape[j] >= 1 && se_shape[j] >= 1, "tm_binary_fill_holes: empty axis");
+  const size_t need = tm_binary_workspace_bytes(batch, ndim, shape);
+  if (!workspace || workspace_bytes < need) {
+    set_error("tm_binary_fill_holes: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  BinTable tab{};
+  std::vector<BinTableWide> wide;
+  bool narrow = true;
+  int n = 0;
+  int status = bin_tables(structure, se_shape, origin, ndim, true, &tab, &wide, &narrow, &n);
+  if (status != TM_OK) return status;
+  TM_CHECK_ARG(n > 0, "tm_binary_fill_holes: structure must contain at least one active position");
+  const Packed pk = bin_packed_shape(ndim, shape);
+  const int64_t nwords = batch * pk.rows * pk.ww;
+  const int64_t total = batch * pk.rows * pk.W;
+  uint64_t* cur = (uint64_t*)workspace;
+  uint64_t* nxt = cur + nwords;
+  uint64_t* zero = nxt + nwords;  // the seed: where(mask, step, seed) restores 0
+  uint64_t* bg = zero + nwords;   // the mask: background voxels
+  int* flags = (int*)(bg + nwords);
+  const hipStream_t st = (hipStream_t)stream;
+  const dim3 wgrid(grid_for(nwords, 256)), blk(256);
+  bin_pack_any(input, input_dtype, cur, pk, nwords, st);
+  hipLaunchKernelGGL(bin_not_kernel, wgrid, blk, 0, st, cur, bg, nwords);
+  hipLaunchKernelGGL(bin_fill_kernel, wgrid, blk, 0, st, zero, nwords, 0ull);
+  hipLaunchKernelGGL(bin_fill_kernel, wgrid, blk, 0, st, cur, nwords, 0ull);
+  TM_LAUNCH_CHECK();
+  // dilation of the empty seed from outside the item (border 1), restricted
+  // to the background, until a step changes nothing (at least two steps, as
+  // the reference compares consecutive results); the host reads the last
+  // step's change flag after chunks of 8, 16, 32, then 64 steps
+  constexpr int kMaxChunk = 64;
+  int chunk = 8;
+  for (int64_t done = 0;;) {
+    hipLaunchKernelGGL(bin_clear_flags, dim3(1), dim3(kMaxChunk), 0, st, flags, chunk);
+    TM_LAUNCH_CHECK();
+    for (int k = 0; k < chunk; ++k) {
+      status = bin_step_into(cur, nxt, pk, nwords, narrow, tab, wide, 1, 1, zero, bg, flags + k,
+                             0, 0, st);
+      if (status != TM_OK) return status;
+      uint64_t* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    done += chunk;
+    int last = 0;
+    TM_CHECK_HIP(hipMemcpyAsync(&last, flags + chunk - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    TM_CHECK_HIP(hipStreamSynchronize(st));
+    if (last == 0 || done > ((int64_t)1 << 40)) break;
+    chunk = chunk * 2 < kMaxChunk ? chunk * 2 : kMaxChunk;
+  }
+  // holes and objects: everything the outside did not reach
+  hipLaunchKernelGGL(bin_not_kernel, wgrid, blk, 0, st, cur, nxt, nwords);
+  TM_LAUNCH_CHECK();
+  bin_unpack_any(nxt, output, pk, total, st);
+  TM_LAUNCH_CHECK();
   return TM_OK;
 }

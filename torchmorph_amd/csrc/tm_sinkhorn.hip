@@ -1,6 +1,6 @@
 // Log-domain Sinkhorn for gfx950: ragged batches of point-cloud pairs with the
-// squared-Euclidean cost computed on the fly, and the reference's histogram
-// entry point (shared (d, d) cost).
+// squared-Euclidean cost computed on the fly (the histogram entry point of the
+// reference, one shared (d, d) cost, is in tm_hist.hip).
 //
 // Point clouds (no reference counterpart: the reference only solves
 // histograms sharing one cost, optimal_transport.py:189-213).  Dual potentials
@@ -346,93 +346,6 @@ __global__ void sk_copy_kernel(const float* __restrict__ src, float* __restrict_
   if (i < n) dst[i] = src[i];
 }
 
-// ------------------------------------------------------------- histograms --
-// Reference sinkhorn_logiter (ot_kernel.cu:235-283): out[b, i] =
-//   log_num[b, i] - LSE_j(-M[i, j] / eps + log_scale[b, j]); one lane per
-// (b, i), two passes (max, then sum) over j with the cost row in cache.
-template <typename T>
-__global__ __launch_bounds__(256) void hist_log_update(const T* __restrict__ lognum,
-                                                       const T* __restrict__ M,
-                                                       const T* __restrict__ lscale,
-                                                       T* __restrict__ out, int64_t nb, int d,
-                                                       T inv_eps) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nb * d) return;
-  const int64_t b = t / d;
-  const int i = (int)(t - b * d);
-  const T* row = M + (int64_t)i * d;
-  const T* ls = lscale + b * d;
-  T mx = -INFINITY;
-  for (int j = 0; j < d; ++j) mx = fmax(mx, -row[j] * inv_eps + ls[j]);
-  T s = 0;
-  if (mx > -INFINITY)
-    for (int j = 0; j < d; ++j) s += exp(-row[j] * inv_eps + ls[j] - mx);
-  const T lse = (mx > -INFINITY) ? mx + log(s) : mx;
-  out[t] = lognum[t] - lse;
-}
-
-// Reference sinkhorn_fastiter (ot_kernel.cu:188-233): out = num / (K scale + 1e-12).
-template <typename T>
-__global__ __launch_bounds__(256) void hist_scaling_update(const T* __restrict__ num,
-                                                           const T* __restrict__ K,
-                                                           const T* __restrict__ scale,
-                                                           T* __restrict__ out, int64_t nb,
-                                                           int d) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nb * d) return;
-  const int64_t b = t / d;
-  const int i = (int)(t - b * d);
-  const T* row = K + (int64_t)i * d;
-  const T* sc = scale + b * d;
-  T s = 0;
-  for (int j = 0; j < d; ++j) s += row[j] * sc[j];
-  out[t] = num[t] / (s + (T)1e-12);
-}
-
-// Runs `launch(k)` (k iterations enqueued on st) n_iter times in total; for
-// n_iter >= 100 a chunk of 25 iterations is captured once as a hipGraph and
-// replayed (reference optimal_transport.py:213-243).  A capture failure
-// falls back to plain launches; extra iterations are never run.
-template <typename F>
-static int run_chunked(F&& launch, int n_iter, hipStream_t st, hipStream_t* lst,
-                       int* fallback_out) {
-  constexpr int kChunk = 25;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  const bool capturing = hipStreamIsCapturing(st, &cs) == hipSuccess &&
-                         cs != hipStreamCaptureStatusNone;
-  const char* force = getenv("TORCHMORPH_AMD_FORCE_GRAPH_FAILURE");
-  if (n_iter < 100 || capturing) return launch(n_iter);
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipStream_t cap = nullptr;
-  bool ok = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) == hipSuccess;
-  if (ok) ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
-  int status = TM_OK;
-  if (ok) {
-    *lst = cap;
-    status = launch(kChunk);
-    *lst = st;
-    ok = hipStreamEndCapture(cap, &graph) == hipSuccess && status == TM_OK;
-  }
-  if (ok && force && force[0] == '1') ok = false;
-  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-  if (!ok) {
-    if (graph) hipGraphDestroy(graph);
-    if (cap) hipStreamDestroy(cap);
-    (void)hipGetLastError();
-    if (fallback_out) *fallback_out = 1;
-    return launch(n_iter);
-  }
-  const int reps = n_iter / kChunk;
-  for (int r = 0; r < reps && status == TM_OK; ++r)
-    if (hipGraphLaunch(exec, st) != hipSuccess) status = TM_ERR_HIP;
-  if (status == TM_OK && n_iter % kChunk) status = launch(n_iter % kChunk);
-  hipGraphExecDestroy(exec);
-  hipGraphDestroy(graph);
-  hipStreamDestroy(cap);
-  return status;
-}
-
 }  // namespace tmh
 
 // ------------------------------------------------------------ C ABI: points --
@@ -722,54 +635,4 @@ extern "C" int tm_sinkhorn_points_grad(const float* x, const float* y, const flo
 #undef SK_GRAD
   TM_LAUNCH_CHECK();
   return TM_OK;
-}
-
-// --------------------------------------------------------- C ABI: histograms --
-
-extern "C" int tm_sinkhorn_hist(int dtype, int log_space, const void* num_a, const void* num_b,
-                                const void* M, const void* M_T, void* u, void* v, int64_t nb,
-                                int d, int n_iter, double epsilon, int* used_fallback,
-                                void* stream) {
-  using namespace tmh;
-  TM_CHECK_ARG(num_a && num_b && M && M_T && u && v, "tm_sinkhorn_hist: NULL argument");
-  TM_CHECK_ARG(nb >= 1 && d >= 1 && n_iter >= 0, "tm_sinkhorn_hist: bad sizes");
-  TM_CHECK_ARG(dtype == TM_F32 || dtype == TM_F64, "tm_sinkhorn_hist: dtype must be f32 or f64");
-  hipStream_t st = (hipStream_t)stream;
-  hipStream_t lst = st;
-  const dim3 grid(grid_for(nb * d, 256)), blk(256);
-  if (used_fallback) *used_fallback = 0;
-  auto launch = [&](int k) -> int {
-    for (int it = 0; it < k; ++it) {
-      if (dtype == TM_F32) {
-        if (log_space) {
-          const float ie = (float)(1.0 / epsilon);
-          hipLaunchKernelGGL(hist_log_update<float>, grid, blk, 0, lst, (const float*)num_a,
-                             (const float*)M, (const float*)v, (float*)u, nb, d, ie);
-          hipLaunchKernelGGL(hist_log_update<float>, grid, blk, 0, lst, (const float*)num_b,
-                             (const float*)M_T, (const float*)u, (float*)v, nb, d, ie);
-        } else {
-          hipLaunchKernelGGL(hist_scaling_update<float>, grid, blk, 0, lst, (const float*)num_a,
-                             (const float*)M, (const float*)v, (float*)u, nb, d);
-          hipLaunchKernelGGL(hist_scaling_update<float>, grid, blk, 0, lst, (const float*)num_b,
-                             (const float*)M_T, (const float*)u, (float*)v, nb, d);
-        }
-      } else {
-        if (log_space) {
-          const double ie = 1.0 / epsilon;
-          hipLaunchKernelGGL(hist_log_update<double>, grid, blk, 0, lst, (const double*)num_a,
-                             (const double*)M, (const double*)v, (double*)u, nb, d, ie);
-          hipLaunchKernelGGL(hist_log_update<double>, grid, blk, 0, lst, (const double*)num_b,
-                             (const double*)M_T, (const double*)u, (double*)v, nb, d, ie);
-        } else {
-          hipLaunchKernelGGL(hist_scaling_update<double>, grid, blk, 0, lst, (const double*)num_a,
-                             (const double*)M, (const double*)v, (double*)u, nb, d);
-          hipLaunchKernelGGL(hist_scaling_update<double>, grid, blk, 0, lst, (const double*)num_b,
-                             (const double*)M_T, (const double*)u, (double*)v, nb, d);
-        }
-      }
-      TM_LAUNCH_CHECK();
-    }
-    return TM_OK;
-  };
-  return run_chunked(launch, n_iter, st, &lst, used_fallback);
 }

@@ -16,7 +16,7 @@ from torch import Tensor
 
 from .. import _lib
 from .._validation import validate_bcs_input, validate_output
-from .structure import _normalize_origin, _validate_origin, generate_binary_structure
+from .structure import check_origin, generate_binary_structure, origin_tuple
 
 
 def _normalize_structure(structure: Tensor, spatial_ndim: int, name: str = "structure") -> Tensor:
@@ -72,8 +72,8 @@ def _prepare(input, structure, origin, mask, output):
     if structure is None:
         structure = _default_structure(spatial_ndim)
     structure = _normalize_structure(structure, spatial_ndim)
-    origin = _normalize_origin(origin, spatial_ndim)
-    _validate_origin(origin, structure)
+    origin = origin_tuple(origin, spatial_ndim)
+    check_origin(origin, structure)
     if mask is not None and mask.shape != input.shape:
         raise ValueError(f"mask shape {mask.shape} must match input shape {input.shape}")
     return structure, origin
@@ -82,7 +82,7 @@ def _prepare(input, structure, origin, mask, output):
 def _sequence(input, structure, origin, border_value, ops, mask, output):
     structure, origin = _prepare(input, structure, origin, mask, output)
     x = input.detach()
-    if not structure.any().item():
+    if not bool(structure.any()):
         # empty structure: every step yields a constant (reference binary.py:54-58),
         # masked against that step's input
         cur = x != 0
@@ -93,10 +93,7 @@ def _sequence(input, structure, origin, border_value, ops, mask, output):
         result = cur
     else:
         result = _run(x.contiguous(), structure, origin, border_value, ops, mask)
-    if output is not None:
-        output.copy_(result)
-        return output
-    return result
+    return _emit(result, output)
 
 
 def binary_erosion(input: Tensor, structure: Tensor | None = None, iterations: int = 1,
@@ -136,50 +133,82 @@ def binary_propagation(input: Tensor, structure: Tensor | None = None,
     return _sequence(input, structure, origin, border_value, [(True, -1)], mask, output)
 
 
+def _emit(result: Tensor, output: Tensor | None) -> Tensor:
+    if output is None:
+        return result
+    output.copy_(result)
+    return output
+
+
+def _packed_call(fn, input: Tensor, *args) -> Tensor:
+    """Runs a packed-pipeline entry point `fn(input, dtype, batch, ndim,
+    shape, *args, output, workspace, bytes, stream)` into a new bool tensor."""
+    L = _lib.lib()
+    x = input.detach().contiguous()
+    xin, in_dtype = _lib.mask_operand(x)
+    shape = _lib.i64_array(x.shape[2:])
+    batch, nd = x.shape[0] * x.shape[1], x.ndim - 2
+    out = torch.empty(x.shape, dtype=torch.bool, device=x.device)
+    nbytes = L.tm_binary_workspace_bytes(batch, nd, shape)
+    ws = _lib.workspace(nbytes, x.device)
+    with torch.cuda.device(x.device):
+        _lib.check(fn(_lib.ptr(xin), in_dtype, batch, nd, shape, *args,
+                      _lib.ptr(out.view(torch.uint8)), _lib.ptr(ws), nbytes,
+                      _lib.stream_of(x.device)), fn.__name__)
+    return out
+
+
+def _host_structure(structure: Tensor):
+    """(uint8 host array, its shape as an int64 array) of a bool structure."""
+    arr = structure.to(torch.uint8).numpy()
+    return arr, arr.ctypes.data_as(_lib.ctypes.c_void_p), _lib.i64_array(structure.shape)
+
+
 def binary_fill_holes(input: Tensor, structure: Tensor | None = None,
                       output: Tensor | None = None, origin=0) -> Tensor:
-    """Fill holes (reference binary.py:160-185)."""
-    validate_bcs_input(input)
+    """Sets every voxel that the outside of the item cannot reach through
+    background voxels (reference binary.py:160-185, SciPy semantics: the
+    reach is a dilation with ``structure`` from a border of ones, restricted
+    to the background, repeated until stable).  One packed pipeline
+    (``tm_binary_fill_holes``)."""
+    spatial_ndim = validate_bcs_input(input)
     validate_output(input, output)
-    mask = input == 0
-    seed = torch.zeros_like(mask, dtype=torch.bool)
-    background = binary_propagation(seed, structure=structure, mask=mask, border_value=True,
-                                    origin=origin)
-    result = torch.logical_not(background)
-    if output is not None:
-        output.copy_(result)
-        return output
-    return result
+    if structure is None:
+        structure = _default_structure(spatial_ndim)
+    structure = _normalize_structure(structure, spatial_ndim)
+    origin = origin_tuple(origin, spatial_ndim)
+    check_origin(origin, structure)
+    if not bool(structure.any()):
+        # nothing propagates: no background is reached, everything is set
+        return _emit(torch.ones(input.shape, dtype=torch.bool, device=input.device), output)
+    keep, sptr, sshape = _host_structure(structure)
+    result = _packed_call(_lib.lib().tm_binary_fill_holes, input, sptr, sshape,
+                          _lib.i64_array(origin))
+    del keep
+    return _emit(result, output)
 
 
 def binary_hit_or_miss(input: Tensor, structure1: Tensor | None = None,
                        structure2: Tensor | None = None, output: Tensor | None = None,
                        origin1=0, origin2=None) -> Tensor:
-    """Hit-or-miss transform (reference binary.py:188-228)."""
+    """Voxels whose ``structure1`` neighbourhood is all set and whose
+    ``structure2`` neighbourhood (default: the complement of structure1) is
+    all clear (reference binary.py:188-228).  Both erosions and their AND run
+    in one packed pipeline (``tm_binary_hit_or_miss``); an empty structure1
+    hits everywhere, an empty structure2 misses nowhere."""
     spatial_ndim = validate_bcs_input(input)
     validate_output(input, output)
-    origin1 = _normalize_origin(origin1, spatial_ndim)
-    origin2 = origin1 if origin2 is None else _normalize_origin(origin2, spatial_ndim)
-    if structure1 is None:
-        structure1 = generate_binary_structure(spatial_ndim, 1)
-    structure1 = _normalize_structure(structure1, spatial_ndim, "structure1")
-    _validate_origin(origin1, structure1, "origin1")
-    if structure2 is None:
-        structure2 = torch.logical_not(structure1).contiguous()
-    else:
-        structure2 = _normalize_structure(structure2, spatial_ndim, "structure2")
-    _validate_origin(origin2, structure2, "origin2")
-    input_bool = input != 0
-    if structure1.any().item():
-        hit = binary_erosion(input_bool, structure=structure1, origin=origin1)
-    else:
-        hit = torch.ones_like(input_bool, dtype=torch.bool)
-    if structure2.any().item():
-        miss = binary_erosion(input_bool == 0, structure=structure2, origin=origin2)
-        result = torch.logical_and(hit, miss)
-    else:
-        result = hit
-    if output is not None:
-        output.copy_(result)
-        return output
-    return result
+    origin1 = origin_tuple(origin1, spatial_ndim)
+    origin2 = origin1 if origin2 is None else origin_tuple(origin2, spatial_ndim)
+    s1 = _default_structure(spatial_ndim) if structure1 is None else structure1
+    s1 = _normalize_structure(s1, spatial_ndim, "structure1")
+    check_origin(origin1, s1, "origin1")
+    s2 = (~s1).contiguous() if structure2 is None else \
+        _normalize_structure(structure2, spatial_ndim, "structure2")
+    check_origin(origin2, s2, "origin2")
+    k1, p1, sh1 = _host_structure(s1)
+    k2, p2, sh2 = _host_structure(s2)
+    result = _packed_call(_lib.lib().tm_binary_hit_or_miss, input, p1, sh1,
+                          _lib.i64_array(origin1), p2, sh2, _lib.i64_array(origin2))
+    del k1, k2
+    return _emit(result, output)

@@ -17,7 +17,7 @@ from torch import Tensor
 
 from .. import _lib
 from .._validation import validate_bcs_input, validate_output
-from .structure import _normalize_origin, _validate_origin
+from .structure import check_origin, origin_tuple as _origin_tuple
 
 _MODE_MAP = {"constant": 0, "reflect": 1, "nearest": 2, "mirror": 3, "wrap": 4}
 
@@ -65,8 +65,8 @@ def _resolve_element_nd(spatial_ndim: int, size, footprint, structure, mode, ori
     if struct.ndim != spatial_ndim:
         raise ValueError(f"Structure dimension {struct.ndim} must match spatial dimension "
                          f"{spatial_ndim}.")
-    origin_tuple = _normalize_origin(origin, spatial_ndim)
-    _validate_origin(origin_tuple, struct)
+    origin_tuple = _origin_tuple(origin, spatial_ndim)
+    check_origin(origin_tuple, struct)
     if mode not in _MODE_MAP:
         raise ValueError(f"Unknown mode '{mode}'. Must be one of {list(_MODE_MAP.keys())}.")
     flat_box = structure is None and footprint_cpu is None

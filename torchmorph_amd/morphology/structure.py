@@ -1,80 +1,77 @@
-"""Structuring-element helpers (CPU torch), same semantics and messages as
-reference torchmorph/morphology/structure.py:5-97 (SciPy's
-generate_binary_structure / iterate_structure, origin rules)."""
-from __future__ import annotations
+"""Structuring-element helpers (CPU torch).
 
-import itertools
+Public names, arguments and error messages are those of reference
+torchmorph/morphology/structure.py:5-97; the definitions are SciPy's
+(``scipy.ndimage.generate_binary_structure`` / ``iterate_structure``):
+
+* the structure of a given connectivity holds the points of the 3^rank cube
+  around the centre that are reached in at most ``connectivity`` unit steps,
+  i.e. whose number of nonzero offsets is <= connectivity;
+* iterating a structure k times is its k-fold Minkowski sum with itself (the
+  structure dilated by itself k - 1 times), on an array of extent
+  k (s - 1) + 1 per axis; the origin scales with k.
+
+Origins follow SciPy's range rule: along an axis of extent s an origin must
+lie in [-(s // 2), (s - 1) // 2].
+"""
+from __future__ import annotations
 
 import torch
 from torch import Tensor
 
 
 def generate_binary_structure(rank: int, connectivity: int) -> Tensor:
-    """Elements within `connectivity` unit steps of the centre of a 3^rank cube."""
+    """(3,) * rank bool tensor: the cube points with at most ``connectivity``
+    nonzero offsets from the centre."""
     if connectivity < 1 or connectivity > rank:
         raise ValueError(f"connectivity must be in [1, rank], got {connectivity}")
     if rank < 1:
         raise ValueError(f"rank must be >= 1, got {rank}")
-    axes = [torch.tensor([-1, 0, 1]) for _ in range(rank)]
-    offsets = torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=0)
-    return (offsets != 0).sum(dim=0) <= connectivity
+    # the number of nonzero offsets of every cube point, one axis at a time
+    steps = torch.zeros((), dtype=torch.int64)
+    for _ in range(rank):
+        steps = (steps[..., None] + torch.tensor([1, 0, 1])).contiguous()
+    return steps <= connectivity
 
 
-def _generated_structure_connectivity(structure: Tensor) -> int | None:
-    if any(size != 3 for size in structure.shape):
-        return None
-    rank = structure.ndim
-    for connectivity in range(1, rank + 1):
-        if torch.equal(structure.cpu(), generate_binary_structure(rank, connectivity)):
-            return connectivity
-    return None
-
-
-def _normalize_origin(origin, ndim: int) -> tuple[int, ...]:
+def origin_tuple(origin, ndim: int) -> tuple[int, ...]:
+    """An int origin broadcast to ``ndim`` axes, or a sequence of exactly
+    ``ndim`` ints."""
     if isinstance(origin, int):
         return (origin,) * ndim
-    origin_tuple = tuple(int(v) for v in origin)
-    if len(origin_tuple) != ndim:
-        raise ValueError(f"origin dimension is not {ndim}, got {len(origin_tuple)}")
-    return origin_tuple
+    values = tuple(int(v) for v in origin)
+    if len(values) != ndim:
+        raise ValueError(f"origin dimension is not {ndim}, got {len(values)}")
+    return values
 
 
-def _validate_origin(origin: tuple[int, ...], structure: Tensor, name: str = "origin") -> None:
-    for value, size in zip(origin, structure.shape):
-        if not -(size // 2) <= value <= (size - 1) // 2:
+def check_origin(origin: tuple[int, ...], structure: Tensor, name: str = "origin") -> None:
+    """Raises ValueError("invalid <name>") when an origin value falls outside
+    [-(s // 2), (s - 1) // 2] for its axis extent s."""
+    for value, extent in zip(origin, structure.shape):
+        if value < -(extent // 2) or value > (extent - 1) // 2:
             raise ValueError(f"invalid {name}")
 
 
+def _minkowski_sum(a: Tensor, b: Tensor) -> Tensor:
+    """Bool array of the sums p + q (p active in a, q active in b), on the
+    full extent a.shape + b.shape - 1: one shifted OR of a per active q."""
+    out = torch.zeros([x + y - 1 for x, y in zip(a.shape, b.shape)], dtype=torch.bool)
+    for q in torch.nonzero(b).tolist():
+        window = tuple(slice(lo, lo + n) for lo, n in zip(q, a.shape))
+        out[window] |= a
+    return out
+
+
 def iterate_structure(structure: Tensor, iterations: int, origin=None):
-    """Dilate a binary structure with itself `iterations - 1` times
-    (reference structure.py:60-97)."""
-    structure = structure != 0
-    if iterations < 2:
-        result = structure.clone()
-    else:
-        conn = _generated_structure_connectivity(structure)
-        shape = tuple(iterations * (s - 1) + 1 for s in structure.shape)
-        if conn is not None:
-            axes = [torch.arange(-iterations, iterations + 1, device=structure.device)
-                    for _ in range(structure.ndim)]
-            offsets = torch.stack(torch.meshgrid(*axes, indexing="ij"), dim=0)
-            result = offsets.abs().sum(dim=0) <= conn * iterations
-        else:
-            coords = torch.nonzero(structure, as_tuple=False)
-            result = torch.zeros(shape, dtype=torch.bool, device=structure.device)
-            if coords.numel():
-                cur = coords
-                for _ in range(iterations - 1):
-                    cur = torch.unique((cur[:, None, :] + coords[None, :, :]).reshape(
-                        -1, structure.ndim), dim=0)
-                result[tuple(cur.unbind(dim=1))] = True
+    """``structure`` dilated by itself ``iterations - 1`` times (SciPy's
+    iterate_structure); with ``origin`` also returns the origin scaled by
+    ``iterations``."""
+    base = (structure != 0).cpu()
+    result = base.clone()
+    for _ in range(iterations - 1):
+        result = _minkowski_sum(result, base)
+    result = result.to(structure.device)
     if origin is None:
         return result
-    return result, [v * iterations for v in _normalize_origin(origin, structure.ndim)]
-
-
-def _cross(ndim: int) -> Tensor:
-    s = torch.zeros((3,) * ndim, dtype=torch.bool)
-    for p in itertools.product(range(3), repeat=ndim):
-        s[p] = sum(abs(v - 1) for v in p) <= 1
-    return s
+    return result, [v * iterations for v in origin_tuple(origin, structure.ndim)]

@@ -3,15 +3,18 @@
 Two entry points:
 
 * :class:`SinkhornSolver` / :func:`build_cost_matrix` -- the reference's
-  histogram API (reference torchmorph/optimal_transport.py:182-424): (n, d)
-  histograms sharing one (d, d) cost, same arguments, preprocessing,
-  envelope-theorem backward and messages.  GPU tensors (float32 and float64)
-  run the HIP kernels of libtorchmorph_hip (``tm_sinkhorn_hist``: one update
-  kernel per half-iteration, chunks of 25 iterations captured as a hipGraph
-  for max_iter >= 100, plain launches below, as the reference does).  CPU
-  tensors run the same iterations with torch operations, as the reference's
-  CPU path does (optimal_transport.py:259-300); that path never handles GPU
-  tensors.
+  histogram API (reference torchmorph/optimal_transport.py:7-18,128-249):
+  (n, d) histograms sharing one (d, d) cost, same arguments, preprocessing,
+  envelope-theorem backward and messages.  GPU tensors run
+  ``tm_sinkhorn_hist_solve``: one update kernel per half-iteration, the value
+  <P, C> and the centred potentials fused into the last update and a finish
+  kernel (no n x d x d tensor), chunks of iterations replayed as a hipGraph
+  for max_iter >= 100.  As in the reference, float32 runs exactly max_iter
+  iterations (its fused kernels ignore ``threshold``) and every other dtype
+  honours ``threshold``; float64 is solved in float64, float16 / bfloat16 in
+  float32 with results cast back.  ``plan()`` is formed only when asked for
+  (``tm_sinkhorn_hist_plan``).  CPU tensors run the same iterations with
+  torch operations, batch rows in chunks.
 * :func:`sinkhorn_points` / :class:`SinkhornPointCloud` -- no reference
   counterpart: ragged batches of weighted point-cloud pairs with the squared
   Euclidean cost computed on the fly, solved to a row-marginal L1 error
@@ -33,107 +36,190 @@ from . import _lib
 
 
 def build_cost_matrix(shape, p=2, device=None) -> Tensor:
-    """Pairwise L^p distances between the points of a grid (reference
-    optimal_transport.py:7-18)."""
-    coords = torch.stack(
-        torch.meshgrid([torch.arange(s, device=device) for s in shape], indexing="ij"), dim=-1)
-    coords = coords.reshape(-1, len(shape)).float()
-    return torch.cdist(coords, coords, p=p)
+    """(d, d) L^p distances between the d points of a grid of ``shape``, points
+    in row-major order (reference optimal_transport.py:7-18)."""
+    axes = [torch.arange(int(s), device=device).float() for s in shape]
+    points = torch.cartesian_prod(*axes) if len(axes) > 1 else axes[0][:, None]
+    return torch.cdist(points, points, p=p)
 
 
-def _transpose_or_self(matrix: Tensor) -> Tensor:
-    return matrix if torch.equal(matrix, matrix.mT) else matrix.mT.contiguous()
+# batch rows per CPU step: bounds the (rows, d, d) temporaries of the CPU path
+_CPU_TERMS = 1 << 24
 
 
-def _hip_iterations(a: Tensor, b: Tensor, cost_matrix: Tensor, epsilon: float, max_iter: int,
-                    log_space: bool):
-    """HIP path for GPU tensors: returns (log_u, log_v)."""
-    L = _lib.lib()
-    dtype = {torch.float32: _lib.TM_F32, torch.float64: _lib.TM_F64}.get(a.dtype)
-    if dtype is None:
-        raise RuntimeError(f"SinkhornSolver: unsupported GPU dtype {a.dtype}")
-    a, b = a.contiguous(), b.contiguous()
+@dataclass
+class _HistSolution:
+    """State of a histogram solve: the final (log-)scalings in the compute
+    dtype, and whichever of the centred potentials and the value were asked
+    for (in the input dtype)."""
+
+    log_space: bool
+    eps: float
+    u: Tensor
+    v: Tensor
+    cost: Tensor
+    iterations: int
+    f: Optional[Tensor] = None
+    g: Optional[Tensor] = None
+    value: Optional[Tensor] = None
+
+    def log_scalings(self):
+        return (self.u, self.v) if self.log_space else (self.u.log(), self.v.log())
+
+
+def _row_chunks(n: int, d: int):
+    step = max(1, _CPU_TERMS // max(1, d * d))
+    return [(r, min(n, r + step)) for r in range(0, n, step)]
+
+
+def _cpu_solve(a, b, cost, eps, max_iter, threshold, log_space, want) -> _HistSolution:
+    """CPU tensors: the reference's torch iterations (optimal_transport.py:71-125).
+
+    Log form ``log u = log a - LSE_j(log v_j - C_ij / eps)`` (then v from u),
+    scaling form ``u = a / (K v + 1e-12)`` with K = exp(-C / eps); with
+    ``threshold`` > 0 the run stops after iteration i (i % 10 == 0) once the
+    largest row L1 change of u is <= threshold.  Log-sum-exps are taken over
+    batch chunks so no (n, d, d) temporary exists at once."""
     n, d = a.shape
-    with torch.cuda.device(a.device):
-        if log_space:
-            num_a, num_b = a.log().contiguous(), b.log().contiguous()
-            M = cost_matrix.contiguous()
-            u = torch.zeros_like(a)
-            v = torch.zeros_like(b)
-        else:
-            num_a, num_b = a, b
-            M = torch.exp(-cost_matrix / epsilon).contiguous()
-            u = torch.ones_like(a)
-            v = torch.ones_like(b)
-        MT = _transpose_or_self(M)
-        fb = _lib.ctypes.c_int(0)
-        _lib.check(L.tm_sinkhorn_hist(dtype, int(log_space), _lib.ptr(num_a), _lib.ptr(num_b),
-                                      _lib.ptr(M), _lib.ptr(MT), _lib.ptr(u), _lib.ptr(v), n, d,
-                                      int(max_iter), float(epsilon), _lib.ctypes.byref(fb),
-                                      _lib.stream_of(a.device)), "tm_sinkhorn_hist")
+    chunks = _row_chunks(n, d)
     if log_space:
-        return u, v
-    return u.log(), v.log()
+        log_k = -cost / eps
+        log_kt = log_k.mT
+        la, lb = a.log(), b.log()
 
+        def half(num, lk, scale):
+            return torch.cat([num[r0:r1] - torch.logsumexp(lk + scale[r0:r1, None, :], dim=-1)
+                              for r0, r1 in chunks]) if n else num.clone()
 
-def _torch_iterations(a, b, cost_matrix, epsilon, max_iter, threshold, log_space):
-    """CPU tensors: the reference's torch iterations (optimal_transport.py:259-300)."""
-    if log_space:
-        log_a, log_b = a.log(), b.log()
-        log_K = -cost_matrix / epsilon
-        log_u = torch.zeros_like(a)
-        log_v = torch.zeros_like(b)
-        for i in range(max_iter):
-            prev = log_u
-            log_u = log_a - torch.logsumexp(log_K + log_v.unsqueeze(-2), dim=-1)
-            log_v = log_b - torch.logsumexp(log_K.mT + log_u.unsqueeze(-2), dim=-1)
-            if threshold > 0 and i % 10 == 0:
-                if (log_u - prev).abs().sum(dim=-1).max() <= threshold:
-                    break
-        return log_u, log_v
-    K = torch.exp(-cost_matrix / epsilon)
-    u = torch.ones_like(a)
-    v = torch.ones_like(b)
-    for i in range(max_iter):
-        prev = u
-        u = a / (v @ K.mT + 1e-12)
-        v = b / (u @ K + 1e-12)
-        if threshold > 0 and i % 10 == 0:
-            if (u - prev).abs().sum(dim=-1).max() <= threshold:
+        u_step = lambda v: half(la, log_k, v)  # noqa: E731
+        v_step = lambda u: half(lb, log_kt, u)  # noqa: E731
+        u, v = torch.zeros_like(a), torch.zeros_like(b)
+    else:
+        k = torch.exp(-cost / eps)
+        u_step = lambda v: a / (v @ k.mT + 1e-12)  # noqa: E731
+        v_step = lambda u: b / (u @ k + 1e-12)  # noqa: E731
+        u, v = torch.ones_like(a), torch.ones_like(b)
+    done = max_iter
+    for it in range(max_iter):
+        before = u
+        u = u_step(v)
+        v = v_step(u)
+        if threshold > 0 and it % 10 == 0 and n:
+            if bool((u - before).abs().sum(dim=-1).max() <= threshold):
+                done = it + 1
                 break
-    return u.log(), v.log()
+    sol = _HistSolution(log_space, eps, u, v, cost, done)
+    lu, lv = sol.log_scalings()
+    if "potentials" in want:
+        sol.f = eps * lu - (eps * lu).mean(dim=-1, keepdim=True)
+        sol.g = eps * lv - (eps * lv).mean(dim=-1, keepdim=True)
+    if "value" in want:
+        parts = []
+        for r0, r1 in chunks:
+            plan = torch.exp(lu[r0:r1, :, None] - cost / eps + lv[r0:r1, None, :])
+            parts.append((plan * cost).sum(dim=(-2, -1)))
+        sol.value = torch.cat(parts) if parts else a.new_zeros((0,))
+    return sol
 
 
-def _iterate(a, b, cost_matrix, epsilon, max_iter, threshold, log_space):
-    if a.is_cuda:
-        return _hip_iterations(a, b, cost_matrix, epsilon, max_iter, log_space)
-    return _torch_iterations(a, b, cost_matrix, epsilon, max_iter, threshold, log_space)
+def _gpu_solve(a, b, cost, eps, max_iter, threshold, log_space, want) -> _HistSolution:
+    """GPU tensors: ``tm_sinkhorn_hist_solve`` (HIP), computing in float32 or
+    float64; float32 ignores ``threshold`` as the reference's fused kernels do."""
+    out_dtype = a.dtype
+    cdt = a.dtype if a.dtype in (torch.float32, torch.float64) else torch.float32
+    if a.dtype == torch.float32:
+        threshold = 0.0
+    a = a.detach().to(cdt).contiguous()
+    b = b.detach().to(cdt).contiguous()
+    cost = cost.detach().to(cdt).contiguous()
+    n, d = a.shape
+    dev = a.device
+    if n == 0:
+        empty = a.new_zeros((0, d))
+        return _HistSolution(log_space, eps, empty, empty, cost, 0, empty.to(out_dtype),
+                             empty.to(out_dtype), a.new_zeros((0,), dtype=out_dtype))
+    cost_t = cost.mT.contiguous()
+    if log_space:
+        num_a, num_b, M, M_t = a.log(), b.log(), cost, cost_t
+        u, v = torch.zeros_like(a), torch.zeros_like(b)
+    else:
+        num_a, num_b = a, b
+        M = torch.exp(-cost / eps).contiguous()
+        M_t = M.mT.contiguous()
+        u, v = torch.ones_like(a), torch.ones_like(b)
+    f = torch.empty_like(a) if "potentials" in want else None
+    g = torch.empty_like(b) if "potentials" in want else None
+    value = a.new_empty((n,)) if "value" in want else None
+    L = _lib.lib()
+    dtype = _lib.TM_F32 if cdt == torch.float32 else _lib.TM_F64
+    iters = _lib.ctypes.c_int(0)
+    fb = _lib.ctypes.c_int(0)
+    with torch.cuda.device(dev):
+        nbytes = L.tm_sinkhorn_hist_workspace_bytes(dtype, n, d, float(threshold),
+                                                    int(value is not None))
+        ws = _lib.workspace(nbytes, dev)
+        _lib.check(L.tm_sinkhorn_hist_solve(
+            dtype, int(log_space), _lib.ptr(num_a), _lib.ptr(num_b), _lib.ptr(M), _lib.ptr(M_t),
+            _lib.ptr(cost), _lib.ptr(cost_t), _lib.ptr(u), _lib.ptr(v), n, d, int(max_iter),
+            float(eps), float(threshold), _lib.ptr(f), _lib.ptr(g), _lib.ptr(value),
+            _lib.ctypes.byref(iters), _lib.ctypes.byref(fb), _lib.ptr(ws), nbytes,
+            _lib.stream_of(dev)), "tm_sinkhorn_hist_solve")
+    cast = (lambda t: None if t is None else t.to(out_dtype))  # noqa: E731
+    return _HistSolution(log_space, eps, u, v, cost, iters.value, cast(f), cast(g), cast(value))
+
+
+def _hist_solve(a, b, cost, eps, max_iter, threshold, log_space, want=()) -> _HistSolution:
+    solver = _gpu_solve if a.is_cuda else _cpu_solve
+    return solver(a, b, cost, float(eps), int(max_iter), float(threshold), bool(log_space),
+                  tuple(want))
+
+
+def _hist_plan(sol: _HistSolution, out_dtype) -> Tensor:
+    """(n, d, d) plan exp(log u_i - C_ij / eps + log v_j) of a solution
+    (reference optimal_transport.py:228-233)."""
+    n, d = sol.u.shape
+    if not sol.u.is_cuda:
+        lu, lv = sol.log_scalings()
+        return torch.exp(lu[:, :, None] - sol.cost / sol.eps + lv[:, None, :]).to(out_dtype)
+    plan = sol.u.new_empty((n, d, d))
+    if n:
+        dtype = _lib.TM_F32 if sol.u.dtype == torch.float32 else _lib.TM_F64
+        with torch.cuda.device(sol.u.device):
+            _lib.check(_lib.lib().tm_sinkhorn_hist_plan(
+                dtype, int(sol.log_space), _lib.ptr(sol.u), _lib.ptr(sol.v), _lib.ptr(sol.cost),
+                n, d, float(sol.eps), _lib.ptr(plan), _lib.stream_of(sol.u.device)),
+                "tm_sinkhorn_hist_plan")
+    return plan.to(out_dtype)
 
 
 class _SinkhornDistance(torch.autograd.Function):
-    """<P, C> with the centred dual potentials as gradients w.r.t. the
-    marginals (reference optimal_transport.py:303-325)."""
+    """<P, C> per batch row; its gradients w.r.t. the two marginals are the
+    centred dual potentials (envelope theorem, reference
+    optimal_transport.py:128-150), the cost gets none."""
 
     @staticmethod
     def forward(ctx, a, b, cost_matrix, epsilon, max_iter, threshold, log_space):
-        log_u, log_v = _iterate(a, b, cost_matrix, epsilon, max_iter, threshold, log_space)
-        plan = torch.exp(log_u.unsqueeze(-1) - cost_matrix / epsilon + log_v.unsqueeze(-2))
-        distance = (plan * cost_matrix).sum(dim=(-2, -1))
-        f = epsilon * log_u
-        g = epsilon * log_v
-        ctx.save_for_backward(f - f.mean(dim=-1, keepdim=True), g - g.mean(dim=-1, keepdim=True))
-        return distance
+        sol = _hist_solve(a, b, cost_matrix, epsilon, max_iter, threshold, log_space,
+                          want=("value", "potentials"))
+        ctx.save_for_backward(sol.f, sol.g)
+        return sol.value
 
     @staticmethod
     def backward(ctx, grad_output):
         f, g = ctx.saved_tensors
-        grad = grad_output.unsqueeze(-1)
-        return grad * f, grad * g, None, None, None, None, None
+        scale = grad_output[:, None]
+        return scale * f, scale * g, None, None, None, None, None
+
+
+def _unit_mass(h: Tensor) -> Tensor:
+    h = h.clamp(min=0)
+    return h / h.sum(dim=-1, keepdim=True).clamp(min=1e-12)
 
 
 class SinkhornSolver(nn.Module):
     """Entropy-regularised balanced OT between (n, d) histograms sharing one
-    (d, d) cost (reference optimal_transport.py:328-424)."""
+    (d, d) cost (reference optimal_transport.py:153-249): forward returns the
+    (n,) values <P, C>, differentiable w.r.t. both histograms."""
 
     def __init__(self, epsilon=1.0, max_iter=100, threshold=0.0, p=2, log_space=False):
         super().__init__()
@@ -154,45 +240,44 @@ class SinkhornSolver(nn.Module):
                 f"threshold={self.threshold}, log_space={self.log_space}")
 
     def data_preprocess(self, source, target, cost_matrix: Optional[Tensor] = None):
+        """Checks the (n, d) pair, resolves the cost (default: the d bins as
+        points on a line) and scales each row to unit mass after clamping
+        negative entries (differentiable)."""
         if source.ndim != 2 or source.shape != target.shape:
             raise ValueError("source and target must be (n, d) tensors of the same shape.")
         if not source.is_floating_point():
             raise ValueError("source and target must be floating-point tensors.")
         d = source.shape[1]
-        if cost_matrix is None:
-            cost_matrix = build_cost_matrix((d,), self.p, source.device)
-        elif cost_matrix.shape != (d, d):
+        if cost_matrix is not None and cost_matrix.shape != (d, d):
             raise ValueError(
                 f"cost_matrix must have shape ({d}, {d}), got {tuple(cost_matrix.shape)}.")
-        cost_matrix = cost_matrix.to(device=source.device, dtype=source.dtype).contiguous()
-        source = source.clamp(min=0)
-        target = target.clamp(min=0)
-        source = source / source.sum(dim=-1, keepdim=True).clamp(min=1e-12)
-        target = target / target.sum(dim=-1, keepdim=True).clamp(min=1e-12)
-        return source, target, cost_matrix
+        cost = build_cost_matrix((d,), self.p, source.device) if cost_matrix is None \
+            else cost_matrix
+        cost = cost.to(device=source.device, dtype=source.dtype).contiguous()
+        return _unit_mass(source), _unit_mass(target), cost
+
+    def _solve(self, source, target, cost_matrix, want):
+        a, b, cost = self.data_preprocess(source, target, cost_matrix)
+        return _hist_solve(a, b, cost, self.epsilon, self.max_iter, self.threshold,
+                           self.log_space, want)
 
     def forward(self, source: Tensor, target: Tensor, cost_matrix: Optional[Tensor] = None):
-        source, target, cost_matrix = self.data_preprocess(source, target, cost_matrix)
-        return _SinkhornDistance.apply(source, target, cost_matrix, self.epsilon, self.max_iter,
-                                       self.threshold, self.log_space)
+        """(n,) transport values between the rows of source and target."""
+        a, b, cost = self.data_preprocess(source, target, cost_matrix)
+        return _SinkhornDistance.apply(a, b, cost, self.epsilon, self.max_iter, self.threshold,
+                                       self.log_space)
 
     @torch.no_grad()
     def plan(self, source: Tensor, target: Tensor, cost_matrix: Optional[Tensor] = None):
-        source, target, cost_matrix = self.data_preprocess(source, target, cost_matrix)
-        log_u, log_v = self._iterate(source, target, cost_matrix)
-        return torch.exp(log_u.unsqueeze(-1) - cost_matrix / self.epsilon + log_v.unsqueeze(-2))
+        """(n, d, d) transport plans, formed from the final log-scalings."""
+        return _hist_plan(self._solve(source, target, cost_matrix, ()), source.dtype)
 
     @torch.no_grad()
     def potentials(self, source: Tensor, target: Tensor, cost_matrix: Optional[Tensor] = None):
-        source, target, cost_matrix = self.data_preprocess(source, target, cost_matrix)
-        log_u, log_v = self._iterate(source, target, cost_matrix)
-        f = self.epsilon * log_u
-        g = self.epsilon * log_v
-        return f - f.mean(dim=-1, keepdim=True), g - g.mean(dim=-1, keepdim=True)
-
-    def _iterate(self, a, b, cost_matrix):
-        return _iterate(a, b, cost_matrix, self.epsilon, self.max_iter, self.threshold,
-                        self.log_space)
+        """Centred dual potentials (f, g), each (n, d): the gradients of the
+        value w.r.t. the marginals."""
+        sol = self._solve(source, target, cost_matrix, ("potentials",))
+        return sol.f, sol.g
 
 
 # ---------------------------------------------------------------- point clouds
