@@ -307,6 +307,23 @@ static int64_t map_coord(int64_t c, int64_t size, int mode, int* use_cval) {
 
 /* ----------------------------------------------------------------- grey -- */
 
+/* min / max as the GPU's v_min_f32 / v_max_f32 evaluate them (measured on
+ * MI355X): a NaN operand yields the other operand, and -0 orders below +0
+ * whichever operand comes first, so a window holding both zeros erodes to -0
+ * and dilates to +0 in any scan order. */
+static inline float tmo_fmax(float a, float b) {
+  if (a != a) return b;
+  if (b != b) return a;
+  if (a == b) return signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+static inline float tmo_fmin(float a, float b) {
+  if (a != a) return b;
+  if (b != b) return a;
+  if (a == b) return signbit(a) ? a : b;
+  return a < b ? a : b;
+}
+
 int tmo_grey(const float* in, int ndim, const int64_t* shape, const float* structure,
              const uint8_t* footprint, const int64_t* se_shape, const int64_t* origin, int mode,
              float cval, int dilation, float* out) {
@@ -340,8 +357,8 @@ int tmo_grey(const float* in, int ndim, const int64_t* shape, const float* struc
                 flat = flat * shape[j] + m;
             }
             float v = uc ? cval : in[flat];
-            if (dilation) acc = fmaxf(acc, v + sv[k]);
-            else acc = fminf(acc, v - sv[k]);
+            if (dilation) acc = tmo_fmax(acc, v + sv[k]);
+            else acc = tmo_fmin(acc, v - sv[k]);
         }
         out[p] = acc;
         for (int j = ndim - 1; j >= 0; --j) {

@@ -410,3 +410,19 @@ def test_bfdt_oracle_equals_edt_oracle_distances(oracle_mod, seed):
     bf, _ = oracle_mod.bfdt(mask[None, None])
     edt, _ = oracle_mod.edt(mask[None, None])
     assert np.array_equal(bf, edt)
+
+
+def test_grey_signed_zero_order_independent(oracle_mod):
+    """A window holding both zeros erodes to -0 and dilates to +0 whatever the
+    scan order (the GPU's min / max instructions order -0 below +0); a plain
+    `a < b ? a : b` scan would depend on which zero comes first."""
+    pos, neg = np.float32(0.0), np.float32(-0.0)
+    for row in ([pos, neg, pos, pos, neg, pos, pos], [neg, pos, pos, neg, neg, pos, neg]):
+        x = np.array(row, np.float32)[None, None]
+        ero = oracle_mod.grey(x, None, None, (3,), (0,), "reflect", 0.0, False)[0, 0]
+        dil = oracle_mod.grey(x, None, None, (3,), (0,), "reflect", 0.0, True)[0, 0]
+        padded = np.concatenate([x[0, 0, :1], x[0, 0], x[0, 0, -1:]])  # reflect: edge repeated
+        for i in range(x.shape[-1]):
+            window = padded[i:i + 3]
+            assert ero[i] == 0 and np.signbit(ero[i]) == bool(np.signbit(window).any()), (row, i)
+            assert dil[i] == 0 and not np.signbit(dil[i]), (row, i)
