@@ -75,6 +75,22 @@ int tm_edt(const void* input, int input_dtype, int64_t batch, int ndim, const in
            const double* sampling, float* distances, void* indices, int index_dtype,
            void* workspace, size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference (its edt_cuda allocates its scratch with
+ * torch on every call).  For callers that cannot allocate -- code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc --
+ * this runs tm_edt with its workspace in a file-scope device array of
+ * 32 MiB (each GPU the library runs on holds one until the process ends),
+ * looked up once per device.  Same arguments and results as tm_edt without
+ * the workspace; a problem whose tm_edt_workspace_bytes exceeds 32 MiB is
+ * rejected with TM_ERR_ARG (the 2-D unit-sampling path needs 2 bytes per
+ * voxel, the N-D path about 4 (2 ndim - 1) + 8 bytes per voxel plus its
+ * stacks).  No hipMalloc, hipFree or other allocating call; the autotuner
+ * times nothing here (cached or default launch variant).  Every call on one
+ * GPU shares the array: calls that overlap on one GPU race. */
+int tm_edt_noalloc(const void* input, int input_dtype, int64_t batch, int ndim,
+                   const int64_t* shape, const double* sampling, float* distances,
+                   void* indices, int index_dtype, void* stream);
+
 /* ---------------------------------------------------------------- CDT --- */
 
 /* Workspace bytes tm_cdt needs (int32 distance + int32 feature per voxel). */

@@ -36,6 +36,7 @@ SIGNATURES: dict[str, tuple] = {
     "tm_autotune_import": (_I, [ctypes.c_char_p]),
     "tm_edt_workspace_bytes": (_SZ, [_I64, _I, _I64P, _DP]),
     "tm_edt": (_I, [_P, _I, _I64, _I, _I64P, _DP, _P, _P, _I, _P, _SZ, _P]),
+    "tm_edt_noalloc": (_I, [_P, _I, _I64, _I, _I64P, _DP, _P, _P, _I, _P]),
     "tm_cdt_workspace_bytes": (_SZ, [_I64, _I, _I64P]),
     "tm_cdt": (_I, [_P, _I, _I64, _I, _I64P, _I, _P, _P, _I, _P, _SZ, _P]),
     "tm_bfdt": (_I, [_P, _I, _I64, _I, _I64P, _DP, _I, _P, _P, _I, _P]),

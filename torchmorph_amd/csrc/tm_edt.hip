@@ -26,6 +26,8 @@
 //
 // Build with -ffp-contract=off: every double multiply and add must round
 // separately, as SciPy's C code does.
+#include <mutex>
+
 #include "tm_dt_chunk.h"
 #include "tm_dt_fast.h"
 #include "tm_dt_lines.h"
@@ -651,4 +653,66 @@ extern "C" int tm_edt(const void* input, int input_dtype, int64_t batch, int ndi
                         workspace_bytes, st);
   set_error("tm_edt: input dtype must be uint8/bool or float32");
   return TM_ERR_ARG;
+}
+
+// ------------------------------------------------------------- no-alloc ---
+// tm_edt_noalloc runs tm_edt with its workspace in a file-scope device array
+// instead of caller memory, for callers that cannot allocate (code under
+// stream capture, tight serving loops).  Each GPU the library runs on holds
+// its own copy of the array until the process ends; every call on a GPU
+// shares it, so calls that overlap on one GPU race.  No kernel of tm_edt
+// reads workspace it has not written in the same call, so nothing is
+// cleared between calls.
+
+namespace tmh {
+
+constexpr size_t kEdtScratchBytes = (size_t)32 << 20;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kEdtScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+// The device address of `scratch` on the current device, looked up once per
+// device (host code must never pass the array's name to a kernel: on the
+// host it is only a stand-in for the symbol).
+static int edt_scratch(void** out) {
+  int dev = 0;
+  TM_CHECK_HIP(hipGetDevice(&dev));
+  TM_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "tm_edt_noalloc: device %d beyond %d", dev,
+               kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    TM_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TM_CHECK_ARG(p != nullptr, "tm_edt_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TM_OK;
+}
+
+}  // namespace tmh
+
+extern "C" int tm_edt_noalloc(const void* input, int input_dtype, int64_t batch, int ndim,
+                              const int64_t* shape, const double* sampling, float* distances,
+                              void* indices, int index_dtype, void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(ndim >= 1 && ndim <= kMaxNdim, "tm_edt_noalloc: ndim must be in 1..8, got %d",
+               ndim);
+  TM_CHECK_ARG(batch >= 1 && shape, "tm_edt_noalloc: batch must be >= 1 and shape given");
+  const size_t need = tm_edt_workspace_bytes(batch, ndim, shape, sampling);
+  TM_CHECK_ARG(need <= kEdtScratchBytes,
+               "tm_edt_noalloc: this problem needs %zu bytes of scratch, above the %zu-byte "
+               "maximum; use tm_edt with a workspace",
+               need, kEdtScratchBytes);
+  void* ws = nullptr;
+  if (need) {
+    const int s = edt_scratch(&ws);
+    if (s != TM_OK) return s;
+  }
+  TuneSuspend no_timing;  // cached or default launch variant: no events, no extra launches
+  return tm_edt(input, input_dtype, batch, ndim, shape, sampling, distances, indices,
+                index_dtype, ws, need, stream);
 }

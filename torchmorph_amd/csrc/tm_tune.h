@@ -15,4 +15,13 @@ namespace tmh {
 int tune_pick(const char* op, const int64_t* key, int nkey, int n, int dflt,
               const std::function<int(int)>& run, hipStream_t st, bool* ran);
 
+// While one of these lives on a host thread, tune_pick on that thread times
+// nothing: it returns the cached variant of the class, or `dflt`.  For entry
+// points that must not create HIP objects (the timing events) or run extra
+// launches.
+struct TuneSuspend {
+  TuneSuspend();
+  ~TuneSuspend();
+};
+
 }  // namespace tmh

@@ -22,6 +22,10 @@ static std::map<std::string, int>& cache() {
   return c;
 }
 static int g_enabled = -1;
+static thread_local int t_suspended = 0;
+
+TuneSuspend::TuneSuspend() { ++t_suspended; }
+TuneSuspend::~TuneSuspend() { --t_suspended; }
 
 static bool tuning_enabled() {
   if (g_enabled < 0) {
@@ -48,8 +52,8 @@ int tune_pick(const char* op, const int64_t* key, int nkey, int n, int dflt,
     if (it != cache().end()) return it->second;
   }
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone ||
-      !tuning_enabled() || n <= 1)
+  if (t_suspended || hipStreamIsCapturing(st, &cs) != hipSuccess ||
+      cs != hipStreamCaptureStatusNone || !tuning_enabled() || n <= 1)
     return dflt;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return dflt;
