@@ -3,10 +3,12 @@
 Headline (``value``): exact EDT throughput in Gvox/s on the plain EDT workload
 -- bool masks B=8, C=1, 256x256, foreground p=0.9 (seeded), unit sampling,
 float32 distances + int64 indices written in place -- bit-exact against the
-CPU oracle.  One step = one call of euclidean_distance_transform on one such
-batch.  Inputs and outputs rotate over enough distinct buffer sets to exceed
-the 256 MB Infinity Cache, so every step reads and writes HBM.  The timed
-steps run as replays of a HIP graph holding one pass over all buffer sets.
+CPU oracle.  One step = one replay of a HIP graph holding one call of
+euclidean_distance_transform per buffer set: 48 batches over 48 distinct
+buffer sets (528 MB, above the 256 MB Infinity Cache), so every call reads and
+writes HBM.  ``value`` counts the voxels of every call of the timed steps.
+The per-call time of eager Python calls is reported beside it
+(``eager_call``), never as ``value``.
 
 Printed: one JSON line (contract of the build driver), with ``roofline`` for
 the dominant kernel (algorithmic bytes per launch / event-timed launch
@@ -38,16 +40,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 C1 = dict(B=8, C=1, H=256, W=256, p=0.9)
 BYTES_PER_VOX = 1 + 4 + 16  # bool mask in, f32 distance + 2 x int64 index out
-# HBM bytes per step of the headline launch variant, measured with the FETCH_SIZE and
-# WRITE_SIZE counters and calibrated against a plain copy (summary in profiles/)
-TRAFFIC_PER_STEP = 13_260_000
+HEADLINE_KEY = "edt2d 256 256 64 1"
+# HBM bytes per call of each profiled headline launch variant: FETCH_SIZE x 2.00 +
+# WRITE_SIZE x 1.00 (factors of a plain 256 MB copy), summed over the call's kernels
+# (profiles/edt2d_b8_256x256_hbm_counters.md)
+TRAFFIC_PER_CALL = {0: 13_260_000, 8: 13_397_504}
+HEADLINE_KERNELS = {v: "edt2d_cols_kernel + edt2d_rows_kernel (one call)" for v in range(8)}
+HEADLINE_KERNELS.update({v: "edt2d_cols_kernel + edt2d_rows4_kernel (one call)"
+                         for v in (8, 9, 10, 11, 12)})
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=480)
-    ap.add_argument("--warmup", type=int, default=48)
+    ap.add_argument("--steps", type=int, default=20, help="graph replays timed")
+    ap.add_argument("--warmup", type=int, default=5, help="graph replays untimed")
     ap.add_argument("--sets", type=int, default=48, help="distinct buffer sets (>=256 MB total)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -157,11 +164,26 @@ def main():
     parity = (torch.equal(dists[0].cpu(), torch.from_numpy(od.astype(np.float32)))
               and np.array_equal(idxs[0].cpu().numpy(), oi))
 
-    for k in range(args.warmup):
+    # eager calls through the Python API (host path included), HIP events
+    # around 2 passes over the buffer sets: reported beside the headline, never
+    # as `value`
+    for k in range(S):
         step(k)
+    stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t_e = time.perf_counter()
+    e0.record(stream)
+    for k in range(2 * S):
+        step(k)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    eager_wall_us = (time.perf_counter() - t_e) * 1e6 / (2 * S)
+    eager_event_us = e0.elapsed_time(e1) * 1e3 / (2 * S)
 
-    # one HIP graph = one pass over all buffer sets
+    # one step = one replay of a HIP graph holding one call per buffer set:
+    # S batches over S distinct buffer sets (> the 256 MB Infinity Cache)
     graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
@@ -172,32 +194,34 @@ def main():
     with torch.cuda.graph(graph):
         for k in range(S):
             step(k)
-    graph.replay()
+    for _ in range(max(1, args.warmup)):
+        graph.replay()
     torch.cuda.synchronize()
+    # the graph's calls wrote the last buffer set: recheck its parity
+    od, oi = oracle.edt(masks[S - 1].cpu().numpy())
+    parity = parity and (torch.equal(dists[S - 1].cpu(), torch.from_numpy(od.astype(np.float32)))
+                         and np.array_equal(idxs[S - 1].cpu().numpy(), oi))
 
     K = args.steps
-    reps, rem = divmod(K, S)
-    stream = torch.cuda.current_stream()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
     if args.no_graph:
-        for k in range(K):
+        for k in range(K * S):
             step(k)
     else:
-        for _ in range(reps):
+        for _ in range(K):
             graph.replay()
-        for k in range(rem):
-            step(k)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_time(ev1)
-    per_launch_s = gpu_ms / 1e3 / K  # one kernel launch per step on this path
-    value = K * nvox / wall / 1e9
-    achieved = nvox * BYTES_PER_VOX / per_launch_s / 1e9
+    calls = K * S
+    per_call_s = gpu_ms / 1e3 / calls
+    value = calls * nvox / wall / 1e9
+    achieved = nvox * BYTES_PER_VOX / per_call_s / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -233,17 +257,26 @@ def main():
         "config": {
             "workload": "plain exact EDT: bool masks B=8 C=1 256x256 p_fg=0.9, unit sampling, "
                         "f32 distances + int64 indices in place",
-            "batch": B * C, "shape": [H, W], "buffer_sets": S,
-            "rotating_footprint_MB": round(footprint_mb, 1), "timing": "eager launches" if args.no_graph else "HIP graph replays",
+            "batch": B * C, "shape": [H, W], "calls_per_step": S,
+            "voxels_per_step": S * nvox, "buffer_sets": S,
+            "rotating_footprint_MB": round(footprint_mb, 1),
+            "timing": (f"eager: {S} Python calls per step" if args.no_graph else
+                       f"one step = one replay of a HIP graph of {S} calls, one per distinct "
+                       f"buffer set"),
         },
         "parity": "bit-exact vs oracle" if parity else "MISMATCH",
+        "eager_call": {"us_per_call_wall": round(eager_wall_us, 2),
+                       "us_per_call_events": round(eager_event_us, 2),
+                       "note": "euclidean_distance_transform called from Python, host path "
+                               "included; not the headline"},
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             # the counter figure belongs to the profiled launch variant
-            "traffic": TRAFFIC_PER_STEP if tuned.get("edt2d 256 256 64 1") == 0 else None,
-            "kernel": "edt2d_cols_kernel + edt2d_rows_kernel (one step)", "bytes_per_launch": nvox * BYTES_PER_VOX,
-            "event_us_per_step": round(per_launch_s * 1e6, 3),
+            "traffic": TRAFFIC_PER_CALL.get(tuned.get(HEADLINE_KEY)),
+            "kernel": HEADLINE_KERNELS.get(tuned.get(HEADLINE_KEY), "edt2d (two launches)"),
+            "bytes_per_launch": nvox * BYTES_PER_VOX,
+            "event_us_per_call": round(per_call_s * 1e6, 3),
             "autotune": tuned,
         },
         "cpu_baseline": cpu,

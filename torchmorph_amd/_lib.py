@@ -152,6 +152,30 @@ def workspace(nbytes: int, device: torch.device) -> torch.Tensor | None:
     return torch.empty(int(nbytes), dtype=torch.uint8, device=device)
 
 
+# Per-(device, stream) workspaces of the hot entry points.  Calls on one
+# stream run in order, so they can share one buffer; a call on another stream
+# gets its own.  A buffer is never freed once handed out (a captured graph may
+# still reference it): growth keeps the old one and allocates double.
+_STREAM_WS: dict = {}
+_STREAM_WS_KEEP: list = []
+
+
+def stream_workspace(nbytes: int, device: torch.device, stream) -> torch.Tensor | None:
+    """A workspace of at least `nbytes` reused across calls on `stream`
+    (a torch.cuda.Stream)."""
+    if nbytes <= 0:
+        return None
+    key = (device.index, stream.cuda_stream)
+    ws = _STREAM_WS.get(key)
+    if ws is None or ws.numel() < nbytes:
+        if ws is not None:
+            _STREAM_WS_KEEP.append(ws)
+        size = max(int(nbytes), 2 * (ws.numel() if ws is not None else 0))
+        ws = torch.empty(size, dtype=torch.uint8, device=device)
+        _STREAM_WS[key] = ws
+    return ws
+
+
 def mask_operand(x: torch.Tensor) -> tuple[torch.Tensor, int]:
     """Contiguous device view of a mask for the kernels: bool/uint8 are read
     as bytes, float32 directly; any other dtype is reduced to ``x != 0``."""

@@ -26,6 +26,7 @@
 //
 // Build with -ffp-contract=off: every double multiply and add must round
 // separately, as SciPy's C code does.
+#include <algorithm>
 #include <mutex>
 
 #include "tm_dt_chunk.h"
@@ -84,24 +85,27 @@ __device__ __forceinline__ uint32_t word_bits(uint32_t w) {
 // CPT adjacent columns per thread (1 or 4), 16 column groups x 16 row
 // segments per workgroup, NCH chunks of 16 rows per segment.  VEC: 4-byte
 // aligned rows of a byte mask (W % 4 == 0).
-template <typename TIn, int NCH, int CPT, bool VEC>
+// CG column groups x (256 / CG) row segments per workgroup: fewer columns
+// per workgroup give more workgroups (CG = 16: 16 columns, 16 segments).
+template <typename TIn, int NCH, int CPT, bool VEC, int CG = 16>
 __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__ in, int H, int W,
                                                          int16_t* __restrict__ f0, int aligned) {
+  constexpr int SEGS = 256 / CG;
   // raw words: one element per row (CPT = 1) or four mask bytes (CPT = 4,
   // VEC), loaded unconditionally so that all of them are in flight at once
   constexpr bool RAWW = CPT == 1 || (VEC && sizeof(TIn) == 1);
-  constexpr int WGC = 16 * CPT;  // columns per workgroup
-  __shared__ int16_t sfirst[kColSegs][WGC];
-  __shared__ int16_t slast[kColSegs][WGC];
-  const int cg = threadIdx.x & 15;
-  const int seg = threadIdx.x >> 4;
+  constexpr int WGC = CG * CPT;  // columns per workgroup
+  __shared__ int16_t sfirst[SEGS][WGC];
+  __shared__ int16_t slast[SEGS][WGC];
+  const int cg = threadIdx.x & (CG - 1);
+  const int seg = threadIdx.x / CG;
   // 1-D grid of (image, column block); the column blocks of one image stay on
   // one XCD (they share the image's cache lines)
   const int ncb = (W + WGC - 1) / WGC;
   const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
   const int img = lb / ncb;
   const int xb = (lb - img * ncb) * WGC + cg * CPT;
-  const int seglen = (H + kColSegs - 1) / kColSegs;
+  const int seglen = (H + SEGS - 1) / SEGS;
   const int ys = seg * seglen;
   const int ye = min(H, ys + seglen);
   const TIn* im = in + (int64_t)img * H * W;
@@ -155,14 +159,14 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
     slast[seg][cg * CPT + c] = (int16_t)last;
   }
   __syncthreads();
-  if (!live) return;
+  if (live) {
   int before[CPT], after[CPT];
 #pragma unroll
   for (int c = 0; c < CPT; ++c) {
     before[c] = -1;
     after[c] = -1;
     for (int s = seg - 1; s >= 0 && before[c] < 0; --s) before[c] = slast[s][cg * CPT + c];
-    for (int s = seg + 1; s < kColSegs && after[c] < 0; ++s) after[c] = sfirst[s][cg * CPT + c];
+    for (int s = seg + 1; s < SEGS && after[c] < 0; ++s) after[c] = sfirst[s][cg * CPT + c];
   }
 #pragma unroll
   for (int k = 0; k < NCH; ++k) {
@@ -213,6 +217,7 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
       }
     }
   }
+  }  // live
 }
 
 // Row pass of one band of `rows` rows starting at y0: squared column
@@ -227,20 +232,32 @@ __device__ __forceinline__ void edt2d_band_rows(const int32_t* hs, const int16_t
     int best = hr[x];
     int bp = x;
     const int reach = max(x, W - 1 - x);
-    for (int k = 1; k <= reach; ++k) {
-      const int kk = k * k;
-      if (best >= 0 && kk > best) break;
-      if (x - k >= 0) {
-        const int hv = hr[x - k];
-        if (hv >= 0 && (best < 0 || hv + kk <= best)) {
-          best = hv + kk;
+    // Candidates in the order x, x-1, x+1, x-2, x+2, ...: a left candidate
+    // replaces an equal best and a right one does not, so the lowest column
+    // attaining the minimum wins.  Steps are taken 8 at a time: the 16 LDS
+    // reads of a block are issued together, then applied in that order.  A
+    // step k with k^2 > best can neither beat nor tie best (hv >= 0), so the
+    // steps of a block past that point change nothing, and the next block is
+    // read only while (k0 + 1)^2 <= best.
+    for (int k0 = 0; k0 < reach; k0 += 8) {
+      if (best >= 0 && (k0 + 1) * (k0 + 1) > best) break;
+      int hl[8], hv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + 1 + u;
+        hl[u] = hr[max(x - k, 0)];
+        hv[u] = hr[min(x + k, W - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + 1 + u;
+        const int kk = k * k;
+        if (x - k >= 0 && hl[u] >= 0 && (best < 0 || hl[u] + kk <= best)) {
+          best = hl[u] + kk;
           bp = x - k;
         }
-      }
-      if (x + k < W) {
-        const int hv = hr[x + k];
-        if (hv >= 0 && (best < 0 || hv + kk < best)) {
-          best = hv + kk;
+        if (x + k < W && hv[u] >= 0 && (best < 0 || hv[u] + kk < best)) {
+          best = hv[u] + kk;
           bp = x + k;
         }
       }
@@ -319,6 +336,145 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
   }
   __syncthreads();
   edt2d_band_rows(hs, fr, npx, img, H, W, y0, o);
+}
+
+// Axis-1 pass with four adjacent pixels per lane (W % 4 == 0, W <= 4096).
+// One workgroup per band of R rows.  The band's squared column distances sit
+// in LDS with 8 entries of -1 (no candidate) on both sides of every row, so a
+// lane reads the 20 entries around its 4 pixels as five 16-byte LDS reads and
+// runs the first 8 window steps of all four pixels on registers, with no
+// bounds checks; a pixel whose best still admits a step beyond 8 (none
+// found, or best >= 81) continues with the one-pixel blocked loop.  The
+// candidate order and tie rule are edt2d_band_rows's, so the results are the
+// same bit for bit.  Outputs leave as 16-byte stores: four float32
+// distances, and per index plane four int32 or two pairs of int64.
+template <int R>
+__global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restrict__ f0, int H,
+                                                          int W, EdtOut o) {
+  extern __shared__ int32_t lds[];
+  const int P = W + 16;                 // padded row pitch of hs
+  int32_t* hs = lds;                    // [R][W + 16], entry x of row r at r * P + 8 + x
+  int16_t* fr = (int16_t*)(lds + R * P);  // [R][W]
+  const int img = blockIdx.y;
+  const int y0 = blockIdx.x * R;
+  const int rows = min(R, H - y0);
+  const int npx = rows * W;
+  const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
+  for (int i = threadIdx.x; i < rows * 16; i += blockDim.x) {
+    const int r = i >> 4, k = i & 15;
+    hs[r * P + (k < 8 ? k : W + k)] = -1;
+  }
+  for (int p0 = threadIdx.x * 8; p0 < npx; p0 += blockDim.x * 8) {
+    const uint4 q = *(const uint4*)(src + p0);  // 8 features, W % 8 == 0 keeps rows whole
+    const int r = p0 / W;
+    const int x = p0 - r * W;
+    const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int f = (int16_t)((w4[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+      const int dy = y0 + r - f;
+      fr[r * W + x + i] = (int16_t)f;
+      hs[r * P + 8 + x + i] = f >= 0 ? dy * dy : -1;
+    }
+  }
+  __syncthreads();
+  for (int qd = threadIdx.x; qd * 4 < npx; qd += blockDim.x) {
+    const int p0 = qd * 4;
+    const int r = p0 / W;
+    const int x0 = p0 - r * W;
+    const int32_t* hr = hs + r * P + 8;  // hr[-8 .. W + 7] readable
+    int h[20];                            // hr[x0 - 8 .. x0 + 11]
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const int4 t = *(const int4*)(hr + x0 - 8 + 4 * v);
+      h[4 * v] = t.x;
+      h[4 * v + 1] = t.y;
+      h[4 * v + 2] = t.z;
+      h[4 * v + 3] = t.w;
+    }
+    int best[4], bp[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      best[i] = h[8 + i];
+      bp[i] = x0 + i;
+#pragma unroll
+      for (int k = 1; k <= 8; ++k) {
+        const int kk = k * k;
+        const int hl = h[8 + i - k], hv = h[8 + i + k];
+        if (hl >= 0 && (best[i] < 0 || hl + kk <= best[i])) {
+          best[i] = hl + kk;
+          bp[i] = x0 + i - k;
+        }
+        if (hv >= 0 && (best[i] < 0 || hv + kk < best[i])) {
+          best[i] = hv + kk;
+          bp[i] = x0 + i + k;
+        }
+      }
+      const int x = x0 + i;
+      const int reach = max(x, W - 1 - x);
+      if (best[i] < 0 || best[i] >= 81) {
+        // steps 9, 10, ...: the blocked loop of edt2d_band_rows
+        for (int k0 = 8; k0 < reach; k0 += 8) {
+          if (best[i] >= 0 && (k0 + 1) * (k0 + 1) > best[i]) break;
+          int hl[8], hv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int k = k0 + 1 + u;
+            hl[u] = hr[max(x - k, 0)];
+            hv[u] = hr[min(x + k, W - 1)];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int k = k0 + 1 + u;
+            const int kk = k * k;
+            if (x - k >= 0 && hl[u] >= 0 && (best[i] < 0 || hl[u] + kk <= best[i])) {
+              best[i] = hl[u] + kk;
+              bp[i] = x - k;
+            }
+            if (x + k < W && hv[u] >= 0 && (best[i] < 0 || hv[u] + kk < best[i])) {
+              best[i] = hv[u] + kk;
+              bp[i] = x + k;
+            }
+          }
+        }
+      }
+    }
+    const int y = y0 + r;
+    const int64_t g = ((int64_t)img * H + y) * W + x0;
+    float dist[4];
+    int32_t fy[4], fx[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double acc;
+      if (best[i] >= 0) {
+        fy[i] = fr[r * W + bp[i]];
+        fx[i] = bp[i];
+        acc = (double)best[i];  // exact integer: same as the double delta/square/sum
+      } else {
+        fy[i] = -1;  // no background in the whole image
+        fx[i] = 0;
+        const double a = (double)(-1 - y), b = (double)(-(x0 + i));
+        acc = a * a + b * b;
+      }
+      dist[i] = (float)sqrt(acc);
+    }
+    if (o.dist) *(float4*)(o.dist + g) = make_float4(dist[0], dist[1], dist[2], dist[3]);
+    if (o.idx) {
+      if (o.idx64) {
+        int64_t* ip = (int64_t*)o.idx;
+        longlong2* a = (longlong2*)(ip + g);
+        longlong2* b = (longlong2*)(ip + o.plane + g);
+        a[0] = make_longlong2(fy[0], fy[1]);
+        a[1] = make_longlong2(fy[2], fy[3]);
+        b[0] = make_longlong2(fx[0], fx[1]);
+        b[1] = make_longlong2(fx[2], fx[3]);
+      } else {
+        int32_t* ip = (int32_t*)o.idx;
+        *(int4*)(ip + g) = make_int4(fy[0], fy[1], fy[2], fy[3]);
+        *(int4*)(ip + o.plane + g) = make_int4(fx[0], fx[1], fx[2], fx[3]);
+      }
+    }
+  }
 }
 
 // --------------------------------------------------------------- N-D --
@@ -438,8 +594,60 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     const int seglen = (H + kColSegs - 1) / kColSegs;
     const int aligned = ((uintptr_t)in % 16) == 0;
     const bool vecw = sizeof(TIn) == 1 && aligned && W % 4 == 0;
-    // variant v: columns per thread = (v & 1) ? 4 : 1, rows per band = 1 << (v >> 1)
+    // variant v < 8: columns per thread = (v & 1) ? 4 : 1, rows per band =
+    // 1 << (v >> 1); v = 8, 9, 10: one column per thread and the four-pixel
+    // row pass (edt2d_rows4_kernel) with bands of 1, 2, 4 rows; v = 11, 12:
+    // bands of 1 row, 8 or 4 columns per column-pass workgroup (more
+    // workgroups for narrow images)
     auto run = [&](int v) -> int {
+      if (v >= 8) {
+        const int R = v <= 10 ? 1 << (v - 8) : 1;  // 1, 2, 4 rows per band
+        const int CG = v <= 10 ? 16 : (v == 11 ? 8 : 4);
+        const size_t lds = (size_t)R * (W + 16) * 4 + (size_t)R * W * 2;
+        auto a16 = [](const void* p) { return ((uintptr_t)p % 16) == 0; };
+        if (W % 8 != 0 || lds > 64 * 1024 || batch > 65535 || !a16(o.dist) || !a16(o.idx)) {
+          set_error("tm_edt: four-pixel row variant does not apply");
+          return TM_ERR_ARG;
+        }
+        const int64_t ncb = (W + CG - 1) / CG;
+        const dim3 cgrid((unsigned)(ncb * batch));
+        const int seg = (H + 256 / CG - 1) / (256 / CG);
+        if (seg > 4 * kChunkRows) {
+          set_error("tm_edt: column segments too long for this variant");
+          return TM_ERR_ARG;
+        }
+#define TM_COLS1(NCH, G)                                                                    \
+  if (vecw)                                                                                 \
+    hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, 1, true, G>), cgrid, dim3(256), 0, st, in, \
+                       H, W, f0, aligned);                                                  \
+  else                                                                                      \
+    hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, 1, false, G>), cgrid, dim3(256), 0, st, \
+                       in, H, W, f0, aligned)
+        if (CG == 16) {
+          if (seg <= 16) TM_COLS1(1, 16);
+          else if (seg <= 32) TM_COLS1(2, 16);
+          else TM_COLS1(4, 16);
+        } else if (CG == 8) {
+          if (seg <= 16) TM_COLS1(1, 8);
+          else if (seg <= 32) TM_COLS1(2, 8);
+          else TM_COLS1(4, 8);
+        } else {
+          if (seg <= 16) TM_COLS1(1, 4);
+          else if (seg <= 32) TM_COLS1(2, 4);
+          else TM_COLS1(4, 4);
+        }
+#undef TM_COLS1
+        TM_LAUNCH_CHECK();
+        const int threads = (int)std::min<int64_t>(256, ((int64_t)R * W / 4 + 63) / 64 * 64);
+        const dim3 rgrid((H + R - 1) / R, (unsigned)batch);
+        switch (R) {
+          case 1: hipLaunchKernelGGL((edt2d_rows4_kernel<1>), rgrid, dim3(threads), lds, st, f0, H, W, o); break;
+          case 2: hipLaunchKernelGGL((edt2d_rows4_kernel<2>), rgrid, dim3(threads), lds, st, f0, H, W, o); break;
+          default: hipLaunchKernelGGL((edt2d_rows4_kernel<4>), rgrid, dim3(threads), lds, st, f0, H, W, o); break;
+        }
+        TM_LAUNCH_CHECK();
+        return TM_OK;
+      }
       const int cpt = (v & 1) ? 4 : 1;
       const int R = 1 << (v >> 1);
       if ((size_t)R * W * 6 > 96 * 1024) {
@@ -485,7 +693,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     const int64_t key[4] = {p2(H), p2(W), o.idx ? (o.idx64 ? 64 : 32) : 0, o.dist ? 1 : 0};
     const int dflt = W >= 512 ? 1 : 3;  // 4 columns/thread; 1 or 2 rows per band
     bool ran = false;
-    const int v = tune_pick("edt2d", key, 4, 8, dflt, run, st, &ran);
+    const int v = tune_pick("edt2d", key, 4, 13, dflt, run, st, &ran);
     if (ran) return TM_OK;
     const int rc = run(v);
     // a cached choice that does not apply here (imported from another

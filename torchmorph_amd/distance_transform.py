@@ -120,18 +120,54 @@ def euclidean_distance_transform(
     L = _lib.lib()
     x, in_dtype = _lib.mask_operand(input.detach())
     batch = input.shape[0] * input.shape[1]
-    shape = list(input.shape[2:])
+    shp, sp, nbytes = _edt_plan(batch, tuple(input.shape[2:]), tuple(samp))
     raw_d, raw_i, idx_dtype = _out_buffers(input, nd, return_distances, return_indices,
                                            distances, indices)
-    with torch.cuda.device(input.device):
+    dev = input.device
+    with _on_device(dev):
+        stream = torch.cuda.current_stream(dev)
+        ws = _lib.stream_workspace(nbytes, dev, stream)
+        _lib.check(L.tm_edt(x.data_ptr(), in_dtype, batch, nd, shp, sp,
+                            raw_d.data_ptr() if raw_d is not None else None,
+                            raw_i.data_ptr() if raw_i is not None else None, idx_dtype,
+                            ws.data_ptr() if ws is not None else None, nbytes,
+                            stream.cuda_stream), "tm_edt")
+    return _finish(raw_d, raw_i, return_distances, return_indices, distances, indices)
+
+
+# (batch, shape, sampling) -> (shape array, sampling array or None, workspace
+# bytes): the host-side constants of a call, built once per problem shape
+_EDT_PLANS: dict = {}
+
+
+def _edt_plan(batch: int, shape: tuple, samp: tuple):
+    key = (batch, shape, samp)
+    plan = _EDT_PLANS.get(key)
+    if plan is None:
         shp = _lib.i64_array(shape)
         sp = None if all(s == 1.0 for s in samp) else _lib.f64_array(samp)
-        nbytes = L.tm_edt_workspace_bytes(batch, nd, shp, sp)
-        ws = _lib.workspace(nbytes, input.device)
-        _lib.check(L.tm_edt(_lib.ptr(x), in_dtype, batch, nd, shp, sp, _lib.ptr(raw_d),
-                            _lib.ptr(raw_i), idx_dtype, _lib.ptr(ws), nbytes,
-                            _lib.stream_of(input.device)), "tm_edt")
-    return _finish(raw_d, raw_i, return_distances, return_indices, distances, indices)
+        plan = (shp, sp, _lib.lib().tm_edt_workspace_bytes(batch, len(shape), shp, sp))
+        if len(_EDT_PLANS) < 4096:
+            _EDT_PLANS[key] = plan
+    return plan
+
+
+class _Nothing:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+_NOTHING = _Nothing()
+
+
+def _on_device(dev: torch.device):
+    """torch.cuda.device(dev) unless dev is the current device already."""
+    if dev.index is None or dev.index == torch.cuda.current_device():
+        return _NOTHING
+    return torch.cuda.device(dev)
 
 
 def chamfer_distance_transform(
