@@ -65,18 +65,41 @@ __device__ __forceinline__ float dotp(const float* a, const float* b) {
   return s;
 }
 
+// Convergence check fused into the g-update of iteration k (1-based): the
+// error of the state after iteration k-1 (f still in fold, g in the g
+// buffer) is the sum of the f-update's tile partials of the pair, in tile
+// order; every g-tile of the pair forms the same sum.  A converged pair (or
+// one at max_iter) updates no g: its first tile copies that state out, sets
+// its iteration count and error and deactivates it.
+struct SkCheck {
+  const int32_t* tile_first;  // f-tiles of pair p: [tile_first[p], tile_first[p + 1])
+  const float* fpart;         // f-update tile partials of the L1 row-marginal error
+  const int* iter_base;       // iterations before this chunk
+  int k_in_chunk;
+  int max_iter;
+  float tol;
+  const float* fold;          // f of the state the error belongs to
+  float* fout;
+  float* gout;
+  int32_t* iters;
+  float* err;
+  int* active;
+  int* n_active;
+};
+
 // One half-step for every tile of the work-list.
 //   rows/cols: concatenated points (N x D); lwc: log2 of the column weights;
 //   pc: column potentials; pin/pout: row potentials (old/new, may alias);
 //   lwr/wr: log2 / plain row weights (for the marginal error).
-// ROWS_ARE_X selects which offsets of SkPair index rows.
-template <int D, bool ROWS_ARE_X>
+// ROWS_ARE_X selects which offsets of SkPair index rows; CHECK (g-updates
+// of the iteration loop) runs the fused convergence check first.
+template <int D, bool ROWS_ARE_X, bool CHECK>
 __global__ __launch_bounds__(256) void sk_half_kernel(
     const float* __restrict__ rows, const float* __restrict__ cols,
     const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ pin,
     float* __restrict__ pout, const float* __restrict__ wr, const SkPair* __restrict__ pairs,
-    const SkTile* __restrict__ tiles, const int* __restrict__ active, float* __restrict__ errpart,
-    int maxc, const float* __restrict__ centre) {
+    const SkTile* __restrict__ tiles, const int* active, float* __restrict__ errpart,
+    int maxc, const float* __restrict__ centre, SkCheck ck) {
   // column j in LDS as one float4: (y_0, .., y_{D-1}, h_j) for D <= 3, so a
   // term is one broadcast LDS read, three FMAs (h_j as the addend), one
   // subtraction, one v_exp_f32 and one add; D = 4 keeps h_j in its own array
@@ -90,6 +113,38 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
   const SkTile tile = tiles[blockIdx.x];
   if (!active[tile.pair]) return;  // uniform per workgroup
   const SkPair pr = pairs[tile.pair];
+  if constexpr (CHECK) {
+    __shared__ int sdone;
+    __shared__ float serr;
+    __shared__ int sk;
+    if (threadIdx.x == 0) {
+      float e = 0.f;
+      for (int t = ck.tile_first[tile.pair]; t < ck.tile_first[tile.pair + 1]; ++t)
+        e += ck.fpart[t];
+      const int k = *ck.iter_base + ck.k_in_chunk;
+      serr = e;
+      sk = k;
+      sdone = (e <= ck.tol || k - 1 >= ck.max_iter) ? 1 : 0;
+    }
+    __syncthreads();
+    if (sdone) {
+      if (tile.r0 == 0) {
+        for (int i = threadIdx.x; i < pr.n; i += blockDim.x)
+          ck.fout[pr.xoff + i] = ck.fold[pr.xoff + i];
+        for (int j = threadIdx.x; j < pr.m; j += blockDim.x)
+          ck.gout[pr.yoff + j] = pin[pr.yoff + j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          ck.iters[tile.pair] = sk - 1;
+          ck.err[tile.pair] = serr;
+          ck.active[tile.pair] = 0;
+          atomicSub(ck.n_active, 1);
+        }
+      }
+      return;
+    }
+    if (tile.r0 == 0 && threadIdx.x == 0) ck.err[tile.pair] = serr;
+  }
   const int64_t roff = ROWS_ARE_X ? pr.xoff : pr.yoff;
   const int64_t coff = ROWS_ARE_X ? pr.yoff : pr.xoff;
   const int nr = ROWS_ARE_X ? pr.n : pr.m;
@@ -270,38 +325,6 @@ __global__ __launch_bounds__(256) void sk_grad_kernel(
   }
 }
 
-// Per-pair convergence check after an f-update of iteration k (1-based):
-// the error is that of the state after iteration k-1, whose f is still in
-// fold and whose g is in g.
-__global__ __launch_bounds__(256) void sk_check_kernel(
-    const SkPair* __restrict__ pairs, const int32_t* __restrict__ tile_first,
-    const float* __restrict__ errpart, int* __restrict__ active, int* __restrict__ n_active,
-    const int* __restrict__ iter_base, int k_in_chunk, int max_iter, float tol,
-    const float* __restrict__ fold, const float* __restrict__ g, float* __restrict__ fout,
-    float* __restrict__ gout, int32_t* __restrict__ iters, float* __restrict__ err) {
-  const int p = blockIdx.x;
-  if (!active[p]) return;
-  __shared__ int done;
-  const int k = *iter_base + k_in_chunk;  // iteration whose f-update just ran
-  if (threadIdx.x == 0) {
-    float e = 0.f;
-    for (int t = tile_first[p]; t < tile_first[p + 1]; ++t) e += errpart[t];
-    err[p] = e;
-    done = (e <= tol || k - 1 >= max_iter) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!done) return;
-  const SkPair pr = pairs[p];
-  for (int i = threadIdx.x; i < pr.n; i += blockDim.x) fout[pr.xoff + i] = fold[pr.xoff + i];
-  for (int j = threadIdx.x; j < pr.m; j += blockDim.x) gout[pr.yoff + j] = g[pr.yoff + j];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    iters[p] = k - 1;
-    active[p] = 0;
-    atomicSub(n_active, 1);
-  }
-}
-
 // One workgroup per pair: centre[p] = mean of the pair's y points.
 __global__ __launch_bounds__(256) void sk_centre_kernel(const float* __restrict__ y, int dim,
                                                         const SkPair* __restrict__ pairs,
@@ -458,26 +481,45 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   float* fB = fbuf + sx;
   hipStream_t lst = st;  // launch stream: a private stream while capturing
 
-#define SK_HALF(D, RX, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, NT, ERR)                     \
-  hipLaunchKernelGGL((sk_half_kernel<D, RX>), dim3((unsigned)(NT)), dim3(256),                  \
+  // tile counts of the current work-list (shrunk as pairs converge)
+  int64_t ntx = tx, nty = ty;
+  SkCheck nock{};
+#define SK_HALF(D, RX, CH, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, NT, ERR, CK)             \
+  hipLaunchKernelGGL((sk_half_kernel<D, RX, CH>), dim3((unsigned)(NT)), dim3(256),              \
                      (size_t)maxc * ((D) <= 3 ? 16 : 20), lst, ROWS, COLS, LWC, PC, PIN, POUT,   \
-                     WR, dpairs, TILES, active, ERR, maxc, centre)
-  auto gstep = [&](const float* fcur) -> int {
-    switch (dim) {
-      case 1: SK_HALF(1, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
-      case 2: SK_HALF(2, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
-      case 3: SK_HALF(3, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
-      default: SK_HALF(4, false, y, x, la, fcur, gw, gw, b, dtiles + tx, ty, nullptr); break;
+                     WR, dpairs, TILES, active, ERR, maxc, centre, CK)
+  // g-update from fcur; with `check`, the fused convergence check of the
+  // state (fold, g) of iteration k_in_chunk - 1 runs first
+  auto gstep = [&](const float* fcur, const float* fold, int k_in_chunk, bool check) -> int {
+    if (nty == 0) return TM_OK;
+    SkCheck ck{dfirst, errpart, iter_base, k_in_chunk, max_iter, tol, fold, f, g,
+               iters, err, active, n_active};
+    const SkTile* gt = dtiles + ntx;
+    if (check) {
+      switch (dim) {
+        case 1: SK_HALF(1, false, true, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, ck); break;
+        case 2: SK_HALF(2, false, true, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, ck); break;
+        case 3: SK_HALF(3, false, true, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, ck); break;
+        default: SK_HALF(4, false, true, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, ck); break;
+      }
+    } else {
+      switch (dim) {
+        case 1: SK_HALF(1, false, false, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, nock); break;
+        case 2: SK_HALF(2, false, false, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, nock); break;
+        case 3: SK_HALF(3, false, false, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, nock); break;
+        default: SK_HALF(4, false, false, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, nock); break;
+      }
     }
     TM_LAUNCH_CHECK();
     return TM_OK;
   };
   auto fstep = [&](const float* fin, float* fo) -> int {
+    if (ntx == 0) return TM_OK;
     switch (dim) {
-      case 1: SK_HALF(1, true, x, y, lb, gw, fin, fo, a, dtiles, tx, errpart); break;
-      case 2: SK_HALF(2, true, x, y, lb, gw, fin, fo, a, dtiles, tx, errpart); break;
-      case 3: SK_HALF(3, true, x, y, lb, gw, fin, fo, a, dtiles, tx, errpart); break;
-      default: SK_HALF(4, true, x, y, lb, gw, fin, fo, a, dtiles, tx, errpart); break;
+      case 1: SK_HALF(1, true, false, x, y, lb, gw, fin, fo, a, dtiles, ntx, errpart, nock); break;
+      case 2: SK_HALF(2, true, false, x, y, lb, gw, fin, fo, a, dtiles, ntx, errpart, nock); break;
+      case 3: SK_HALF(3, true, false, x, y, lb, gw, fin, fo, a, dtiles, ntx, errpart, nock); break;
+      default: SK_HALF(4, true, false, x, y, lb, gw, fin, fo, a, dtiles, ntx, errpart, nock); break;
     }
     TM_LAUNCH_CHECK();
     return TM_OK;
@@ -486,18 +528,17 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   // state 0: g from the initial f (g's own initial values only seed the shift)
   hipLaunchKernelGGL(sk_copy_kernel, dim3(grid_for(sy, 256)), dim3(256), 0, st, g, gw, sy);
   TM_LAUNCH_CHECK();
-  if (gstep(fA) != TM_OK) return TM_ERR_HIP;
+  if (gstep(fA, nullptr, 0, false) != TM_OK) return TM_ERR_HIP;
 
-  // one chunk = `chunk` iterations (chunk even keeps the f ping-pong aligned)
+  // one chunk = `chunk` iterations (chunk even keeps the f ping-pong aligned);
+  // an iteration is an f-update and a g-update with the fused check
   const int K = chunk % 2 ? chunk + 1 : chunk;
   auto enqueue_chunk = [&]() -> int {
     float* fo = fA;
     float* fn = fB;
     for (int k = 1; k <= K; ++k) {
       if (fstep(fo, fn) != TM_OK) return TM_ERR_HIP;
-      hipLaunchKernelGGL(sk_check_kernel, dim3(npairs), dim3(256), 0, lst, dpairs, dfirst, errpart,
-                         active, n_active, iter_base, k, max_iter, tol, fo, gw, f, g, iters, err);
-      if (gstep(fn) != TM_OK) return TM_ERR_HIP;
+      if (gstep(fn, fo, k, true) != TM_OK) return TM_ERR_HIP;
       float* t = fo;
       fo = fn;
       fn = t;
@@ -514,23 +555,67 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   TM_CHECK_ARG(!outer_capture, "tm_sinkhorn_points synchronises on convergence and cannot be "
                                "captured; capture tm_sinkhorn_points_iterate instead");
   // capture one chunk on a private stream (the caller's may be the null
-  // stream, which cannot capture), replay it on the caller's stream
+  // stream, which cannot capture), replay it on the caller's stream; captured
+  // again whenever the work-list shrinks
   hipStream_t cap = nullptr;
   bool use_graph = getenv("TORCHMORPH_AMD_NO_GRAPH") == nullptr &&
                    hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) == hipSuccess;
-  if (use_graph) use_graph = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
-  if (use_graph) {
-    lst = cap;
-    const int s1 = enqueue_chunk();
-    lst = st;
-    use_graph = hipStreamEndCapture(cap, &graph) == hipSuccess && s1 == TM_OK;
-    if (use_graph) use_graph = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-    if (!use_graph) {
-      if (graph) hipGraphDestroy(graph);
+  auto build_graph = [&]() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      lst = cap;
+      const int s1 = enqueue_chunk();
+      lst = st;
+      ok = hipStreamEndCapture(cap, &graph) == hipSuccess && s1 == TM_OK;
+      if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    }
+    if (!ok) {
+      if (graph) (void)hipGraphDestroy(graph);
       graph = nullptr;
+      exec = nullptr;
       (void)hipGetLastError();
     }
-  }
+    return ok;
+  };
+  if (use_graph) use_graph = build_graph();
+  // Work-list compaction: once the active pairs fall below 3/4 of the pairs
+  // in the list, the tile tables are rebuilt for the active pairs only (their
+  // f-tile ranges in tile_first; converged pairs get empty ranges), so the
+  // long tail of a batch launches only the tiles of its stragglers.
+  std::vector<int> hact(npairs);
+  int listed = npairs;
+  auto compact = [&]() -> int {
+    TM_CHECK_HIP(hipMemcpyAsync(hact.data(), active, sizeof(int) * npairs,
+                                hipMemcpyDeviceToHost, st));
+    TM_CHECK_HIP(hipStreamSynchronize(st));
+    htile.clear();
+    int kept = 0;
+    for (int p = 0; p < npairs; ++p) {
+      hfirst[p] = (int32_t)htile.size();
+      if (!hact[p]) continue;
+      ++kept;
+      for (int r = 0; r < n[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+    }
+    hfirst[npairs] = (int32_t)htile.size();
+    ntx = (int64_t)htile.size();
+    for (int p = 0; p < npairs; ++p)
+      if (hact[p])
+        for (int r = 0; r < m[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+    nty = (int64_t)htile.size() - ntx;
+    if (!htile.empty())
+      TM_CHECK_HIP(hipMemcpyAsync(dtiles, htile.data(), sizeof(SkTile) * htile.size(),
+                                  hipMemcpyHostToDevice, st));
+    TM_CHECK_HIP(hipMemcpyAsync(dfirst, hfirst.data(), sizeof(int32_t) * (npairs + 1),
+                                hipMemcpyHostToDevice, st));
+    TM_CHECK_HIP(hipStreamSynchronize(st));  // the host tables are reused by the next compaction
+    listed = kept;
+    if (use_graph) use_graph = build_graph();
+    return TM_OK;
+  };
   int status = TM_OK;
   int host_active = npairs;
   for (int done = 0; done < max_iter + K && host_active > 0; done += K) {
@@ -545,10 +630,11 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
       status = TM_ERR_HIP;
       break;
     }
+    if (host_active > 0 && 4 * host_active < 3 * listed && (status = compact()) != TM_OK) break;
   }
-  if (exec) hipGraphExecDestroy(exec);
-  if (graph) hipGraphDestroy(graph);
-  if (cap) hipStreamDestroy(cap);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (cap) (void)hipStreamDestroy(cap);
   if (status != TM_OK) {
     set_error("tm_sinkhorn_points: HIP failure in the iteration loop");
     return status;
