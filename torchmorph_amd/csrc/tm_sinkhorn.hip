@@ -43,7 +43,7 @@ struct SkPair {
   int64_t xoff, yoff;  // first row of the pair in the concatenated x / y
   int32_t n, m;
   float eps;
-  int32_t pad;
+  int32_t gbase;       // first 4-row error group of the pair (see sk_half_kernel)
 };
 
 // Per-pair centre (mean of the y cloud) subtracted from both clouds when
@@ -67,13 +67,12 @@ __device__ __forceinline__ float dotp(const float* a, const float* b) {
 
 // Convergence check fused into the g-update of iteration k (1-based): the
 // error of the state after iteration k-1 (f still in fold, g in the g
-// buffer) is the sum of the f-update's tile partials of the pair, in tile
-// order; every g-tile of the pair forms the same sum.  A converged pair (or
-// one at max_iter) updates no g: its first tile copies that state out, sets
-// its iteration count and error and deactivates it.
+// buffer) is the sum of the f-update's 4-row group partials of the pair, by
+// a fixed tree; every g-tile of the pair forms the same sum.  A converged
+// pair (or one at max_iter) updates no g: its first tile copies that state
+// out, sets its iteration count and error and deactivates it.
 struct SkCheck {
-  const int32_t* tile_first;  // f-tiles of pair p: [tile_first[p], tile_first[p + 1])
-  const float* fpart;         // f-update tile partials of the L1 row-marginal error
+  const float* fpart;         // f-update 4-row group partials of the L1 row-marginal error
   const int* iter_base;       // iterations before this chunk
   int k_in_chunk;
   int max_iter;
@@ -90,25 +89,41 @@ struct SkCheck {
 // One half-step for every tile of the work-list.
 //   rows/cols: concatenated points (N x D); lwc: log2 of the column weights;
 //   pc: column potentials; pin/pout: row potentials (old/new, may alias);
-//   lwr/wr: log2 / plain row weights (for the marginal error).
+//   wr: plain row weights (for the marginal error).
 // ROWS_ARE_X selects which offsets of SkPair index rows; CHECK (g-updates
 // of the iteration loop) runs the fused convergence check first.
-template <int D, bool ROWS_ARE_X, bool CHECK>
+//
+// A tile is RPT rows (64, 16 or 4) with 256 / RPT lanes per row.  The
+// arithmetic does not depend on RPT, so the schedule can change tile shapes
+// between iterations (more, smaller tiles for a few straggling pairs) with
+// identical results: a row's sum runs over 64 interleaved column groups
+// (group g holds the columns j = g mod 64, summed in column order; columns
+// are padded to a multiple of 64 with terms that are exactly 0), and the 64
+// group sums combine by one fixed pairwise tree (g + 32 into g, then + 16,
+// ..., + 1), in registers where the partners share a lane and in LDS where
+// they do not.  The row-marginal error is written per group of 4 rows
+// (summed in row order) and a pair's error is summed over its groups by a
+// fixed tree as well.
+template <int D, bool ROWS_ARE_X, bool CHECK, int RPT>
 __global__ __launch_bounds__(256) void sk_half_kernel(
     const float* __restrict__ rows, const float* __restrict__ cols,
     const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ pin,
     float* __restrict__ pout, const float* __restrict__ wr, const SkPair* __restrict__ pairs,
     const SkTile* __restrict__ tiles, const int* active, float* __restrict__ errpart,
     int maxc, const float* __restrict__ centre, SkCheck ck) {
+  static_assert(RPT == 64 || RPT == 16 || RPT == 4, "tile rows");
+  constexpr int NQ = 256 / RPT;  // lanes per row
+  constexpr int GPL = 64 / NQ;   // column groups per lane
   // column j in LDS as one float4: (y_0, .., y_{D-1}, h_j) for D <= 3, so a
-  // term is one broadcast LDS read, three FMAs (h_j as the addend), one
-  // subtraction, one v_exp_f32 and one add; D = 4 keeps h_j in its own array
+  // term is one LDS read, three FMAs (h_j as the addend), one subtraction,
+  // one v_exp_f32 and one add; D = 4 keeps h_j in its own array
   constexpr bool PACK = D <= 3;
   extern __shared__ float4 dyn4[];
-  float4* s4 = dyn4;                                 // [maxc] points (+ h_j when PACK)
-  float* shx = (float*)(dyn4 + maxc);                // [maxc] h_j when !PACK
-  __shared__ float ssum[4][kRowsPerTile];
-  __shared__ float smax[4][kRowsPerTile];
+  const int maxcp = (maxc + 63) & ~63;
+  float4* s4 = dyn4;                    // [maxcp] points (+ h_j when PACK)
+  float* shx = (float*)(dyn4 + maxcp);  // [maxcp] h_j when !PACK
+  __shared__ float ssum[NQ][RPT];
+  __shared__ float serr4[RPT];
   __shared__ int sbad;
   const SkTile tile = tiles[blockIdx.x];
   if (!active[tile.pair]) return;  // uniform per workgroup
@@ -117,14 +132,20 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
     __shared__ int sdone;
     __shared__ float serr;
     __shared__ int sk;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+      // the pair's error over its ceil(n / 4) groups: lane l sums groups
+      // l, l + 64, ... in order, then a fixed tree over the lanes
+      const int ng = (pr.n + 3) >> 2;
       float e = 0.f;
-      for (int t = ck.tile_first[tile.pair]; t < ck.tile_first[tile.pair + 1]; ++t)
-        e += ck.fpart[t];
-      const int k = *ck.iter_base + ck.k_in_chunk;
-      serr = e;
-      sk = k;
-      sdone = (e <= ck.tol || k - 1 >= ck.max_iter) ? 1 : 0;
+      for (int t = threadIdx.x; t < ng; t += 64) e += ck.fpart[pr.gbase + t];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) e += __shfl_down(e, off, 64);
+      if (threadIdx.x == 0) {
+        const int k = *ck.iter_base + ck.k_in_chunk;
+        serr = e;
+        sk = k;
+        sdone = (e <= ck.tol || k - 1 >= ck.max_iter) ? 1 : 0;
+      }
     }
     __syncthreads();
     if (sdone) {
@@ -149,20 +170,24 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
   const int64_t coff = ROWS_ARE_X ? pr.yoff : pr.xoff;
   const int nr = ROWS_ARE_X ? pr.n : pr.m;
   const int nc = ROWS_ARE_X ? pr.m : pr.n;
+  const int ncp = (nc + 63) & ~63;
   const float k1 = kL2E / pr.eps;  // L / eps
   float ctr[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) ctr[d] = centre[tile.pair * kCtrStride + d];
-  // stage the pair's columns
-  for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+  // stage the pair's columns; padding columns have h = -inf (terms exactly 0)
+  for (int j = threadIdx.x; j < ncp; j += blockDim.x) {
     float y[4] = {0.f, 0.f, 0.f, 0.f};
-    float yy = 0.f;
+    float h = -INFINITY;
+    if (j < nc) {
+      float yy = 0.f;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      y[d] = cols[(coff + j) * D + d] - ctr[d];
-      yy = fmaf(y[d], y[d], yy);
+      for (int d = 0; d < D; ++d) {
+        y[d] = cols[(coff + j) * D + d] - ctr[d];
+        yy = fmaf(y[d], y[d], yy);
+      }
+      h = fmaf(k1, pc[coff + j] - yy, lwc[coff + j]);
     }
-    const float h = fmaf(k1, pc[coff + j] - yy, lwc[coff + j]);
     if (PACK) {
       s4[j] = make_float4(y[0], y[1], y[2], h);
     } else {
@@ -172,8 +197,8 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
   }
   if (threadIdx.x == 0) sbad = 0;
   __syncthreads();
-  const int r = threadIdx.x & (kRowsPerTile - 1);
-  const int q = threadIdx.x >> 6;  // column quarter (one per wavefront)
+  const int r = threadIdx.x % RPT;
+  const int q = threadIdx.x / RPT;  // this lane's first column group
   const int i = tile.r0 + r;
   const bool live = i < nr;
   float xt[4] = {0.f, 0.f, 0.f, 0.f};
@@ -199,51 +224,69 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
     if (D >= 4) t = fmaf(xt[3], v.w, t);
     return t;
   };
-  float acc = 0.f;
-  if (live) {
-#pragma unroll 4
-    for (int j = q; j < nc; j += 4) acc += __builtin_amdgcn_exp2f(term(j) - s);
-  }
-  ssum[q][r] = acc;
-  __syncthreads();
-  float S = ssum[0][r] + ssum[1][r] + ssum[2][r] + ssum[3][r];
-  const bool bad = live && !(S >= 0x1p-60f && S <= 0x1p60f);
+  // canonical row sum of 2^(t_j - shift); the result is valid in the q == 0
+  // lanes.  Every thread of the workgroup calls it (barriers inside).
+  auto row_sum = [&](float shift) -> float {
+    float acc[GPL];
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) acc[k] = 0.f;
+    if (live)
+      for (int j0 = 0; j0 < ncp; j0 += 64) {
+#pragma unroll
+        for (int k = 0; k < GPL; ++k) acc[k] += __builtin_amdgcn_exp2f(term(j0 + q + NQ * k) - shift);
+      }
+#pragma unroll
+    for (int dk = GPL / 2; dk >= 1; dk >>= 1)
+#pragma unroll
+      for (int k = 0; k < dk; ++k) acc[k] += acc[k + dk];
+    ssum[q][r] = acc[0];
+    __syncthreads();
+#pragma unroll
+    for (int dq = NQ / 2; dq >= 1; dq >>= 1) {
+      if (q < dq) ssum[q][r] += ssum[q + dq][r];
+      __syncthreads();
+    }
+    const float v = ssum[0][r];
+    __syncthreads();  // ssum is reused by the next call
+    return v;
+  };
+  float S = row_sum(s);
+  const bool bad = live && q == 0 && !(S >= 0x1p-60f && S <= 0x1p60f);
   if (bad) sbad = 1;
   __syncthreads();
   if (sbad) {
-    // exact path for the whole workgroup: per-row maximum, then the sum
+    // exact path for the whole workgroup: the row maximum (any order), then
+    // the canonical sum relative to it
     float mx = -INFINITY;
     if (live)
-      for (int j = q; j < nc; j += 4) mx = fmaxf(mx, term(j));
-    smax[q][r] = mx;
+      for (int j = q; j < nc; j += NQ) mx = fmaxf(mx, term(j));
+    ssum[q][r] = mx;
     __syncthreads();
-    mx = fmaxf(fmaxf(smax[0][r], smax[1][r]), fmaxf(smax[2][r], smax[3][r]));
-    float a2 = 0.f;
-    if (live)
-      for (int j = q; j < nc; j += 4) a2 += __builtin_amdgcn_exp2f(term(j) - mx);
+    if (q == 0)
+      for (int k = 1; k < NQ; ++k) mx = fmaxf(mx, ssum[k][r]);
     __syncthreads();
-    ssum[q][r] = a2;
+    if (q == 0) ssum[0][r] = mx;
     __syncthreads();
-    const float S2 = ssum[0][r] + ssum[1][r] + ssum[2][r] + ssum[3][r];
+    mx = ssum[0][r];
+    __syncthreads();
+    const float S2 = row_sum(mx);
     // rebase to the stale shift: log2 S = mx - s + log2 S2
-    if (live) {
+    if (live && q == 0) {
       const float l2 = mx - s + __builtin_amdgcn_logf(S2);
       S = exp2f(l2);  // may be inf/0 for the error only
-      if (q == 0) pout[roff + i] = fold - l2 / k1;
+      pout[roff + i] = fold - l2 / k1;
     }
   } else if (live && q == 0) {
     pout[roff + i] = fold - __builtin_amdgcn_logf(S) / k1;
   }
   if (errpart) {
-    // deterministic tile partial of sum_i a_i |S_i - 1| (row-marginal L1 error)
+    // row errors a_i |S_i - 1| summed per 4-row group in row order
+    if (q == 0) serr4[r] = live ? wr[roff + i] * fabsf(S - 1.f) : 0.f;
     __syncthreads();
-    float e = (live && q == 0) ? wr[roff + i] * fabsf(S - 1.f) : 0.f;
-    if (q == 0) ssum[0][r] = e;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      float t = 0.f;
-      for (int k = 0; k < kRowsPerTile; ++k) t += ssum[0][k];
-      errpart[blockIdx.x] = t;
+    if (threadIdx.x < RPT / 4 && tile.r0 + 4 * (int)threadIdx.x < nr) {
+      const int b = 4 * threadIdx.x;
+      errpart[pr.gbase + ((tile.r0 + b) >> 2)] =
+          ((serr4[b] + serr4[b + 1]) + serr4[b + 2]) + serr4[b + 3];
     }
   }
 }
@@ -375,19 +418,20 @@ __global__ void sk_copy_kernel(const float* __restrict__ src, float* __restrict_
 
 extern "C" size_t tm_sinkhorn_points_workspace_bytes(int npairs, const int32_t* n,
                                                      const int32_t* m) {
-  int64_t sx = 0, sy = 0, tx = 0, ty = 0;
+  // tiles sized for the smallest tile shape (4 rows), error groups of 4 rows
+  int64_t sx = 0, sy = 0, gx = 0, gy = 0;
   for (int p = 0; p < npairs; ++p) {
     sx += n[p];
     sy += m[p];
-    tx += (n[p] + tmh::kRowsPerTile - 1) / tmh::kRowsPerTile;
-    ty += (m[p] + tmh::kRowsPerTile - 1) / tmh::kRowsPerTile;
+    gx += (n[p] + 3) / 4;
+    gy += (m[p] + 3) / 4;
   }
   size_t b = 0;
   auto add = [&](size_t x) { b += (x + 255) & ~(size_t)255; };
   add(sizeof(tmh::SkPair) * npairs);
-  add(sizeof(tmh::SkTile) * (tx + ty));
+  add(sizeof(tmh::SkTile) * (gx + gy));
   add(sizeof(int32_t) * (npairs + 1));
-  add(sizeof(float) * tx);        // errpart
+  add(sizeof(float) * gx);        // errpart: one per 4-row group of x
   add(sizeof(int) * npairs);      // active
   add(sizeof(int) * 4);           // n_active, iter_base
   add(sizeof(float) * (sx + sy)); // log2 weights
@@ -415,7 +459,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   std::vector<SkPair> hp(npairs);
   std::vector<SkTile> htile;
   std::vector<int32_t> hfirst(npairs + 1);
-  int64_t sx = 0, sy = 0;
+  int64_t sx = 0, sy = 0, gx = 0;
   int maxc = 1;
   for (int p = 0; p < npairs; ++p) {
     TM_CHECK_ARG(n[p] >= 1 && m[p] >= 1, "tm_sinkhorn_points: empty cloud in pair %d", p);
@@ -424,9 +468,12 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
     TM_CHECK_ARG(n[p] <= kMaxCols && m[p] <= kMaxCols,
                  "tm_sinkhorn_points: clouds are limited to %d points (pair %d)", kMaxCols, p);
     TM_CHECK_ARG(eps[p] > 0.f, "tm_sinkhorn_points: eps must be > 0 (pair %d)", p);
-    hp[p] = SkPair{sx, sy, n[p], m[p], eps[p], 0};
+    TM_CHECK_ARG(gx + (n[p] + 3) / 4 < ((int64_t)1 << 31),
+                 "tm_sinkhorn_points: more than 2^33 source points");
+    hp[p] = SkPair{sx, sy, n[p], m[p], eps[p], (int32_t)gx};
     sx += n[p];
     sy += m[p];
+    gx += (n[p] + 3) / 4;
   }
   for (int p = 0; p < npairs; ++p) {
     hfirst[p] = (int32_t)htile.size();
@@ -450,9 +497,11 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
     return (void*)p;
   };
   SkPair* dpairs = (SkPair*)take(sizeof(SkPair) * npairs);
-  SkTile* dtiles = (SkTile*)take(sizeof(SkTile) * (tx + ty));
+  int64_t gy = 0;
+  for (int p = 0; p < npairs; ++p) gy += (m[p] + 3) / 4;
+  SkTile* dtiles = (SkTile*)take(sizeof(SkTile) * (gx + gy));
   int32_t* dfirst = (int32_t*)take(sizeof(int32_t) * (npairs + 1));
-  float* errpart = (float*)take(sizeof(float) * tx);
+  float* errpart = (float*)take(sizeof(float) * gx);
   int* active = (int*)take(sizeof(int) * npairs);
   int* ctr = (int*)take(sizeof(int) * 4);
   int* n_active = ctr;
@@ -481,19 +530,28 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   float* fB = fbuf + sx;
   hipStream_t lst = st;  // launch stream: a private stream while capturing
 
-  // tile counts of the current work-list (shrunk as pairs converge)
+  // tile counts and tile rows of the current work-list (rebuilt as pairs
+  // converge)
   int64_t ntx = tx, nty = ty;
+  int rpt = kRowsPerTile;
   SkCheck nock{};
-#define SK_HALF(D, RX, CH, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, NT, ERR, CK)             \
-  hipLaunchKernelGGL((sk_half_kernel<D, RX, CH>), dim3((unsigned)(NT)), dim3(256),              \
-                     (size_t)maxc * ((D) <= 3 ? 16 : 20), lst, ROWS, COLS, LWC, PC, PIN, POUT,   \
-                     WR, dpairs, TILES, active, ERR, maxc, centre, CK)
+  const size_t half_lds = (size_t)((maxc + 63) & ~63) * 20;
+#define SK_HALF_R(D, RX, CH, R, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, NT, ERR, CK)         \
+  hipLaunchKernelGGL((sk_half_kernel<D, RX, CH, R>), dim3((unsigned)(NT)), dim3(256), half_lds, \
+                     lst, ROWS, COLS, LWC, PC, PIN, POUT, WR, dpairs, TILES, active, ERR, maxc,  \
+                     centre, CK)
+#define SK_HALF(D, RX, CH, ...)                                                              \
+  do {                                                                                       \
+    if (rpt == 64) SK_HALF_R(D, RX, CH, 64, __VA_ARGS__);                                    \
+    else if (rpt == 16) SK_HALF_R(D, RX, CH, 16, __VA_ARGS__);                               \
+    else SK_HALF_R(D, RX, CH, 4, __VA_ARGS__);                                               \
+  } while (0)
   // g-update from fcur; with `check`, the fused convergence check of the
   // state (fold, g) of iteration k_in_chunk - 1 runs first
   auto gstep = [&](const float* fcur, const float* fold, int k_in_chunk, bool check) -> int {
     if (nty == 0) return TM_OK;
-    SkCheck ck{dfirst, errpart, iter_base, k_in_chunk, max_iter, tol, fold, f, g,
-               iters, err, active, n_active};
+    SkCheck ck{errpart, iter_base, k_in_chunk, max_iter, tol, fold, f, g, iters, err, active,
+               n_active};
     const SkTile* gt = dtiles + ntx;
     if (check) {
       switch (dim) {
@@ -525,6 +583,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
     return TM_OK;
   };
 #undef SK_HALF
+#undef SK_HALF_R
   // state 0: g from the initial f (g's own initial values only seed the shift)
   hipLaunchKernelGGL(sk_copy_kernel, dim3(grid_for(sy, 256)), dim3(256), 0, st, g, gw, sy);
   TM_LAUNCH_CHECK();
@@ -583,28 +642,44 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   };
   if (use_graph) use_graph = build_graph();
   // Work-list compaction: once the active pairs fall below 3/4 of the pairs
-  // in the list, the tile tables are rebuilt for the active pairs only (their
-  // f-tile ranges in tile_first; converged pairs get empty ranges), so the
-  // long tail of a batch launches only the tiles of its stragglers.
+  // in the list, the tile tables are rebuilt for the active pairs only, with
+  // the largest tile shape (64, 16 or 4 rows) that still gives at least
+  // kTailTiles f-tiles, or else the smallest shape whose tiles' column
+  // staging (half_lds each) fits the GPU's LDS at once (kResidentLds): the
+  // long tail of a batch launches only the tiles of its stragglers, spread
+  // over many small tiles (the results do not depend on the tile shape).
+  constexpr int64_t kTailTiles = 1024;
+  constexpr int64_t kResidentLds = (int64_t)256 * 128 * 1024;  // 256 CUs x 128 KB
   std::vector<int> hact(npairs);
   int listed = npairs;
   auto compact = [&]() -> int {
     TM_CHECK_HIP(hipMemcpyAsync(hact.data(), active, sizeof(int) * npairs,
                                 hipMemcpyDeviceToHost, st));
     TM_CHECK_HIP(hipStreamSynchronize(st));
+    int64_t t64 = 0, t16 = 0, t4 = 0;
+    for (int p = 0; p < npairs; ++p)
+      if (hact[p]) {
+        t64 += (n[p] + 63) / 64 + (m[p] + 63) / 64;
+        t16 += (n[p] + 15) / 16 + (m[p] + 15) / 16;
+        t4 += (n[p] + 3) / 4 + (m[p] + 3) / 4;
+      }
+    const int64_t L = (int64_t)half_lds;
+    if (t64 >= 2 * kTailTiles) rpt = 64;
+    else if (t16 >= 2 * kTailTiles || t4 * L > kResidentLds) rpt = t16 * L > kResidentLds ? 64 : 16;
+    else rpt = 4;
     htile.clear();
     int kept = 0;
     for (int p = 0; p < npairs; ++p) {
       hfirst[p] = (int32_t)htile.size();
       if (!hact[p]) continue;
       ++kept;
-      for (int r = 0; r < n[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+      for (int r = 0; r < n[p]; r += rpt) htile.push_back(SkTile{p, r});
     }
     hfirst[npairs] = (int32_t)htile.size();
     ntx = (int64_t)htile.size();
     for (int p = 0; p < npairs; ++p)
       if (hact[p])
-        for (int r = 0; r < m[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+        for (int r = 0; r < m[p]; r += rpt) htile.push_back(SkTile{p, r});
     nty = (int64_t)htile.size() - ntx;
     if (!htile.empty())
       TM_CHECK_HIP(hipMemcpyAsync(dtiles, htile.data(), sizeof(SkTile) * htile.size(),
