@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
-    ap.add_argument("--only", choices=["morph3d", "ragged", "grad", "warmstart"], default=None,
+    ap.add_argument("--only", choices=["morph3d", "ragged", "grad", "hist", "warmstart"],
+                    default=None,
                     help="run one extra workload alone and print its JSON (for profiling)")
     ap.add_argument("--tuning", default=os.path.join(ROOT, "torchmorph_amd", "tuning", "mi355x.json"),
                     help="autotune table to preload ('' = tune live)")
@@ -137,7 +138,7 @@ def main():
         return
     if args.only:
         fn = {"morph3d": extra_morph3d, "ragged": extra_ragged, "grad": extra_grad_and_sinkhorn,
-              "warmstart": extra_warmstart}[args.only]
+              "hist": extra_hist, "warmstart": extra_warmstart}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return
     B, C, H, W, p = C1["B"], C1["C"], C1["H"], C1["W"], C1["p"]
@@ -226,12 +227,16 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline([m.cpu().numpy() for m in masks[:4]], args.cpu_seconds)
+        # SciPy, the library the reference's tests compare against, per image
+        cpu["scipy"] = _scipy_rates("edt_indices", [m[b, 0].cpu().numpy() for m in masks[:2]
+                                                    for b in range(B)], args.cpu_seconds / 2)
 
     extras = None
     if not args.no_extras:
         extras = {}
         for name, fn in (("morph3d", extra_morph3d), ("ragged", extra_ragged),
-                         ("grad", extra_grad_and_sinkhorn), ("warmstart", extra_warmstart)):
+                         ("grad", extra_grad_and_sinkhorn), ("hist", extra_hist),
+                         ("warmstart", extra_warmstart)):
             t_x = time.perf_counter()
             extras[name] = fn()
             extras[name]["wall_s"] = round(time.perf_counter() - t_x, 1)
@@ -303,12 +308,34 @@ def _timed(fn, reps, stream):
     return e0.elapsed_time(e1) / reps
 
 
+def _timed_sets(fn, sets, reps, stream):
+    """Mean milliseconds per call of fn(k), k cycling over `sets` distinct
+    buffer sets (together larger than the 256 MB Infinity Cache), reps passes,
+    HIP events on `stream`."""
+    import torch
+
+    for k in range(sets):
+        fn(k)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        for k in range(sets):
+            fn(k)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / (reps * sets)
+
+
 def _gb(bytes_, ms):
     return round(bytes_ / ms / 1e6, 1)
 
 
 def extra_morph3d():
-    """4x2x128^3 float32 quantised to 256 levels; parity on item 0."""
+    """4x2x128^3 float32 quantised to 256 levels (134 MB per volume set); each
+    workload rotates over several input / output sets (> 256 MB together);
+    parity on item 0."""
     import numpy as np
     import torch
     import torch.nn.functional as F
@@ -317,15 +344,23 @@ def extra_morph3d():
     import torchmorph_amd as tm
 
     g = torch.Generator(device="cuda").manual_seed(7)
-    x = torch.floor(torch.rand((4, 2, 128, 128, 128), generator=g, device="cuda") * 256) / 256
-    mask = torch.rand((4, 2, 128, 128, 128), generator=g, device="cuda") < 0.7
+    SG, SB = 3, 8  # grey sets (3 x 268 MB in + out), binary sets (8 x 33.6 MB)
+    xs = [torch.floor(torch.rand((4, 2, 128, 128, 128), generator=g, device="cuda") * 256) / 256
+          for _ in range(SG)]
+    ys = [torch.empty_like(xs[0]) for _ in range(SG)]
+    masks = [torch.rand((4, 2, 128, 128, 128), generator=g, device="cuda") < 0.7
+             for _ in range(SB)]
+    mouts = [torch.empty_like(masks[0]) for _ in range(SB)]
+    x, mask = xs[0], masks[0]
     n = x.numel()
     st = torch.cuda.current_stream()
-    out = {}
+    out = {"rotating_sets": {"grey": SG, "binary": SB,
+                             "grey_MB": round(SG * 2 * 4 * n / 1e6, 1),
+                             "binary_MB": round(SB * 2 * n / 1e6, 1)}}
     x0 = x[:1, :1].cpu().numpy()
     for size in (3, 7, 15):
         y = tm.grey_dilation(x, size=size)
-        ms = _timed(lambda: tm.grey_dilation(x, size=size), 5, st)
+        ms = _timed_sets(lambda k: tm.grey_dilation(xs[k], size=size, output=ys[k]), SG, 3, st)
         # flat cube, reflect border: the max over the in-image part of the window
         ref = F.max_pool3d(x.view(8, 1, 128, 128, 128), size, 1, size // 2).view_as(x)
         out[f"grey_dilation_cube{size}"] = {"ms": round(ms, 3), "Gvox_s": round(n / ms / 1e6, 2),
@@ -337,13 +372,15 @@ def extra_morph3d():
     s_t = torch.from_numpy(heights)
     f_t = torch.from_numpy(fp)
     y = tm.grey_dilation(x, structure=s_t, footprint=f_t)
-    ms = _timed(lambda: tm.grey_dilation(x, structure=s_t, footprint=f_t), 3, st)
+    ms = _timed_sets(lambda k: tm.grey_dilation(xs[k], structure=s_t, footprint=f_t,
+                                                output=ys[k]), SG, 2, st)
     want = oracle.grey(x0, heights, fp, (7, 7, 7), (0, 0, 0), dilation=True)
     out["grey_dilation_ball_r3"] = {"ms": round(ms, 3), "Gvox_s": round(n / ms / 1e6, 2),
                                     "GB_s": _gb(8 * n, ms),
                                     "parity": bool(np.array_equal(y[:1, :1].cpu().numpy(), want))}
     yb = tm.binary_erosion(mask, iterations=4)
-    ms = _timed(lambda: tm.binary_erosion(mask, iterations=4), 5, st)
+    ms = _timed_sets(lambda k: tm.binary_erosion(masks[k], iterations=4, output=mouts[k]), SB,
+                     5, st)
     cross = np.zeros((3, 3, 3), bool)
     cross[1, 1, :] = cross[1, :, 1] = cross[:, 1, 1] = True
     want = oracle.binary(mask[:1, :1].cpu().numpy(), cross, (0, 0, 0), iterations=4)
@@ -353,20 +390,22 @@ def extra_morph3d():
                                                               want.astype(bool)))}
     # composites, fused epilogue against the plain composition of the same kernels
     composed = {
-        "morphological_gradient": lambda: tm.grey_dilation(x, size=3) - tm.grey_erosion(x, size=3),
-        "white_tophat": lambda: x - tm.grey_opening(x, size=3),
-        "morphological_laplace": lambda: (tm.grey_dilation(x, size=3) + tm.grey_erosion(x, size=3)
-                                          - 2 * x),
+        "morphological_gradient": lambda v: tm.grey_dilation(v, size=3) - tm.grey_erosion(v, size=3),
+        "white_tophat": lambda v: v - tm.grey_opening(v, size=3),
+        "morphological_laplace": lambda v: (tm.grey_dilation(v, size=3) + tm.grey_erosion(v, size=3)
+                                            - 2 * v),
     }
     for name, plain in composed.items():
         fn = getattr(tm, name)
         y = fn(x, size=3)
-        ms = _timed(lambda: fn(x, size=3), 5, st)
-        ms_plain = _timed(plain, 5, st)
+        ms = _timed_sets(lambda k: fn(xs[k], size=3, output=ys[k]), SG, 3, st)
+        ms_plain = _timed_sets(lambda k: plain(xs[k]), SG, 3, st)
         out[f"{name}_cube3"] = {"ms": round(ms, 3), "composed_ms": round(ms_plain, 3),
-                                "parity_with_composed": bool(torch.equal(y, plain()))}
+                                "parity_with_composed": bool(torch.equal(y, plain(x)))}
     out["bytes_per_vox"] = "grey 4 in + 4 out; binary 1 in + 1 out (algorithmic)"
-    # CPU oracle beside it: one 128^3 item, single thread
+    # CPU beside it: the C oracle (one 128^3 item, one thread), and SciPy
+    # (the library the reference's tests compare against) per item, one core
+    # and a pool over the host cores
     t0 = time.perf_counter()
     oracle.grey(x0, None, None, (3, 3, 3), (0, 0, 0), dilation=True)
     t_grey = time.perf_counter() - t0
@@ -377,7 +416,49 @@ def extra_morph3d():
                            "binary_erosion_x4_Mvox_s": round(x0.size / t_bin / 1e6, 2),
                            "cores": 1, "kind": "port",
                            "sample": "one 128^3 item through oracle/tm_oracle.c"}
+    items = [xs[0][b, c].cpu().numpy() for b in range(2) for c in range(2)]
+    out["scipy"] = _scipy_rates("grey_dilation_size3", items, 3.0)
+    mitems = [masks[0][b, c].cpu().numpy() for b in range(2) for c in range(2)]
+    out["scipy"].update(_scipy_rates("binary_erosion_x4", mitems, 3.0))
     return out
+
+
+def _scipy_job(args):
+    """Pool worker: SciPy `op` over the items for `seconds`; (voxels, seconds)."""
+    op, items, seconds = args
+    from scipy import ndimage as ndi
+
+    fns = {"grey_dilation_size3": lambda v: ndi.grey_dilation(v, size=3),
+           "binary_erosion_x4": lambda v: ndi.binary_erosion(v, iterations=4),
+           "edt_indices": lambda v: ndi.distance_transform_edt(v, return_indices=True)}
+    fn = fns[op]
+    t0 = time.perf_counter()
+    vox = k = 0
+    while time.perf_counter() - t0 < seconds:
+        fn(items[k % len(items)])
+        vox += items[k % len(items)].size
+        k += 1
+    return vox, time.perf_counter() - t0
+
+
+def _scipy_rates(op, items, seconds):
+    """SciPy on one core and on a pool over the host cores (16 at most), in
+    Mvox/s, or a note when SciPy is not importable."""
+    try:
+        import scipy
+    except ImportError:
+        return {f"{op}_scipy": "scipy not importable"}
+    import multiprocessing as mp
+
+    v1, t1 = _scipy_job((op, items, seconds / 2))
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    with mp.get_context("spawn").Pool(cores) as pool:
+        t0 = time.perf_counter()
+        res = pool.map(_scipy_job, [(op, items, seconds / 2)] * cores)
+        wall = time.perf_counter() - t0
+    return {f"{op}_scipy": {"one_core_Mvox_s": round(v1 / t1 / 1e6, 2),
+                            "pooled_Mvox_s": round(sum(v for v, _ in res) / wall / 1e6, 2),
+                            "cores": cores, "version": scipy.__version__}}
 
 
 def _smooth_noise_mask(shape, gen, sigma=4.0):
@@ -529,13 +610,16 @@ def extra_grad_and_sinkhorn():
     st = torch.cuda.current_stream()
     out = {}
     g = torch.Generator(device="cuda").manual_seed(3)
-    x = torch.rand((16, 1, 512, 512), generator=g, device="cuda").requires_grad_(True)
-    y = tm.grey_dilation(x, size=3, differentiable=True)
-    go = torch.rand_like(y)
-    n = x.numel()
-    ms = _timed(lambda: torch.autograd.grad(y, x, go, retain_graph=True), 10, st)
+    SETS = 6  # 6 x (x, grad_out, grad_in) of 16.8 MB each = 302 MB
+    xs = [torch.rand((16, 1, 512, 512), generator=g, device="cuda").requires_grad_(True)
+          for _ in range(SETS)]
+    ys = [tm.grey_dilation(x, size=3, differentiable=True) for x in xs]
+    gos = [torch.rand_like(ys[0]) for _ in range(SETS)]
+    n = xs[0].numel()
+    ms = _timed_sets(lambda k: torch.autograd.grad(ys[k], xs[k], gos[k], retain_graph=True),
+                     SETS, 5, st)
     out["grey_dilation_backward"] = {"ms": round(ms, 3), "Gvox_s": round(n / ms / 1e6, 2),
-                                     "GB_s": _gb(12 * n, ms),
+                                     "GB_s": _gb(12 * n, ms), "rotating_sets": SETS,
                                      "bytes_per_vox": "x 4 + grad_out 4 + grad_in 4"}
 
     rng = np.random.default_rng(11)
@@ -583,6 +667,46 @@ def extra_grad_and_sinkhorn():
                               "iters_max": int(it.max()), "tol_f32": tol,
                               "fp64_marginal_err_max_of_8": float(max(errs)),
                               "all_converged": bool((res.err <= tol).all())}
+    return out
+
+
+def extra_hist():
+    """Histogram Sinkhorn (the reference's SinkhornSolver) on grid costs
+    d = 32x32 = 1024 and 64x64 = 4096, batches of 1 and 64 histograms, 100
+    iterations, both forms; per iteration the two update kernels read the
+    (d, d) matrix once each (8 d^2 bytes in float32), the bytes counted here.
+    At d = 4096 the 64 MB matrix stays in the 256 MB Infinity Cache between
+    half-iterations, so the rate can exceed the HBM figure.  Parity: float32
+    GPU values against the CPU path at d = 1024 (rtol / atol 1e-4)."""
+    import torch
+
+    import torchmorph_amd as tm
+
+    st = torch.cuda.current_stream()
+    out = {}
+    g = torch.Generator().manual_seed(42)
+    for side in (32, 64):
+        d = side * side
+        cost = tm.build_cost_matrix((side, side), device="cuda")
+        for nb in (1, 64):
+            src = torch.rand((nb, d), generator=g).cuda()
+            tgt = torch.rand((nb, d), generator=g).cuda()
+            for log_space in (False, True):
+                eps = 4.0 if log_space else 16.0
+                solver = tm.SinkhornSolver(epsilon=eps, max_iter=100, log_space=log_space)
+                ms = _timed(lambda: solver(src, tgt, cost), 3, st)
+                us_it = ms * 1e3 / 100
+                key = f"d{d}_n{nb}_{'log' if log_space else 'scaling'}"
+                bytes_it = 8 * d * d + 16 * nb * d
+                rec = {"ms_per_solve": round(ms, 3), "us_per_iteration": round(us_it, 2),
+                       "GB_s": round(bytes_it / us_it / 1e3, 1),
+                       "frac_of_hbm": round(bytes_it / us_it / 1e3 / HBM_PEAK_GBS, 4)}
+                if side == 32 and nb == 1:
+                    want = solver(src.cpu(), tgt.cpu(), cost.cpu())
+                    got = solver(src, tgt, cost).cpu()
+                    rec["parity_vs_cpu"] = bool(torch.allclose(got, want, rtol=1e-4, atol=1e-4))
+                out[key] = rec
+    out["bytes_per_iteration"] = "8 d^2 (matrix read by both updates) + 16 n d (scalings)"
     return out
 
 

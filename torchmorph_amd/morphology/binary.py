@@ -25,14 +25,17 @@ def _normalize_structure(structure: Tensor, spatial_ndim: int, name: str = "stru
     return (structure != 0).detach().to(device="cpu", dtype=torch.bool).contiguous()
 
 
-def _run(x: Tensor, structure: Tensor, origin, border_value: bool, ops, mask: Tensor | None):
-    """Runs [(dilation?, iterations), ...] on x (bool/float, contiguous)."""
+def _run(x: Tensor, structure: Tensor, origin, border_value: bool, ops, mask: Tensor | None,
+         into: Tensor | None = None):
+    """Runs [(dilation?, iterations), ...] on x (bool/float, contiguous) into
+    `into` (a bool contiguous tensor sharing no storage with x) or a new
+    tensor."""
     L = _lib.lib()
     xin, in_dtype = _lib.mask_operand(x)
     nd = x.ndim - 2
     batch = x.shape[0] * x.shape[1]
     shp = _lib.i64_array(x.shape[2:])
-    out = torch.empty(x.shape, dtype=torch.bool, device=x.device)
+    out = into if into is not None else torch.empty(x.shape, dtype=torch.bool, device=x.device)
     st = structure.to(torch.uint8).numpy()
     nops = len(ops)
     dil = (_lib.ctypes.c_int * nops)(*[int(d) for d, _ in ops])
@@ -92,7 +95,13 @@ def _sequence(input, structure, origin, border_value, ops, mask, output):
             cur = val if maskb is None else torch.where(maskb, val, cur)
         result = cur
     else:
-        result = _run(x.contiguous(), structure, origin, border_value, ops, mask)
+        x = x.contiguous()
+        if (output is not None and output.dtype == torch.bool and output.is_contiguous()
+                and output.untyped_storage().data_ptr() != x.untyped_storage().data_ptr()
+                and (mask is None
+                     or output.untyped_storage().data_ptr() != mask.untyped_storage().data_ptr())):
+            return _run(x, structure, origin, border_value, ops, mask, into=output)
+        result = _run(x, structure, origin, border_value, ops, mask)
     return _emit(result, output)
 
 

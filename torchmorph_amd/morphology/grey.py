@@ -186,6 +186,10 @@ def _grey_morphology(input: Tensor, size=None, footprint=None, structure=None, o
     else:
         x = input.detach()
         x = x.contiguous() if x.dtype == torch.float32 else x.float().contiguous()
+        if _writable_in_place(output, x):
+            _launch(x, struct, fp, se_shape, origin_t, mode_int, cval, operation == "dilation",
+                    output)
+            return output
         result = torch.empty(x.shape, dtype=torch.float32, device=x.device)
         _launch(x, struct, fp, se_shape, origin_t, mode_int, cval, operation == "dilation",
                 result)
@@ -193,6 +197,13 @@ def _grey_morphology(input: Tensor, size=None, footprint=None, structure=None, o
         output.copy_(result)
         return output
     return result
+
+
+def _writable_in_place(output, x) -> bool:
+    """A preallocated float32 contiguous output that shares no storage with
+    the input is written by the kernels directly (no copy pass)."""
+    return (output is not None and output.dtype == torch.float32 and output.is_contiguous()
+            and output.untyped_storage().data_ptr() != x.untyped_storage().data_ptr())
 
 
 def grey_erosion(input: Tensor, size=None, footprint=None, structure=None, output=None,
