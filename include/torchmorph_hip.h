@@ -386,6 +386,18 @@ int tm_sinkhorn_points_grad(const float* x, const float* y, const float* a, cons
                             const float* eps, const float* f, const float* g, float* gx,
                             float* gy, void* workspace, size_t workspace_bytes, void* stream);
 
+/* No reference counterpart (the backward of the implicit Sinkhorn adjoint).
+ * Products with the plan of every pair's potentials f, g, formed on the fly
+ * (never materialised): transpose = 0: out (sum n, k) = P V for V (sum m, k);
+ * transpose = 1: out (sum m, k) = P^T V for V (sum n, k); k in 1..8, V and
+ * out row-major.  P_ij = a_i b_j exp((f_i + g_j - |x_i - y_j|^2) / eps).
+ * Layouts as tm_sinkhorn_points; workspace as tm_sinkhorn_points_grad. */
+int tm_sinkhorn_points_planmat(const float* x, const float* y, const float* a, const float* b,
+                               int dim, int npairs, const int32_t* n, const int32_t* m,
+                               const float* eps, const float* f, const float* g, int transpose,
+                               int k, const float* V, float* out, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
 /* ----------------------------------------------------------- warm start --- */
 /* No reference counterpart.  DeepSets network predicting the dual potential f
  * of every x point of a point-cloud pair (dim 3) from both clouds and eps:

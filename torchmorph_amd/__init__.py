@@ -34,9 +34,11 @@ from .morphology import (
 )
 from .optimal_transport import (
     SinkhornPointCloud,
+    SinkhornPotentials,
     SinkhornSolver,
     build_cost_matrix,
     sinkhorn_points,
+    sinkhorn_potentials,
 )
 from .ragged import (
     RaggedBinaryMorphology,
@@ -68,6 +70,8 @@ __all__ = [
     "build_cost_matrix",
     "sinkhorn_points",
     "SinkhornPointCloud",
+    "sinkhorn_potentials",
+    "SinkhornPotentials",
     "RaggedDistanceTransform",
     "euclidean_distance_transform_ragged",
     "chamfer_distance_transform_ragged",

@@ -67,6 +67,10 @@ SIGNATURES: dict[str, tuple] = {
                                      ctypes.POINTER(ctypes.c_int32),
                                      ctypes.POINTER(ctypes.c_float), _P, _P, _P, _P, _P, _SZ,
                                      _P]),
+    "tm_sinkhorn_points_planmat": (_I, [_P, _P, _P, _P, _I, _I, ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_float), _P, _P, _I, _I, _P, _P,
+                                        _P, _SZ, _P]),
     "tm_mlp_gemm_bf16": (_I, [_P, _I64, _I64, _P, _I64, _I64, _P, _I, _I, _I, _P, _SZ, _P]),
     "tm_cast_bf16": (_I, [_P, _P, _I64, _P]),
     "tm_warmstart_param_count": (_I64, []),

@@ -368,6 +368,80 @@ __global__ __launch_bounds__(256) void sk_grad_kernel(
   }
 }
 
+// Products with the plan of converged potentials, formed on the fly:
+// out[i, k] = sum_j P_ij V[j, k] for the tile's rows (K vectors), with
+// P_ij = a_i 2^(e_ij - s_i) as in sk_grad_kernel.  Rows are x (P V) or y
+// (P^T V).  Four lanes per row take the columns j = q mod 4; their partial
+// sums add in lane order (deterministic).  V is read from global memory
+// (every lane of a wavefront reads the same V[j], one broadcast load).
+template <int D, bool ROWS_ARE_X, int K>
+__global__ __launch_bounds__(256) void sk_planmat_kernel(
+    const float* __restrict__ rows, const float* __restrict__ cols,
+    const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ prow,
+    const float* __restrict__ wr, const SkPair* __restrict__ pairs,
+    const SkTile* __restrict__ tiles, int maxc, const float* __restrict__ centre,
+    const float* __restrict__ V, float* __restrict__ out) {
+  extern __shared__ float dyn[];
+  float* sy = dyn;
+  float* sh = dyn + maxc * D;
+  __shared__ float part[4][kRowsPerTile][K];
+  const SkTile tile = tiles[blockIdx.x];
+  const SkPair pr = pairs[tile.pair];
+  const int64_t roff = ROWS_ARE_X ? pr.xoff : pr.yoff;
+  const int64_t coff = ROWS_ARE_X ? pr.yoff : pr.xoff;
+  const int nr = ROWS_ARE_X ? pr.n : pr.m;
+  const int nc = ROWS_ARE_X ? pr.m : pr.n;
+  const float k1 = kL2E / pr.eps;
+  float ctr[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) ctr[d] = centre[tile.pair * kCtrStride + d];
+  for (int j = threadIdx.x; j < nc; j += blockDim.x) {
+    float yy = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float y = cols[(coff + j) * D + d] - ctr[d];
+      sy[j * D + d] = y;
+      yy = fmaf(y, y, yy);
+    }
+    sh[j] = fmaf(k1, pc[coff + j] - yy, lwc[coff + j]);
+  }
+  __syncthreads();
+  const int r = threadIdx.x & (kRowsPerTile - 1);
+  const int q = threadIdx.x >> 6;
+  const int i = tile.r0 + r;
+  const bool live = i < nr;
+  float xt[D];
+  float ci = 0.f, s = 0.f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const float x = live ? rows[(roff + i) * D + d] - ctr[d] : 0.f;
+    xt[d] = 2.f * k1 * x;
+    ci = fmaf(x, x, ci);
+  }
+  if (live) s = k1 * ci - k1 * prow[roff + i];
+  float acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.f;
+  if (live) {
+    for (int j = q; j < nc; j += 4) {
+      const float w = __builtin_amdgcn_exp2f(sh[j] + dotp<D>(xt, &sy[j * D]) - s);
+      const float* vj = V + (coff + j) * K;
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = fmaf(w, vj[k], acc[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) part[q][r][k] = acc[k];
+  __syncthreads();
+  if (live && q == 0) {
+    const float wi = wr[roff + i];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      out[(roff + i) * K + k] =
+          (((part[0][r][k] + part[1][r][k]) + part[2][r][k]) + part[3][r][k]) * wi;
+  }
+}
+
 // One workgroup per pair: centre[p] = mean of the pair's y points.
 __global__ __launch_bounds__(256) void sk_centre_kernel(const float* __restrict__ y, int dim,
                                                         const SkPair* __restrict__ pairs,
@@ -794,6 +868,96 @@ extern "C" int tm_sinkhorn_points_grad(const float* x, const float* y, const flo
     default: SK_GRAD(4); break;
   }
 #undef SK_GRAD
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+// out = P V (transpose = 0: V is (sum m, k), out (sum n, k)) or P^T V
+// (transpose = 1: V (sum n, k), out (sum m, k)) for every pair, with the plan
+// of the potentials f, g formed on the fly; k in 1..8.  Workspace as
+// tm_sinkhorn_points_grad.
+extern "C" int tm_sinkhorn_points_planmat(const float* x, const float* y, const float* a,
+                                          const float* b, int dim, int npairs, const int32_t* n,
+                                          const int32_t* m, const float* eps, const float* f,
+                                          const float* g, int transpose, int k, const float* V,
+                                          float* out, void* workspace, size_t workspace_bytes,
+                                          void* stream) {
+  using namespace tmh;
+  TM_CHECK_ARG(x && y && a && b && f && g && V && out, "tm_sinkhorn_points_planmat: NULL argument");
+  TM_CHECK_ARG(dim >= 1 && dim <= 4, "tm_sinkhorn_points_planmat: dim must be 1..4");
+  TM_CHECK_ARG(npairs >= 1, "tm_sinkhorn_points_planmat: npairs must be >= 1");
+  TM_CHECK_ARG(k >= 1 && k <= 8, "tm_sinkhorn_points_planmat: k must be 1..8, got %d", k);
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<SkPair> hp(npairs);
+  std::vector<SkTile> htile;
+  int64_t sx = 0, sy = 0;
+  int maxc = 1;
+  for (int p = 0; p < npairs; ++p) {
+    TM_CHECK_ARG(n[p] >= 1 && m[p] >= 1 && n[p] <= kMaxCols && m[p] <= kMaxCols,
+                 "tm_sinkhorn_points_planmat: cloud sizes must be in 1..%d", kMaxCols);
+    TM_CHECK_ARG(eps[p] > 0.f, "tm_sinkhorn_points_planmat: eps must be > 0");
+    hp[p] = SkPair{sx, sy, n[p], m[p], eps[p], 0};
+    sx += n[p];
+    sy += m[p];
+    maxc = n[p] > maxc ? n[p] : maxc;
+    maxc = m[p] > maxc ? m[p] : maxc;
+  }
+  for (int p = 0; p < npairs; ++p) {
+    const int rows = transpose ? m[p] : n[p];
+    for (int r = 0; r < rows; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+  }
+  const int64_t nt = (int64_t)htile.size();
+  const size_t need = tm_sinkhorn_points_grad_workspace_bytes(npairs, n, m);
+  if (!workspace || workspace_bytes < need) {
+    set_error("tm_sinkhorn_points_planmat: workspace of %zu bytes is smaller than the %zu "
+              "required", workspace_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  char* w = (char*)workspace;
+  auto take = [&](size_t bytes) {
+    char* p = w;
+    w += (bytes + 255) & ~(size_t)255;
+    return (void*)p;
+  };
+  SkPair* dpairs = (SkPair*)take(sizeof(SkPair) * npairs);
+  SkTile* dtiles = (SkTile*)take(sizeof(SkTile) * nt);
+  float* lw = (float*)take(sizeof(float) * (sx + sy));
+  float* centre = (float*)take(sizeof(float) * kCtrStride * npairs);
+  float* la = lw;
+  float* lb = lw + sx;
+  TM_CHECK_HIP(hipMemcpyAsync(dpairs, hp.data(), sizeof(SkPair) * npairs, hipMemcpyHostToDevice, st));
+  TM_CHECK_HIP(hipMemcpyAsync(dtiles, htile.data(), sizeof(SkTile) * nt, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(sk_centre_kernel, dim3(npairs), dim3(256), 0, st, y, dim, dpairs, centre);
+  hipLaunchKernelGGL(sk_log2_kernel, dim3(grid_for(sx, 256)), dim3(256), 0, st, a, la, sx);
+  hipLaunchKernelGGL(sk_log2_kernel, dim3(grid_for(sy, 256)), dim3(256), 0, st, b, lb, sy);
+  TM_LAUNCH_CHECK();
+  const size_t lds = (size_t)maxc * (dim + 1) * sizeof(float);
+#define SK_PM(D, K)                                                                            \
+  if (transpose)                                                                               \
+    hipLaunchKernelGGL((sk_planmat_kernel<D, false, K>), dim3((unsigned)nt), dim3(256), lds, st, \
+                       y, x, la, f, g, b, dpairs, dtiles, maxc, centre, V, out);               \
+  else                                                                                         \
+    hipLaunchKernelGGL((sk_planmat_kernel<D, true, K>), dim3((unsigned)nt), dim3(256), lds, st, \
+                       x, y, lb, g, f, a, dpairs, dtiles, maxc, centre, V, out)
+#define SK_PM_K(D)                                                                             \
+  switch (k) {                                                                                 \
+    case 1: SK_PM(D, 1); break;                                                                \
+    case 2: SK_PM(D, 2); break;                                                                \
+    case 3: SK_PM(D, 3); break;                                                                \
+    case 4: SK_PM(D, 4); break;                                                                \
+    case 5: SK_PM(D, 5); break;                                                                \
+    case 6: SK_PM(D, 6); break;                                                                \
+    case 7: SK_PM(D, 7); break;                                                                \
+    default: SK_PM(D, 8); break;                                                               \
+  }
+  switch (dim) {
+    case 1: SK_PM_K(1); break;
+    case 2: SK_PM_K(2); break;
+    case 3: SK_PM_K(3); break;
+    default: SK_PM_K(4); break;
+  }
+#undef SK_PM_K
+#undef SK_PM
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

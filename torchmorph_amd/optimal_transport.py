@@ -438,3 +438,219 @@ class SinkhornPointCloud(torch.autograd.Function):
         gb = g - g.mean()
         return (grad * gx).to(x.dtype), (grad * gy).to(y.dtype), (grad * ga).to(a.dtype), \
             (grad * gb).to(b.dtype), None, None, None
+
+
+# ------------------------------------------------ implicit adjoint (one pair)
+
+
+class PlanOps:
+    """Products with the entropic plan P of one solved pair, P_ij = a_i b_j
+    exp((f_i + g_j - |x_i - y_j|^2) / eps): ``rows(V) = P V`` ((m, k) ->
+    (n, k)) and ``cols(U) = P^T U`` ((n, k) -> (m, k)).  The adjoint below
+    needs nothing else, so the plan is never materialised on the GPU."""
+
+    def rows(self, V: Tensor) -> Tensor:
+        raise NotImplementedError
+
+    def cols(self, U: Tensor) -> Tensor:
+        raise NotImplementedError
+
+
+class HipPlanOps(PlanOps):
+    """PlanOps of GPU float32 tensors (``tm_sinkhorn_points_planmat``: the
+    plan formed on the fly per tile, 8 vectors per launch)."""
+
+    def __init__(self, x, y, a, b, f, g, eps):
+        self.x, self.y = x.detach().float().contiguous(), y.detach().float().contiguous()
+        self.a, self.b = a.detach().float().contiguous(), b.detach().float().contiguous()
+        self.f, self.g = f.detach().float().contiguous(), g.detach().float().contiguous()
+        self.n_arr = (_lib.ctypes.c_int32 * 1)(self.x.shape[0])
+        self.m_arr = (_lib.ctypes.c_int32 * 1)(self.y.shape[0])
+        self.e_arr = (_lib.ctypes.c_float * 1)(float(eps))
+        L = _lib.lib()
+        self.nbytes = L.tm_sinkhorn_points_grad_workspace_bytes(1, self.n_arr, self.m_arr)
+        self.ws = _lib.workspace(self.nbytes, self.x.device)
+
+    def _prod(self, V: Tensor, transpose: int) -> Tensor:
+        V = V.detach().float()
+        rows = self.y.shape[0] if transpose else self.x.shape[0]
+        outs = []
+        L = _lib.lib()
+        with torch.cuda.device(self.x.device):
+            for k0 in range(0, V.shape[1], 8):
+                Vk = V[:, k0:k0 + 8].contiguous()
+                out = torch.empty((rows, Vk.shape[1]), device=V.device, dtype=torch.float32)
+                _lib.check(L.tm_sinkhorn_points_planmat(
+                    _lib.ptr(self.x), _lib.ptr(self.y), _lib.ptr(self.a), _lib.ptr(self.b),
+                    self.x.shape[1], 1, self.n_arr, self.m_arr, self.e_arr, _lib.ptr(self.f),
+                    _lib.ptr(self.g), transpose, Vk.shape[1], _lib.ptr(Vk), _lib.ptr(out),
+                    _lib.ptr(self.ws), self.nbytes, _lib.stream_of(self.x.device)),
+                    "tm_sinkhorn_points_planmat")
+                outs.append(out)
+        return torch.cat(outs, dim=1)
+
+    def rows(self, V):
+        return self._prod(V, 0)
+
+    def cols(self, U):
+        return self._prod(U, 1)
+
+
+def solve_adjoint(ops: PlanOps, a: Tensor, b: Tensor, vf: Tensor, vg: Tensor,
+                  mode: str = "implicit", steps: int = 200, tol: float = 1e-7):
+    """The adjoint (alpha, beta) of the Sinkhorn fixed point for upstream
+    potential gradients (vf, vg), gauge invariant (sum vf = sum vg).
+
+    The fixed point f = T_f(g), g = T_g(f) has Jacobian blocks
+    df/dg = -P / a (rows) and dg/df = -(P / b)^T; with alpha = w_f / a and
+    beta = w_g / b the adjoint system (I - J^T) w = v reads
+        a * alpha + P beta = vf,    P^T alpha + b * beta = vg,
+    a symmetric positive semi-definite system (null space (1, -1)).
+    mode "implicit": conjugate gradients with the Jacobi preconditioner
+    (a, b) to a relative residual `tol` (or `steps` iterations).
+    mode "unrolled": reverse mode through `steps` Sinkhorn iterations
+    recomputed from the converged checkpoint (each recomputed iterate equals
+    the checkpoint, so every step uses the checkpoint's plan): the truncated
+    Neumann series of the implicit solution."""
+    vf = vf.reshape(-1, 1).to(a.dtype)
+    vg = vg.reshape(-1, 1).to(a.dtype)
+    a = a.reshape(-1, 1)
+    b = b.reshape(-1, 1)
+    if mode == "unrolled":
+        alpha = torch.zeros_like(vf)
+        beta = torch.zeros_like(vg)
+        gbar = vg
+        fbar = vf - ops.rows(gbar / b).to(a.dtype)
+        for t in range(steps):
+            beta = beta + gbar / b
+            alpha = alpha + fbar / a
+            gbar = -ops.cols(fbar / a).to(a.dtype)
+            fbar = -ops.rows(gbar / b).to(a.dtype)
+        return alpha.reshape(-1), beta.reshape(-1)
+    if mode != "implicit":
+        raise ValueError("mode must be 'implicit' or 'unrolled'")
+
+    def apply_m(al, be):
+        return a * al + ops.rows(be).to(a.dtype), ops.cols(al).to(a.dtype) + b * be
+
+    al = torch.zeros_like(vf)
+    be = torch.zeros_like(vg)
+    rf, rg = vf.clone(), vg.clone()
+    zf, zg = rf / a, rg / b
+    pf, pg = zf.clone(), zg.clone()
+    rz = (rf * zf).sum() + (rg * zg).sum()
+    r0 = float(torch.sqrt((rf * rf).sum() + (rg * rg).sum()))
+    if r0 == 0.0:
+        return al.reshape(-1), be.reshape(-1)
+    for _ in range(steps):
+        qf, qg = apply_m(pf, pg)
+        pq = (pf * qf).sum() + (pg * qg).sum()
+        if float(pq) <= 0.0:
+            break
+        step = rz / pq
+        al = al + step * pf
+        be = be + step * pg
+        rf = rf - step * qf
+        rg = rg - step * qg
+        if float(torch.sqrt((rf * rf).sum() + (rg * rg).sum())) <= tol * r0:
+            break
+        zf, zg = rf / a, rg / b
+        rz_new = (rf * zf).sum() + (rg * zg).sum()
+        pf = zf + (rz_new / rz) * pf
+        pg = zg + (rz_new / rz) * pg
+        rz = rz_new
+    # the gauge component (1, -1) changes nothing downstream except a constant
+    # in the weight gradients, which are centred; drop it anyway
+    shift = (al.sum() - be.sum()) / (al.numel() + be.numel())
+    return (al - shift).reshape(-1), (be + shift).reshape(-1)
+
+
+def adjoint_param_grads(ops: PlanOps, x: Tensor, y: Tensor, a: Tensor, b: Tensor, eps: float,
+                        alpha: Tensor, beta: Tensor):
+    """Gradients through the fixed point for the adjoint (alpha, beta):
+    dL/dC_ij = P_ij (alpha_i + beta_j), so dL/dx_i = 2 sum_j P_ij (alpha_i +
+    beta_j)(x_i - y_j) (likewise y), dL/da_i = -eps (P beta)_i / a_i and
+    dL/db_j = -eps (P^T alpha)_j / b_j.  Coordinates are taken relative to
+    the y mean (the cost is translation invariant)."""
+    dt = a.dtype
+    ctr = y.mean(dim=0, keepdim=True)
+    xc, yc = (x - ctr).to(dt), (y - ctr).to(dt)
+    al, be = alpha.reshape(-1, 1).to(dt), beta.reshape(-1, 1).to(dt)
+    one_m = torch.ones_like(be)
+    one_n = torch.ones_like(al)
+    R = ops.rows(torch.cat([one_m, yc, be, be * yc], dim=1)).to(dt)  # (n, 2D + 2)
+    Cc = ops.cols(torch.cat([one_n, xc, al, al * xc], dim=1)).to(dt)  # (m, 2D + 2)
+    D = x.shape[1]
+    r0, ry, rb, rby = R[:, :1], R[:, 1:1 + D], R[:, 1 + D:2 + D], R[:, 2 + D:]
+    c0, cx, ca, cax = Cc[:, :1], Cc[:, 1:1 + D], Cc[:, 1 + D:2 + D], Cc[:, 2 + D:]
+    gx = 2 * (xc * (al * r0 + rb) - (al * ry + rby))
+    gy = 2 * (yc * (be * c0 + ca) - (be * cx + cax))
+    ga = -eps * rb.reshape(-1) / a.reshape(-1)
+    gb = -eps * ca.reshape(-1) / b.reshape(-1)
+    return gx, gy, ga, gb
+
+
+class SinkhornPotentials(torch.autograd.Function):
+    """Converged dual potentials (f, g) of one point-cloud pair (cost
+    |x - y|^2, weights a, b summing to 1), differentiable w.r.t. x, y, a, b
+    through the fixed point (no reference counterpart).
+
+    Gauge: f, g are shifted by c = (<g, b> - <f, a>) / 2 so that <f, a> =
+    <g, b>, which makes them unique.  Backward: the adjoint of the fixed
+    point (:func:`solve_adjoint`, ``mode`` "implicit" or "unrolled" with
+    ``steps``), then :func:`adjoint_param_grads`; weight gradients are
+    centred (mass-preserving directions).  Plan-based losses differentiate
+    through P = a b exp((f + g - C) / eps) built from these potentials with
+    torch operations, whose direct terms autograd adds."""
+
+    @staticmethod
+    def forward(ctx, x, y, a, b, eps, tol=1e-6, max_iter=20000, mode="implicit", steps=300):
+        res = sinkhorn_points(x[None], y[None], eps, a[None], b[None], tol=tol,
+                              max_iter=max_iter)
+        f = res.f.to(x.dtype)
+        g = res.g.to(x.dtype)
+        c = ((g * b).sum() - (f * a).sum()) / 2
+        f, g = f + c, g - c
+        ctx.save_for_backward(x, y, a, b, f, g)
+        ctx.eps, ctx.mode, ctx.steps = float(eps), mode, int(steps)
+        return f, g
+
+    @staticmethod
+    def backward(ctx, gf, gg):
+        x, y, a, b, f, g = ctx.saved_tensors
+        gf = torch.zeros_like(f) if gf is None else gf
+        gg = torch.zeros_like(g) if gg is None else gg
+        ops = HipPlanOps(x, y, a, b, f, g, ctx.eps)
+        gx, gy, ga, gb = potentials_backward(ops, x.float(), y.float(), a.float(), b.float(),
+                                             f.float(), g.float(), ctx.eps, gf.float(),
+                                             gg.float(), ctx.mode, ctx.steps)
+        return (gx.to(x.dtype), gy.to(y.dtype), ga.to(a.dtype), gb.to(b.dtype), None, None, None,
+                None, None)
+
+
+def potentials_backward(ops: PlanOps, x, y, a, b, f, g, eps, gf, gg, mode="implicit",
+                        steps=300):
+    """Gradients w.r.t. (x, y, a, b) of a loss of the gauge-fixed converged
+    potentials (f, g) (<f, a> = <g, b>), given its gradients (gf, gg).  The
+    gauge shift c = (<g, b> - <f, a>) / 2 passes s = sum gf - sum gg on:
+    v = (gf - s a / 2, gg + s b / 2) (gauge invariant), plus direct weight
+    terms -s f / 2 and s g / 2; weight gradients are centred."""
+    s = gf.sum() - gg.sum()
+    vf = gf - s * a / 2
+    vg = gg + s * b / 2
+    alpha, beta = solve_adjoint(ops, a, b, vf, vg, mode, steps)
+    gx, gy, ga, gb = adjoint_param_grads(ops, x, y, a, b, eps, alpha, beta)
+    ga = ga - s * f / 2
+    gb = gb + s * g / 2
+    return gx, gy, ga - ga.mean(), gb - gb.mean()
+
+
+def sinkhorn_potentials(x: Tensor, y: Tensor, eps: float, a: Optional[Tensor] = None,
+                        b: Optional[Tensor] = None, tol: float = 1e-6, max_iter: int = 20000,
+                        mode: str = "implicit", steps: int = 300):
+    """Differentiable converged potentials (f, g) of one pair of GPU point
+    clouds; see :class:`SinkhornPotentials`."""
+    n, m = x.shape[0], y.shape[0]
+    a = torch.full((n,), 1.0 / n, device=x.device, dtype=x.dtype) if a is None else a
+    b = torch.full((m,), 1.0 / m, device=x.device, dtype=x.dtype) if b is None else b
+    return SinkhornPotentials.apply(x, y, a, b, eps, tol, max_iter, mode, steps)
