@@ -19,16 +19,18 @@
 // recomputes with an exact per-row maximum.
 //
 // One iteration = f-update (into the other half of a ping-pong buffer, with
-// the marginal error) -> per-pair check -> g-update (in place).  A pair whose
-// error is <= tol (or that reached max_iter) is frozen at the state the error
-// was measured on: its f, g and iteration count are copied out and later
+// the marginal error) -> g-update (in place) with the convergence check of
+// the state before the f-update fused in front of it.  A pair whose error is
+// <= tol (or that reached max_iter) is frozen at the state the error was
+// measured on: its f, g and iteration count are copied out and later
 // launches skip it.  The host replays a captured hipGraph of K iterations and
 // reads the number of active pairs between replays.
 //
-// Work distribution: a device work-list of (pair, first row) tiles of 64 rows
-// per half-step covers ragged pairs in one launch; a workgroup of 256 lanes
-// stages the pair's column points and h_j in LDS and splits each row's sum
-// over 4 lanes.
+// Work distribution: a device work-list of (pair, first row) tiles of 64, 16
+// or 4 rows per half-step covers ragged pairs in one launch; a workgroup of
+// 256 lanes stages the pair's column points and h_j in LDS and splits each
+// row's sum over 256 / rows lanes.  The list is rebuilt for the active pairs
+// as pairs converge.
 #include <vector>
 
 #include "tm_common.h"

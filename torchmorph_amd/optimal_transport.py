@@ -21,6 +21,11 @@ Two entry points:
   tolerance with device-side convergence (``tm_sinkhorn_points``), optional
   warm-start potentials, and a differentiable OT value (gradients from the
   converged plan, see :class:`SinkhornPointCloud`).
+* :func:`sinkhorn_potentials` / :class:`SinkhornPotentials` -- no reference
+  counterpart: the gauge-fixed potentials themselves, differentiable through
+  the implicit adjoint of the fixed point (conjugate gradients) or reverse
+  mode from the converged checkpoint, with every plan product formed on the
+  fly (``tm_sinkhorn_points_planmat``).
 """
 from __future__ import annotations
 
