@@ -208,40 +208,112 @@ __device__ __forceinline__ float grey_epi(float acc, const GreyEpi& e, int64_t p
   return acc;
 }
 
-// Windows of at least kVhgw along an axis are reduced by van Herk /
-// Gil-Werman: the axis is cut into blocks of k, each block gets its suffix
-// and prefix min/max by one serial scan, and the window [c, c + k - 1] is
-// suffix[c] combined with prefix[c + k - 1] (just suffix[c] when c starts a
-// block): three operations per value instead of k, for two more barriers
-// per plane (measured on 4 x 2 x 128^3: a cube of 7 is faster reduced
-// directly, a cube of 15 by the scans).  min/max are exact, so any grouping
-// gives the same values.
+// Windows of at least kVhgw planes along z are reduced by van Herk /
+// Gil-Werman: the planes are cut into blocks of k, each block gets its
+// suffix min/max in place once complete and a running prefix per output
+// voxel, and the window [z, z + k - 1] is suffix[z] combined with
+// prefix[z + k - 1] (just the prefix when z starts a block).  min/max are
+// exact, so any grouping gives the same values.
 constexpr int kVhgw = 10;
 
-// Suffixes and prefixes of the blocks of `len` values of `lines` lines: line
-// l, value j at a[l * ls + j * es].  Suffix j goes to sfx[l * ss + j * es2]
-// for j < nout (the outputs' window starts); the prefix overwrites a in
-// place (only values at j >= k - 1 are read afterwards).
-template <bool MAX>
-__device__ __forceinline__ void vhgw_scan(float* a, int ls, int es, float* sfx, int ss, int es2,
-                                          int lines, int len, int nout, int k) {
-  const int nb = (len + k - 1) / k;
-  const float ID = gid<MAX>();
-  for (int task = threadIdx.x; task < lines * nb; task += blockDim.x) {
-    const int l = task / nb, b = task - l * nb;
-    const int j0 = b * k, j1 = min(len, j0 + k);
-    float* al = a + l * ls;
-    float acc = ID;
-    for (int j = j1 - 1; j >= j0; --j) {
-      acc = gop<MAX>(al[j * es], acc);
-      if (j < nout) sfx[l * ss + j * es2] = acc;
+// x and y: each lane reduces a run of T consecutive outputs of one line from
+// LDS, out[i] = op(v[(i + j) es], j < k).  For k >= T the windows share the
+// middle M = v[T-1 .. k-1]; window i is S_i op M op P_i with the suffix
+// S_i = v[i .. T-2] and the prefix P_i = v[k .. k+i-1], so T outputs take
+// k + 3 T - 4 operations and k + T - 1 loads, all independent of the other
+// lanes (no scans, no extra barriers).  For k < T the k + T - 1 values are
+// held in registers and each window is folded directly.
+template <bool MAX, int T>
+__device__ __forceinline__ void win_run(const float* v, int es, int k, float (&out)[T]) {
+  if (k >= T) {
+    float lo[T - 1], hi[T - 1];
+#pragma unroll
+    for (int i = 0; i < T - 1; ++i) {
+      lo[i] = v[i * es];
+      hi[i] = v[(k + i) * es];
     }
-    acc = ID;
-    for (int j = j0; j < j1; ++j) {
-      acc = gop<MAX>(acc, al[j * es]);
-      al[j * es] = acc;
+    float m = v[(T - 1) * es];
+    for (int j = T; j < k; ++j) m = gop<MAX>(m, v[j * es]);
+    // out[i] = S_i op M op P_i: S from the right, P from the left
+    float sfx = lo[T - 2];
+    out[T - 2] = gop<MAX>(sfx, m);
+#pragma unroll
+    for (int i = T - 3; i >= 0; --i) {
+      sfx = gop<MAX>(lo[i], sfx);
+      out[i] = gop<MAX>(sfx, m);
+    }
+    float pre = hi[0];
+    out[T - 1] = m;
+#pragma unroll
+    for (int i = 1; i < T; ++i) {
+      if (i > 1) pre = gop<MAX>(pre, hi[i - 1]);
+      out[i] = gop<MAX>(out[i], pre);
+    }
+  } else {
+    float w[2 * T - 1];
+#pragma unroll
+    for (int j = 0; j < 2 * T - 1; ++j) w[j] = j < T + k - 1 ? v[j * es] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+      float acc = w[i];
+#pragma unroll
+      for (int j = 1; j < T; ++j)
+        if (j < k) acc = gop<MAX>(acc, w[i + j]);
+      out[i] = acc;
     }
   }
+}
+
+// Runs per line of the x and y passes: 8 outputs along x (8 runs per
+// 64-wide row, rows in sweeps of 32 lanes), 4 along y (4 runs per 16-high
+// column, 64 columns).  Windows shorter than a run are folded per output,
+// one output per lane at a time (consecutive lanes on consecutive LDS words,
+// which measured faster than runs for such windows).
+constexpr int kRunX = 8, kRunY = 4;
+
+// x pass of one staged plane: rows of SX values (pitch SX) to rows of kTX
+// window results (pitch kTX).
+template <bool MAX>
+__device__ __forceinline__ void xpass(const float* stage, int SX, int SY, int kx, float* xr) {
+  if (kx < kRunX) {
+    for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
+      const int r = e >> 6, c = e & 63;
+      const float* s = stage + r * SX + c;
+      float acc = s[0];
+      for (int l = 1; l < kx; ++l) acc = gop<MAX>(acc, s[l]);
+      xr[e] = acc;
+    }
+    return;
+  }
+  const int g = threadIdx.x >> 5, r0 = threadIdx.x & 31;  // run, first row
+  for (int r = r0; r < SY; r += 32) {
+    float o[kRunX];
+    win_run<MAX, kRunX>(stage + r * SX + g * kRunX, 1, kx, o);
+#pragma unroll
+    for (int i = 0; i < kRunX; ++i) xr[r * kTX + g * kRunX + i] = o[i];
+  }
+}
+
+// y pass: SY rows of kTX x-results to kTY rows of window results.  Lane
+// (g, c) writes rows kRunY g .. kRunY g + kRunY - 1 of column c: the outputs
+// it later folds along z, so a z ring slot is only ever touched by its lane.
+template <bool MAX>
+__device__ __forceinline__ void ypass(const float* xr, int ky, float* slot) {
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const float* col = xr + g * kRunY * kTX + c;
+  if (ky < kRunY) {
+#pragma unroll
+    for (int i = 0; i < kRunY; ++i) {
+      float acc = col[i * kTX];
+      for (int j = 1; j < ky; ++j) acc = gop<MAX>(acc, col[(i + j) * kTX]);
+      slot[(g * kRunY + i) * kTX + c] = acc;
+    }
+    return;
+  }
+  float o[kRunY];
+  win_run<MAX, kRunY>(col, kTX, ky, o);
+#pragma unroll
+  for (int i = 0; i < kRunY; ++i) slot[(g * kRunY + i) * kTX + c] = o[i];
 }
 
 // One (x, y) tile of one item over output planes [zb, ze).
@@ -260,7 +332,7 @@ __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, MAX ? 0.0f : -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
   const PlaneMap<NR> pm = plane_map<NR>(g, y0 + sy, x0 + sx);
-  const bool vx = kx >= kVhgw, vy = ky >= kVhgw, vz = kz >= kVhgw;
+  const bool vz = kz >= kVhgw;
   // z by blocks of kz planes: a running prefix per output voxel (this
   // thread's four), and the ring holding the previous block's suffixes
   float zpre[kTY * kTX / 256];
@@ -272,45 +344,18 @@ __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
     __syncthreads();
     // the next plane's loads are in flight while this one is reduced
     if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
-    if (vx) {
-      vhgw_scan<MAX>(stage, SX, 1, xr, kTX, 1, SY, SX, kTX, kx);
-      __syncthreads();
-      for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
-        if (c % kx) xr[e] = gop<MAX>(xr[e], stage[r * SX + c + kx - 1]);
-      }
-    } else {
-      for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
-        const float* s = stage + r * SX + c;
-        float acc = ID;
-        for (int l = 0; l < kx; ++l) acc = gop<MAX>(acc, s[l]);
-        xr[e] = acc;
-      }
-    }
+    xpass<MAX>(stage, SX, SY, kx, xr);
     __syncthreads();
     float* slot = ring + (n % kz) * (kTY * kTX);
-    if (vy) {
-      vhgw_scan<MAX>(xr, 1, kTX, slot, 1, kTX, kTX, SY, kTY, ky);
-      __syncthreads();
-      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
-        if (r % ky) slot[e] = gop<MAX>(slot[e], xr[(r + ky - 1) * kTX + c]);
-      }
-    } else {
-      for (int e = threadIdx.x; e < kTY * kTX; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
-        float acc = ID;
-        for (int j = 0; j < ky; ++j) acc = gop<MAX>(acc, xr[(r + j) * kTX + c]);
-        slot[e] = acc;
-      }
-    }
-    __syncthreads();
+    ypass<MAX>(xr, ky, slot);
+    // no barrier: this lane's z fold reads only the ring entries its own
+    // y pass wrote (and the next plane's stage store waits for the barrier
+    // after its x pass, which every lane reaches after this y pass)
     const int z = zb + n - (kz - 1);
     const int pos = n % kz;  // place of plane n in its block of kz
 #pragma unroll
-    for (int q = 0; q < kTY * kTX / 256; ++q) {
-      const int e = q * 256 + (int)threadIdx.x;
+    for (int q = 0; q < kRunY; ++q) {
+      const int e = ((threadIdx.x >> 6) * kRunY + q) * kTX + (threadIdx.x & 63);
       float acc;
       if (vz) {
         const float v = ring[pos * (kTY * kTX) + e];
