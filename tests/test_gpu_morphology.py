@@ -111,6 +111,20 @@ def test_grey_large_box_uses_fallback(tm, oracle_mod):
     _grey_vs_oracle(tm, oracle_mod, x, False, size=(3, 25, 5))
 
 
+@pytest.mark.parametrize("mode", ["reflect", "constant", "wrap"])
+def test_grey_halo_tiles_beyond_register_staging(tm, oracle_mod, mode):
+    """Halo tiles of more than 16 x 256 values: the elements past the lanes'
+    registers are staged directly (flat box and sparse table, 2-D and 3-D)."""
+    rng = np.random.default_rng(4)
+    x = rng.random((1, 2, 5, 50, 140)).astype(np.float32)
+    _grey_vs_oracle(tm, oracle_mod, x, True, size=(1, 33, 65), mode=mode, cval=0.25)
+    _grey_vs_oracle(tm, oracle_mod, x, False, size=(2, 30, 80), origin=(0, 3, -5), mode=mode,
+                    cval=0.25)
+    fp = np.zeros((1, 31, 101), bool)
+    fp[0, rng.integers(0, 31, 120), rng.integers(0, 101, 120)] = True
+    _grey_vs_oracle(tm, oracle_mod, x, True, footprint=fp, mode=mode, cval=0.25)
+
+
 def test_grey_signed_zero_follows_reference(tm, oracle_mod):
     x = np.zeros((1, 1, 6, 6), np.float32)
     x[0, 0, ::2] = -0.0

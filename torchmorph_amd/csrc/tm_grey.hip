@@ -68,15 +68,17 @@ __device__ __forceinline__ float gid() {
 }
 
 constexpr int kTX = 64, kTY = 16;
-// Halo rows a thread keeps in registers (4 rows per pass): NR = 6 covers
-// structuring elements up to 9 high (more waves per SIMD), NR = 12 up to 33.
-constexpr int kPlaneRowsSmall = 6, kPlaneRowsLarge = 12;
+// Halo tile elements a lane keeps in registers per plane: NE = 8 covers
+// tiles of up to 2048 values (structuring elements up to about 9 x 9 in y,
+// x: more waves per SIMD), NE = 16 up to 4096; elements beyond are staged
+// directly (large elements).
+constexpr int kPlaneElemsSmall = 8, kPlaneElemsLarge = 16;
 
 // One mode-extended input plane halo tile [SY][SX] starting at (y0, x0),
-// distributed over the workgroup as 4 rows x 64 columns per pass: thread
-// (rr, cc) holds rows rr + 4k and columns cc, cc + 64 (coalesced along x, no
-// divisions).  Rows beyond 4 * NR are staged directly (large SEs).
-// The registers of plane n + 1 are filled while plane n is reduced.
+// flattened over the workgroup: lane t holds elements e = t + 256 i, so the
+// loads of a row are coalesced and the LDS stores are consecutive words.
+// Two planes are in flight: the registers of plane n + 2 are filled while
+// planes n and n + 1 are reduced.
 // `bias` is added to every value: +0.0f for flat dilation (the reference
 // adds its zero structure, which turns -0.0 into +0.0), -0.0f otherwise (an
 // exact identity).
@@ -94,83 +96,76 @@ __device__ __forceinline__ float plane_value(const float* __restrict__ vol, cons
   return ((yc || xc) ? g.cval : vol[(zm * g.H + ym) * g.W + xm]) + g.bias;
 }
 
-// The rows and columns a thread loads are the same for every plane of its
-// tile: their mode-mapped coordinates (-1 = constant border) are computed
-// once per workgroup.
-template <int NR>
+// The in-plane offsets a lane loads are the same for every plane of its
+// tile: mode-mapped once per workgroup (-1 = constant border, or past the
+// tile's last element).  Plane offsets fit 32 bits (H W < 2^31, checked by
+// the host).
+template <int NE>
 struct PlaneMap {
-  int32_t ym[NR];
-  int32_t xm[2];
+  int32_t off[NE];
+  int ne;  // register elements in use, ceil(SY SX / 256) capped at NE (uniform)
 };
 
-template <int NR>
-__device__ __forceinline__ PlaneMap<NR> plane_map(const PlaneGeom& g, int y0, int x0) {
-  PlaneMap<NR> m;
-  const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
+template <int NE>
+__device__ __forceinline__ PlaneMap<NE> plane_map(const PlaneGeom& g, int y0, int x0) {
+  PlaneMap<NE> m;
+  const int n = g.SY * g.SX;
+  m.ne = min(NE, (n + 255) >> 8);
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    bool c;
-    const int64_t v = gmap(y0 + rr + 4 * k, g.H, g.mode, &c);
-    m.ym[k] = c ? -1 : (int32_t)v;
-  }
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    bool c;
-    const int64_t v = gmap(x0 + cc + 64 * h, g.W, g.mode, &c);
-    m.xm[h] = c ? -1 : (int32_t)v;
+  for (int i = 0; i < NE; ++i) {
+    const int e = (int)threadIdx.x + 256 * i;
+    int32_t o = -1;
+    if (i < m.ne && e < n) {
+      const int r = e / g.SX, c = e - r * g.SX;
+      bool yc, xc;
+      const int64_t ym = gmap(y0 + r, g.H, g.mode, &yc);
+      const int64_t xm = gmap(x0 + c, g.W, g.mode, &xc);
+      o = (yc || xc) ? -1 : (int32_t)(ym * g.W + xm);
+    }
+    m.off[i] = o;
   }
   return m;
 }
 
 // Raw loads of one plane's halo tile, in flight until plane_store: every
-// load has an in-range address (border positions read a clamped one), and
-// the border substitution and bias are applied at the store, so nothing
-// consumes the loaded registers early and the loads overlap the reduction
-// of the previous plane.
-template <int NR>
+// load has an in-range address (border positions read offset 0), and the
+// border substitution and bias are applied at the store, so nothing
+// consumes the loaded registers early.
+template <int NE>
 struct PlaneRegs {
-  float v[NR][2];
+  float v[NE];
   bool zc;  // the plane lies in the constant border
 };
 
-template <int NR>
-__device__ __forceinline__ void plane_fetch(PlaneRegs<NR>& R, const float* __restrict__ vol,
-                                            const PlaneGeom& g, const PlaneMap<NR>& m, int zi) {
+template <int NE>
+__device__ __forceinline__ void plane_fetch(PlaneRegs<NE>& R, const float* __restrict__ vol,
+                                            const PlaneGeom& g, const PlaneMap<NE>& m, int zi) {
   bool zc;
   const int64_t zm = gmap(zi, g.D, g.mode, &zc);
   R.zc = zc;
   const float* pl = vol + zm * g.H * g.W;
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int64_t row = (int64_t)(m.ym[k] < 0 ? 0 : m.ym[k]) * g.W;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) R.v[k][h] = pl[row + (m.xm[h] < 0 ? 0 : m.xm[h])];
-  }
+  for (int i = 0; i < NE; ++i)
+    if (i < m.ne) R.v[i] = pl[m.off[i] < 0 ? 0 : m.off[i]];
 }
 
-template <int NR>
-__device__ __forceinline__ void plane_store(const PlaneRegs<NR>& R, const PlaneMap<NR>& m,
+template <int NE>
+__device__ __forceinline__ void plane_store(const PlaneRegs<NE>& R, const PlaneMap<NE>& m,
                                             float* stage, const float* __restrict__ vol,
                                             const PlaneGeom& g, int zi, int y0, int x0) {
-  const int rr = threadIdx.x >> 6, cc = threadIdx.x & 63;
+  const int n = g.SY * g.SX;
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int r = rr + 4 * k;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int c = cc + 64 * h;
-      if (r < g.SY && c < g.SX) {
-        const bool border = R.zc || m.ym[k] < 0 || m.xm[h] < 0;
-        stage[r * g.SX + c] = (border ? g.cval : R.v[k][h]) + g.bias;
-      }
-    }
+  for (int i = 0; i < NE; ++i) {
+    const int e = (int)threadIdx.x + 256 * i;
+    if (i < m.ne && e < n) stage[e] = (R.zc || m.off[i] < 0 ? g.cval : R.v[i]) + g.bias;
   }
-  if (g.SY > 4 * NR) {
+  if (n > 256 * NE) {
     bool zc;
     const int64_t zm = gmap(zi, g.D, g.mode, &zc);
-    for (int r = 4 * NR + rr; r < g.SY; r += 4)
-      for (int c = cc; c < g.SX; c += 64)
-        stage[r * g.SX + c] = plane_value(vol, g, zm, zc, y0 + r, x0 + c);
+    for (int e = 256 * NE + (int)threadIdx.x; e < n; e += 256) {
+      const int r = e / g.SX, c = e - r * g.SX;
+      stage[e] = plane_value(vol, g, zm, zc, y0 + r, x0 + c);
+    }
   }
 }
 
@@ -270,9 +265,12 @@ __device__ __forceinline__ void win_run(const float* v, int es, int k, float (&o
 // one output per lane at a time (consecutive lanes on consecutive LDS words,
 // which measured faster than runs for such windows).
 constexpr int kRunX = 8, kRunY = 4;
+// Pitch of the x-pass results: one word more than a row, so the 32 lanes of
+// a run position that store rows r .. r + 31 hit 32 different LDS banks.
+constexpr int kXP = kTX + 1;
 
 // x pass of one staged plane: rows of SX values (pitch SX) to rows of kTX
-// window results (pitch kTX).
+// window results (pitch kXP).
 template <bool MAX>
 __device__ __forceinline__ void xpass(const float* stage, int SX, int SY, int kx, float* xr) {
   if (kx < kRunX) {
@@ -281,7 +279,7 @@ __device__ __forceinline__ void xpass(const float* stage, int SX, int SY, int kx
       const float* s = stage + r * SX + c;
       float acc = s[0];
       for (int l = 1; l < kx; ++l) acc = gop<MAX>(acc, s[l]);
-      xr[e] = acc;
+      xr[r * kXP + c] = acc;
     }
     return;
   }
@@ -290,7 +288,7 @@ __device__ __forceinline__ void xpass(const float* stage, int SX, int SY, int kx
     float o[kRunX];
     win_run<MAX, kRunX>(stage + r * SX + g * kRunX, 1, kx, o);
 #pragma unroll
-    for (int i = 0; i < kRunX; ++i) xr[r * kTX + g * kRunX + i] = o[i];
+    for (int i = 0; i < kRunX; ++i) xr[r * kXP + g * kRunX + i] = o[i];
   }
 }
 
@@ -300,18 +298,18 @@ __device__ __forceinline__ void xpass(const float* stage, int SX, int SY, int kx
 template <bool MAX>
 __device__ __forceinline__ void ypass(const float* xr, int ky, float* slot) {
   const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const float* col = xr + g * kRunY * kTX + c;
+  const float* col = xr + g * kRunY * kXP + c;
   if (ky < kRunY) {
 #pragma unroll
     for (int i = 0; i < kRunY; ++i) {
-      float acc = col[i * kTX];
-      for (int j = 1; j < ky; ++j) acc = gop<MAX>(acc, col[(i + j) * kTX]);
+      float acc = col[i * kXP];
+      for (int j = 1; j < ky; ++j) acc = gop<MAX>(acc, col[(i + j) * kXP]);
       slot[(g * kRunY + i) * kTX + c] = acc;
     }
     return;
   }
   float o[kRunY];
-  win_run<MAX, kRunY>(col, kTX, ky, o);
+  win_run<MAX, kRunY>(col, kXP, ky, o);
 #pragma unroll
   for (int i = 0; i < kRunY; ++i) slot[(g * kRunY + i) * kTX + c] = o[i];
 }
@@ -326,7 +324,7 @@ __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
   const int SX = kTX + kx - 1, SY = kTY + ky - 1;
   float* stage = sm;
   float* xr = stage + SY * SX;
-  float* ring = xr + SY * kTX;
+  float* ring = xr + SY * kXP;
   const int x0 = tx * kTX, y0 = ty * kTY;
   const float ID = gid<MAX>();
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, MAX ? 0.0f : -0.0f};
@@ -336,14 +334,12 @@ __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
   // z by blocks of kz planes: a running prefix per output voxel (this
   // thread's four), and the ring holding the previous block's suffixes
   float zpre[kTY * kTX / 256];
-  PlaneRegs<NR> R;
-  plane_fetch(R, vol, g, pm, zb + sz);
-  for (int n = 0; n < nplanes; ++n) {
+  auto step = [&](PlaneRegs<NR>& R, int n) {
     const int zi = zb + sz + n;
     plane_store(R, pm, stage, vol, g, zi, y0 + sy, x0 + sx);
     __syncthreads();
-    // the next plane's loads are in flight while this one is reduced
-    if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
+    // plane n + 2's loads are in flight while planes n and n + 1 are reduced
+    if (n + 2 < nplanes) plane_fetch(R, vol, g, pm, zi + 2);
     xpass<MAX>(stage, SX, SY, kx, xr);
     __syncthreads();
     float* slot = ring + (n % kz) * (kTY * kTX);
@@ -380,6 +376,13 @@ __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
         ovol[p] = grey_epi<EPI>(acc, epi, p);
       }
     }
+  };
+  PlaneRegs<NR> RA, RB;
+  plane_fetch(RA, vol, g, pm, zb + sz);
+  if (nplanes > 1) plane_fetch(RB, vol, g, pm, zb + sz + 1);
+  for (int n = 0; n < nplanes; n += 2) {
+    step(RA, n);
+    if (n + 1 < nplanes) step(RB, n + 1);
   }
 }
 
@@ -424,13 +427,11 @@ __device__ __forceinline__ void grey_box3_dual_tile(const float* __restrict__ vo
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
   const PlaneMap<NR> pm = plane_map<NR>(g, y0 + sy, x0 + sx);
-  PlaneRegs<NR> R;
-  plane_fetch(R, vol, g, pm, zb + sz);
-  for (int n = 0; n < nplanes; ++n) {
+  auto step = [&](PlaneRegs<NR>& R, int n) {
     const int zi = zb + sz + n;
     plane_store(R, pm, stage, vol, g, zi, y0 + sy, x0 + sx);
     __syncthreads();
-    if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
+    if (n + 2 < nplanes) plane_fetch(R, vol, g, pm, zi + 2);
     for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
       const int r = e >> 6, c = e & 63;
       const float* s = stage + r * SX + c;
@@ -471,6 +472,13 @@ __device__ __forceinline__ void grey_box3_dual_tile(const float* __restrict__ vo
         }
       }
     }
+  };
+  PlaneRegs<NR> RA, RB;
+  plane_fetch(RA, vol, g, pm, zb + sz);
+  if (nplanes > 1) plane_fetch(RB, vol, g, pm, zb + sz + 1);
+  for (int n = 0; n < nplanes; n += 2) {
+    step(RA, n);
+    if (n + 1 < nplanes) step(RB, n + 1);
   }
 }
 
@@ -519,9 +527,7 @@ __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
   const PlaneGeom g{D, H, W, SY, SX, mode, cval, -0.0f};
   const int nplanes = (ze - zb) + kz - 1;
   const PlaneMap<NR> pm = plane_map<NR>(g, y0 + ylo, x0 + xlo);
-  PlaneRegs<NR> R;
-  plane_fetch(R, vol, g, pm, zb + zlo);
-  for (int n = 0; n < nplanes; ++n) {
+  auto step = [&](PlaneRegs<NR>& R, int n) {
     const int zi = zb + zlo + n;
     plane_store(R, pm, sm + (n % kz) * plane, vol, g, zi, y0 + ylo, x0 + xlo);
     // plane of offset dz sits at ring index (n - (zhi - dz)) mod kz; the
@@ -539,7 +545,7 @@ __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
       }
     }
     __syncthreads();
-    if (n + 1 < nplanes) plane_fetch(R, vol, g, pm, zi + 1);
+    if (n + 2 < nplanes) plane_fetch(R, vol, g, pm, zi + 2);
     const int z = zb + n - (kz - 1);
     if (z >= zb) {
       // four adjacent outputs of one row per lane: each element's offset and
@@ -573,6 +579,13 @@ __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
       }
     }
     __syncthreads();
+  };
+  PlaneRegs<NR> RA, RB;
+  plane_fetch(RA, vol, g, pm, zb + zlo);
+  if (nplanes > 1) plane_fetch(RB, vol, g, pm, zb + zlo + 1);
+  for (int n = 0; n < nplanes; n += 2) {
+    step(RA, n);
+    if (n + 1 < nplanes) step(RB, n + 1);
   }
 }
 
@@ -708,7 +721,7 @@ __global__ __launch_bounds__(256) void grey_gather_kernel(const float* __restric
 
 static size_t box3_lds(int kz, int ky, int kx) {
   const size_t SX = kTX + kx - 1, SY = kTY + ky - 1;
-  return (SY * SX + SY * kTX + (size_t)kz * kTY * kTX) * sizeof(float);
+  return (SY * SX + SY * kXP + (size_t)kz * kTY * kTX) * sizeof(float);
 }
 
 static size_t table3_lds(int kz, int ky, int kx) {
@@ -810,6 +823,15 @@ static bool grey_table3_fits(const GreyElems& E, const Grey3& g) {
   for (int j = 0; j < 3; ++j) ok = ok && g.lo[j] >= -128 && g.hi[j] <= 127;
   return ok && table3_lds(g.kz, g.ky, g.kx) <= kLdsBudget;
 }
+
+// Register size class of a streaming launch: the halo tile of a plane,
+// (kTY + ky - 1) x (kTX + kx - 1) values, in at most 8 per lane.
+static bool plane_small(int ky, int kx) {
+  return (int64_t)(kTY + ky - 1) * (kTX + kx - 1) <= 256 * kPlaneElemsSmall;
+}
+
+// The streaming kernels address a plane with 32-bit offsets.
+static bool plane32(int64_t H, int64_t W) { return H * W < ((int64_t)1 << 31); }
 
 static GreyTable grey_table3_of(int ndim, const GreyElems& E) {
   GreyTable tab{};
@@ -919,7 +941,7 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
     };
     auto zchunk_of = [&](int v) { return std::min(D, 8 << v); };
     auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
-    if (grey_box3_fits(E, g3)) {
+    if (grey_box3_fits(E, g3) && plane32(H, W)) {
       const size_t lds = box3_lds(kz, ky, kx);
       auto run = [&](int v) -> int {
         const int zchunk = zchunk_of(v);
@@ -928,16 +950,16 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
           set_error("tm_grey: batch too large for one launch");
           return TM_ERR_ARG;
         }
-        const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
+        const bool small_se = plane_small(ky, kx);
 #define TM_BOX(MX, NR)                                                                         \
   launch_box3<MX, NR>(epi, (unsigned)G, lds, st, input, output, D, H, W, kz, ky, kx, (int)lo3[0], \
                       (int)lo3[1], (int)lo3[2], mode, cval, zchunk, ge)
         if (is_max) {
-          if (small_se) TM_BOX(true, kPlaneRowsSmall);
-          else TM_BOX(true, kPlaneRowsLarge);
+          if (small_se) TM_BOX(true, kPlaneElemsSmall);
+          else TM_BOX(true, kPlaneElemsLarge);
         } else {
-          if (small_se) TM_BOX(false, kPlaneRowsSmall);
-          else TM_BOX(false, kPlaneRowsLarge);
+          if (small_se) TM_BOX(false, kPlaneElemsSmall);
+          else TM_BOX(false, kPlaneElemsLarge);
         }
 #undef TM_BOX
         TM_LAUNCH_CHECK();
@@ -948,7 +970,7 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
       const int v = tune_pick("grey3", key, 8, 4, 1, run, st, &ran);
       return ran ? TM_OK : run(v);
     }
-    if (grey_table3_fits(E, g3)) {
+    if (grey_table3_fits(E, g3) && plane32(H, W)) {
       const GreyTable tab = grey_table3_of(ndim, E);
       const size_t lds = table3_lds(kz, ky, kx);
       auto run = [&](int v) -> int {
@@ -958,16 +980,16 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
           set_error("tm_grey: batch too large for one launch");
           return TM_ERR_ARG;
         }
-        const bool small_se = kTY + (int)(hi3[1] - lo3[1]) <= 4 * kPlaneRowsSmall;
+        const bool small_se = plane_small(ky, kx);
 #define TM_TAB(MX, NR)                                                                        \
   launch_table3<MX, NR>(epi, (unsigned)G, lds, st, input, output, D, H, W, tab, g3, mode, cval, \
                         zchunk, ge)
         if (is_max) {
-          if (small_se) TM_TAB(true, kPlaneRowsSmall);
-          else TM_TAB(true, kPlaneRowsLarge);
+          if (small_se) TM_TAB(true, kPlaneElemsSmall);
+          else TM_TAB(true, kPlaneElemsLarge);
         } else {
-          if (small_se) TM_TAB(false, kPlaneRowsSmall);
-          else TM_TAB(false, kPlaneRowsLarge);
+          if (small_se) TM_TAB(false, kPlaneElemsSmall);
+          else TM_TAB(false, kPlaneElemsLarge);
         }
 #undef TM_TAB
         TM_LAUNCH_CHECK();
@@ -981,7 +1003,8 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
   }
   TM_CHECK_ARG(epi == 0,
                "tm_grey_ex: a composite epilogue needs items of at most 3 axes and a "
-               "structuring element of the streaming kernels (tm_grey_fused_ok)");
+               "structuring element of the streaming kernels (tm_grey_fused_ok), and planes of "
+               "at most 2^31 - 1 voxels");
   const size_t need = tm_grey_workspace_bytes(batch, ndim, shape, se_shape);
   if (!workspace || workspace_bytes < need) {
     set_error("tm_grey: workspace of %zu bytes is smaller than the %zu required",
@@ -1082,10 +1105,11 @@ static int grey_dual_run(const float* input, int64_t batch, int ndim, const int6
   int64_t s3[3] = {1, 1, 1};
   for (int j = 0; j < ndim; ++j) s3[3 - ndim + j] = shape[j];
   const int D = (int)s3[0], H = (int)s3[1], W = (int)s3[2];
+  TM_CHECK_ARG(plane32(H, W), "tm_grey_ex: planes of at most 2^31 - 1 voxels");
   const int kz = g3.kz, ky = g3.ky, kx = g3.kx;
   const int64_t ntiles = ((W + kTX - 1) / kTX) * (int64_t)((H + kTY - 1) / kTY);
   const size_t lds = box3_dual_lds(kz, ky, kx);
-  const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
+  const bool small_se = plane_small(ky, kx);
   auto run = [&](int v) -> int {
     const int zchunk = std::min(D, 8 << v);
     const int64_t G = ntiles * ((D + zchunk - 1) / zchunk) * batch;
@@ -1098,11 +1122,11 @@ static int grey_dual_run(const float* input, int64_t batch, int ndim, const int6
                      output, D, H, W, kz, ky, kx, (int)g3.lo[0], (int)g3.lo[1], (int)g3.lo[2],      \
                      mode, cval, zchunk)
     if (comb == 0) {
-      if (small_se) TM_DUAL(kPlaneRowsSmall, 0);
-      else TM_DUAL(kPlaneRowsLarge, 0);
+      if (small_se) TM_DUAL(kPlaneElemsSmall, 0);
+      else TM_DUAL(kPlaneElemsLarge, 0);
     } else {
-      if (small_se) TM_DUAL(kPlaneRowsSmall, 1);
-      else TM_DUAL(kPlaneRowsLarge, 1);
+      if (small_se) TM_DUAL(kPlaneElemsSmall, 1);
+      else TM_DUAL(kPlaneElemsLarge, 1);
     }
 #undef TM_DUAL
     TM_LAUNCH_CHECK();
@@ -1189,6 +1213,7 @@ static int grey_rg_plan(const tm_ragged_job* jobs, int64_t njobs, GreyRgPlan& p)
     it.D = (int)s3[0];
     it.H = (int)s3[1];
     it.W = (int)s3[2];
+    TM_CHECK_ARG(plane32(it.H, it.W), "tm_ragged_grey: planes of at most 2^31 - 1 voxels");
     const int64_t ntx = (s3[2] + kTX - 1) / kTX, nty = (s3[1] + kTY - 1) / kTY;
     const int64_t nz = (s3[0] + kRgZChunk - 1) / kRgZChunk;
     TM_CHECK_ARG(ntx * nty * nz < ((int64_t)1 << 31) - blocks,
@@ -1267,33 +1292,33 @@ extern "C" int tm_ragged_grey(const tm_ragged_job* jobs, int64_t njobs, const fl
   const int kz = g3.kz, ky = g3.ky, kx = g3.kx;
   if (box) {
     const size_t lds = box3_lds(kz, ky, kx);
-    const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
+    const bool small_se = plane_small(ky, kx);
 #define TM_BOX(MX, NR)                                                                            \
   hipLaunchKernelGGL((grey_box3_ragged_kernel<MX, NR>), dim3(G), dim3(256), lds, st, input, output, \
                      items, blk_item, kz, ky, kx, (int)g3.lo[0], (int)g3.lo[1], (int)g3.lo[2],      \
                      mode, cval)
     if (is_max) {
-      if (small_se) TM_BOX(true, kPlaneRowsSmall);
-      else TM_BOX(true, kPlaneRowsLarge);
+      if (small_se) TM_BOX(true, kPlaneElemsSmall);
+      else TM_BOX(true, kPlaneElemsLarge);
     } else {
-      if (small_se) TM_BOX(false, kPlaneRowsSmall);
-      else TM_BOX(false, kPlaneRowsLarge);
+      if (small_se) TM_BOX(false, kPlaneElemsSmall);
+      else TM_BOX(false, kPlaneElemsLarge);
     }
 #undef TM_BOX
   } else {
     const GreyTable tab = grey_table3_of(nd, E);
     const size_t lds = table3_lds(kz, ky, kx);
-    const bool small_se = kTY + ky - 1 <= 4 * kPlaneRowsSmall;
+    const bool small_se = plane_small(ky, kx);
 #define TM_TAB(MX, NR)                                                                             \
   hipLaunchKernelGGL((grey_table3_ragged_kernel<MX, NR>), dim3(G), dim3(256), lds, st, input,       \
                      output, items, blk_item, tab, (int)g3.lo[0], (int)g3.hi[0], (int)g3.lo[1],     \
                      (int)g3.hi[1], (int)g3.lo[2], (int)g3.hi[2], mode, cval)
     if (is_max) {
-      if (small_se) TM_TAB(true, kPlaneRowsSmall);
-      else TM_TAB(true, kPlaneRowsLarge);
+      if (small_se) TM_TAB(true, kPlaneElemsSmall);
+      else TM_TAB(true, kPlaneElemsLarge);
     } else {
-      if (small_se) TM_TAB(false, kPlaneRowsSmall);
-      else TM_TAB(false, kPlaneRowsLarge);
+      if (small_se) TM_TAB(false, kPlaneElemsSmall);
+      else TM_TAB(false, kPlaneElemsLarge);
     }
 #undef TM_TAB
   }
