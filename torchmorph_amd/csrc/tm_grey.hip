@@ -144,9 +144,11 @@ __device__ __forceinline__ void plane_fetch(PlaneRegs<NE>& R, const float* __res
   const int64_t zm = gmap(zi, g.D, g.mode, &zc);
   R.zc = zc;
   const float* pl = vol + zm * g.H * g.W;
+  // all NE loads, unconditionally (unused ones read offset 0): a fixed
+  // number of loads per plane lets the compiler wait for exactly the older
+  // plane's loads while the newer plane's stay in flight
 #pragma unroll
-  for (int i = 0; i < NE; ++i)
-    if (i < m.ne) R.v[i] = pl[m.off[i] < 0 ? 0 : m.off[i]];
+  for (int i = 0; i < NE; ++i) R.v[i] = pl[m.off[i] < 0 ? 0 : m.off[i]];
 }
 
 template <int NE>
@@ -339,7 +341,7 @@ __device__ __forceinline__ void grey_box3_tile(const float* __restrict__ vol,
     plane_store(R, pm, stage, vol, g, zi, y0 + sy, x0 + sx);
     __syncthreads();
     // plane n + 2's loads are in flight while planes n and n + 1 are reduced
-    if (n + 2 < nplanes) plane_fetch(R, vol, g, pm, zi + 2);
+    plane_fetch(R, vol, g, pm, zi + 2);  // past the last plane: mapped, unused
     xpass<MAX>(stage, SX, SY, kx, xr);
     __syncthreads();
     float* slot = ring + (n % kz) * (kTY * kTX);
@@ -431,7 +433,7 @@ __device__ __forceinline__ void grey_box3_dual_tile(const float* __restrict__ vo
     const int zi = zb + sz + n;
     plane_store(R, pm, stage, vol, g, zi, y0 + sy, x0 + sx);
     __syncthreads();
-    if (n + 2 < nplanes) plane_fetch(R, vol, g, pm, zi + 2);
+    plane_fetch(R, vol, g, pm, zi + 2);  // past the last plane: mapped, unused
     for (int e = threadIdx.x; e < SY * kTX; e += blockDim.x) {
       const int r = e >> 6, c = e & 63;
       const float* s = stage + r * SX + c;
@@ -545,7 +547,7 @@ __device__ __forceinline__ void grey_table3_tile(const float* __restrict__ vol,
       }
     }
     __syncthreads();
-    if (n + 2 < nplanes) plane_fetch(R, vol, g, pm, zi + 2);
+    plane_fetch(R, vol, g, pm, zi + 2);  // past the last plane: mapped, unused
     const int z = zb + n - (kz - 1);
     if (z >= zb) {
       // four adjacent outputs of one row per lane: each element's offset and
