@@ -291,6 +291,29 @@ def test_points_graph_and_plain_launches_agree(tm, monkeypatch):
     assert torch.equal(a.iterations, b.iterations)
 
 
+def test_points_compacted_schedule_equals_single_pairs(tm):
+    """A batch whose work-list is compacted several times (64-, 16- and
+    4-row tiles as pairs converge, replays pipelined one chunk ahead) gives
+    every pair the bits and iteration count of solving it alone."""
+    rng = np.random.default_rng(8)
+    B = 24
+    sizes = [(int(rng.integers(100, 1025)), int(rng.integers(100, 1025))) for _ in range(B)]
+    fracs = rng.choice([0.01, 0.03, 0.1], size=B)
+    xs, ys, eps = [], [], []
+    for (n, m), fr in zip(sizes, fracs):
+        x, y = _gmm_pair(rng, n, m)
+        xs.append(torch.tensor(x, dtype=torch.float32, device="cuda"))
+        ys.append(torch.tensor(y, dtype=torch.float32, device="cuda"))
+        eps.append(float(fr) * float(torch.cdist(xs[-1], ys[-1]).square().median()))
+    res = tm.sinkhorn_points(xs, ys, torch.tensor(eps), tol=1e-5)
+    assert int(res.iterations.max()) > 4 * int(res.iterations.min())
+    for p in range(B):
+        one = tm.sinkhorn_points([xs[p]], [ys[p]], eps[p], tol=1e-5)
+        assert torch.equal(res.split_f()[p], one.f), p
+        assert torch.equal(res.split_g()[p], one.g), p
+        assert int(res.iterations[p]) == int(one.iterations[0]), p
+
+
 def test_points_warm_start_from_solution_stops_immediately(tm):
     rng = np.random.default_rng(2)
     x, y = _gmm_pair(rng, 120, 110)

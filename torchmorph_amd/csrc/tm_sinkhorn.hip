@@ -31,6 +31,8 @@
 // 256 lanes stages the pair's column points and h_j in LDS and splits each
 // row's sum over 256 / rows lanes.  The list is rebuilt for the active pairs
 // as pairs converge.
+#include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "tm_common.h"
@@ -106,13 +108,29 @@ struct SkCheck {
 // they do not.  The row-marginal error is written per group of 4 rows
 // (summed in row order) and a pair's error is summed over its groups by a
 // fixed tree as well.
-template <int D, bool ROWS_ARE_X, bool CHECK, int RPT>
-__global__ __launch_bounds__(256) void sk_half_kernel(
-    const float* __restrict__ rows, const float* __restrict__ cols,
-    const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ pin,
-    float* __restrict__ pout, const float* __restrict__ wr, const SkPair* __restrict__ pairs,
-    const SkTile* __restrict__ tiles, const int* active, float* __restrict__ errpart,
-    int maxc, const float* __restrict__ centre, SkCheck ck) {
+// The pair's L1 row-marginal error over its ceil(n / 4) error groups, called
+// by lanes 0..63: lane l sums groups l, l + 64, ... in order, then a fixed
+// tree over the lanes; the result is valid in lane 0.
+__device__ __forceinline__ float sk_pair_error(const float* __restrict__ fpart, const SkPair& pr) {
+  const int ng = (pr.n + 3) >> 2;
+  float e = 0.f;
+  for (int t = threadIdx.x; t < ng; t += 64) e += fpart[pr.gbase + t];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) e += __shfl_down(e, off, 64);
+  return e;
+}
+
+// One half-step tile: rows tile.r0 .. tile.r0 + RPT - 1 of the pair (every
+// thread of the workgroup calls it; barriers inside).
+template <int D, bool ROWS_ARE_X, int RPT>
+__device__ __forceinline__ void sk_tile(const SkTile tile, const SkPair& pr,
+                                        const float* __restrict__ rows,
+                                        const float* __restrict__ cols,
+                                        const float* __restrict__ lwc,
+                                        const float* __restrict__ pc,
+                                        const float* __restrict__ pin, float* __restrict__ pout,
+                                        const float* __restrict__ wr, float* __restrict__ errpart,
+                                        int maxc, const float* __restrict__ centre) {
   static_assert(RPT == 64 || RPT == 16 || RPT == 4, "tile rows");
   constexpr int NQ = 256 / RPT;  // lanes per row
   constexpr int GPL = 64 / NQ;   // column groups per lane
@@ -127,47 +145,6 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
   __shared__ float ssum[NQ][RPT];
   __shared__ float serr4[RPT];
   __shared__ int sbad;
-  const SkTile tile = tiles[blockIdx.x];
-  if (!active[tile.pair]) return;  // uniform per workgroup
-  const SkPair pr = pairs[tile.pair];
-  if constexpr (CHECK) {
-    __shared__ int sdone;
-    __shared__ float serr;
-    __shared__ int sk;
-    if (threadIdx.x < 64) {
-      // the pair's error over its ceil(n / 4) groups: lane l sums groups
-      // l, l + 64, ... in order, then a fixed tree over the lanes
-      const int ng = (pr.n + 3) >> 2;
-      float e = 0.f;
-      for (int t = threadIdx.x; t < ng; t += 64) e += ck.fpart[pr.gbase + t];
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) e += __shfl_down(e, off, 64);
-      if (threadIdx.x == 0) {
-        const int k = *ck.iter_base + ck.k_in_chunk;
-        serr = e;
-        sk = k;
-        sdone = (e <= ck.tol || k - 1 >= ck.max_iter) ? 1 : 0;
-      }
-    }
-    __syncthreads();
-    if (sdone) {
-      if (tile.r0 == 0) {
-        for (int i = threadIdx.x; i < pr.n; i += blockDim.x)
-          ck.fout[pr.xoff + i] = ck.fold[pr.xoff + i];
-        for (int j = threadIdx.x; j < pr.m; j += blockDim.x)
-          ck.gout[pr.yoff + j] = pin[pr.yoff + j];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-          ck.iters[tile.pair] = sk - 1;
-          ck.err[tile.pair] = serr;
-          ck.active[tile.pair] = 0;
-          atomicSub(ck.n_active, 1);
-        }
-      }
-      return;
-    }
-    if (tile.r0 == 0 && threadIdx.x == 0) ck.err[tile.pair] = serr;
-  }
   const int64_t roff = ROWS_ARE_X ? pr.xoff : pr.yoff;
   const int64_t coff = ROWS_ARE_X ? pr.yoff : pr.xoff;
   const int nr = ROWS_ARE_X ? pr.n : pr.m;
@@ -292,6 +269,53 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
     }
   }
 }
+
+template <int D, bool ROWS_ARE_X, bool CHECK, int RPT>
+__global__ __launch_bounds__(256) void sk_half_kernel(
+    const float* __restrict__ rows, const float* __restrict__ cols,
+    const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ pin,
+    float* __restrict__ pout, const float* __restrict__ wr, const SkPair* __restrict__ pairs,
+    const SkTile* __restrict__ tiles, const int* active, float* __restrict__ errpart,
+    int maxc, const float* __restrict__ centre, SkCheck ck) {
+  const SkTile tile = tiles[blockIdx.x];
+  if (!active[tile.pair]) return;  // uniform per workgroup
+  const SkPair pr = pairs[tile.pair];
+  if constexpr (CHECK) {
+    __shared__ int sdone;
+    __shared__ float serr;
+    __shared__ int sk;
+    if (threadIdx.x < 64) {
+      const float e = sk_pair_error(ck.fpart, pr);
+      if (threadIdx.x == 0) {
+        const int k = *ck.iter_base + ck.k_in_chunk;
+        serr = e;
+        sk = k;
+        sdone = (e <= ck.tol || k - 1 >= ck.max_iter) ? 1 : 0;
+      }
+    }
+    __syncthreads();
+    if (sdone) {
+      if (tile.r0 == 0) {
+        for (int i = threadIdx.x; i < pr.n; i += blockDim.x)
+          ck.fout[pr.xoff + i] = ck.fold[pr.xoff + i];
+        for (int j = threadIdx.x; j < pr.m; j += blockDim.x)
+          ck.gout[pr.yoff + j] = pin[pr.yoff + j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          ck.iters[tile.pair] = sk - 1;
+          ck.err[tile.pair] = serr;
+          ck.active[tile.pair] = 0;
+          atomicSub(ck.n_active, 1);
+        }
+      }
+      return;
+    }
+    if (tile.r0 == 0 && threadIdx.x == 0) ck.err[tile.pair] = serr;
+  }
+  sk_tile<D, ROWS_ARE_X, RPT>(tile, pr, rows, cols, lwc, pc, pin, pout, wr, errpart, maxc,
+                              centre);
+}
+
 
 
 // Gradient of the OT value <f, a> + <g, b> w.r.t. the row points at the
@@ -479,14 +503,56 @@ __global__ void sk_log2_kernel(const float* __restrict__ w, float* __restrict__ 
   if (i < n) lw[i] = __builtin_amdgcn_logf(w[i]);
 }
 
-__global__ void sk_advance_kernel(int* iter_base, int k) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *iter_base += k;
+// End of a chunk of k iterations: advances the iteration base and writes
+// the active count and the active flags of every pair to host memory
+// (pinned, mapped, fine-grained), where the host reads them once the chunk's
+// event has completed.
+__global__ void sk_chunk_end_kernel(int* iter_base, int k, const int* __restrict__ n_active,
+                                    const int* __restrict__ active, int npairs,
+                                    int* __restrict__ host) {
+  if (threadIdx.x == 0) {
+    *iter_base += k;
+    host[0] = *n_active;
+  }
+  for (int p = threadIdx.x; p < npairs; p += blockDim.x) host[1 + p] = active[p];
 }
 
 __global__ void sk_copy_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
+
+// Host read-back of the iteration loop, one per host thread: (active count,
+// active flags per pair) in pinned, device-mapped, coherent host memory that
+// every chunk's last kernel overwrites, and the events of the last two
+// chunks.  The host reads it after a chunk's event, possibly while the next
+// chunk writes it: the count and the flags only decrease, so any mix of two
+// consecutive chunks' values is still a valid (conservative) state.
+struct SkReadback {
+  int* host = nullptr;
+  int* dev = nullptr;  // the same memory as the device addresses it
+  size_t cap = 0;      // ints
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+inline int sk_readback_acquire(SkReadback& rb, size_t ints) {
+  // a previous call ended with a stream synchronisation: nothing is pending
+  if (rb.cap < ints) {
+    if (rb.host) TM_CHECK_HIP(hipHostFree(rb.host));
+    rb.host = rb.dev = nullptr;
+    rb.cap = 0;
+    TM_CHECK_HIP(hipHostMalloc((void**)&rb.host, ints * sizeof(int),
+                               hipHostMallocMapped | hipHostMallocCoherent));
+    TM_CHECK_HIP(hipHostGetDevicePointer((void**)&rb.dev, rb.host, 0));
+    rb.cap = ints;
+  }
+  for (hipEvent_t& e : rb.ev)
+    if (!e) TM_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return TM_OK;
+}
+
+static thread_local SkReadback g_sk_readback;
+static thread_local PinnedStage g_sk_tiles[2];
 
 }  // namespace tmh
 
@@ -668,6 +734,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   // one chunk = `chunk` iterations (chunk even keeps the f ping-pong aligned);
   // an iteration is an f-update and a g-update with the fused check
   const int K = chunk % 2 ? chunk + 1 : chunk;
+  if (sk_readback_acquire(g_sk_readback, (size_t)npairs + 1) != TM_OK) return TM_ERR_HIP;
   auto enqueue_chunk = [&]() -> int {
     float* fo = fA;
     float* fn = fB;
@@ -678,7 +745,8 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
       fo = fn;
       fn = t;
     }
-    hipLaunchKernelGGL(sk_advance_kernel, dim3(1), dim3(64), 0, lst, iter_base, K);
+    hipLaunchKernelGGL(sk_chunk_end_kernel, dim3(1), dim3(256), 0, lst, iter_base, K, n_active,
+                       active, npairs, g_sk_readback.dev);
     TM_LAUNCH_CHECK();
     return TM_OK;
   };
@@ -695,8 +763,11 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   hipStream_t cap = nullptr;
   bool use_graph = getenv("TORCHMORPH_AMD_NO_GRAPH") == nullptr &&
                    hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) == hipSuccess;
+  // executable graphs replaced by a compaction may still have a replay in
+  // flight: they are destroyed after the final synchronisation
+  std::vector<hipGraphExec_t> retired;
   auto build_graph = [&]() {
-    if (exec) (void)hipGraphExecDestroy(exec);
+    if (exec) retired.push_back(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
@@ -720,29 +791,47 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   // Work-list compaction: once the active pairs fall below 3/4 of the pairs
   // in the list, the tile tables are rebuilt for the active pairs only, with
   // the largest tile shape (64, 16 or 4 rows) that still gives at least
-  // kTailTiles f-tiles, or else the smallest shape whose tiles' column
-  // staging (half_lds each) fits the GPU's LDS at once (kResidentLds): the
-  // long tail of a batch launches only the tiles of its stragglers, spread
-  // over many small tiles (the results do not depend on the tile shape).
+  // kTailTiles tiles per launch, or else the smallest shape whose tiles per
+  // launch the GPU holds at once (256 CUs, each with up to 8 workgroups of
+  // half_lds bytes of column staging in its 160 KB of LDS): the long tail
+  // of a batch launches only the tiles of its stragglers, spread over many
+  // small tiles (the results do not depend on the tile shape).
+  // The new tables go up through pinned stages (two, alternating), ordered
+  // on the stream after the chunk in flight, which still uses the old ones.
   constexpr int64_t kTailTiles = 1024;
-  constexpr int64_t kResidentLds = (int64_t)256 * 128 * 1024;  // 256 CUs x 128 KB
-  std::vector<int> hact(npairs);
+  const int64_t resident =
+      (int64_t)256 * std::max<int64_t>(1, std::min<int64_t>(8, (160 << 10) / (int64_t)half_lds));
   int listed = npairs;
-  auto compact = [&]() -> int {
-    TM_CHECK_HIP(hipMemcpyAsync(hact.data(), active, sizeof(int) * npairs,
-                                hipMemcpyDeviceToHost, st));
-    TM_CHECK_HIP(hipStreamSynchronize(st));
-    int64_t t64 = 0, t16 = 0, t4 = 0;
+  int nstage = 0;
+  auto upload = [&](const void* tiles, size_t tb, const void* firsts, size_t fb) -> int {
+    PinnedStage& ps = g_sk_tiles[nstage++ & 1];
+    int rc = stage_acquire(ps, align256(tb) + fb);
+    if (rc != TM_OK) return rc;
+    memcpy(ps.host, tiles, tb);
+    memcpy((char*)ps.host + align256(tb), firsts, fb);
+    if (tb) TM_CHECK_HIP(hipMemcpyAsync(dtiles, ps.host, tb, hipMemcpyHostToDevice, st));
+    TM_CHECK_HIP(hipMemcpyAsync(dfirst, (char*)ps.host + align256(tb), fb, hipMemcpyHostToDevice,
+                                st));
+    TM_CHECK_HIP(hipEventRecord(ps.ev, st));
+    ps.pending = true;
+    return TM_OK;
+  };
+  auto compact = [&](const int* hact) -> int {
+    // tiles per launch at each shape: the larger of the f and g lists
+    int64_t tf[3] = {0, 0, 0}, tg[3] = {0, 0, 0};
+    const int shapes[3] = {64, 16, 4};
     for (int p = 0; p < npairs; ++p)
-      if (hact[p]) {
-        t64 += (n[p] + 63) / 64 + (m[p] + 63) / 64;
-        t16 += (n[p] + 15) / 16 + (m[p] + 15) / 16;
-        t4 += (n[p] + 3) / 4 + (m[p] + 3) / 4;
-      }
-    const int64_t L = (int64_t)half_lds;
-    if (t64 >= 2 * kTailTiles) rpt = 64;
-    else if (t16 >= 2 * kTailTiles || t4 * L > kResidentLds) rpt = t16 * L > kResidentLds ? 64 : 16;
-    else rpt = 4;
+      if (hact[p])
+        for (int k = 0; k < 3; ++k) {
+          tf[k] += (n[p] + shapes[k] - 1) / shapes[k];
+          tg[k] += (m[p] + shapes[k] - 1) / shapes[k];
+        }
+    const int64_t T64 = std::max(tf[0], tg[0]), T16 = std::max(tf[1], tg[1]),
+                  T4 = std::max(tf[2], tg[2]);
+    if (T64 >= kTailTiles) rpt = 64;
+    else if (T16 >= kTailTiles) rpt = T16 <= resident ? 16 : 64;
+    else if (T4 <= resident) rpt = 4;
+    else rpt = T16 <= resident ? 16 : 64;
     htile.clear();
     int kept = 0;
     for (int p = 0; p < npairs; ++p) {
@@ -757,32 +846,52 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
       if (hact[p])
         for (int r = 0; r < m[p]; r += rpt) htile.push_back(SkTile{p, r});
     nty = (int64_t)htile.size() - ntx;
-    if (!htile.empty())
-      TM_CHECK_HIP(hipMemcpyAsync(dtiles, htile.data(), sizeof(SkTile) * htile.size(),
-                                  hipMemcpyHostToDevice, st));
-    TM_CHECK_HIP(hipMemcpyAsync(dfirst, hfirst.data(), sizeof(int32_t) * (npairs + 1),
-                                hipMemcpyHostToDevice, st));
-    TM_CHECK_HIP(hipStreamSynchronize(st));  // the host tables are reused by the next compaction
+    const int rc = upload(htile.data(), sizeof(SkTile) * htile.size(), hfirst.data(),
+                          sizeof(int32_t) * (npairs + 1));
+    if (rc != TM_OK) return rc;
     listed = kept;
     if (use_graph) use_graph = build_graph();
     return TM_OK;
   };
+  // Pipelined replays: chunk c + 1 is enqueued before the host waits for
+  // chunk c's active count, so the GPU does not idle through the host's
+  // turnaround (or a compaction's capture).  The active flags come back with
+  // the count, so a compaction works from flags one chunk old: pairs that
+  // converged since stay listed until the next compaction and skip all work.
+  // A chunk enqueued after the last pair converged only launches kernels
+  // that return at once.
+  SkReadback& rb = g_sk_readback;
   int status = TM_OK;
-  int host_active = npairs;
-  for (int done = 0; done < max_iter + K && host_active > 0; done += K) {
+  const int64_t max_chunks = ((int64_t)max_iter + K) / K + ((max_iter + K) % K != 0);
+  auto launch_chunk = [&](int slot) -> int {
     if (use_graph) {
-      if (hipGraphLaunch(exec, st) != hipSuccess) { status = TM_ERR_HIP; break; }
-    } else if ((status = enqueue_chunk()) != TM_OK) {
-      break;
+      if (hipGraphLaunch(exec, st) != hipSuccess) return TM_ERR_HIP;
+    } else if (enqueue_chunk() != TM_OK) {
+      return TM_ERR_HIP;
     }
-    if (hipMemcpyAsync(&host_active, n_active, sizeof(int), hipMemcpyDeviceToHost, st) !=
-            hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
+    return hipEventRecord(rb.ev[slot], st) == hipSuccess ? TM_OK : TM_ERR_HIP;
+  };
+  int64_t launched = 0;
+  status = launch_chunk(0);
+  if (status == TM_OK) launched = 1;
+  for (int64_t c = 0; status == TM_OK; ++c) {
+    if (launched < max_chunks) {
+      if ((status = launch_chunk((int)(launched & 1))) != TM_OK) break;
+      ++launched;
+    }
+    if (hipEventSynchronize(rb.ev[c & 1]) != hipSuccess) {
       status = TM_ERR_HIP;
       break;
     }
-    if (host_active > 0 && 4 * host_active < 3 * listed && (status = compact()) != TM_OK) break;
+    const int* h = rb.host;
+    const int host_active = h[0];
+    if (host_active == 0 || c + 1 >= launched) break;
+    if (4 * host_active < 3 * listed && (status = compact(h + 1)) != TM_OK) break;
   }
+  // the replays still in flight return at once; the graphs they use are
+  // released after them
+  if (hipStreamSynchronize(st) != hipSuccess && status == TM_OK) status = TM_ERR_HIP;
+  for (hipGraphExec_t e : retired) (void)hipGraphExecDestroy(e);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
   if (cap) (void)hipStreamDestroy(cap);
