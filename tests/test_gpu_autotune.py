@@ -74,6 +74,61 @@ def test_edt2d_variants_all_foreground_and_background(tm, oracle_mod):
         autotune.clear()
 
 
+def _sparse_masks():
+    """Masks whose row windows pass the step cap, so rows (or parts of rows)
+    are resolved through the row envelope: one site, an all-background
+    column far from most pixels, equidistant sites (ties at the midpoints),
+    rows whose pixels split between the window and the envelope."""
+    rng = np.random.default_rng(5)
+    out = []
+    m = np.ones((1, 1, 64, 1024), bool)
+    m[0, 0, 30, 500] = False
+    out.append(m)
+    m = np.ones((1, 1, 40, 512), bool)
+    m[0, 0, :, 0] = False
+    out.append(m)
+    m = np.ones((2, 1, 48, 800), bool)
+    m[0, 0, [5, 5, 40], [100, 300, 700]] = False
+    m[1, 0, [0, 47, 47], [399, 401, 600]] = False
+    out.append(m)
+    out.append(rng.random((2, 1, 100, 600)) > 0.002)
+    m = rng.random((1, 1, 256, 256)) > 0.2
+    m[0, 0, :, :200] = True  # columns without background next to dense ones
+    out.append(m)
+    m = np.ones((1, 1, 9, 4096), bool)
+    m[0, 0, 4, [0, 4095]] = False
+    out.append(m)
+    return out
+
+
+@pytest.mark.parametrize("case", range(6))
+@pytest.mark.parametrize("idx", ["i64", "none"])
+def test_edt2d_sparse_rows_every_variant(tm, oracle_mod, case, idx):
+    from torchmorph_amd import autotune
+
+    mask = _sparse_masks()[case]
+    od, oi = oracle_mod.edt(mask)
+    want_d = torch.from_numpy(od.astype(np.float32))
+    H, W = mask.shape[2:]
+    bits = {"i64": 64, "none": 0}[idx]
+    key = f"edt2d {_p2(H)} {_p2(W)} {bits} 1"
+    x = torch.as_tensor(mask, device="cuda")
+    try:
+        for v in range(13):
+            autotune.clear()
+            autotune.load({key: v})
+            if idx == "none":
+                d = tm.euclidean_distance_transform(x)
+                assert torch.equal(d.cpu(), want_d), v
+            else:
+                ind = torch.empty((2, *mask.shape), dtype=torch.int64, device="cuda")
+                d = tm.euclidean_distance_transform(x, indices=ind)
+                assert torch.equal(d.cpu(), want_d), v
+                assert np.array_equal(ind.cpu().numpy(), oi), v
+    finally:
+        autotune.clear()
+
+
 def test_inapplicable_cached_variant_falls_back(tm, oracle_mod):
     """A cached choice that cannot run here (variant 7 = 8-row bands needs
     more LDS than a 4096-wide row allows) falls back to the default."""

@@ -78,6 +78,21 @@ def test_edt_sparse_background(tm, oracle_mod):
     _check_vs_oracle(tm, oracle_mod, mask)
 
 
+@pytest.mark.parametrize("kind", ["one_site", "left_column", "p_bg_1e-4"])
+def test_edt_sparse_background_1024x4096(tm, oracle_mod, kind):
+    """The largest 2-D-path shape with sparse background: row windows pass
+    the step cap and rows are resolved through their envelopes."""
+    rng = np.random.default_rng(3)
+    mask = np.ones((1, 1, 1024, 4096), bool)
+    if kind == "one_site":
+        mask[0, 0, 500, 2000] = False
+    elif kind == "left_column":
+        mask[0, 0, :, 0] = False
+    else:
+        mask = rng.random((1, 1, 1024, 4096)) > 1e-4
+    _check_vs_oracle(tm, oracle_mod, mask, idx_dtype=torch.int64)
+
+
 @pytest.mark.parametrize("sampling", [None, [1.0, 1.0, 1.0], [1.3, 0.7, 1.9], [0.5, 2.0, 1.0]])
 def test_edt_3d_vs_oracle(tm, oracle_mod, sampling):
     rng = np.random.default_rng(5)

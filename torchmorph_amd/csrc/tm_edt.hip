@@ -16,7 +16,9 @@
 //    W <= 4096.  Axis 0: nearest background row per pixel from bit-packed
 //    column segments (one load round), int16 into the workspace.  Axis 1:
 //    band of rows staged in LDS, exact expanding-window argmin per pixel with
-//    the lower-index tie rule (exact integers, so equal to the envelope).
+//    the lower-index tie rule (exact integers, so equal to the envelope); a
+//    row where some window passes kWinCap steps (sparse background) is
+//    resolved through its exact integer lower envelope, built in LDS.
 //  * edt_pass0_kernel   -- N-D, axis 0 from the mask: two sweeps per line.
 //  * edt_stack_kernel   -- N-D, axis d >= 1: one lane per line runs the
 //    double-precision envelope (stack of sites with their partial squared
@@ -220,11 +222,251 @@ __global__ __launch_bounds__(256) void edt2d_cols_kernel(const TIn* __restrict__
   }  // live
 }
 
+// Window steps a pixel may take before its row is resolved through the row's
+// lower envelope instead.  The expanding window costs O(distance to the
+// nearest site) per pixel, which makes a row with sparse background
+// quadratic in its length; the envelope costs O(W) per row plus a binary
+// search per pixel.
+constexpr int kWinCap = 64;
+
+__device__ __forceinline__ int floor_div(int n, int d) {  // d > 0
+  return n >= 0 ? n / d : -((-n + d - 1) / d);
+}
+
+// Site heights of one row: h(x) = (y - fr[x])^2 for the feature row fr[x]
+// of column x (fr[x] < 0: the column has no background, no site).
+__device__ __forceinline__ int site_h(const int16_t* frr, int y, int x) {
+  const int d = y - (int)frr[x];
+  return d * d;
+}
+
+// Lower envelope of the parabolas h(x) + (q - x)^2 over the integer points
+// q = 0 .. W-1, in exact integer arithmetic, built in place over the row's
+// sites listed in column order as E[i] = (h << 12) | x (n entries; h <
+// 2^20, x < 4096).  Entry k becomes (b << 12) | site: the site and the first
+// q at which it wins (b = 0 for entry 0, strictly increasing).  A site wins
+// from the first q where it is strictly below the entry before it, so the
+// lower column keeps every tie: the result at q is the lowest column
+// attaining the minimum, the window's rule.  A site that wins at no q of the
+// row is dropped.  Entry k is written only after list entry k was read.  Run
+// by one thread; returns the index of the last entry (-1: no site).
+__device__ int row_env_build(uint32_t* E, int n, int W, const int16_t* frr, int y) {
+  int top = -1;
+  int tp = 0, tc = 0, tb = 0;  // top entry: site, h(site) + site^2, first q
+  uint32_t nxt = n > 0 ? E[0] : 0u;
+  for (int i = 0; i < n; ++i) {
+    const uint32_t v = nxt;
+    if (i + 1 < n) nxt = E[i + 1];  // entries above i are never written here
+    const int x = (int)(v & 0xfffu);
+    const int cq = (int)(v >> 12) + x * x;
+    int bq = 0;
+    while (top >= 0) {
+      // first q with h + (q - x)^2 < h(tp) + (q - tp)^2, i.e. 2q(x - tp) > cq - tc
+      const int B = floor_div(cq - tc, 2 * (x - tp)) + 1;
+      if (B > tb) {
+        bq = B;
+        break;
+      }
+      --top;  // the top site wins at no q
+      if (top >= 0) {
+        const uint32_t t = E[top];
+        tp = (int)(t & 0xfffu);
+        tb = (int)(t >> 12);
+        tc = site_h(frr, y, tp) + tp * tp;
+      }
+    }
+    if (bq > W - 1) continue;
+    ++top;
+    E[top] = (uint32_t)x | ((uint32_t)bq << 12);
+    tp = x;
+    tc = cq;
+    tb = bq;
+  }
+  return top;
+}
+
+// The envelope's site at q: the last entry that starts at or before q.
+__device__ __forceinline__ int row_env_site(const uint32_t* E, int top, int q) {
+  int lo = 0, hi = top;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((int)(E[mid] >> 12) <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  return (int)(E[lo] & 0xfffu);
+}
+
+// LDS of the envelope fallback: per row of the band a flag and one mark bit
+// per pixel group left unresolved by the window, the site count and four
+// wave totals of the compaction.  The envelope itself is built in the row's
+// squared-distance area, which the window no longer reads by then.
+struct RowEnvLds {
+  uint32_t* mark;  // [R][mw]
+  int* flag;       // [R]
+  int* nsite;      // [1]
+  int* wsum;       // [4]
+  int mw;          // mark words per row
+};
+
+__host__ __device__ constexpr size_t row_env_lds_bytes(int R, int W, int G) {
+  return (size_t)R * (((W + G - 1) / G + 31) / 32) * 4 + (size_t)R * 4 + 32;
+}
+
+__device__ __forceinline__ RowEnvLds row_env_lds(char* base, int R, int W, int G) {
+  RowEnvLds e;
+  e.mw = ((W + G - 1) / G + 31) / 32;
+  e.mark = (uint32_t*)base;
+  e.flag = (int*)(e.mark + R * e.mw);
+  e.nsite = e.flag + R;
+  e.wsum = e.nsite + 1;
+  return e;
+}
+
+// Clears the marks and flags (before the workgroup's first barrier).
+__device__ __forceinline__ void row_env_clear(const RowEnvLds& e, int R) {
+  for (int i = threadIdx.x; i < R * e.mw; i += blockDim.x) e.mark[i] = 0;
+  for (int i = threadIdx.x; i < R; i += blockDim.x) e.flag[i] = 0;
+}
+
+__device__ __forceinline__ void row_env_mark(const RowEnvLds& e, int r, int bit) {
+  atomicOr(&e.mark[r * e.mw + (bit >> 5)], 1u << (bit & 31));
+  e.flag[r] = 1;
+}
+
+// Writes G adjacent pixels (x0 .. x0+G-1 of row y) from their minima best
+// (-1: no background in the whole image) and winning columns bp.
+template <int G>
+__device__ __forceinline__ void edt2d_store(const EdtOut& o, const int16_t* frr, int img, int H,
+                                            int W, int y, int x0, const int* best,
+                                            const int* bp) {
+  const int64_t g = ((int64_t)img * H + y) * W + x0;
+  float dist[G];
+  int32_t fy[G], fx[G];
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    double acc;
+    if (best[i] >= 0) {
+      fy[i] = frr[bp[i]];
+      fx[i] = bp[i];
+      acc = (double)best[i];  // exact integer: same as the double delta/square/sum
+    } else {
+      fy[i] = -1;  // no background in the whole image
+      fx[i] = 0;
+      const double a = (double)(-1 - y), b = (double)(-(x0 + i));
+      acc = a * a + b * b;
+    }
+    dist[i] = (float)sqrt(acc);
+  }
+  if (G == 4) {
+    if (o.dist) *(float4*)(o.dist + g) = make_float4(dist[0], dist[1 % G], dist[2 % G], dist[3 % G]);
+    if (o.idx) {
+      if (o.idx64) {
+        int64_t* ip = (int64_t*)o.idx;
+        longlong2* a = (longlong2*)(ip + g);
+        longlong2* b = (longlong2*)(ip + o.plane + g);
+        a[0] = make_longlong2(fy[0], fy[1 % G]);
+        a[1] = make_longlong2(fy[2 % G], fy[3 % G]);
+        b[0] = make_longlong2(fx[0], fx[1 % G]);
+        b[1] = make_longlong2(fx[2 % G], fx[3 % G]);
+      } else {
+        int32_t* ip = (int32_t*)o.idx;
+        *(int4*)(ip + g) = make_int4(fy[0], fy[1 % G], fy[2 % G], fy[3 % G]);
+        *(int4*)(ip + o.plane + g) = make_int4(fx[0], fx[1 % G], fx[2 % G], fx[3 % G]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      if (o.dist) o.dist[g + i] = dist[i];
+      if (o.idx) {
+        if (o.idx64) {
+          int64_t* ip = (int64_t*)o.idx;
+          ip[g + i] = fy[i];
+          ip[o.plane + g + i] = fx[i];
+        } else {
+          int32_t* ip = (int32_t*)o.idx;
+          ip[g + i] = fy[i];
+          ip[o.plane + g + i] = fx[i];
+        }
+      }
+    }
+  }
+}
+
+// Resolves the marked pixel groups of every flagged row of the band through
+// the row's envelope.  Called by every thread of the workgroup, whose size is
+// a multiple of 64 (barriers inside, at the top level of loops whose trip
+// counts come from LDS values read after a barrier).  hs:
+// squared column distances, row pitch P, entry x of row r at r * P + off (W
+// entries, overwritten by the envelope); fr: feature rows, pitch W.
+template <int G>
+__device__ void edt2d_rows_resolve(const RowEnvLds& e, int32_t* hs, int P, int off,
+                                   const int16_t* fr, int rows, int img, int H, int W, int y0,
+                                   const EdtOut& o) {
+  __syncthreads();  // marks and flags are complete, the window reads are done
+  const int T = blockDim.x;
+  const int C = (W + T - 1) / T;  // columns per thread in the compaction
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int r = 0; r < rows; ++r) {
+    if (!e.flag[r]) continue;
+    const int y = y0 + r;
+    const int16_t* frr = fr + r * W;
+    uint32_t* E = (uint32_t*)(hs + r * P + off);
+    // the row's sites in column order, as (h << 12) | x: per-thread counts,
+    // wave prefix sums, then every thread writes its own at its offset
+    const int c0 = min(W, (int)threadIdx.x * C), c1 = min(W, c0 + C);
+    int cnt = 0;
+    for (int x = c0; x < c1; ++x) cnt += frr[x] >= 0 ? 1 : 0;
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) e.wsum[wave] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wave; ++w) base += e.wsum[w];
+    int pos = base + incl - cnt;
+    for (int x = c0; x < c1; ++x) {
+      const int f = frr[x];
+      if (f >= 0) E[pos++] = ((uint32_t)((y - f) * (y - f)) << 12) | (uint32_t)x;
+    }
+    if ((int)threadIdx.x == T - 1) *e.nsite = base + incl;
+    __syncthreads();
+    if (threadIdx.x == 0) *e.nsite = row_env_build(E, *e.nsite, W, frr, y);
+    __syncthreads();
+    const int top = *e.nsite;
+    const int nb = (W + G - 1) / G;
+    for (int q = threadIdx.x; q < nb; q += T) {
+      if (!((e.mark[r * e.mw + (q >> 5)] >> (q & 31)) & 1u)) continue;
+      int best[G], bp[G];
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int x = q * G + i;
+        if (top < 0) {
+          best[i] = -1;
+          bp[i] = 0;
+        } else {
+          const int s = row_env_site(E, top, x);
+          best[i] = site_h(frr, y, s) + (x - s) * (x - s);
+          bp[i] = s;
+        }
+      }
+      edt2d_store<G>(o, frr, img, H, W, y, q * G, best, bp);
+    }
+    __syncthreads();  // wsum, nsite and the next row's area are reused
+  }
+}
+
 // Row pass of one band of `rows` rows starting at y0: squared column
 // distances hs and feature rows fr in LDS, exact expanding-window argmin per
-// pixel (the lower column wins ties), outputs written to HBM.
-__device__ __forceinline__ void edt2d_band_rows(const int32_t* hs, const int16_t* fr, int npx,
-                                                int img, int H, int W, int y0, const EdtOut& o) {
+// pixel (the lower column wins ties), outputs written to HBM; pixels whose
+// window reaches kWinCap steps are resolved by their row's envelope.
+__device__ __forceinline__ void edt2d_band_rows(int32_t* hs, const int16_t* fr, int rows,
+                                                int img, int H, int W, int y0, const EdtOut& o,
+                                                const RowEnvLds& e) {
+  const int npx = rows * W;
   for (int p = threadIdx.x; p < npx; p += blockDim.x) {
     const int r = p / W;
     const int x = p - r * W;
@@ -232,6 +474,7 @@ __device__ __forceinline__ void edt2d_band_rows(const int32_t* hs, const int16_t
     int best = hr[x];
     int bp = x;
     const int reach = max(x, W - 1 - x);
+    bool capped = false;
     // Candidates in the order x, x-1, x+1, x-2, x+2, ...: a left candidate
     // replaces an equal best and a right one does not, so the lowest column
     // attaining the minimum wins.  Steps are taken 8 at a time: the 16 LDS
@@ -241,6 +484,10 @@ __device__ __forceinline__ void edt2d_band_rows(const int32_t* hs, const int16_t
     // read only while (k0 + 1)^2 <= best.
     for (int k0 = 0; k0 < reach; k0 += 8) {
       if (best >= 0 && (k0 + 1) * (k0 + 1) > best) break;
+      if (k0 >= kWinCap) {
+        capped = true;
+        break;
+      }
       int hl[8], hv[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -262,33 +509,21 @@ __device__ __forceinline__ void edt2d_band_rows(const int32_t* hs, const int16_t
         }
       }
     }
-    const int y = y0 + r;
-    const int64_t g = ((int64_t)img * H + y) * W + x;
-    int32_t fy, fx;
-    double acc;
-    if (best >= 0) {
-      fy = fr[r * W + bp];
-      fx = bp;
-      acc = (double)best;  // exact integer: same as the double delta/square/sum
-    } else {
-      fy = -1;  // no background in the whole image
-      fx = 0;
-      const double a = (double)(-1 - y), b = (double)(-x);
-      acc = a * a + b * b;
+    if (capped) {
+      row_env_mark(e, r, x);
+      continue;
     }
-    if (o.dist) o.dist[g] = (float)sqrt(acc);
-    if (o.idx) {
-      if (o.idx64) {
-        int64_t* ip = (int64_t*)o.idx;
-        ip[g] = fy;
-        ip[o.plane + g] = fx;
-      } else {
-        int32_t* ip = (int32_t*)o.idx;
-        ip[g] = fy;
-        ip[o.plane + g] = fx;
-      }
-    }
+    edt2d_store<1>(o, fr + r * W, img, H, W, y0 + r, x, &best, &bp);
   }
+  edt2d_rows_resolve<1>(e, hs, W, 0, fr, rows, img, H, W, y0, o);
+}
+
+// LDS bytes of edt2d_rows_kernel<R> and edt2d_rows4_kernel<R>.
+inline size_t edt2d_rows_lds(int R, int W) {
+  return align256((size_t)R * W * 6) + row_env_lds_bytes(R, W, 1);
+}
+inline size_t edt2d_rows4_lds(int R, int W) {
+  return align256((size_t)R * (W + 16) * 4 + (size_t)R * W * 2) + row_env_lds_bytes(R, W, 4);
 }
 
 // Axis-1 pass.  One workgroup per band of R rows: the band's column features
@@ -303,12 +538,14 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
   extern __shared__ int32_t lds[];
   int32_t* hs = lds;                       // [R][W] squared column distance, -1 = none
   int16_t* fr = (int16_t*)(lds + R * W);   // [R][W] feature row
+  const RowEnvLds e = row_env_lds((char*)lds + ((R * W * 6 + 255) & ~255), R, W, 1);
   const int img = blockIdx.y;
   const int y0 = blockIdx.x * R;
   const int rows = min(R, H - y0);
   const int npx = rows * W;
   const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
   const bool vec = (W % 8 == 0);
+  row_env_clear(e, R);
   for (int base = 0; base < npx; base += 256 * 8) {
     const int p0 = base + threadIdx.x * 8;
     int16_t v[8];
@@ -335,7 +572,7 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
     }
   }
   __syncthreads();
-  edt2d_band_rows(hs, fr, npx, img, H, W, y0, o);
+  edt2d_band_rows(hs, fr, rows, img, H, W, y0, o, e);
 }
 
 // Axis-1 pass with four adjacent pixels per lane (W % 4 == 0, W <= 4096).
@@ -355,11 +592,14 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
   const int P = W + 16;                 // padded row pitch of hs
   int32_t* hs = lds;                    // [R][W + 16], entry x of row r at r * P + 8 + x
   int16_t* fr = (int16_t*)(lds + R * P);  // [R][W]
+  const RowEnvLds e =
+      row_env_lds((char*)lds + ((R * P * 4 + R * W * 2 + 255) & ~255), R, W, 4);
   const int img = blockIdx.y;
   const int y0 = blockIdx.x * R;
   const int rows = min(R, H - y0);
   const int npx = rows * W;
   const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
+  row_env_clear(e, R);
   for (int i = threadIdx.x; i < rows * 16; i += blockDim.x) {
     const int r = i >> 4, k = i & 15;
     hs[r * P + (k < 8 ? k : W + k)] = -1;
@@ -393,6 +633,7 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
       h[4 * v + 3] = t.w;
     }
     int best[4], bp[4];
+    bool capped = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       best[i] = h[8 + i];
@@ -416,6 +657,10 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
         // steps 9, 10, ...: the blocked loop of edt2d_band_rows
         for (int k0 = 8; k0 < reach; k0 += 8) {
           if (best[i] >= 0 && (k0 + 1) * (k0 + 1) > best[i]) break;
+          if (k0 >= kWinCap) {
+            capped = true;
+            break;
+          }
           int hl[8], hv[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
@@ -439,42 +684,13 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
         }
       }
     }
-    const int y = y0 + r;
-    const int64_t g = ((int64_t)img * H + y) * W + x0;
-    float dist[4];
-    int32_t fy[4], fx[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      double acc;
-      if (best[i] >= 0) {
-        fy[i] = fr[r * W + bp[i]];
-        fx[i] = bp[i];
-        acc = (double)best[i];  // exact integer: same as the double delta/square/sum
-      } else {
-        fy[i] = -1;  // no background in the whole image
-        fx[i] = 0;
-        const double a = (double)(-1 - y), b = (double)(-(x0 + i));
-        acc = a * a + b * b;
-      }
-      dist[i] = (float)sqrt(acc);
+    if (capped) {
+      row_env_mark(e, r, x0 >> 2);
+      continue;
     }
-    if (o.dist) *(float4*)(o.dist + g) = make_float4(dist[0], dist[1], dist[2], dist[3]);
-    if (o.idx) {
-      if (o.idx64) {
-        int64_t* ip = (int64_t*)o.idx;
-        longlong2* a = (longlong2*)(ip + g);
-        longlong2* b = (longlong2*)(ip + o.plane + g);
-        a[0] = make_longlong2(fy[0], fy[1]);
-        a[1] = make_longlong2(fy[2], fy[3]);
-        b[0] = make_longlong2(fx[0], fx[1]);
-        b[1] = make_longlong2(fx[2], fx[3]);
-      } else {
-        int32_t* ip = (int32_t*)o.idx;
-        *(int4*)(ip + g) = make_int4(fy[0], fy[1], fy[2], fy[3]);
-        *(int4*)(ip + o.plane + g) = make_int4(fx[0], fx[1], fx[2], fx[3]);
-      }
-    }
+    edt2d_store<4>(o, fr + r * W, img, H, W, y0 + r, x0, best, bp);
   }
+  edt2d_rows_resolve<4>(e, hs, P, 8, fr, rows, img, H, W, y0, o);
 }
 
 // --------------------------------------------------------------- N-D --
@@ -603,9 +819,9 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
       if (v >= 8) {
         const int R = v <= 10 ? 1 << (v - 8) : 1;  // 1, 2, 4 rows per band
         const int CG = v <= 10 ? 16 : (v == 11 ? 8 : 4);
-        const size_t lds = (size_t)R * (W + 16) * 4 + (size_t)R * W * 2;
+        const size_t lds = edt2d_rows4_lds(R, W);
         auto a16 = [](const void* p) { return ((uintptr_t)p % 16) == 0; };
-        if (W % 8 != 0 || lds > 64 * 1024 || batch > 65535 || !a16(o.dist) || !a16(o.idx)) {
+        if (W % 8 != 0 || lds > 96 * 1024 || batch > 65535 || !a16(o.dist) || !a16(o.idx)) {
           set_error("tm_edt: four-pixel row variant does not apply");
           return TM_ERR_ARG;
         }
@@ -650,7 +866,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
       }
       const int cpt = (v & 1) ? 4 : 1;
       const int R = 1 << (v >> 1);
-      if ((size_t)R * W * 6 > 96 * 1024) {
+      if (edt2d_rows_lds(R, W) > 112 * 1024) {
         set_error("tm_edt: band does not fit LDS");
         return TM_ERR_ARG;
       }
@@ -679,7 +895,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
 #undef TM_COLS
       TM_LAUNCH_CHECK();
       const dim3 rgrid((H + R - 1) / R, (unsigned)batch);
-      const size_t lds = (size_t)R * W * 6;
+      const size_t lds = edt2d_rows_lds(R, W);
       switch (R) {
         case 1: hipLaunchKernelGGL((edt2d_rows_kernel<1>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
         case 2: hipLaunchKernelGGL((edt2d_rows_kernel<2>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
