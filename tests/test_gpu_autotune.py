@@ -38,7 +38,7 @@ def test_edt2d_every_variant_matches_oracle(tm, oracle_mod, shape, idx):
     key = f"edt2d {_p2(H)} {_p2(W)} {bits} 1"
     x = torch.as_tensor(mask, device="cuda")
     try:
-        for v in range(13):
+        for v in range(16):
             autotune.clear()
             autotune.load({key: v})
             if idx == "none":
@@ -58,7 +58,7 @@ def test_edt2d_variants_all_foreground_and_background(tm, oracle_mod):
     from torchmorph_amd import autotune
 
     try:
-        for v in range(13):
+        for v in range(16):
             autotune.clear()
             autotune.load({"edt2d 64 128 64 1": v})
             for fill in (True, False):
@@ -92,16 +92,20 @@ def _sparse_masks():
     m[1, 0, [0, 47, 47], [399, 401, 600]] = False
     out.append(m)
     out.append(rng.random((2, 1, 100, 600)) > 0.002)
-    m = rng.random((1, 1, 256, 256)) > 0.2
+    m = rng.random((1, 1, 256, 304)) > 0.2
     m[0, 0, :, :200] = True  # columns without background next to dense ones
     out.append(m)
     m = np.ones((1, 1, 9, 4096), bool)
     m[0, 0, 4, [0, 4095]] = False
     out.append(m)
+    m = np.ones((2, 1, 64, 256), bool)  # rows too short for the fallback: long windows
+    m[0, 0, 10, 0] = False
+    m[1, 0, [3, 60], [17, 255]] = False
+    out.append(m)
     return out
 
 
-@pytest.mark.parametrize("case", range(6))
+@pytest.mark.parametrize("case", range(7))
 @pytest.mark.parametrize("idx", ["i64", "none"])
 def test_edt2d_sparse_rows_every_variant(tm, oracle_mod, case, idx):
     from torchmorph_amd import autotune
@@ -114,7 +118,7 @@ def test_edt2d_sparse_rows_every_variant(tm, oracle_mod, case, idx):
     key = f"edt2d {_p2(H)} {_p2(W)} {bits} 1"
     x = torch.as_tensor(mask, device="cuda")
     try:
-        for v in range(13):
+        for v in range(16):
             autotune.clear()
             autotune.load({key: v})
             if idx == "none":

@@ -399,16 +399,15 @@ __device__ __forceinline__ void edt2d_store(const EdtOut& o, const int16_t* frr,
 // counts come from LDS values read after a barrier).  hs:
 // squared column distances, row pitch P, entry x of row r at r * P + off (W
 // entries, overwritten by the envelope); fr: feature rows, pitch W.
+// One flagged row r.
 template <int G>
-__device__ void edt2d_rows_resolve(const RowEnvLds& e, int32_t* hs, int P, int off,
-                                   const int16_t* fr, int rows, int img, int H, int W, int y0,
-                                   const EdtOut& o) {
-  __syncthreads();  // marks and flags are complete, the window reads are done
+__device__ __forceinline__ void edt2d_resolve_row(const RowEnvLds& e, int32_t* hs, int P, int off,
+                                               const int16_t* fr, int r, int img, int H, int W,
+                                               int y0, const EdtOut& o) {
   const int T = blockDim.x;
   const int C = (W + T - 1) / T;  // columns per thread in the compaction
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int r = 0; r < rows; ++r) {
-    if (!e.flag[r]) continue;
+  {
     const int y = y0 + r;
     const int16_t* frr = fr + r * W;
     uint32_t* E = (uint32_t*)(hs + r * P + off);
@@ -459,10 +458,20 @@ __device__ void edt2d_rows_resolve(const RowEnvLds& e, int32_t* hs, int P, int o
   }
 }
 
+template <int G>
+__device__ __forceinline__ void edt2d_rows_resolve(const RowEnvLds& e, int32_t* hs, int P,
+                                                   int off, const int16_t* fr, int rows, int img,
+                                                   int H, int W, int y0, const EdtOut& o) {
+  __syncthreads();  // marks and flags are complete, the window reads are done
+  for (int r = 0; r < rows; ++r)
+    if (e.flag[r]) edt2d_resolve_row<G>(e, hs, P, off, fr, r, img, H, W, y0, o);
+}
+
 // Row pass of one band of `rows` rows starting at y0: squared column
 // distances hs and feature rows fr in LDS, exact expanding-window argmin per
 // pixel (the lower column wins ties), outputs written to HBM; pixels whose
 // window reaches kWinCap steps are resolved by their row's envelope.
+template <bool ENV>
 __device__ __forceinline__ void edt2d_band_rows(int32_t* hs, const int16_t* fr, int rows,
                                                 int img, int H, int W, int y0, const EdtOut& o,
                                                 const RowEnvLds& e) {
@@ -482,9 +491,10 @@ __device__ __forceinline__ void edt2d_band_rows(int32_t* hs, const int16_t* fr, 
     // step k with k^2 > best can neither beat nor tie best (hv >= 0), so the
     // steps of a block past that point change nothing, and the next block is
     // read only while (k0 + 1)^2 <= best.
+#pragma unroll 1
     for (int k0 = 0; k0 < reach; k0 += 8) {
       if (best >= 0 && (k0 + 1) * (k0 + 1) > best) break;
-      if (k0 >= kWinCap) {
+      if (ENV && k0 >= kWinCap) {
         capped = true;
         break;
       }
@@ -509,21 +519,26 @@ __device__ __forceinline__ void edt2d_band_rows(int32_t* hs, const int16_t* fr, 
         }
       }
     }
-    if (capped) {
+    if (ENV && capped) {
       row_env_mark(e, r, x);
       continue;
     }
     edt2d_store<1>(o, fr + r * W, img, H, W, y0 + r, x, &best, &bp);
   }
-  edt2d_rows_resolve<1>(e, hs, W, 0, fr, rows, img, H, W, y0, o);
+  if (ENV) edt2d_rows_resolve<1>(e, hs, W, 0, fr, rows, img, H, W, y0, o);
 }
 
-// LDS bytes of edt2d_rows_kernel<R> and edt2d_rows4_kernel<R>.
+// Rows longer than this take the envelope fallback (ENV): shorter rows bound
+// every window by the row length.
+constexpr int kEnvMinW = 257;
+
+// LDS bytes of edt2d_rows_kernel<R, ENV> and edt2d_rows4_kernel<R, ENV>.
 inline size_t edt2d_rows_lds(int R, int W) {
-  return align256((size_t)R * W * 6) + row_env_lds_bytes(R, W, 1);
+  return align256((size_t)R * W * 6) + (W >= kEnvMinW ? row_env_lds_bytes(R, W, 1) : 0);
 }
 inline size_t edt2d_rows4_lds(int R, int W) {
-  return align256((size_t)R * (W + 16) * 4 + (size_t)R * W * 2) + row_env_lds_bytes(R, W, 4);
+  return align256((size_t)R * (W + 16) * 4 + (size_t)R * W * 2) +
+         (W >= kEnvMinW ? row_env_lds_bytes(R, W, 4) : 0);
 }
 
 // Axis-1 pass.  One workgroup per band of R rows: the band's column features
@@ -531,8 +546,9 @@ inline size_t edt2d_rows4_lds(int R, int W) {
 // column distances in LDS, then every pixel runs an exact expanding-window
 // argmin over its row (the lower column wins ties).  With unit sampling all
 // quantities are exact integers, so this equals SciPy's double-precision
-// envelope result bit for bit.
-template <int R>
+// envelope result bit for bit.  ENV: rows of at least kEnvMinW pixels, with
+// the envelope fallback.
+template <int R, bool ENV>
 __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restrict__ f0, int H,
                                                          int W, EdtOut o) {
   extern __shared__ int32_t lds[];
@@ -545,7 +561,7 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
   const int npx = rows * W;
   const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
   const bool vec = (W % 8 == 0);
-  row_env_clear(e, R);
+  if (ENV) row_env_clear(e, R);
   for (int base = 0; base < npx; base += 256 * 8) {
     const int p0 = base + threadIdx.x * 8;
     int16_t v[8];
@@ -572,7 +588,7 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
     }
   }
   __syncthreads();
-  edt2d_band_rows(hs, fr, rows, img, H, W, y0, o, e);
+  edt2d_band_rows<ENV>(hs, fr, rows, img, H, W, y0, o, e);
 }
 
 // Axis-1 pass with four adjacent pixels per lane (W % 4 == 0, W <= 4096).
@@ -585,7 +601,7 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
 // candidate order and tie rule are edt2d_band_rows's, so the results are the
 // same bit for bit.  Outputs leave as 16-byte stores: four float32
 // distances, and per index plane four int32 or two pairs of int64.
-template <int R>
+template <int R, bool ENV>
 __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restrict__ f0, int H,
                                                           int W, EdtOut o) {
   extern __shared__ int32_t lds[];
@@ -599,7 +615,7 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
   const int rows = min(R, H - y0);
   const int npx = rows * W;
   const int16_t* src = f0 + ((int64_t)img * H + y0) * W;
-  row_env_clear(e, R);
+  if (ENV) row_env_clear(e, R);
   for (int i = threadIdx.x; i < rows * 16; i += blockDim.x) {
     const int r = i >> 4, k = i & 15;
     hs[r * P + (k < 8 ? k : W + k)] = -1;
@@ -654,10 +670,12 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
       const int x = x0 + i;
       const int reach = max(x, W - 1 - x);
       if (best[i] < 0 || best[i] >= 81) {
-        // steps 9, 10, ...: the blocked loop of edt2d_band_rows
+        // steps 9, 10, ...: the blocked loop of edt2d_band_rows (rolled: the
+        // cap bounds its trip count, and unrolled it would quadruple the code)
+#pragma unroll 1
         for (int k0 = 8; k0 < reach; k0 += 8) {
           if (best[i] >= 0 && (k0 + 1) * (k0 + 1) > best[i]) break;
-          if (k0 >= kWinCap) {
+          if (ENV && k0 >= kWinCap) {
             capped = true;
             break;
           }
@@ -684,13 +702,13 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
         }
       }
     }
-    if (capped) {
+    if (ENV && capped) {
       row_env_mark(e, r, x0 >> 2);
       continue;
     }
     edt2d_store<4>(o, fr + r * W, img, H, W, y0 + r, x0, best, bp);
   }
-  edt2d_rows_resolve<4>(e, hs, P, 8, fr, rows, img, H, W, y0, o);
+  if (ENV) edt2d_rows_resolve<4>(e, hs, P, 8, fr, rows, img, H, W, y0, o);
 }
 
 // --------------------------------------------------------------- N-D --
@@ -814,18 +832,21 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     // 1 << (v >> 1); v = 8, 9, 10: one column per thread and the four-pixel
     // row pass (edt2d_rows4_kernel) with bands of 1, 2, 4 rows; v = 11, 12:
     // bands of 1 row, 8 or 4 columns per column-pass workgroup (more
-    // workgroups for narrow images)
+    // workgroups for narrow images); v = 13, 14, 15: bands of 1 row, four
+    // adjacent columns per column-pass thread (one 4-byte load per row, one
+    // 8-byte store per row), 16, 8 or 4 column groups per workgroup
     auto run = [&](int v) -> int {
       if (v >= 8) {
         const int R = v <= 10 ? 1 << (v - 8) : 1;  // 1, 2, 4 rows per band
-        const int CG = v <= 10 ? 16 : (v == 11 ? 8 : 4);
+        const int CPT = v >= 13 ? 4 : 1;
+        const int CG = (v <= 10 || v == 13) ? 16 : ((v == 11 || v == 14) ? 8 : 4);
         const size_t lds = edt2d_rows4_lds(R, W);
         auto a16 = [](const void* p) { return ((uintptr_t)p % 16) == 0; };
         if (W % 8 != 0 || lds > 96 * 1024 || batch > 65535 || !a16(o.dist) || !a16(o.idx)) {
           set_error("tm_edt: four-pixel row variant does not apply");
           return TM_ERR_ARG;
         }
-        const int64_t ncb = (W + CG - 1) / CG;
+        const int64_t ncb = (W + CG * CPT - 1) / (CG * CPT);
         const dim3 cgrid((unsigned)(ncb * batch));
         const int seg = (H + 256 / CG - 1) / (256 / CG);
         if (seg > 4 * kChunkRows) {
@@ -839,28 +860,41 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   else                                                                                      \
     hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, 1, false, G>), cgrid, dim3(256), 0, st, \
                        in, H, W, f0, aligned)
-        if (CG == 16) {
-          if (seg <= 16) TM_COLS1(1, 16);
-          else if (seg <= 32) TM_COLS1(2, 16);
-          else TM_COLS1(4, 16);
-        } else if (CG == 8) {
-          if (seg <= 16) TM_COLS1(1, 8);
-          else if (seg <= 32) TM_COLS1(2, 8);
-          else TM_COLS1(4, 8);
+#define TM_COLS4(NCH, G)                                                                    \
+  if (vecw)                                                                                 \
+    hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, 4, true, G>), cgrid, dim3(256), 0, st, in, \
+                       H, W, f0, aligned);                                                  \
+  else                                                                                      \
+    hipLaunchKernelGGL((edt2d_cols_kernel<TIn, NCH, 4, false, G>), cgrid, dim3(256), 0, st, \
+                       in, H, W, f0, aligned)
+#define TM_COLS_CG(M, G)                     \
+  if (seg <= 16) M(1, G);                    \
+  else if (seg <= 32) M(2, G);               \
+  else M(4, G)
+        if (CPT == 1) {
+          if (CG == 16) { TM_COLS_CG(TM_COLS1, 16); }
+          else if (CG == 8) { TM_COLS_CG(TM_COLS1, 8); }
+          else { TM_COLS_CG(TM_COLS1, 4); }
         } else {
-          if (seg <= 16) TM_COLS1(1, 4);
-          else if (seg <= 32) TM_COLS1(2, 4);
-          else TM_COLS1(4, 4);
+          if (CG == 16) { TM_COLS_CG(TM_COLS4, 16); }
+          else if (CG == 8) { TM_COLS_CG(TM_COLS4, 8); }
+          else { TM_COLS_CG(TM_COLS4, 4); }
         }
+#undef TM_COLS_CG
+#undef TM_COLS4
 #undef TM_COLS1
         TM_LAUNCH_CHECK();
         const int threads = (int)std::min<int64_t>(256, ((int64_t)R * W / 4 + 63) / 64 * 64);
         const dim3 rgrid((H + R - 1) / R, (unsigned)batch);
+#define TM_ROWS4(RR, E) \
+  hipLaunchKernelGGL((edt2d_rows4_kernel<RR, E>), rgrid, dim3(threads), lds, st, f0, H, W, o)
+        const bool env = W >= kEnvMinW;
         switch (R) {
-          case 1: hipLaunchKernelGGL((edt2d_rows4_kernel<1>), rgrid, dim3(threads), lds, st, f0, H, W, o); break;
-          case 2: hipLaunchKernelGGL((edt2d_rows4_kernel<2>), rgrid, dim3(threads), lds, st, f0, H, W, o); break;
-          default: hipLaunchKernelGGL((edt2d_rows4_kernel<4>), rgrid, dim3(threads), lds, st, f0, H, W, o); break;
+          case 1: if (env) TM_ROWS4(1, true); else TM_ROWS4(1, false); break;
+          case 2: if (env) TM_ROWS4(2, true); else TM_ROWS4(2, false); break;
+          default: if (env) TM_ROWS4(4, true); else TM_ROWS4(4, false); break;
         }
+#undef TM_ROWS4
         TM_LAUNCH_CHECK();
         return TM_OK;
       }
@@ -896,12 +930,16 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
       TM_LAUNCH_CHECK();
       const dim3 rgrid((H + R - 1) / R, (unsigned)batch);
       const size_t lds = edt2d_rows_lds(R, W);
+#define TM_ROWS(RR, E) \
+  hipLaunchKernelGGL((edt2d_rows_kernel<RR, E>), rgrid, dim3(256), lds, st, f0, H, W, o)
+      const bool env = W >= kEnvMinW;
       switch (R) {
-        case 1: hipLaunchKernelGGL((edt2d_rows_kernel<1>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
-        case 2: hipLaunchKernelGGL((edt2d_rows_kernel<2>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
-        case 4: hipLaunchKernelGGL((edt2d_rows_kernel<4>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
-        default: hipLaunchKernelGGL((edt2d_rows_kernel<8>), rgrid, dim3(256), lds, st, f0, H, W, o); break;
+        case 1: if (env) TM_ROWS(1, true); else TM_ROWS(1, false); break;
+        case 2: if (env) TM_ROWS(2, true); else TM_ROWS(2, false); break;
+        case 4: if (env) TM_ROWS(4, true); else TM_ROWS(4, false); break;
+        default: if (env) TM_ROWS(8, true); else TM_ROWS(8, false); break;
       }
+#undef TM_ROWS
       TM_LAUNCH_CHECK();
       return TM_OK;
     };
@@ -909,7 +947,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     const int64_t key[4] = {p2(H), p2(W), o.idx ? (o.idx64 ? 64 : 32) : 0, o.dist ? 1 : 0};
     const int dflt = W >= 512 ? 1 : 3;  // 4 columns/thread; 1 or 2 rows per band
     bool ran = false;
-    const int v = tune_pick("edt2d", key, 4, 13, dflt, run, st, &ran);
+    const int v = tune_pick("edt2d", key, 4, 16, dflt, run, st, &ran);
     if (ran) return TM_OK;
     const int rc = run(v);
     // a cached choice that does not apply here (imported from another
