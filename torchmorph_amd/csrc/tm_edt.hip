@@ -591,16 +591,99 @@ __global__ __launch_bounds__(256) void edt2d_rows_kernel(const int16_t* __restri
   edt2d_band_rows<ENV>(hs, fr, rows, img, H, W, y0, o, e);
 }
 
+// Window keys of the 4-pixel row pass: column x of a band row holds
+// key = (h << 12) | x, h its squared column distance, or kKeyNone for a
+// column without background (and the 8 pad entries on both sides).  With H
+// <= 1024 and k <= 8, every h + k^2 of a real column is below kKeyNone, and
+// (kKeyNone + 64) << 12 still fits 32 bits, so the candidate of step k is
+// key + (k^2 << 12) and the minimum key is the lowest column attaining the
+// minimum: one add and one unsigned min per candidate.
+constexpr uint32_t kKeyNone = (1u << 20) - 65u;
+
+__device__ __forceinline__ uint32_t window_key(int f, int y, int x) {
+  const int dy = y - f;
+  return ((f >= 0 ? (uint32_t)(dy * dy) : kKeyNone) << 12) | (uint32_t)x;
+}
+
+// The window argmin of the four pixels x0 .. x0 + 3 of one row (W <= 4096):
+// kr[-8 .. W + 7] readable, 16-byte aligned at kr - 8.  The 20 keys around
+// the four pixels come as five 16-byte LDS reads and the first 8 window steps
+// are key minima on registers; a pixel whose best still admits a step beyond
+// 8 (none found, or best >= 81) continues with the one-pixel blocked loop of
+// edt2d_band_rows (same candidate order and tie rule).  best = -1: no
+// candidate.  With CAP, returns whether a pixel reached kWinCap steps (its
+// best is then not final).
+template <bool CAP>
+__device__ __forceinline__ bool rows4_quad(const uint32_t* kr, int x0, int W, int* best,
+                                           int* bp) {
+  uint32_t h[20];  // kr[x0 - 8 .. x0 + 11]
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const uint4 t = *(const uint4*)(kr + x0 - 8 + 4 * v);
+    h[4 * v] = t.x;
+    h[4 * v + 1] = t.y;
+    h[4 * v + 2] = t.z;
+    h[4 * v + 3] = t.w;
+  }
+  bool capped = false;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t m = h[8 + i];
+#pragma unroll
+    for (int k = 1; k <= 8; ++k) {
+      const uint32_t add = (uint32_t)(k * k) << 12;
+      m = min(m, h[8 + i - k] + add);
+      m = min(m, h[8 + i + k] + add);
+    }
+    const uint32_t mv = m >> 12;
+    best[i] = mv >= kKeyNone ? -1 : (int)mv;
+    bp[i] = (int)(m & 4095u);
+    const int x = x0 + i;
+    const int reach = max(x, W - 1 - x);
+    if (best[i] < 0 || best[i] >= 81) {
+      // steps 9, 10, ...: the blocked loop of edt2d_band_rows (rolled: the
+      // cap bounds its trip count, and unrolled it would quadruple the code)
+#pragma unroll 1
+      for (int k0 = 8; k0 < reach; k0 += 8) {
+        if (best[i] >= 0 && (k0 + 1) * (k0 + 1) > best[i]) break;
+        if (CAP && k0 >= kWinCap) {
+          capped = true;
+          break;
+        }
+        int hl[8], hv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int k = k0 + 1 + u;
+          const uint32_t a = kr[max(x - k, 0)] >> 12, c = kr[min(x + k, W - 1)] >> 12;
+          hl[u] = a >= kKeyNone ? -1 : (int)a;
+          hv[u] = c >= kKeyNone ? -1 : (int)c;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int k = k0 + 1 + u;
+          const int kk = k * k;
+          if (x - k >= 0 && hl[u] >= 0 && (best[i] < 0 || hl[u] + kk <= best[i])) {
+            best[i] = hl[u] + kk;
+            bp[i] = x - k;
+          }
+          if (x + k < W && hv[u] >= 0 && (best[i] < 0 || hv[u] + kk < best[i])) {
+            best[i] = hv[u] + kk;
+            bp[i] = x + k;
+          }
+        }
+      }
+    }
+  }
+  return capped;
+}
+
 // Axis-1 pass with four adjacent pixels per lane (W % 4 == 0, W <= 4096).
-// One workgroup per band of R rows.  The band's squared column distances sit
-// in LDS with 8 entries of -1 (no candidate) on both sides of every row, so a
-// lane reads the 20 entries around its 4 pixels as five 16-byte LDS reads and
-// runs the first 8 window steps of all four pixels on registers, with no
-// bounds checks; a pixel whose best still admits a step beyond 8 (none
-// found, or best >= 81) continues with the one-pixel blocked loop.  The
-// candidate order and tie rule are edt2d_band_rows's, so the results are the
-// same bit for bit.  Outputs leave as 16-byte stores: four float32
-// distances, and per index plane four int32 or two pairs of int64.
+// One workgroup per band of R rows.  The band's window keys sit in LDS with
+// 8 kKeyNone entries on both sides of every row; every lane runs rows4_quad
+// on its four pixels, so the results are edt2d_band_rows's bit for bit.
+// Outputs leave as 16-byte stores: four float32 distances, and per index
+// plane four int32 or two pairs of int64.  (The envelope fallback rebuilds a
+// flagged row from the feature rows fr, overwriting its keys.)
 template <int R, bool ENV>
 __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restrict__ f0, int H,
                                                           int W, EdtOut o) {
@@ -618,7 +701,7 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
   if (ENV) row_env_clear(e, R);
   for (int i = threadIdx.x; i < rows * 16; i += blockDim.x) {
     const int r = i >> 4, k = i & 15;
-    hs[r * P + (k < 8 ? k : W + k)] = -1;
+    hs[r * P + (k < 8 ? k : W + k)] = (int32_t)(kKeyNone << 12);
   }
   for (int p0 = threadIdx.x * 8; p0 < npx; p0 += blockDim.x * 8) {
     const uint4 q = *(const uint4*)(src + p0);  // 8 features, W % 8 == 0 keeps rows whole
@@ -628,9 +711,8 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int f = (int16_t)((w4[i >> 1] >> (16 * (i & 1))) & 0xffffu);
-      const int dy = y0 + r - f;
       fr[r * W + x + i] = (int16_t)f;
-      hs[r * P + 8 + x + i] = f >= 0 ? dy * dy : -1;
+      hs[r * P + 8 + x + i] = (int32_t)window_key(f, y0 + r, x + i);
     }
   }
   __syncthreads();
@@ -638,70 +720,9 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
     const int p0 = qd * 4;
     const int r = p0 / W;
     const int x0 = p0 - r * W;
-    const int32_t* hr = hs + r * P + 8;  // hr[-8 .. W + 7] readable
-    int h[20];                            // hr[x0 - 8 .. x0 + 11]
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const int4 t = *(const int4*)(hr + x0 - 8 + 4 * v);
-      h[4 * v] = t.x;
-      h[4 * v + 1] = t.y;
-      h[4 * v + 2] = t.z;
-      h[4 * v + 3] = t.w;
-    }
+    const uint32_t* kr = (const uint32_t*)(hs + r * P + 8);  // kr[-8 .. W + 7] readable
     int best[4], bp[4];
-    bool capped = false;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      best[i] = h[8 + i];
-      bp[i] = x0 + i;
-#pragma unroll
-      for (int k = 1; k <= 8; ++k) {
-        const int kk = k * k;
-        const int hl = h[8 + i - k], hv = h[8 + i + k];
-        if (hl >= 0 && (best[i] < 0 || hl + kk <= best[i])) {
-          best[i] = hl + kk;
-          bp[i] = x0 + i - k;
-        }
-        if (hv >= 0 && (best[i] < 0 || hv + kk < best[i])) {
-          best[i] = hv + kk;
-          bp[i] = x0 + i + k;
-        }
-      }
-      const int x = x0 + i;
-      const int reach = max(x, W - 1 - x);
-      if (best[i] < 0 || best[i] >= 81) {
-        // steps 9, 10, ...: the blocked loop of edt2d_band_rows (rolled: the
-        // cap bounds its trip count, and unrolled it would quadruple the code)
-#pragma unroll 1
-        for (int k0 = 8; k0 < reach; k0 += 8) {
-          if (best[i] >= 0 && (k0 + 1) * (k0 + 1) > best[i]) break;
-          if (ENV && k0 >= kWinCap) {
-            capped = true;
-            break;
-          }
-          int hl[8], hv[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int k = k0 + 1 + u;
-            hl[u] = hr[max(x - k, 0)];
-            hv[u] = hr[min(x + k, W - 1)];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int k = k0 + 1 + u;
-            const int kk = k * k;
-            if (x - k >= 0 && hl[u] >= 0 && (best[i] < 0 || hl[u] + kk <= best[i])) {
-              best[i] = hl[u] + kk;
-              bp[i] = x - k;
-            }
-            if (x + k < W && hv[u] >= 0 && (best[i] < 0 || hv[u] + kk < best[i])) {
-              best[i] = hv[u] + kk;
-              bp[i] = x + k;
-            }
-          }
-        }
-      }
-    }
+    const bool capped = rows4_quad<ENV>(kr, x0, W, best, bp);
     if (ENV && capped) {
       row_env_mark(e, r, x0 >> 2);
       continue;
