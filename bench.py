@@ -47,7 +47,9 @@ HEADLINE_KEY = "edt2d 256 256 64 1"
 TRAFFIC_PER_CALL = {0: 13_260_000, 8: 13_397_504}
 HEADLINE_KERNELS = {v: "edt2d_cols_kernel + edt2d_rows_kernel (one call)" for v in range(8)}
 HEADLINE_KERNELS.update({v: "edt2d_cols_kernel + edt2d_rows4_kernel (one call)"
-                         for v in (8, 9, 10, 11, 12)})
+                         for v in (8, 9, 10, 11, 12, 13, 14, 15)})
+HEADLINE_KERNELS.update({v: "edt2d_colsum_kernel + edt2d_rowsum_kernel (one call)"
+                         for v in (16, 17)})
 
 
 def parse():

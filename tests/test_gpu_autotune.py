@@ -38,7 +38,7 @@ def test_edt2d_every_variant_matches_oracle(tm, oracle_mod, shape, idx):
     key = f"edt2d {_p2(H)} {_p2(W)} {bits} 1"
     x = torch.as_tensor(mask, device="cuda")
     try:
-        for v in range(16):
+        for v in range(18):
             autotune.clear()
             autotune.load({key: v})
             if idx == "none":
@@ -58,7 +58,7 @@ def test_edt2d_variants_all_foreground_and_background(tm, oracle_mod):
     from torchmorph_amd import autotune
 
     try:
-        for v in range(16):
+        for v in range(18):
             autotune.clear()
             autotune.load({"edt2d 64 128 64 1": v})
             for fill in (True, False):
@@ -118,7 +118,7 @@ def test_edt2d_sparse_rows_every_variant(tm, oracle_mod, case, idx):
     key = f"edt2d {_p2(H)} {_p2(W)} {bits} 1"
     x = torch.as_tensor(mask, device="cuda")
     try:
-        for v in range(16):
+        for v in range(18):
             autotune.clear()
             autotune.load({key: v})
             if idx == "none":

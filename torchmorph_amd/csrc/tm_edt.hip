@@ -84,6 +84,24 @@ __device__ __forceinline__ uint32_t word_bits(uint32_t w) {
          ((uint32_t)((w & 0xff0000u) == 0) << 2) | ((uint32_t)((w & 0xff000000u) == 0) << 3);
 }
 
+// Column summary of one (column, segment of at most 32 rows): background
+// bits of the segment's rows, the last background row above the segment and
+// the first below it (-1: none).
+__device__ __forceinline__ uint64_t colsum_pack(uint32_t mask, int before, int after) {
+  return (uint64_t)mask | ((uint64_t)(uint16_t)before << 32) | ((uint64_t)(uint16_t)after << 48);
+}
+
+// Nearest background row of row y in a column from its summary (segment
+// starting at row ys): the row above wins ties, as in the column pass.
+__device__ __forceinline__ int colsum_nearest(uint64_t q, int y, int ys) {
+  const uint32_t m = (uint32_t)q;
+  const int b = y - ys;
+  const uint32_t up = m & (b >= 31 ? 0xffffffffu : ((2u << b) - 1u)), dn = m >> b;
+  const int pv = up ? ys + 31 - __builtin_clz(up) : (int16_t)(uint16_t)(q >> 32);
+  const int nx = dn ? y + __builtin_ctz(dn) : (int16_t)(uint16_t)(q >> 48);
+  return (pv >= 0 && (nx < 0 || (y - pv) <= (nx - y))) ? pv : nx;
+}
+
 // CPT adjacent columns per thread (1 or 4), 16 column groups x 16 row
 // segments per workgroup, NCH chunks of 16 rows per segment.  VEC: 4-byte
 // aligned rows of a byte mask (W % 4 == 0).
@@ -705,7 +723,7 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
   }
   for (int p0 = threadIdx.x * 8; p0 < npx; p0 += blockDim.x * 8) {
     const uint4 q = *(const uint4*)(src + p0);  // 8 features, W % 8 == 0 keeps rows whole
-    const int r = p0 / W;
+    const int r = R == 1 ? 0 : p0 / W;
     const int x = p0 - r * W;
     const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
@@ -718,7 +736,7 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
   __syncthreads();
   for (int qd = threadIdx.x; qd * 4 < npx; qd += blockDim.x) {
     const int p0 = qd * 4;
-    const int r = p0 / W;
+    const int r = R == 1 ? 0 : p0 / W;
     const int x0 = p0 - r * W;
     const uint32_t* kr = (const uint32_t*)(hs + r * P + 8);  // kr[-8 .. W + 7] readable
     int best[4], bp[4];
@@ -730,6 +748,95 @@ __global__ __launch_bounds__(256) void edt2d_rows4_kernel(const int16_t* __restr
     edt2d_store<4>(o, fr + r * W, img, H, W, y0 + r, x0, best, bp);
   }
   if (ENV) edt2d_rows_resolve<4>(e, hs, P, 8, fr, rows, img, H, W, y0, o);
+}
+
+// Summary path (unit sampling, H <= 1024, W % 8 == 0).  Column pass: a
+// workgroup covers CG columns x (256 / CG) segments of seglen <= 32 rows;
+// each thread loads its segment's rows of one column, keeps them as a
+// background bit mask, the segments exchange first/last background rows
+// through LDS, and each thread writes one summary (colsum_pack) to
+// sm[image][segment][column] -- 8 bytes per 16 or 32 pixels instead of one
+// int16 feature row per pixel.
+template <typename TIn, int NCH, int CG>
+__global__ __launch_bounds__(256) void edt2d_colsum_kernel(const TIn* __restrict__ in, int H,
+                                                           int W, int seglen,
+                                                           uint64_t* __restrict__ sm) {
+  constexpr int SEGS = 256 / CG;
+  __shared__ int16_t sfirst[SEGS][CG];
+  __shared__ int16_t slast[SEGS][CG];
+  const int cg = threadIdx.x & (CG - 1);
+  const int seg = threadIdx.x / CG;
+  const int ncb = (W + CG - 1) / CG;
+  const int lb = xcd_swizzle(blockIdx.x, gridDim.x);  // an image's column blocks share an XCD
+  const int img = lb / ncb;
+  const int xb = (lb - img * ncb) * CG + cg;
+  const int ys = seg * seglen;
+  const int nr = min(seglen, H - ys);
+  const bool live = xb < W && nr > 0;
+  TIn raw[NCH * 16];
+  if (live) {
+    const TIn* col = in + ((int64_t)img * H + ys) * W + xb;
+#pragma unroll
+    for (int b = 0; b < NCH * 16; ++b) raw[b] = col[(int64_t)min(b, nr - 1) * W];
+  }
+  uint32_t m = 0;
+  if (live) {
+#pragma unroll
+    for (int b = 0; b < NCH * 16; ++b) m |= (uint32_t)(b < nr && raw[b] == TIn(0)) << b;
+  }
+  sfirst[seg][cg] = m ? (int16_t)(ys + __builtin_ctz(m)) : (int16_t)-1;
+  slast[seg][cg] = m ? (int16_t)(ys + 31 - __builtin_clz(m)) : (int16_t)-1;
+  __syncthreads();
+  if (!live) return;
+  int before = -1, after = -1;
+  for (int s = seg - 1; s >= 0 && before < 0; --s) before = slast[s][cg];
+  for (int s = seg + 1; s < SEGS && after < 0; ++s) after = sfirst[s][cg];
+  sm[((int64_t)img * SEGS + seg) * W + xb] = colsum_pack(m, before, after);
+}
+
+// Summary path, row pass: one workgroup per row (min(256, W / 4) lanes,
+// rounded up to whole waves).  Every lane resolves four columns of the row
+// from their summaries (two 16-byte loads), writes their window keys and
+// feature rows to LDS, then runs rows4_quad on four pixels -- the results
+// of edt2d_rows4_kernel bit for bit.  ENV: rows of at least kEnvMinW pixels,
+// with the envelope fallback.
+template <bool ENV>
+__global__ __launch_bounds__(256) void edt2d_rowsum_kernel(const uint64_t* __restrict__ sm,
+                                                           int H, int W, int segs, int seglen,
+                                                           EdtOut o) {
+  extern __shared__ int32_t lds[];
+  const int P = W + 16;
+  uint32_t* kb = (uint32_t*)lds;           // [W + 16] window keys, entry x at 8 + x
+  int16_t* fr = (int16_t*)(lds + P);       // [W] feature rows
+  const RowEnvLds e = row_env_lds((char*)lds + ((P * 4 + W * 2 + 255) & ~255), 1, W, 4);
+  const int img = blockIdx.y;
+  const int y = blockIdx.x;
+  const int sg = y / seglen;
+  const uint64_t* srow = sm + ((int64_t)img * segs + sg) * W;
+  if (ENV) row_env_clear(e, 1);
+  if (threadIdx.x < 16) kb[threadIdx.x < 8 ? threadIdx.x : W + threadIdx.x] = kKeyNone << 12;
+  for (int x = threadIdx.x * 4; x < W; x += blockDim.x * 4) {
+    const ulonglong2 q0 = *(const ulonglong2*)(srow + x);
+    const ulonglong2 q1 = *(const ulonglong2*)(srow + x + 2);
+    const uint64_t qq[4] = {q0.x, q0.y, q1.x, q1.y};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int f = colsum_nearest(qq[i], y, sg * seglen);
+      fr[x + i] = (int16_t)f;
+      kb[8 + x + i] = window_key(f, y, x + i);
+    }
+  }
+  __syncthreads();
+  for (int x0 = threadIdx.x * 4; x0 < W; x0 += blockDim.x * 4) {
+    int best[4], bp[4];
+    const bool capped = rows4_quad<ENV>(kb + 8, x0, W, best, bp);
+    if (ENV && capped) {
+      row_env_mark(e, 0, x0 >> 2);
+      continue;
+    }
+    edt2d_store<4>(o, fr, img, H, W, y, x0, best, bp);
+  }
+  if (ENV) edt2d_rows_resolve<4>(e, (int32_t*)kb, P, 8, fr, 1, img, H, W, y, o);
 }
 
 // --------------------------------------------------------------- N-D --
@@ -855,12 +962,21 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     // bands of 1 row, 8 or 4 columns per column-pass workgroup (more
     // workgroups for narrow images); v = 13, 14, 15: bands of 1 row, four
     // adjacent columns per column-pass thread (one 4-byte load per row, one
-    // 8-byte store per row), 16, 8 or 4 column groups per workgroup
+    // 8-byte store per row), 16, 8 or 4 column groups per workgroup; v = 16,
+    // 17: bands of 1 row, the column pass writes one summary per (column,
+    // segment of at most 32 rows) instead of one feature row per pixel, 16 or
+    // 8 column groups per workgroup
     auto run = [&](int v) -> int {
       if (v >= 8) {
         const int R = v <= 10 ? 1 << (v - 8) : 1;  // 1, 2, 4 rows per band
-        const int CPT = v >= 13 ? 4 : 1;
-        const int CG = (v <= 10 || v == 13) ? 16 : ((v == 11 || v == 14) ? 8 : 4);
+        const int CPT = (v >= 13 && v <= 15) ? 4 : 1;
+        const int CG = (v <= 10 || v == 13 || v == 16) ? 16 : ((v == 11 || v == 14 || v == 17) ? 8 : 4);
+        const bool sum = v >= 16;
+        const int segs = 256 / CG, seglen = (H + segs - 1) / segs;
+        if (sum && (seglen > 32 || (size_t)segs * 8 > (size_t)H * 2)) {
+          set_error("tm_edt: column summaries do not apply to this height");
+          return TM_ERR_ARG;
+        }
         const size_t lds = edt2d_rows4_lds(R, W);
         auto a16 = [](const void* p) { return ((uintptr_t)p % 16) == 0; };
         if (W % 8 != 0 || lds > 96 * 1024 || batch > 65535 || !a16(o.dist) || !a16(o.idx)) {
@@ -892,7 +1008,19 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
   if (seg <= 16) M(1, G);                    \
   else if (seg <= 32) M(2, G);               \
   else M(4, G)
-        if (CPT == 1) {
+#define TM_COLSS(NCH, G)                                                                \
+  hipLaunchKernelGGL((edt2d_colsum_kernel<TIn, NCH, G>), cgrid, dim3(256), 0, st, in, H, W, \
+                     seglen, sm)
+        if (sum) {
+          uint64_t* sm = (uint64_t*)f0;
+          if (CG == 16) {
+            if (seg <= 16) TM_COLSS(1, 16);
+            else TM_COLSS(2, 16);
+          } else {
+            if (seg <= 16) TM_COLSS(1, 8);
+            else TM_COLSS(2, 8);
+          }
+        } else if (CPT == 1) {
           if (CG == 16) { TM_COLS_CG(TM_COLS1, 16); }
           else if (CG == 8) { TM_COLS_CG(TM_COLS1, 8); }
           else { TM_COLS_CG(TM_COLS1, 4); }
@@ -901,6 +1029,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
           else if (CG == 8) { TM_COLS_CG(TM_COLS4, 8); }
           else { TM_COLS_CG(TM_COLS4, 4); }
         }
+#undef TM_COLSS
 #undef TM_COLS_CG
 #undef TM_COLS4
 #undef TM_COLS1
@@ -910,6 +1039,17 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
 #define TM_ROWS4(RR, E) \
   hipLaunchKernelGGL((edt2d_rows4_kernel<RR, E>), rgrid, dim3(threads), lds, st, f0, H, W, o)
         const bool env = W >= kEnvMinW;
+        if (sum) {
+          const uint64_t* sm = (const uint64_t*)f0;
+          if (env)
+            hipLaunchKernelGGL((edt2d_rowsum_kernel<true>), rgrid, dim3(threads), lds, st, sm, H,
+                               W, segs, seglen, o);
+          else
+            hipLaunchKernelGGL((edt2d_rowsum_kernel<false>), rgrid, dim3(threads), lds, st, sm, H,
+                               W, segs, seglen, o);
+          TM_LAUNCH_CHECK();
+          return TM_OK;
+        }
         switch (R) {
           case 1: if (env) TM_ROWS4(1, true); else TM_ROWS4(1, false); break;
           case 2: if (env) TM_ROWS4(2, true); else TM_ROWS4(2, false); break;
@@ -968,7 +1108,7 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
     const int64_t key[4] = {p2(H), p2(W), o.idx ? (o.idx64 ? 64 : 32) : 0, o.dist ? 1 : 0};
     const int dflt = W >= 512 ? 1 : 3;  // 4 columns/thread; 1 or 2 rows per band
     bool ran = false;
-    const int v = tune_pick("edt2d", key, 4, 16, dflt, run, st, &ran);
+    const int v = tune_pick("edt2d", key, 4, 18, dflt, run, st, &ran);
     if (ran) return TM_OK;
     const int rc = run(v);
     // a cached choice that does not apply here (imported from another
