@@ -518,6 +518,25 @@ def extra_ragged():
         outs[sched] = plan.distances.clone()
         del plan
     res["schedules_agree"] = bool(all(torch.equal(outs["per_pass"], v) for v in outs.values()))
+    # the per-pass launches captured once as a HIP graph and replayed; the
+    # row-pass chunk depth the autotuner picked for this shape class
+    from torchmorph_amd import autotune
+    gplan = RaggedDistanceTransform(shapes, ops, samp, schedule="per_pass", graph=True,
+                                    device="cuda")
+    gflat = gplan.pack(masks)
+    gplan.run(gflat)
+    ms = _timed(lambda: gplan.run(gflat, check=False), 5, st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        gplan.run(gflat, check=False)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / 5 * 1e3
+    res["per_pass_graph"] = {"ms": round(ms, 3), "wall_ms": round(wall, 3),
+                             "equal": bool(torch.equal(gplan.distances, outs["per_pass"])),
+                             "tuned": {k: v for k, v in autotune.export().items()
+                                       if k.startswith("ragged_rows")}}
+    del gplan
     plan = RaggedDistanceTransform(shapes, ops, samp, schedule="per_pass", device="cuda")
     plan.run(plan.pack(masks))
     out = plan.results()

@@ -139,3 +139,81 @@ def test_ragged_argument_errors(tm):
         plan([torch.zeros(4, 4, device="cuda")])
     with pytest.raises(ValueError):
         plan.run(torch.zeros(10, dtype=torch.uint8, device="cuda"))
+
+
+def _mixed_case(seed):
+    rng = np.random.default_rng(seed)
+    shapes = [tuple(int(s) for s in rng.integers(20, 300, size=2)) for _ in range(20)]
+    shapes += [tuple(int(s) for s in rng.integers(8, 60, size=3)) for _ in range(6)]
+    masks = _masks(rng, shapes, 0.85)
+    samp = [list(rng.uniform(0.5, 2.0, size=len(sh))) for sh in shapes]
+    return shapes, masks, samp
+
+
+def _check_mixed(oracle_mod, masks, samp, edt, taxi, cheb, items):
+    for k in items:
+        m = masks[k][None, None]
+        od, _ = oracle_mod.edt(m, samp[k])
+        assert torch.equal(edt[k].cpu(), _f32(od[0, 0])), ("edt", k)
+        for metric, res in (("taxicab", taxi), ("chessboard", cheb)):
+            oc, _ = oracle_mod.cdt(m, metric)
+            assert torch.equal(res[k].cpu(), _f32(oc[0, 0])), (metric, k)
+
+
+def test_ragged_graph_replays_match_eager(tm, oracle_mod):
+    """A per-pass plan captured as a HIP graph: every replay, with eager work
+    and synchronisations between replays, equals the eager run bit for bit;
+    new masks copied into the plan's input buffer replay the same graph."""
+    from torchmorph_amd.ragged import RaggedDistanceTransform
+
+    shapes, masks, samp = _mixed_case(11)
+    ops = ("edt", "cdt_taxicab", "cdt_chessboard")
+    eager = RaggedDistanceTransform(shapes, ops, samp, device="cuda")
+    want = [[t.clone() for t in r] for r in eager([torch.as_tensor(m, device="cuda") for m in masks])]
+    _check_mixed(oracle_mod, masks, samp, *want, items=range(len(shapes)))
+    plan = RaggedDistanceTransform(shapes, ops, samp, graph=True, device="cuda")
+    dm = [torch.as_tensor(m, device="cuda") for m in masks]
+    for rep in range(4):
+        plan.distances.fill_(-7.0)  # stale outputs must be overwritten by the replay
+        got = plan(dm)
+        for o in range(3):
+            for k in range(len(shapes)):
+                assert torch.equal(got[o][k], want[o][k]), (rep, o, k)
+        # eager work between replays
+        tm.euclidean_distance_transform(dm[0][None, None])
+        torch.cuda.synchronize()
+    assert len(plan._graphs) == 1
+    # new masks through the same graph
+    rng = np.random.default_rng(12)
+    masks2 = _masks(rng, shapes, 0.8)
+    got2 = plan([torch.as_tensor(m, device="cuda") for m in masks2])
+    assert len(plan._graphs) == 1
+    _check_mixed(oracle_mod, masks2, samp, *got2, items=(0, 5, 19, 20, 25))
+
+
+def test_ragged_every_row_chunk_variant(tm, oracle_mod):
+    """Each chunk depth the tuner may pick for the row-staged passes gives the
+    same bits (forced through the cache)."""
+    from torchmorph_amd import autotune
+    from torchmorph_amd.ragged import RaggedDistanceTransform
+
+    shapes, masks, samp = _mixed_case(13)
+    ops = ("edt", "cdt_taxicab", "cdt_chessboard")
+    dm = [torch.as_tensor(m, device="cuda") for m in masks]
+    try:
+        autotune.clear()
+        plan = RaggedDistanceTransform(shapes, ops, samp, device="cuda")
+        ref = [[t.clone() for t in r] for r in plan(dm)]
+        _check_mixed(oracle_mod, masks, samp, *ref, items=range(len(shapes)))
+        keys = [k for k in autotune.export() if k.startswith("ragged_rows")]
+        assert len(keys) == 1, keys
+        for v in range(4):
+            autotune.clear()
+            autotune.load({keys[0]: v})
+            plan.distances.fill_(-7.0)
+            got = plan(dm)
+            for o in range(3):
+                for k in range(len(shapes)):
+                    assert torch.equal(got[o][k], ref[o][k]), (v, o, k)
+    finally:
+        autotune.clear()

@@ -55,6 +55,7 @@
 #include "tm_dt_chunk.h"
 #include "tm_dt_fast.h"
 #include "tm_dt_lines.h"
+#include "tm_tune.h"
 
 namespace tmh {
 
@@ -367,8 +368,10 @@ __global__ __launch_bounds__(kRgThreads) void rg_cheb_window_kernel(
 }
 
 // Schedule 2, row-staged passes along the last axis (d = ndim - 1): each
-// workgroup owns kCLanes consecutive rows of one job.
-template <int KIND>
+// workgroup owns kCLanes consecutive rows of one job and walks them in chunks
+// of CH voxels (a launch variant: more voxels per chunk mean fewer barriers
+// and more LDS and registers per workgroup).
+template <int KIND, int CH>
 __global__ __launch_bounds__(kCLanes) void rg_rows_kernel(RgArgs a,
                                                           const RgBlk* __restrict__ blks) {
   extern __shared__ int32_t lds_rows[];
@@ -389,22 +392,27 @@ __global__ __launch_bounds__(kCLanes) void rg_rows_kernel(RgArgs a,
     uint32_t* E = rows_stack_e(area);
     uint16_t* F1 = rows_stack_f1(area, L, nl_axis);
     if (KIND == RG_EDT_ROWS2)
-      edt_rows_final<1, 32>(src, numel, sh, bk.line0, bk.nlines, jb.sp, E, F1, nl_axis,
+      edt_rows_final<1, CH>(src, numel, sh, bk.line0, bk.nlines, jb.sp, E, F1, nl_axis,
                             a.dist + jb.out_off, lds_rows);
     else
-      edt_rows_final<2, 16>(src, numel, sh, bk.line0, bk.nlines, jb.sp, E, F1, nl_axis,
+      edt_rows_final<2, CH>(src, numel, sh, bk.line0, bk.nlines, jb.sp, E, F1, nl_axis,
                             a.dist + jb.out_off, lds_rows);
   } else {
-    cdt_rows_taxicab<32>((int32_t*)(a.dist + jb.out_off), sh, bk.line0, bk.nlines, lds_rows);
+    cdt_rows_taxicab<CH>((int32_t*)(a.dist + jb.out_off), sh, bk.line0, bk.nlines, lds_rows);
   }
 }
 
-// LDS bytes of rg_rows_kernel<KIND>.
-static size_t rows_lds(int kind) {
-  if (kind == RG_EDT_ROWS2) return edt_rows_lds<1, 32>();
-  if (kind == RG_EDT_ROWS3) return edt_rows_lds<2, 16>();
-  return (size_t)kCLanes * 33 * sizeof(int32_t);
+// LDS bytes of rg_rows_kernel<KIND, CH>.
+static size_t rows_lds(int kind, int ch) {
+  if (kind == RG_EDT_ROWS3) return (size_t)2 * kCLanes * (ch + 1) * sizeof(int32_t);
+  return (size_t)kCLanes * (ch + 1) * sizeof(int32_t);
 }
+
+// Chunk depths (2-D EDT rows, 3-D EDT rows, taxicab rows) of per-pass launch
+// variant v; variant 0 is the default.
+constexpr int kRgRowVariants = 4;
+static const int kRgRowCH[kRgRowVariants][3] = {{32, 16, 32}, {16, 8, 16}, {64, 32, 64},
+                                                {16, 16, 16}};
 
 // Schedule 2, chamfer with indices: one workgroup per job runs raster pass d
 // (every thread takes part: the pass has workgroup barriers).
@@ -685,7 +693,7 @@ static int grid_limit(int dtype_slot, const void* kern) {
 
 template <typename TIn>
 static int rg_launch(const TIn* in, const RgPlan& p, RgArgs a, int schedule, int dslot,
-                     hipStream_t st) {
+                     hipStream_t st, int variant = 0) {
   const int nctr = kRgJob0 + (int)p.jobs.size();
   hipLaunchKernelGGL(rg_zero_kernel, dim3(grid_for(nctr, 256)), dim3(256), 0, st, a.ctr, nctr);
   TM_LAUNCH_CHECK();
@@ -716,17 +724,31 @@ static int rg_launch(const TIn* in, const RgPlan& p, RgArgs a, int schedule, int
           hipLaunchKernelGGL(rg_cheb_window_kernel, g, dim3(kRgThreads), 0, st, a, b, d);
           break;
         case RG_EDT_ROWS2:
-          hipLaunchKernelGGL((rg_rows_kernel<RG_EDT_ROWS2>), g, dim3(kCLanes),
-                             rows_lds(RG_EDT_ROWS2), st, a, b);
-          break;
         case RG_EDT_ROWS3:
-          hipLaunchKernelGGL((rg_rows_kernel<RG_EDT_ROWS3>), g, dim3(kCLanes),
-                             rows_lds(RG_EDT_ROWS3), st, a, b);
+        case RG_CDT_TAXICAB_ROWS: {
+          const int slot = ln.kind == RG_EDT_ROWS2 ? 0 : (ln.kind == RG_EDT_ROWS3 ? 1 : 2);
+          const int ch = kRgRowCH[variant][slot];
+          const size_t lds = rows_lds(ln.kind, ch);
+#define TM_RG_ROWS(K, C) \
+  hipLaunchKernelGGL((rg_rows_kernel<K, C>), g, dim3(kCLanes), lds, st, a, b)
+#define TM_RG_ROWS_CH(K)             \
+  switch (ch) {                      \
+    case 8: TM_RG_ROWS(K, 8); break;   \
+    case 16: TM_RG_ROWS(K, 16); break; \
+    case 32: TM_RG_ROWS(K, 32); break; \
+    default: TM_RG_ROWS(K, 64); break; \
+  }
+          if (ln.kind == RG_EDT_ROWS2) {
+            TM_RG_ROWS_CH(RG_EDT_ROWS2);
+          } else if (ln.kind == RG_EDT_ROWS3) {
+            TM_RG_ROWS_CH(RG_EDT_ROWS3);
+          } else {
+            TM_RG_ROWS_CH(RG_CDT_TAXICAB_ROWS);
+          }
+#undef TM_RG_ROWS_CH
+#undef TM_RG_ROWS
           break;
-        case RG_CDT_TAXICAB_ROWS:
-          hipLaunchKernelGGL((rg_rows_kernel<RG_CDT_TAXICAB_ROWS>), g, dim3(kCLanes),
-                             rows_lds(RG_CDT_TAXICAB_ROWS), st, a, b);
-          break;
+        }
         default:
           hipLaunchKernelGGL((rg_raster_kernel<TIn>), g, dim3(kRgThreads), 0, st, in, a, b, d);
           break;
@@ -823,8 +845,28 @@ extern "C" int tm_ragged_dt(const tm_ragged_job* jobs, int64_t njobs, const void
   a.qslot = 0;
   a.lo = a.hi = 0;
   a.wait = 0;
-  if (input_dtype == TM_U8) return rg_launch((const uint8_t*)input, p, a, schedule, 0, st);
-  return rg_launch((const float*)input, p, a, schedule, 1, st);
+  auto run = [&](int v) -> int {
+    if (input_dtype == TM_U8) return rg_launch((const uint8_t*)input, p, a, schedule, 0, st, v);
+    return rg_launch((const float*)input, p, a, schedule, 1, st, v);
+  };
+  if (schedule != 2) return run(0);
+  // per-pass schedule: the row-staged chunk depth is tuned per shape class
+  // of the job list (item count, voxels, longest row, kinds of row pass)
+  int64_t vox = 0, maxrow = 0, kinds = 0;
+  for (const RgJob& j : p.jobs) {
+    vox += j.sh.numel;
+    maxrow = std::max<int64_t>(maxrow, j.sh.size[j.sh.ndim - 1]);
+  }
+  for (const RgLaunch& ln : p.launches)
+    if (ln.kind == RG_EDT_ROWS2 || ln.kind == RG_EDT_ROWS3 || ln.kind == RG_CDT_TAXICAB_ROWS)
+      kinds |= 1 << (ln.kind - RG_EDT_ROWS2);
+  if (!kinds) return run(0);
+  auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
+  const int64_t key[4] = {kinds, p2((int64_t)p.jobs.size()), p2((vox + 65535) >> 16), p2(maxrow)};
+  bool ran = false;
+  const int v = tune_pick("ragged_rows", key, 4, kRgRowVariants, 0, run, st, &ran);
+  if (ran) return TM_OK;
+  return run(v);
 }
 
 extern "C" int tm_ragged_dt_check(const void* workspace, void* stream) {

@@ -124,15 +124,22 @@ class RaggedDistanceTransform:
             of pass, one line per lane, row-staged passes along the contiguous
             axis), ``"persistent"`` (one launch, device dependency counters) or
             ``"per_level"`` (one launch per pass level).
+        graph: per-pass schedule only -- after the first call (which uploads
+            the plan's tables and lets the autotuner pick the row-pass chunk
+            depth for this shape class), the plan's launches are captured once
+            per input buffer as a HIP graph and replayed.  ``__call__`` then
+            copies the masks into one persistent input buffer.
         device: CUDA device of the masks and outputs.
     """
 
     def __init__(self, shapes: Sequence[Sequence[int]], ops: Sequence[str] = ("edt",),
                  sampling=None, return_distances: bool = True, return_indices: bool = False,
                  index_dtype: torch.dtype = torch.int64, schedule: str = "per_pass",
-                 device: torch.device | str | None = None):
+                 graph: bool = False, device: torch.device | str | None = None):
         if not shapes:
             raise ValueError("at least one mask shape is required")
+        if graph and schedule != "per_pass":
+            raise ValueError("graph replay is offered for the per_pass schedule only")
         if not return_distances and not return_indices:
             raise ValueError("at least one of return_distances/return_indices must be True")
         if index_dtype not in (torch.int32, torch.int64):
@@ -201,6 +208,9 @@ class RaggedDistanceTransform:
         self.indices = torch.empty(idx_off, dtype=index_dtype, device=self.device) \
             if return_indices else None
         self._uploaded = False
+        self.graph = bool(graph)
+        self._graphs: dict = {}       # (input pointer, dtype) -> captured launches
+        self._inbuf: Tensor | None = None
 
     @property
     def numel(self) -> int:
@@ -236,16 +246,33 @@ class RaggedDistanceTransform:
             raise ValueError("packed input must be bool, uint8 or float32")
         L = _lib.lib()
         with torch.cuda.device(self.device):
-            st = _lib.stream_of(self.device)
-            _lib.check(L.tm_ragged_dt(
-                self._jobs, self._njobs, _lib.ptr(flat), dt, _lib.ptr(self.distances),
-                _lib.ptr(self.indices),
-                _lib.TM_I64 if self.index_dtype == torch.int64 else _lib.TM_I32,
-                self.schedule, 0 if self._uploaded else 1, _lib.ptr(self.workspace),
-                self.workspace.numel(), st), "tm_ragged_dt")
-            self._uploaded = True
+            if self.graph and self._uploaded:
+                key = (flat.data_ptr(), flat.dtype)
+                g = self._graphs.get(key)
+                if g is None:
+                    g = torch.cuda.CUDAGraph()
+                    side = torch.cuda.Stream(device=self.device)
+                    side.wait_stream(torch.cuda.current_stream(self.device))
+                    with torch.cuda.graph(g, stream=side):
+                        self._launch(L, flat, dt)
+                    torch.cuda.current_stream(self.device).wait_stream(side)
+                    self._graphs[key] = g
+                g.replay()
+            else:
+                self._launch(L, flat, dt)
             if check:
+                st = _lib.stream_of(self.device)
                 _lib.check(L.tm_ragged_dt_check(_lib.ptr(self.workspace), st), "tm_ragged_dt")
+
+    def _launch(self, L, flat: Tensor, dt: int) -> None:
+        st = _lib.stream_of(self.device)
+        _lib.check(L.tm_ragged_dt(
+            self._jobs, self._njobs, _lib.ptr(flat), dt, _lib.ptr(self.distances),
+            _lib.ptr(self.indices),
+            _lib.TM_I64 if self.index_dtype == torch.int64 else _lib.TM_I32,
+            self.schedule, 0 if self._uploaded else 1, _lib.ptr(self.workspace),
+            self.workspace.numel(), st), "tm_ragged_dt")
+        self._uploaded = True
 
     def results(self):
         """One list per op, one entry per mask: distances, indices
@@ -266,6 +293,12 @@ class RaggedDistanceTransform:
 
     def __call__(self, masks: Sequence[Tensor] | Tensor, check: bool = True):
         flat = masks if isinstance(masks, Tensor) else self.pack(masks)
+        if self.graph and not isinstance(masks, Tensor):
+            # one persistent input buffer, so the captured graph is reused
+            if self._inbuf is None or self._inbuf.dtype != flat.dtype:
+                self._inbuf = torch.empty_like(flat)
+            self._inbuf.copy_(flat)
+            flat = self._inbuf
         self.run(flat, check=check)
         return self.results()
 
