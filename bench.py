@@ -68,6 +68,9 @@ def parse():
                     help="autotune table to preload ('' = tune live)")
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches instead of graph replays (for counter collection)")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="streams the headline graph spreads its independent calls over "
+                         "(MI355X has 4 hardware queues per process; 1 = one chain)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run a plain device-to-device copy of MB megabytes 8 times "
                          "(reference for the HBM byte counters) and exit")
@@ -185,18 +188,56 @@ def main():
     eager_wall_us = (time.perf_counter() - t_e) * 1e6 / (2 * S)
     eager_event_us = e0.elapsed_time(e1) * 1e3 / (2 * S)
 
-    # one step = one replay of a HIP graph holding one call per buffer set:
-    # S batches over S distinct buffer sets (> the 256 MB Infinity Cache)
-    graph = torch.cuda.CUDAGraph()
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        for k in range(S):
-            step(k)
-    torch.cuda.current_stream().wait_stream(side)
-    with torch.cuda.graph(graph):
-        for k in range(S):
-            step(k)
+    # The per-call reference: a HIP graph of the S calls as one chain on one
+    # stream (buffer sets > the 256 MB Infinity Cache), timed with HIP events;
+    # the calls do not overlap, so the per-call time is the two kernels'
+    # duration and the roofline below is a per-kernel figure.
+    def capture(nstreams):
+        streams = [torch.cuda.Stream() for _ in range(nstreams)]
+        for st_ in streams:
+            st_.wait_stream(torch.cuda.current_stream())
+        for k in range(S):  # warm-up, and each stream's workspace, outside capture
+            with torch.cuda.stream(streams[k * nstreams // S]):
+                step(k)
+        for st_ in streams:
+            torch.cuda.current_stream().wait_stream(st_)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cur = torch.cuda.current_stream()
+            for st_ in streams:
+                st_.wait_stream(cur)
+            # consecutive calls per branch (measured faster than round robin)
+            for k in range(S):
+                with torch.cuda.stream(streams[k * nstreams // S]):
+                    step(k)
+            for st_ in streams:
+                cur.wait_stream(st_)
+        return g, streams
+
+    def replay_ms(g, reps):
+        e0_ = torch.cuda.Event(enable_timing=True)
+        e1_ = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0_.record(stream)
+        for _ in range(reps):
+            g.replay()
+        e1_.record(stream)
+        torch.cuda.synchronize()
+        return e0_.elapsed_time(e1_)
+
+    graph1, _ = capture(1)
+    for _ in range(max(1, args.warmup)):
+        graph1.replay()
+    per_call_s = replay_ms(graph1, max(args.steps, 10)) / 1e3 / (max(args.steps, 10) * S)
+    achieved = nvox * BYTES_PER_VOX / per_call_s / 1e9
+
+    # One step = one replay of a HIP graph of the S independent calls spread
+    # over `streams` forked streams (parallel graph branches): one call's
+    # latency-bound kernels overlap another's, as independent requests do on
+    # MI355X's hardware queues.
+    NS = max(1, args.streams)
+    graph, _ = capture(NS) if NS > 1 else (graph1, None)
     for _ in range(max(1, args.warmup)):
         graph.replay()
     torch.cuda.synchronize()
@@ -222,9 +263,9 @@ def main():
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_time(ev1)
     calls = K * S
-    per_call_s = gpu_ms / 1e3 / calls
     value = calls * nvox / wall / 1e9
-    achieved = nvox * BYTES_PER_VOX / per_call_s / 1e9
+    step_share_s = gpu_ms / 1e3 / calls  # GPU time per call inside the concurrent step
+    aggregate = nvox * BYTES_PER_VOX / step_share_s / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -267,9 +308,10 @@ def main():
             "batch": B * C, "shape": [H, W], "calls_per_step": S,
             "voxels_per_step": S * nvox, "buffer_sets": S,
             "rotating_footprint_MB": round(footprint_mb, 1),
+            "streams": 1 if args.no_graph else NS,
             "timing": (f"eager: {S} Python calls per step" if args.no_graph else
-                       f"one step = one replay of a HIP graph of {S} calls, one per distinct "
-                       f"buffer set"),
+                       f"one step = one replay of a HIP graph of {S} independent calls, one per "
+                       f"distinct buffer set, spread over {NS} parallel streams"),
         },
         "parity": "bit-exact vs oracle" if parity else "MISMATCH",
         "eager_call": {"us_per_call_wall": round(eager_wall_us, 2),
@@ -284,6 +326,13 @@ def main():
             "kernel": HEADLINE_KERNELS.get(tuned.get(HEADLINE_KEY), "edt2d (two launches)"),
             "bytes_per_launch": nvox * BYTES_PER_VOX,
             "event_us_per_call": round(per_call_s * 1e6, 3),
+            "measured_on": f"a HIP graph of the {S} calls as one chain (no overlap): per-call "
+                           f"time = the two kernels back to back",
+            "concurrent": {"achieved": round(aggregate, 1),
+                           "frac": round(aggregate / HBM_PEAK_GBS, 4),
+                           "us_per_call": round(step_share_s * 1e6, 3),
+                           "note": f"algorithmic bytes over GPU time per call in the headline "
+                                   f"step ({NS} streams): an aggregate rate, the calls overlap"},
             "autotune": tuned,
         },
         "cpu_baseline": cpu,
