@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--streams", type=int, default=4,
                     help="streams the headline graph spreads its independent calls over "
                          "(MI355X has 4 hardware queues per process; 1 = one chain)")
+    ap.add_argument("--counters", choices=["morph3d"], default=None,
+                    help="only run the named workload's kernels a few times, untimed "
+                         "(the program the hardware-counter passes profile)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run a plain device-to-device copy of MB megabytes 8 times "
                          "(reference for the HBM byte counters) and exit")
@@ -140,6 +143,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 8}), flush=True)
+        return
+    if args.counters == "morph3d":
+        print(json.dumps({"counters_morph3d": counters_morph3d()}), flush=True)
         return
     if args.only:
         fn = {"morph3d": extra_morph3d, "ragged": extra_ragged, "grad": extra_grad_and_sinkhorn,
@@ -226,10 +232,14 @@ def main():
         torch.cuda.synchronize()
         return e0_.elapsed_time(e1_)
 
-    graph1, _ = capture(1)
-    for _ in range(max(1, args.warmup)):
-        graph1.replay()
-    per_call_s = replay_ms(graph1, max(args.steps, 10)) / 1e3 / (max(args.steps, 10) * S)
+    if args.no_graph:  # eager calls only (hardware-counter passes see every dispatch)
+        graph1 = None
+        per_call_s = eager_event_us / 1e6
+    else:
+        graph1, _ = capture(1)
+        for _ in range(max(1, args.warmup)):
+            graph1.replay()
+        per_call_s = replay_ms(graph1, max(args.steps, 10)) / 1e3 / (max(args.steps, 10) * S)
     achieved = nvox * BYTES_PER_VOX / per_call_s / 1e9
 
     # One step = one replay of a HIP graph of the S independent calls spread
@@ -237,9 +247,14 @@ def main():
     # latency-bound kernels overlap another's, as independent requests do on
     # MI355X's hardware queues.
     NS = max(1, args.streams)
-    graph, _ = capture(NS) if NS > 1 else (graph1, None)
-    for _ in range(max(1, args.warmup)):
-        graph.replay()
+    graph = None
+    if not args.no_graph:
+        graph, _ = capture(NS) if NS > 1 else (graph1, None)
+        for _ in range(max(1, args.warmup)):
+            graph.replay()
+    else:
+        for k in range(S):
+            step(k)
     torch.cuda.synchronize()
     # the graph's calls wrote the last buffer set: recheck its parity
     od, oi = oracle.edt(masks[S - 1].cpu().numpy())
@@ -381,6 +396,25 @@ def _timed_sets(fn, sets, reps, stream):
 
 def _gb(bytes_, ms):
     return round(bytes_ / ms / 1e6, 1)
+
+
+def counters_morph3d():
+    """Grey dilation size 3 and binary erosion x 4 (default cross) on the
+    C2 volumes (4 x 2 x 128^3, 256-level values; Bernoulli(0.7) masks), three
+    calls each: the program of the morphology HBM-counter passes."""
+    import torch
+
+    import torchmorph_amd as tm
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.floor(torch.rand((4, 2, 128, 128, 128), generator=g, device="cuda") * 256) / 256
+    m = torch.rand((4, 2, 128, 128, 128), generator=g, device="cuda") < 0.7
+    y = torch.empty_like(x)
+    for _ in range(3):
+        tm.grey_dilation(x, size=3, output=y)
+        tm.binary_erosion(m, iterations=4)
+    torch.cuda.synchronize()
+    return {"grey_dilation_size3": 3, "binary_erosion_x4": 3}
 
 
 def extra_morph3d():
