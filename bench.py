@@ -44,7 +44,7 @@ HEADLINE_KEY = "edt2d 256 256 64 1"
 # HBM bytes per call of each profiled headline launch variant: FETCH_SIZE x 2.00 +
 # WRITE_SIZE x 1.00 (factors of a plain 256 MB copy), summed over the call's kernels
 # (profiles/edt2d_b8_256x256_hbm_counters.md)
-TRAFFIC_PER_CALL = {0: 13_260_000, 8: 13_397_504}
+TRAFFIC_PER_CALL = {0: 13_260_000, 8: 13_397_504, 16: 11_724_954}
 HEADLINE_KERNELS = {v: "edt2d_cols_kernel + edt2d_rows_kernel (one call)" for v in range(8)}
 HEADLINE_KERNELS.update({v: "edt2d_cols_kernel + edt2d_rows4_kernel (one call)"
                          for v in (8, 9, 10, 11, 12, 13, 14, 15)})

@@ -795,7 +795,10 @@ __global__ __launch_bounds__(256) void edt2d_colsum_kernel(const TIn* __restrict
 }
 
 // Summary path, row pass: one workgroup per row (min(256, W / 4) lanes,
-// rounded up to whole waves).  Every lane resolves four columns of the row
+// rounded up to whole waves), a 1-D grid of batch * H rows whose launch
+// order is remapped so that every XCD takes a contiguous range of rows: the
+// rows of one segment read the same summaries, which one L2 then fetches
+// once instead of once per XCD.  Every lane resolves four columns of the row
 // from their summaries (two 16-byte loads), writes their window keys and
 // feature rows to LDS, then runs rows4_quad on four pixels -- the results
 // of edt2d_rows4_kernel bit for bit.  ENV: rows of at least kEnvMinW pixels,
@@ -809,8 +812,9 @@ __global__ __launch_bounds__(256) void edt2d_rowsum_kernel(const uint64_t* __res
   uint32_t* kb = (uint32_t*)lds;           // [W + 16] window keys, entry x at 8 + x
   int16_t* fr = (int16_t*)(lds + P);       // [W] feature rows
   const RowEnvLds e = row_env_lds((char*)lds + ((P * 4 + W * 2 + 255) & ~255), 1, W, 4);
-  const int img = blockIdx.y;
-  const int y = blockIdx.x;
+  const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int img = lb / H;
+  const int y = lb - img * H;
   const int sg = y / seglen;
   const uint64_t* srow = sm + ((int64_t)img * segs + sg) * W;
   if (ENV) row_env_clear(e, 1);
@@ -1041,11 +1045,16 @@ static int edt_dispatch(const TIn* in, int64_t batch, int ndim, const int64_t* s
         const bool env = W >= kEnvMinW;
         if (sum) {
           const uint64_t* sm = (const uint64_t*)f0;
+          if ((int64_t)H * batch >= ((int64_t)1 << 31)) {
+            set_error("tm_edt: too many rows for one launch of this variant");
+            return TM_ERR_ARG;
+          }
+          const dim3 sgrid((unsigned)(H * batch));
           if (env)
-            hipLaunchKernelGGL((edt2d_rowsum_kernel<true>), rgrid, dim3(threads), lds, st, sm, H,
+            hipLaunchKernelGGL((edt2d_rowsum_kernel<true>), sgrid, dim3(threads), lds, st, sm, H,
                                W, segs, seglen, o);
           else
-            hipLaunchKernelGGL((edt2d_rowsum_kernel<false>), rgrid, dim3(threads), lds, st, sm, H,
+            hipLaunchKernelGGL((edt2d_rowsum_kernel<false>), sgrid, dim3(threads), lds, st, sm, H,
                                W, segs, seglen, o);
           TM_LAUNCH_CHECK();
           return TM_OK;
