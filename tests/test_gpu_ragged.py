@@ -207,7 +207,7 @@ def test_ragged_every_row_chunk_variant(tm, oracle_mod):
         _check_mixed(oracle_mod, masks, samp, *ref, items=range(len(shapes)))
         keys = [k for k in autotune.export() if k.startswith("ragged_rows")]
         assert len(keys) == 1, keys
-        for v in range(4):
+        for v in range(5):
             autotune.clear()
             autotune.load({keys[0]: v})
             plan.distances.fill_(-7.0)

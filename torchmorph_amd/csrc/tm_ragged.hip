@@ -410,9 +410,9 @@ static size_t rows_lds(int kind, int ch) {
 
 // Chunk depths (2-D EDT rows, 3-D EDT rows, taxicab rows) of per-pass launch
 // variant v; variant 0 is the default.
-constexpr int kRgRowVariants = 4;
+constexpr int kRgRowVariants = 5;
 static const int kRgRowCH[kRgRowVariants][3] = {{32, 16, 32}, {16, 8, 16}, {64, 32, 64},
-                                                {16, 16, 16}};
+                                                {16, 16, 16}, {32, 16, 64}};
 
 // Schedule 2, chamfer with indices: one workgroup per job runs raster pass d
 // (every thread takes part: the pass has workgroup barriers).
