@@ -110,6 +110,8 @@ def cpu_baseline(masks_np, seconds):
         t0 = time.perf_counter()
         res = pool.map(_oracle_edt_worker, [(masks_np, seconds / 2)] * cores)
         wall = time.perf_counter() - t0
+        pool.close()
+        pool.join()  # workers gone before the run goes on
     vox = sum(v for v, _ in res)
     return dict(value=vox / wall / 1e6, unit="Mvox/s", cores=cores, kind="port",
                 single_core_value=vox1 / dt1 / 1e6,
@@ -541,6 +543,8 @@ def _scipy_rates(op, items, seconds):
         t0 = time.perf_counter()
         res = pool.map(_scipy_job, [(op, items, seconds / 2)] * cores)
         wall = time.perf_counter() - t0
+        pool.close()
+        pool.join()
     return {f"{op}_scipy": {"one_core_Mvox_s": round(v1 / t1 / 1e6, 2),
                             "pooled_Mvox_s": round(sum(v for v, _ in res) / wall / 1e6, 2),
                             "cores": cores, "version": scipy.__version__}}

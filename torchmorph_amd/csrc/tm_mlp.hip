@@ -4,9 +4,13 @@
 // network predicts Sinkhorn dual potentials (see tm_warmstart.hip); this file
 // holds the hand-written kernels it is built from:
 //  * gemm_bf16_kernel: C = A . B with bf16 operands and fp32 accumulation on
-//    v_mfma_f32_16x16x16_bf16 (64 x 64 output tile per 256-lane workgroup,
-//    each wave a 32 x 32 quadrant = 2 x 2 MFMA tiles, K staged through LDS in
-//    steps of 32).  Operands are addressed through (row, col) strides, so the
+//    gfx950's v_mfma_f32_16x16x32_bf16 (64 x 64 output tile per 256-lane
+//    workgroup, each wave a 32 x 32 quadrant = 2 x 2 MFMA tiles; K staged
+//    through LDS in steps of 32, one MFMA per tile and step).  Both operands
+//    sit K-contiguous in LDS (B transposed while staged), so a lane reads its
+//    eight K values of a row or column as one 16-byte LDS read; 80-byte rows
+//    keep a quarter-wave's reads on distinct banks.  Operands are addressed
+//    through (row, col) strides, so the
 //    same kernel computes A.B, A.B^T and A^T.B (weight gradients, with the
 //    long reduction split over workgroups and summed in a fixed order).
 //    Fused epilogues: fp32 store, fp32 + bias, bias (+ per-row-group add) +
@@ -17,7 +21,7 @@
 
 namespace tmh {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -39,8 +43,8 @@ enum GemmEpi {
 constexpr int kBM = 64, kBN = 64, kBK = 32;
 
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
-  __shared__ uint16_t As[kBM][kBK + 4];
-  __shared__ uint16_t Bs[kBK][kBN + 4];
+  __shared__ __align__(16) uint16_t As[kBM][kBK + 8];  // [m][k]
+  __shared__ __align__(16) uint16_t Bs[kBN][kBK + 8];  // [n][k]: B transposed
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
@@ -64,27 +68,24 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
       As[r][c] = (gm < g.M && gk < ke) ? g.A[gm * g.sam + gk * g.sak] : (uint16_t)0;
       const int rb = idx / kBN, cb = idx % kBN;
       const int gk2 = k0 + rb, gn = n0 + cb;
-      Bs[rb][cb] = (gk2 < ke && gn < g.N) ? g.B[gk2 * g.sbk + gn * g.sbn] : (uint16_t)0;
+      Bs[cb][rb] = (gk2 < ke && gn < g.N) ? g.B[gk2 * g.sbk + gn * g.sbn] : (uint16_t)0;
     }
     __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < kBK; ks += 16) {
-      bf16x4 af[2], bfr[2];
+    {
+      // lane: row / column lane % 16 of its 16 x 16 tile, K values
+      // 8 (lane / 16) .. 8 (lane / 16) + 7 of this step
+      bf16x8 af[2], bfr[2];
+      const int kk = 8 * (lane >> 4);
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        const int row = wm + t * 16 + (lane & 15);
-        const int kk = ks + 4 * (lane >> 4);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) af[t][q] = (short)As[row][kk + q];
-        const int col = wn + t * 16 + (lane & 15);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) bfr[t][q] = (short)Bs[kk + q][col];
+        af[t] = *(const bf16x8*)&As[wm + t * 16 + (lane & 15)][kk];
+        bfr[t] = *(const bf16x8*)&Bs[wn + t * 16 + (lane & 15)][kk];
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
     }
     __syncthreads();
   }
