@@ -71,7 +71,7 @@ def parse():
     ap.add_argument("--streams", type=int, default=4,
                     help="streams the headline graph spreads its independent calls over "
                          "(MI355X has 4 hardware queues per process; 1 = one chain)")
-    ap.add_argument("--counters", choices=["morph3d"], default=None,
+    ap.add_argument("--counters", choices=["morph3d", "rows"], default=None,
                     help="only run the named workload's kernels a few times, untimed "
                          "(the program the hardware-counter passes profile)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -148,6 +148,9 @@ def main():
         return
     if args.counters == "morph3d":
         print(json.dumps({"counters_morph3d": counters_morph3d()}), flush=True)
+        return
+    if args.counters == "rows":
+        print(json.dumps({"counters_rows": counters_rows()}), flush=True)
         return
     if args.only:
         fn = {"morph3d": extra_morph3d, "ragged": extra_ragged, "grad": extra_grad_and_sinkhorn,
@@ -417,6 +420,26 @@ def counters_morph3d():
         tm.binary_erosion(m, iterations=4)
     torch.cuda.synchronize()
     return {"grey_dilation_size3": 3, "binary_erosion_x4": 3}
+
+
+def counters_rows():
+    """Anisotropic 2-D EDT (sampling 1.3, 0.7, distances only) of one and of
+    64 smooth-noise images of 256 x 1024, two eager calls each, autotuner
+    off: the program of the row-envelope counter passes (edt_pass0_kernel,
+    then edt_rows_kernel<1, 32> over 256 and 16,384 rows of 1,024 voxels)."""
+    import torch
+
+    import torchmorph_amd as tm
+    from torchmorph_amd import autotune
+
+    autotune.enable(False)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    for b in (1, 64):
+        m = _smooth_noise_mask((b * 256, 1024), gen).view(b, 1, 256, 1024)
+        for _ in range(2):
+            tm.euclidean_distance_transform(m, sampling=[1.3, 0.7])
+    torch.cuda.synchronize()
+    return {"images": [1, 64], "shape": [256, 1024], "calls_each": 2}
 
 
 def extra_morph3d():
