@@ -4,6 +4,8 @@ import numpy as np
 import pytest
 import torch
 
+from tests._golden import assert_bits_equal
+
 pytestmark = pytest.mark.gpu
 
 
@@ -153,7 +155,8 @@ def test_inapplicable_cached_variant_falls_back(tm, oracle_mod):
 @pytest.mark.parametrize("se", ["cube3", "cube7", "ball2"])
 @pytest.mark.parametrize("mode", ["reflect", "constant", "wrap"])
 def test_grey3_every_zchunk_variant_matches_oracle(tm, oracle_mod, se, mode):
-    """The streaming 3-D grey kernels give the same bits for every z-chunk."""
+    """The streaming 3-D grey kernels give the same bits for every z-chunk and,
+    for the 3 x 3 x 3 box, for every register-streaming tile."""
     from torchmorph_amd import autotune
 
     rng = np.random.default_rng(len(se) + len(mode))
@@ -174,11 +177,86 @@ def test_grey3_every_zchunk_variant_matches_oracle(tm, oracle_mod, se, mode):
     xt = torch.as_tensor(x, device="cuda")
     key = "grey3 {} {} {} {} 64 32 128 2".format(key_kind, *kk)
     try:
-        for v in range(4):
+        for v in range(8):  # 4..7: register-streaming 3 x 3 x 3 boxes (others fall back)
             autotune.clear()
             autotune.load({key: v})
             y = tm.grey_dilation(xt, mode=mode, cval=0.25, **kw)
             assert np.array_equal(y.cpu().numpy(), want), v
+    finally:
+        autotune.clear()
+
+
+def _boxr_cases():
+    """(input, size, origin): 3-D and 2-D items whose 3 x 3 (x 3) flat box the
+    register-streaming grey kernel runs: ragged tiles in x, y and z, items
+    smaller than a tile or the window, shifted origins, signed zeros,
+    infinities and NaN."""
+    rng = np.random.default_rng(21)
+    q = lambda shape: (np.floor(rng.random(shape) * 16) / 16 - 0.5).astype(np.float32)
+    out = [(q((2, 1, 37, 21, 70)), 3, 0),
+           (q((1, 2, 9, 40, 130)), 3, (1, -1, 0)),
+           (q((1, 1, 2, 5, 3)), 3, 0),
+           (q((3, 1, 50, 200)), 3, 0),
+           (q((1, 1, 19, 65)), 3, (-1, 1)),
+           (q((1, 1, 1, 1)), 3, 0)]
+    x = q((1, 1, 6, 17, 66))
+    flat = x.reshape(-1)
+    idx = rng.choice(flat.size, 40, replace=False)
+    flat[idx[:10]] = -0.0
+    flat[idx[10:20]] = 0.0
+    flat[idx[20:25]] = np.inf
+    flat[idx[25:30]] = -np.inf
+    flat[idx[30:]] = np.nan
+    out.append((x, 3, 0))
+    return out
+
+
+def _p2s(v):
+    return str(_p2(v))
+
+
+@pytest.mark.parametrize("case", range(7))
+@pytest.mark.parametrize("mode", ["reflect", "constant", "nearest", "mirror", "wrap"])
+def test_grey_register_box_every_tile_matches_oracle(tm, oracle_mod, case, mode):
+    """Launch variants 4..7 (register-streaming 3 x 3 x 3 / 3 x 3 boxes) give
+    the oracle's bits for dilation, erosion, the white tophat's fused
+    subtraction and (zero origin) the one-pass gradient and laplace, for every
+    border mode (NaNs compared as one canonical NaN: the payload of a NaN
+    that arithmetic creates is the platform's)."""
+    from torchmorph_amd import autotune
+
+    x, size, origin = _boxr_cases()[case]
+    sp = x.shape[2:]
+    nd = len(sp)
+    s3 = (1,) * (3 - nd) + tuple(sp)
+    k3 = (1,) * (3 - nd) + (size,) * nd
+    cls = [*map(str, k3), *map(_p2s, s3), _p2s(x.shape[0] * x.shape[1])]
+    key, key_dual = " ".join(["grey3", "1", *cls]), " ".join(["grey3", "3", *cls])
+    o = (origin,) * nd if isinstance(origin, int) else tuple(origin)
+    shape = (size,) * nd
+    want_d = oracle_mod.grey(x, None, None, shape, o, mode=mode, cval=0.25, dilation=True)
+    want_e = oracle_mod.grey(x, None, None, shape, o, mode=mode, cval=0.25, dilation=False)
+    opened = oracle_mod.grey(want_e, None, None, shape, o, mode=mode, cval=0.25, dilation=True)
+    with np.errstate(invalid="ignore"):  # inf - inf in the special-values case
+        want_wt = x - opened
+        want_g = want_d - want_e
+        want_l = (want_d + want_e) - 2 * x
+    xt = torch.as_tensor(x, device="cuda")
+    try:
+        for v in range(4, 8):
+            autotune.clear()
+            autotune.load({key: v, key_dual: v})
+            d = tm.grey_dilation(xt, size=size, mode=mode, cval=0.25, origin=origin)
+            e = tm.grey_erosion(xt, size=size, mode=mode, cval=0.25, origin=origin)
+            wt = tm.white_tophat(xt, size=size, mode=mode, cval=0.25, origin=origin)
+            assert_bits_equal(d.cpu(), want_d, f"dilation variant {v}")
+            assert_bits_equal(e.cpu(), want_e, f"erosion variant {v}")
+            assert_bits_equal(wt.cpu(), want_wt, f"white_tophat variant {v}")
+            if origin == 0:
+                g = tm.morphological_gradient(xt, size=size, mode=mode, cval=0.25)
+                lp = tm.morphological_laplace(xt, size=size, mode=mode, cval=0.25)
+                assert_bits_equal(g.cpu(), want_g, f"gradient variant {v}")
+                assert_bits_equal(lp.cpu(), want_l, f"laplace variant {v}")
     finally:
         autotune.clear()
 

@@ -407,6 +407,210 @@ __global__ __launch_bounds__(256) void grey_box3_kernel(const float* __restrict_
                                mode, cval, tx, ty, zb, min(D, zb + zchunk), sm, ei);
 }
 
+// Flat boxes of at most 3 voxels per axis, register streaming.  A wave owns a
+// 64-wide x strip of TY output rows and a chunk of z planes; the four waves
+// of a workgroup take four consecutive y tiles.  Lane l holds input column
+// x0 + sx + l of every row it streams, and lanes 0 .. KX - 2 also the voxels
+// past the strip.  The x window reads the right-hand neighbours through DPP
+// lane shifts, the wave's last lanes taking the voxels past the strip.  The
+// y window folds KY consecutive x-results, and the z window keeps the last
+// KZ - 1 y-results of each output row in registers.  No LDS and no barriers.
+// The next plane's rows are loaded while the current plane is reduced: two
+// row buffers alternate, and a buffer is reloaded right after it is
+// consumed, so the loads land in the registers they are consumed from.
+// Every load address is in range (border-mapped; constant borders read
+// offset 0 and are substituted when consumed), and a load is a uniform plane
+// base plus a 32-bit byte offset (planes of at most 2^30 voxels).
+// EPI 0..3: one erosion / dilation with grey_epi's epilogue.  EPI 4 / 5: the
+// one-pass gradient (max - min) / laplace ((max + min) - 2 input) of
+// grey_box3_dual_tile, min and max folded side by side (MAX unused).
+constexpr int kRwWaves = 4;
+
+template <int KX, int KY, int TY>
+struct RwRows {
+  float v[TY + KY - 1];  // this lane's column of each input row
+  float e[KX > 1 ? TY + KY - 1 : 1];  // lanes 0 .. KX - 2: the voxels past the strip
+  bool zc;               // the plane lies in the constant border
+};
+
+// Value of lane l + 1 (lane 63: `last`), by a DPP wave shift.
+__device__ __forceinline__ float rw_next_lane(float v, float last) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(last), __float_as_int(v),
+                                                    0x130 /* wave_shl:1 */, 0xf, 0xf, false));
+}
+
+template <bool MAX, int KX, int KY, int KZ, int TY, int EPI>
+__global__ __launch_bounds__(256) void grey_boxr_kernel(const float* __restrict__ in,
+                                                        float* __restrict__ out, int D, int H,
+                                                        int W, int sz, int sy, int sx, int mode,
+                                                        float cval, int zchunk, GreyEpi epi) {
+  constexpr int NR = TY + KY - 1;
+  const int ntx = (W + 63) >> 6;
+  const int nty = (H + TY - 1) / TY;
+  const int nyb = (nty + kRwWaves - 1) / kRwWaves;
+  const int nz = (D + zchunk - 1) / zchunk;
+  int tx, tyb, zci;
+  int64_t item;
+  grey_block(ntx, ntx * nyb, nz, &tx, &tyb, &zci, &item);
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int ty = tyb * kRwWaves + wave;
+  if (ty >= nty) return;  // whole wave; no barriers in this kernel
+  const int zb = zci * zchunk, ze = min(D, zb + zchunk);
+  const int x0 = tx * 64, y0 = ty * TY;
+  const int64_t n = (int64_t)D * H * W;
+  const float* vol = in + item * n;
+  float* ovol = out + item * n;
+  const GreyEpi ei{epi.a ? epi.a + item * n : nullptr, epi.b ? epi.b + item * n : nullptr};
+  const float bias = MAX ? 0.0f : -0.0f;  // the flat dilation's +0.0f; an identity otherwise
+  constexpr bool DU = EPI >= 4;
+
+  // border maps (byte offsets) of this lane's column, of the voxel past the
+  // strip this lane loads (lanes 0 .. KX - 2) and of the rows
+  bool xc, ec;
+  const uint32_t xo = (uint32_t)gmap(x0 + sx + lane, W, mode, &xc) * 4u;
+  const uint32_t eo =
+      (uint32_t)gmap(x0 + sx + 64 + min(lane, KX > 1 ? KX - 2 : 0), W, mode, &ec) * 4u;
+  uint32_t hcm = 0;  // voxels past the strip in the constant border
+#pragma unroll
+  for (int j = 0; j < KX - 1; ++j) hcm |= (__builtin_amdgcn_readlane((int)ec, j) ? 1u : 0u) << j;
+  uint32_t ro[NR];
+  uint32_t rcm = 0;  // rows in the constant border
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    bool c;
+    ro[r] = (uint32_t)gmap(y0 + sy + r, H, mode, &c) * (uint32_t)W * 4u;
+    rcm |= (c ? 1u : 0u) << r;
+  }
+
+  auto fetch = [&](RwRows<KX, KY, TY>& R, int zi) {
+    bool zc;
+    const int64_t zm = gmap(zi, D, mode, &zc);
+    R.zc = zc;
+    const char* pl = (const char*)(vol + zm * H * W);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      R.v[r] = *(const float*)(pl + (ro[r] + xo));
+      if (KX > 1) R.e[r] = *(const float*)(pl + (ro[r] + eo));
+    }
+  };
+
+  const int nplanes = (ze - zb) + KZ - 1;
+  constexpr int NZ = KZ > 1 ? KZ - 1 : 1;
+  float zf[NZ][TY];               // y-results of the previous KZ - 1 planes
+  float zn[DU ? NZ : 1][TY];      // DU: their minima (zf: maxima)
+#pragma unroll
+  for (int k = 0; k < NZ; ++k)
+#pragma unroll
+    for (int t = 0; t < TY; ++t) {
+      zf[k][t] = DU ? -INFINITY : gid<MAX>();
+      if (DU) zn[k][t] = INFINITY;
+    }
+  // epilogue operands of an output plane: a (EPI 1..3) or the input (EPI 5),
+  // and b (EPI 3); loaded when the plane's step starts, consumed at its
+  // stores (clamped coordinates: every address in range, unused values)
+  constexpr bool EA = (EPI >= 1 && EPI <= 3) || EPI == 5, EB = EPI == 3;
+  const float* pa = EPI == 5 ? vol : ei.a;
+  const int xl = min(x0 + lane, W - 1);
+  auto step = [&](RwRows<KX, KY, TY>& R, int p) {
+    const int z = zb + p - (KZ - 1);
+    float ea[EA ? TY : 1], eb[EB ? TY : 1];
+    if (EA) {
+      const int64_t zq = (int64_t)max(z, zb) * H;
+#pragma unroll
+      for (int t = 0; t < TY; ++t) {
+        const int64_t q = (zq + min(y0 + t, H - 1)) * W + xl;
+        ea[t] = pa[q];
+        if (EB) eb[EB ? t : 0] = ei.b[q];
+      }
+    }
+    float xw[NR], xn[DU ? NR : 1];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const bool cr = R.zc || ((rcm >> r) & 1u);
+      const float raw = (cr || xc) ? cval : R.v[r];
+      if (!DU) {
+        float acc = raw + bias;
+        float s = acc;
+#pragma unroll
+        for (int j = 0; j < KX - 1; ++j) {
+          const float h = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(R.e[r]), j));
+          s = rw_next_lane(s, ((cr || ((hcm >> j) & 1u)) ? cval : h) + bias);
+          acc = gop<MAX>(acc, s);
+        }
+        xw[r] = acc;
+      } else {  // max of the values + 0.0f, min of the raw values
+        float hi = raw + 0.0f, lo = raw, s = raw;
+#pragma unroll
+        for (int j = 0; j < KX - 1; ++j) {
+          const float h = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(R.e[r]), j));
+          s = rw_next_lane(s, (cr || ((hcm >> j) & 1u)) ? cval : h);
+          hi = fmaxf(hi, s + 0.0f);
+          lo = fminf(lo, s);
+        }
+        xw[r] = hi;
+        xn[DU ? r : 0] = lo;
+      }
+    }
+    // this buffer's registers are free: plane p + 2 (past the last plane:
+    // mapped, unused)
+    fetch(R, zb + sz + p + 2);
+#pragma unroll
+    for (int t = 0; t < TY; ++t) {
+      const int y = y0 + t, x = x0 + lane;
+      const bool st = z >= zb && y < H && x < W;
+      const int64_t q = ((int64_t)z * H + y) * W + x;
+      if (!DU) {
+        float yw = xw[t];
+#pragma unroll
+        for (int j = 1; j < KY; ++j) yw = gop<MAX>(yw, xw[t + j]);
+        float acc = yw;
+#pragma unroll
+        for (int k = 0; k < KZ - 1; ++k) acc = gop<MAX>(acc, zf[k][t]);
+#pragma unroll
+        for (int k = 0; k + 1 < KZ - 1; ++k) zf[k][t] = zf[k + 1][t];
+        if (KZ > 1) zf[KZ - 2][t] = yw;
+        if (st) {
+          float o = acc;  // grey_epi's operations on the preloaded operands
+          if (EPI == 1) o = acc - ea[EA ? t : 0];
+          if (EPI == 2) o = ea[EA ? t : 0] - acc;
+          if (EPI == 3) o = (acc + ea[EA ? t : 0]) - 2.0f * eb[EB ? t : 0];
+          ovol[q] = o;
+        }
+      } else {
+        float yh = xw[t], yl = xn[DU ? t : 0];
+#pragma unroll
+        for (int j = 1; j < KY; ++j) {
+          yh = fmaxf(yh, xw[t + j]);
+          yl = fminf(yl, xn[DU ? t + j : 0]);
+        }
+        float hi = yh, lo = yl;
+#pragma unroll
+        for (int k = 0; k < KZ - 1; ++k) {
+          hi = fmaxf(hi, zf[k][t]);
+          lo = fminf(lo, zn[DU ? k : 0][t]);
+        }
+#pragma unroll
+        for (int k = 0; k + 1 < KZ - 1; ++k) {
+          zf[k][t] = zf[k + 1][t];
+          zn[DU ? k : 0][t] = zn[DU ? k + 1 : 0][t];
+        }
+        if (KZ > 1) {
+          zf[KZ - 2][t] = yh;
+          zn[DU ? KZ - 2 : 0][t] = yl;
+        }
+        if (st) ovol[q] = EPI == 4 ? hi - lo : (hi + lo) - 2.0f * ea[EA ? t : 0];
+      }
+    }
+  };
+  RwRows<KX, KY, TY> A, B;
+  fetch(A, zb + sz);
+  fetch(B, zb + sz + 1);
+  for (int p = 0; p < nplanes; p += 2) {
+    step(A, p);
+    if (p + 1 < nplanes) step(B, p + 1);
+  }
+}
+
 // Gradient (dilation - erosion, COMB 0) or laplace ((dilation + erosion) -
 // 2 input, COMB 1) of a flat box whose erosion and dilation windows coincide
 // (odd sizes, zero origin), in one pass: each staged plane is reduced to its
@@ -881,6 +1085,45 @@ static void launch_box3(int epi, unsigned G, size_t lds, hipStream_t st, const f
 #undef TM_BOX
 }
 
+// Register-streaming flat box (grey_boxr_kernel): 3 x 3 x 3 or, for 2-D
+// items, 3 x 3; TY output rows per wave; epi 4 / 5 the one-pass gradient /
+// laplace.  Returns false when the box has another size.
+template <int TY>
+static bool launch_boxr(bool mx, int epi, int64_t batch, int D, int H, int W, int kz, int ky,
+                        int kx, int sz, int sy, int sx, int mode, float cval, int zchunk,
+                        GreyEpi e, const float* in, float* out, hipStream_t st) {
+  if (kx != 3 || ky != 3 || (kz != 3 && kz != 1)) return false;
+  if ((int64_t)H * W > ((int64_t)1 << 30)) return false;  // 32-bit byte offsets in a plane
+  const int64_t ntx = (W + 63) / 64, nty = (H + TY - 1) / TY;
+  const int64_t nyb = (nty + kRwWaves - 1) / kRwWaves, nz = (D + zchunk - 1) / zchunk;
+  const int64_t G = ntx * nyb * nz * batch;
+  if (G >= ((int64_t)1 << 31)) return false;
+#define TM_RW(MX, KZ, EP)                                                                      \
+  hipLaunchKernelGGL((grey_boxr_kernel<MX, 3, 3, KZ, TY, EP>), dim3((unsigned)G), dim3(256), 0, \
+                     st, in, out, D, H, W, sz, sy, sx, mode, cval, zchunk, e)
+#define TM_RW_EPI(MX, KZ)            \
+  switch (epi) {                     \
+    case 1: TM_RW(MX, KZ, 1); break; \
+    case 2: TM_RW(MX, KZ, 2); break; \
+    case 3: TM_RW(MX, KZ, 3); break; \
+    default: TM_RW(MX, KZ, 0); break; \
+  }
+  if (epi >= 4) {  // one-pass gradient / laplace
+    if (kz == 3) {
+      if (epi == 4) TM_RW(true, 3, 4); else TM_RW(true, 3, 5);
+    } else {
+      if (epi == 4) TM_RW(true, 1, 4); else TM_RW(true, 1, 5);
+    }
+  } else if (mx) {
+    if (kz == 3) TM_RW_EPI(true, 3) else TM_RW_EPI(true, 1)
+  } else {
+    if (kz == 3) TM_RW_EPI(false, 3) else TM_RW_EPI(false, 1)
+  }
+#undef TM_RW_EPI
+#undef TM_RW
+  return true;
+}
+
 template <bool MX, int NR>
 static void launch_table3(int epi, unsigned G, size_t lds, hipStream_t st, const float* in,
                           float* out, int D, int H, int W, const GreyTable& tab, const Grey3& g3,
@@ -945,7 +1188,24 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
     auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
     if (grey_box3_fits(E, g3) && plane32(H, W)) {
       const size_t lds = box3_lds(kz, ky, kx);
+      // variants 4..7: the register-streaming kernel for 3 x 3 (x 3) boxes,
+      // (rows per wave, z planes per workgroup) = (8, 8), (8, 16), (16, 16),
+      // (8, 32); not applicable to other boxes
       auto run = [&](int v) -> int {
+        if (v >= 4) {
+          static const int kRw[4][2] = {{8, 8}, {8, 16}, {16, 16}, {8, 32}};
+          const int ty = kRw[v - 4][0], zc = std::min(D, kRw[v - 4][1]);
+          const bool ok =
+              ty == 8 ? launch_boxr<8>(is_max, epi, batch, D, H, W, kz, ky, kx, (int)lo3[0],
+                                       (int)lo3[1], (int)lo3[2], mode, cval, zc, ge, input,
+                                       output, st)
+                      : launch_boxr<16>(is_max, epi, batch, D, H, W, kz, ky, kx, (int)lo3[0],
+                                        (int)lo3[1], (int)lo3[2], mode, cval, zc, ge, input,
+                                        output, st);
+          if (!ok) return TM_ERR_ARG;
+          TM_LAUNCH_CHECK();
+          return TM_OK;
+        }
         const int zchunk = zchunk_of(v);
         const int64_t G = grid_of(zchunk);
         if (G >= ((int64_t)1 << 31)) {
@@ -969,8 +1229,10 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
       };
       const int64_t key[8] = {1, kz, ky, kx, p2(D), p2(H), p2(W), p2(batch)};
       bool ran = false;
-      const int v = tune_pick("grey3", key, 8, 4, 1, run, st, &ran);
-      return ran ? TM_OK : run(v);
+      const int v = tune_pick("grey3", key, 8, 8, 1, run, st, &ran);
+      if (ran) return TM_OK;
+      const int rc = run(v);
+      return (rc == TM_ERR_ARG && v != 1) ? run(1) : rc;  // a cached variant that does not apply
     }
     if (grey_table3_fits(E, g3) && plane32(H, W)) {
       const GreyTable tab = grey_table3_of(ndim, E);
@@ -1112,7 +1374,22 @@ static int grey_dual_run(const float* input, int64_t batch, int ndim, const int6
   const int64_t ntiles = ((W + kTX - 1) / kTX) * (int64_t)((H + kTY - 1) / kTY);
   const size_t lds = box3_dual_lds(kz, ky, kx);
   const bool small_se = plane_small(ky, kx);
+  // variants 4..7: the register-streaming kernel (3 x 3 (x 3) boxes), tiles
+  // as grey_run's
   auto run = [&](int v) -> int {
+    if (v >= 4) {
+      static const int kRw[4][2] = {{8, 8}, {8, 16}, {16, 16}, {8, 32}};
+      const int ty = kRw[v - 4][0], zc = std::min(D, kRw[v - 4][1]);
+      const int lz = (int)g3.lo[0], ly = (int)g3.lo[1], lx = (int)g3.lo[2];
+      const bool ok = ty == 8 ? launch_boxr<8>(true, 4 + comb, batch, D, H, W, kz, ky, kx, lz,
+                                               ly, lx, mode, cval, zc, GreyEpi{}, input, output, st)
+                              : launch_boxr<16>(true, 4 + comb, batch, D, H, W, kz, ky, kx, lz,
+                                                ly, lx, mode, cval, zc, GreyEpi{}, input, output,
+                                                st);
+      if (!ok) return TM_ERR_ARG;
+      TM_LAUNCH_CHECK();
+      return TM_OK;
+    }
     const int zchunk = std::min(D, 8 << v);
     const int64_t G = ntiles * ((D + zchunk - 1) / zchunk) * batch;
     if (G >= ((int64_t)1 << 31)) {
@@ -1137,8 +1414,10 @@ static int grey_dual_run(const float* input, int64_t batch, int ndim, const int6
   auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
   const int64_t key[8] = {3, kz, ky, kx, p2(D), p2(H), p2(W), p2(batch)};
   bool ran = false;
-  const int v = tune_pick("grey3", key, 8, 4, 1, run, st, &ran);
-  return ran ? TM_OK : run(v);
+  const int v = tune_pick("grey3", key, 8, 8, 1, run, st, &ran);
+  if (ran) return TM_OK;
+  const int rc2 = run(v);
+  return (rc2 == TM_ERR_ARG && v != 1) ? run(1) : rc2;  // a cached variant that does not apply
 }
 
 }  // namespace tmh

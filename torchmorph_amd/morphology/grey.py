@@ -159,13 +159,13 @@ def _composite(input: Tensor, output, kw, first: bool, second: bool, second_on_f
     x = x.contiguous() if x.dtype == torch.float32 else x.float().contiguous()
     if one_pass and struct is None and fp is None and _lib.lib().tm_grey_dual_ok(
             nd, _lib.i64_array(se_shape), _lib.i64_array(origin_t)) == 1:
-        result = torch.empty_like(x)
+        result = _result_buffer(x, output)
         _launch_ex(x, None, None, se_shape, origin_t, mode_int, kw["cval"], True, one_pass, x,
                    None, result)
         return _finish(result, output)
     tmp = torch.empty_like(x)
     _launch(x, struct, fp, se_shape, origin_t, mode_int, kw["cval"], first, tmp)
-    result = torch.empty_like(x)
+    result = _result_buffer(x, output)
     _launch_ex(tmp if second_on_first else x, struct, fp, se_shape, origin_t, mode_int, kw["cval"],
                second, epi, tmp if a_is_first else x, x if b_is_input else None, result)
     return _finish(result, output)
@@ -237,10 +237,26 @@ def grey_closing(input: Tensor, size=None, footprint=None, structure=None, outpu
 
 
 def _finish(result, output):
-    if output is not None:
+    if output is not None and result is not output:
         output.copy_(result)
         return output
     return result
+
+
+def _disjoint(a: Tensor, b: Tensor) -> bool:
+    a0, b0 = a.data_ptr(), b.data_ptr()
+    return (a0 + a.numel() * a.element_size() <= b0
+            or b0 + b.numel() * b.element_size() <= a0)
+
+
+def _result_buffer(x: Tensor, output) -> Tensor:
+    """Where a composite's last pass writes: the caller's output itself when
+    it is contiguous float32 memory apart from the input (the passes read the
+    input to the end), else a new tensor that _finish copies into it."""
+    if (output is not None and output.dtype == torch.float32 and output.is_contiguous()
+            and output.shape == x.shape and _disjoint(output, x)):
+        return output
+    return torch.empty_like(x)
 
 
 def morphological_gradient(input: Tensor, size=None, footprint=None, structure=None, output=None,
