@@ -177,7 +177,7 @@ def test_grey3_every_zchunk_variant_matches_oracle(tm, oracle_mod, se, mode):
     xt = torch.as_tensor(x, device="cuda")
     key = "grey3 {} {} {} {} 64 32 128 2".format(key_kind, *kk)
     try:
-        for v in range(8):  # 4..7: register-streaming 3 x 3 x 3 boxes (others fall back)
+        for v in range(8):  # 4..7: register-streaming boxes (the ball falls back)
             autotune.clear()
             autotune.load({key: v})
             y = tm.grey_dilation(xt, mode=mode, cval=0.25, **kw)
@@ -187,9 +187,9 @@ def test_grey3_every_zchunk_variant_matches_oracle(tm, oracle_mod, se, mode):
 
 
 def _boxr_cases():
-    """(input, size, origin): 3-D and 2-D items whose 3 x 3 (x 3) flat box the
-    register-streaming grey kernel runs: ragged tiles in x, y and z, items
-    smaller than a tile or the window, shifted origins, signed zeros,
+    """(input, size, origin): 3-D and 2-D items whose K x K (x K) flat box, K =
+    3, 5, 7, the register-streaming grey kernel runs: ragged tiles in x, y and
+    z, items smaller than a tile or the window, shifted origins, signed zeros,
     infinities and NaN."""
     rng = np.random.default_rng(21)
     q = lambda shape: (np.floor(rng.random(shape) * 16) / 16 - 0.5).astype(np.float32)
@@ -208,6 +208,10 @@ def _boxr_cases():
     flat[idx[25:30]] = -np.inf
     flat[idx[30:]] = np.nan
     out.append((x, 3, 0))
+    out += [(q((1, 1, 20, 30, 80)), 5, 0),
+            (q((2, 1, 11, 9, 70)), 7, (1, -2, 3)),
+            (q((2, 1, 40, 150)), 7, 0),
+            (q((1, 1, 6, 5)), 5, 0)]
     return out
 
 
@@ -215,10 +219,10 @@ def _p2s(v):
     return str(_p2(v))
 
 
-@pytest.mark.parametrize("case", range(7))
+@pytest.mark.parametrize("case", range(11))
 @pytest.mark.parametrize("mode", ["reflect", "constant", "nearest", "mirror", "wrap"])
 def test_grey_register_box_every_tile_matches_oracle(tm, oracle_mod, case, mode):
-    """Launch variants 4..7 (register-streaming 3 x 3 x 3 / 3 x 3 boxes) give
+    """Launch variants 4..7 (register-streaming K x K x K / K x K boxes) give
     the oracle's bits for dilation, erosion, the white tophat's fused
     subtraction and (zero origin) the one-pass gradient and laplace, for every
     border mode (NaNs compared as one canonical NaN: the payload of a NaN

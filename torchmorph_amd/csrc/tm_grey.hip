@@ -1085,21 +1085,18 @@ static void launch_box3(int epi, unsigned G, size_t lds, hipStream_t st, const f
 #undef TM_BOX
 }
 
-// Register-streaming flat box (grey_boxr_kernel): 3 x 3 x 3 or, for 2-D
-// items, 3 x 3; TY output rows per wave; epi 4 / 5 the one-pass gradient /
-// laplace.  Returns false when the box has another size.
-template <int TY>
-static bool launch_boxr(bool mx, int epi, int64_t batch, int D, int H, int W, int kz, int ky,
-                        int kx, int sz, int sy, int sx, int mode, float cval, int zchunk,
-                        GreyEpi e, const float* in, float* out, hipStream_t st) {
-  if (kx != 3 || ky != 3 || (kz != 3 && kz != 1)) return false;
-  if ((int64_t)H * W > ((int64_t)1 << 30)) return false;  // 32-bit byte offsets in a plane
-  const int64_t ntx = (W + 63) / 64, nty = (H + TY - 1) / TY;
-  const int64_t nyb = (nty + kRwWaves - 1) / kRwWaves, nz = (D + zchunk - 1) / zchunk;
-  const int64_t G = ntx * nyb * nz * batch;
-  if (G >= ((int64_t)1 << 31)) return false;
-#define TM_RW(MX, KZ, EP)                                                                      \
-  hipLaunchKernelGGL((grey_boxr_kernel<MX, 3, 3, KZ, TY, EP>), dim3((unsigned)G), dim3(256), 0, \
+// Register-streaming flat box (grey_boxr_kernel): K x K x K or, for 2-D
+// items, K x K, for K = 3, 5, 7; rw_rows(K) output rows per wave (fewer for
+// K = 7, whose row buffers and z window fill the registers); epi 4 / 5 the
+// one-pass gradient / laplace.  Returns false when the box has another shape.
+constexpr int rw_rows(int k) { return k >= 7 ? 4 : 8; }
+
+template <int K>
+static void launch_boxr_k(bool mx, int epi, unsigned G, int kz, const float* in, float* out,
+                          int D, int H, int W, int sz, int sy, int sx, int mode, float cval,
+                          int zchunk, GreyEpi e, hipStream_t st) {
+#define TM_RW(MX, KZ, EP)                                                                   \
+  hipLaunchKernelGGL((grey_boxr_kernel<MX, K, K, KZ, rw_rows(K), EP>), dim3(G), dim3(256), 0, \
                      st, in, out, D, H, W, sz, sy, sx, mode, cval, zchunk, e)
 #define TM_RW_EPI(MX, KZ)            \
   switch (epi) {                     \
@@ -1109,18 +1106,38 @@ static bool launch_boxr(bool mx, int epi, int64_t batch, int D, int H, int W, in
     default: TM_RW(MX, KZ, 0); break; \
   }
   if (epi >= 4) {  // one-pass gradient / laplace
-    if (kz == 3) {
-      if (epi == 4) TM_RW(true, 3, 4); else TM_RW(true, 3, 5);
+    if (kz == K) {
+      if (epi == 4) TM_RW(true, K, 4); else TM_RW(true, K, 5);
     } else {
       if (epi == 4) TM_RW(true, 1, 4); else TM_RW(true, 1, 5);
     }
   } else if (mx) {
-    if (kz == 3) TM_RW_EPI(true, 3) else TM_RW_EPI(true, 1)
+    if (kz == K) TM_RW_EPI(true, K) else TM_RW_EPI(true, 1)
   } else {
-    if (kz == 3) TM_RW_EPI(false, 3) else TM_RW_EPI(false, 1)
+    if (kz == K) TM_RW_EPI(false, K) else TM_RW_EPI(false, 1)
   }
 #undef TM_RW_EPI
 #undef TM_RW
+}
+
+static bool launch_boxr(bool mx, int epi, int64_t batch, int D, int H, int W, int kz, int ky,
+                        int kx, int sz, int sy, int sx, int mode, float cval, int zchunk,
+                        GreyEpi e, const float* in, float* out, hipStream_t st) {
+  if (kx != ky || (kz != kx && kz != 1) || (kx != 3 && kx != 5 && kx != 7)) return false;
+  if ((int64_t)H * W > ((int64_t)1 << 30)) return false;  // 32-bit byte offsets in a plane
+  const int rows = rw_rows(kx);
+  const int64_t ntx = (W + 63) / 64, nty = (H + rows - 1) / rows;
+  const int64_t nyb = (nty + kRwWaves - 1) / kRwWaves, nz = (D + zchunk - 1) / zchunk;
+  const int64_t G = ntx * nyb * nz * batch;
+  if (G >= ((int64_t)1 << 31)) return false;
+  const unsigned g = (unsigned)G;
+  auto go = [&](auto kk) {
+    launch_boxr_k<decltype(kk)::value>(mx, epi, g, kz, in, out, D, H, W, sz, sy, sx, mode, cval,
+                                       zchunk, e, st);
+  };
+  if (kx == 3) go(std::integral_constant<int, 3>{});
+  else if (kx == 5) go(std::integral_constant<int, 5>{});
+  else go(std::integral_constant<int, 7>{});
   return true;
 }
 
@@ -1188,21 +1205,15 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
     auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
     if (grey_box3_fits(E, g3) && plane32(H, W)) {
       const size_t lds = box3_lds(kz, ky, kx);
-      // variants 4..7: the register-streaming kernel for 3 x 3 (x 3) boxes,
-      // (rows per wave, z planes per workgroup) = (8, 8), (8, 16), (16, 16),
-      // (8, 32); not applicable to other boxes
+      // variants 4..7: the register-streaming kernel for K x K (x K) boxes,
+      // K = 3, 5, 7, with z chunks of 8 << (v - 4) planes; not applicable to
+      // other boxes
       auto run = [&](int v) -> int {
         if (v >= 4) {
-          static const int kRw[4][2] = {{8, 8}, {8, 16}, {16, 16}, {8, 32}};
-          const int ty = kRw[v - 4][0], zc = std::min(D, kRw[v - 4][1]);
-          const bool ok =
-              ty == 8 ? launch_boxr<8>(is_max, epi, batch, D, H, W, kz, ky, kx, (int)lo3[0],
-                                       (int)lo3[1], (int)lo3[2], mode, cval, zc, ge, input,
-                                       output, st)
-                      : launch_boxr<16>(is_max, epi, batch, D, H, W, kz, ky, kx, (int)lo3[0],
-                                        (int)lo3[1], (int)lo3[2], mode, cval, zc, ge, input,
-                                        output, st);
-          if (!ok) return TM_ERR_ARG;
+          const int zc = std::min(D, 8 << (v - 4));
+          if (!launch_boxr(is_max, epi, batch, D, H, W, kz, ky, kx, (int)lo3[0], (int)lo3[1],
+                           (int)lo3[2], mode, cval, zc, ge, input, output, st))
+            return TM_ERR_ARG;
           TM_LAUNCH_CHECK();
           return TM_OK;
         }
@@ -1374,19 +1385,15 @@ static int grey_dual_run(const float* input, int64_t batch, int ndim, const int6
   const int64_t ntiles = ((W + kTX - 1) / kTX) * (int64_t)((H + kTY - 1) / kTY);
   const size_t lds = box3_dual_lds(kz, ky, kx);
   const bool small_se = plane_small(ky, kx);
-  // variants 4..7: the register-streaming kernel (3 x 3 (x 3) boxes), tiles
-  // as grey_run's
+  // variants 4..7: the register-streaming kernel (K x K (x K) boxes, K = 3,
+  // 5, 7), z chunks as grey_run's
   auto run = [&](int v) -> int {
     if (v >= 4) {
-      static const int kRw[4][2] = {{8, 8}, {8, 16}, {16, 16}, {8, 32}};
-      const int ty = kRw[v - 4][0], zc = std::min(D, kRw[v - 4][1]);
-      const int lz = (int)g3.lo[0], ly = (int)g3.lo[1], lx = (int)g3.lo[2];
-      const bool ok = ty == 8 ? launch_boxr<8>(true, 4 + comb, batch, D, H, W, kz, ky, kx, lz,
-                                               ly, lx, mode, cval, zc, GreyEpi{}, input, output, st)
-                              : launch_boxr<16>(true, 4 + comb, batch, D, H, W, kz, ky, kx, lz,
-                                                ly, lx, mode, cval, zc, GreyEpi{}, input, output,
-                                                st);
-      if (!ok) return TM_ERR_ARG;
+      const int zc = std::min(D, 8 << (v - 4));
+      if (!launch_boxr(true, 4 + comb, batch, D, H, W, kz, ky, kx, (int)g3.lo[0],
+                       (int)g3.lo[1], (int)g3.lo[2], mode, cval, zc, GreyEpi{}, input, output,
+                       st))
+        return TM_ERR_ARG;
       TM_LAUNCH_CHECK();
       return TM_OK;
     }
