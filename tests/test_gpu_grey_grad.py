@@ -120,3 +120,40 @@ def test_grey_backward_tiny_items_and_large_elements(tm, oracle_mod, mode, dilat
         want = oracle_mod.grey_backward(x[b, 0], st, fp, (7, 5), (1, -2), mode, 0.5, dilation,
                                         dy[b, 0])
         np.testing.assert_allclose(dx[b, 0], want, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("mode", ["reflect", "mirror", "nearest", "constant"])
+@pytest.mark.parametrize("dilation", [False, True])
+@pytest.mark.parametrize("shape", [(2, 1, 23, 131), (1, 1, 70, 64), (1, 1, 3, 2), (2, 1, 1, 5),
+                                   (1, 1, 5, 1), (1, 1, 2, 63), (1, 1, 17, 62)])
+def test_grey_backward_flat3_every_variant(tm, oracle_mod, mode, dilation, shape):
+    """The two launch variants of the flat 3 x 3 backward (0: LDS tiles, 1:
+    register streaming) route and sum identically (bit for bit: both sum in
+    element order) and match the oracle (which sums in its own order, hence
+    the tolerance), on quantised input (many ties), across strip and tile
+    edges and items smaller than the element; signed zeros and infinities in
+    the upstream gradient."""
+    from torchmorph_amd import autotune
+    from tests._golden import assert_bits_equal
+
+    rng = np.random.default_rng(sum(shape) + 3)
+    x = (np.floor(rng.random(shape) * 5) / 5).astype(np.float32)
+    dy = rng.normal(size=shape).astype(np.float32)
+    flat = dy.reshape(-1)
+    if flat.size > 8:
+        flat[:4] = [-0.0, 0.0, np.inf, -np.inf]
+    p2 = lambda v: 1 << (int(v) - 1).bit_length()
+    key = f"greybw {int(dilation)} {p2(shape[2])} {p2(shape[3])} {p2(shape[0] * shape[1])}"
+    got = []
+    try:
+        for v in (0, 1):
+            autotune.clear()
+            autotune.load({key: v})
+            got.append(_grad(tm, x, dy, dilation, size=3, mode=mode, cval=0.25)[1])
+    finally:
+        autotune.clear()
+    assert_bits_equal(got[1], got[0], "register variant vs LDS variant")
+    for b in range(shape[0]):
+        want = oracle_mod.grey_backward(x[b, 0], None, None, (3, 3), (0, 0), mode, 0.25,
+                                        dilation, dy[b, 0])
+        np.testing.assert_allclose(got[1][b, 0], want, rtol=1e-6, atol=1e-6)

@@ -1974,6 +1974,187 @@ __global__ __launch_bounds__(256) void grey_bw2d_kernel(const float* __restrict_
   }
 }
 
+// Backward of the flat 3 x 3 element with zero origin on 2-D items (modes
+// other than wrap), register streaming: grey_bw2d_kernel's routing and sums,
+// term for term, with no LDS.  A wave owns input columns x0 .. x0 + 61
+// (lanes 1 .. 62; lane l sits at column X = x0 - 1 + l) and kBwrRows input
+// rows.  It loads the mode-extended x rows y0 - 2 .. y0 + kBwrRows + 1 and
+// the dy rows y0 - 1 .. y0 + kBwrRows at once, forms the winning element of
+// every output of rows y0 - 1 .. y0 + kBwrRows (neighbouring columns by DPP
+// lane shifts; the columns past the wave's ends are loaded by lanes 0 and 1),
+// and each input then sums, in element order, over the outputs that route to
+// it.  Element k of the table (raster order) reads p + o_k: o_k = (1 - k / 3,
+// 1 - k % 3) for a dilation (the offsets are negated), (k / 3 - 1, k % 3 - 1)
+// for an erosion; its value is +0.0f (dilation) or -0.0f (erosion).  Every
+// in-item output that routes to an input q lies within one voxel of q (a
+// read outside the item at c maps back to c's reflection, one voxel from
+// the border), so border inputs sum over the same register window, with
+// grey_preimages' candidates in its order.
+constexpr int kBwrRows = 8;
+
+__device__ __forceinline__ float lane_prev(float v, float first) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(first), __float_as_int(v),
+                                                    0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+__device__ __forceinline__ int lane_prev_i(int v, int first) {
+  return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int lane_next_i(int v, int last) {
+  return __builtin_amdgcn_update_dpp(last, v, 0x130, 0xf, 0xf, false);
+}
+
+template <bool MAX>
+__global__ __launch_bounds__(256) void grey_bwr_kernel(const float* __restrict__ in,
+                                                       const float* __restrict__ dy,
+                                                       float* __restrict__ dx, int H, int W,
+                                                       int mode, float cval) {
+  constexpr int NXR = kBwrRows + 4, NOR = kBwrRows + 2;
+  const int ntx = (W + 61) / 62, nty = (H + kBwrRows - 1) / kBwrRows;
+  const int nyb = (nty + 3) / 4;
+  const int lb = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = lb % (ntx * nyb);
+  const int64_t item = lb / (ntx * nyb);
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int ty = (t / ntx) * 4 + wave;
+  if (ty >= nty) return;  // whole wave; no barriers in this kernel
+  const int x0 = (t % ntx) * 62, y0 = ty * kBwrRows;
+  const int X = x0 - 1 + lane;
+  const int64_t n = (int64_t)H * W;
+  const float* vol = in + item * n;
+  const float* dyv = dy + item * n;
+  float* dxv = dx + item * n;
+  const float val = MAX ? 0.0f : -0.0f;
+
+  // loads: x rows (this lane's column; lanes 0 / 1 also the columns left of
+  // lane 0 / right of lane 63), dy rows; every address in range
+  bool ux, uh;
+  const int xm = gmap32(X, W, mode, &ux);
+  const int xh = gmap32(lane == 1 ? x0 + 63 : x0 - 2, W, mode, &uh);
+  const bool xin = (unsigned)X < (unsigned)W;
+  const int xd = xin ? X : 0;
+  float xv[NXR], xe[NXR], dv[NOR];
+  bool ur[NXR];
+#pragma unroll
+  for (int i = 0; i < NXR; ++i) {
+    const int ym = gmap32(y0 - 2 + i, H, mode, &ur[i]);
+    xv[i] = vol[(int64_t)ym * W + xm];
+    xe[i] = vol[(int64_t)ym * W + xh];
+  }
+#pragma unroll
+  for (int i = 0; i < NOR; ++i) {
+    const int py = y0 - 1 + i;
+    dv[i] = dyv[(int64_t)((unsigned)py < (unsigned)H ? py : 0) * W + xd];
+  }
+  const bool hl = __builtin_amdgcn_readlane((int)uh, 0) != 0;
+  const bool hr = __builtin_amdgcn_readlane((int)uh, 1) != 0;
+
+  // x rows with the border substitution, and their values one column left
+  // and right
+  float xc[NXR], xl[NXR], xr[NXR];
+#pragma unroll
+  for (int i = 0; i < NXR; ++i) {
+    xc[i] = (ur[i] || ux) ? cval : xv[i];
+    const float el = (ur[i] || hl) ? cval
+                                   : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xe[i]), 0));
+    const float er = (ur[i] || hr) ? cval
+                                   : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(xe[i]), 1));
+    xl[i] = lane_prev(xc[i], el);
+    xr[i] = rw_next_lane(xc[i], er);
+  }
+  // winning element and dy of outputs y0 - 1 + i (-1 / 0 outside the item),
+  // and of the outputs one column left and right
+  int wc[NOR], wl[NOR], wr[NOR];
+  float dc[NOR], dl[NOR], dr[NOR];
+#pragma unroll
+  for (int i = 0; i < NOR; ++i) {
+    const int py = y0 - 1 + i;
+    const bool out = xin && (unsigned)py < (unsigned)H;
+    float best = 0.f;
+    int bi = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int oy = MAX ? 1 - k / 3 : k / 3 - 1, ox = MAX ? 1 - k % 3 : k % 3 - 1;
+      const int r = i + 1 + oy;  // x row of p + o_k
+      const float v = (ox < 0 ? xl[r] : ox > 0 ? xr[r] : xc[r]) + val;
+      if (k == 0 || (MAX ? v > best : v < best)) {
+        best = v;
+        bi = k;
+      }
+    }
+    wc[i] = out ? bi : -1;
+    dc[i] = out ? dv[i] : 0.f;
+    wl[i] = lane_prev_i(wc[i], -1);
+    wr[i] = lane_next_i(wc[i], -1);
+    dl[i] = lane_prev(dc[i], 0.f);
+    dr[i] = rw_next_lane(dc[i], 0.f);
+  }
+
+  // inputs
+  const bool lane_st = lane >= 1 && lane <= 62 && xin;
+  const uint64_t mxm = grey_preimages(X < 0 ? 0 : X, W, 2, mode);
+  const bool border_lanes = __builtin_amdgcn_ballot_w64(mxm != 1) != 0;  // uniform
+#pragma unroll
+  for (int j = 0; j < kBwrRows; ++j) {
+    const int qy = y0 + j;
+    const uint64_t mym = grey_preimages(qy, H, 2, mode);  // uniform
+    float acc = 0.f;
+    if (mym == 1 && !border_lanes) {
+      // q is its own only preimage: p = q - o_k
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const int oy = MAX ? 1 - k / 3 : k / 3 - 1, ox = MAX ? 1 - k % 3 : k % 3 - 1;
+        const int r = j + 1 - oy;  // output row of p
+        const int w = ox > 0 ? wl[r] : ox < 0 ? wr[r] : wc[r];
+        const float d = ox > 0 ? dl[r] : ox < 0 ? dr[r] : dc[r];
+        if (w == k) acc += d;
+      }
+    } else {
+      // candidates c of q per axis in grey_preimages' bit order (q, -2, -1,
+      // n, n + 1); p = c - o_k routes to q when it lies in the item with
+      // winner k.  Of the candidates outside the item only -1 (with o = -1:
+      // p = 0) and n (with o = +1: p = n - 1) reach the item, so each
+      // element has at most (1 + |oy|)(1 + |ox|) terms, in that order; p then
+      // lies within one voxel of q (rows j .. j + 2 of the window).  The
+      // window's values pass an empty asm so the selects below stay selects
+      // of registers (not of addresses into a stack copy of the arrays).
+      int w0 = wl[j], w1 = wc[j], w2 = wr[j], w3 = wl[j + 1], w4 = wc[j + 1], w5 = wr[j + 1],
+          w6 = wl[j + 2], w7 = wc[j + 2], w8 = wr[j + 2];
+      float e0 = dl[j], e1 = dc[j], e2 = dr[j], e3 = dl[j + 1], e4 = dc[j + 1], e5 = dr[j + 1],
+            e6 = dl[j + 2], e7 = dc[j + 2], e8 = dr[j + 2];
+      asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3), "+v"(w4), "+v"(w5), "+v"(w6),
+                   "+v"(w7), "+v"(w8));
+      asm volatile("" : "+v"(e0), "+v"(e1), "+v"(e2), "+v"(e3), "+v"(e4), "+v"(e5), "+v"(e6),
+                   "+v"(e7), "+v"(e8));
+      const bool yl = (mym >> 2) & 1u, yh = (mym >> 3) & 1u;  // -1 / H map to qy (uniform)
+      const bool xlo = (mxm >> 2) & 1u, xhi = (mxm >> 3) & 1u;  // -1 / W map to X
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const int oy = MAX ? 1 - k / 3 : k / 3 - 1, ox = MAX ? 1 - k % 3 : k % 3 - 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if ((a == 1 && oy != -1) || (a == 2 && oy != 1)) continue;
+          const bool vy = a == 0 ? true : a == 1 ? yl : yh;
+          const int ry = a == 0 ? -oy : a == 1 ? -qy : H - 1 - qy;
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            if ((b == 1 && ox != -1) || (b == 2 && ox != 1)) continue;
+            const bool vx = b == 0 ? true : b == 1 ? xlo : xhi;
+            const int rx = b == 0 ? -ox : b == 1 ? -X : W - 1 - X;
+            const int w = ry < 0 ? (rx < 0 ? w0 : rx > 0 ? w2 : w1)
+                                 : ry > 0 ? (rx < 0 ? w6 : rx > 0 ? w8 : w7)
+                                          : (rx < 0 ? w3 : rx > 0 ? w5 : w4);
+            const float d = ry < 0 ? (rx < 0 ? e0 : rx > 0 ? e2 : e1)
+                                   : ry > 0 ? (rx < 0 ? e6 : rx > 0 ? e8 : e7)
+                                            : (rx < 0 ? e3 : rx > 0 ? e5 : e4);
+            if (vy && vx && w == k) acc += d;
+          }
+        }
+      }
+    }
+    if (lane_st && qy < H) dxv[(int64_t)qy * W + X] = acc;
+  }
+}
+
 static size_t grey_bw2d_lds(const GreyTabN& tab) {
   const int py = tab.ext[0] + 1, px = tab.ext[1] + 1;
   const size_t OR = kBwTY + 2 * py, OC = kBwTX + 2 * px;
@@ -2041,22 +2222,53 @@ extern "C" int tm_grey_backward(const float* input, int64_t batch, int ndim, con
   if (ndim == 2 && mode != 4 && ext[0] <= 16 && ext[1] <= 16) {
     const int H = (int)shape[0], W = (int)shape[1];
     const int64_t G = (int64_t)((W + kBwTX - 1) / kBwTX) * ((H + kBwTY - 1) / kBwTY) * batch;
-    if (G < ((int64_t)1 << 31)) {
+    // the flat 3 x 3 element with zero origin: every element in raster
+    // order, offsets +-(r - 1), values +0.0f / -0.0f (grey_bwr_kernel's table)
+    bool flat3 = n == 9 && se_shape[0] == 3 && se_shape[1] == 3;
+    for (int k = 0; k < n && flat3; ++k) {
+      const int oy = dilation ? 1 - k / 3 : k / 3 - 1, ox = dilation ? 1 - k % 3 : k % 3 - 1;
+      uint32_t vb, want = dilation ? 0u : 0x80000000u;
+      memcpy(&vb, &tab.v[k], 4);
+      flat3 = tab.o[k][0] == oy && tab.o[k][1] == ox && vb == want;
+    }
+    const int64_t Gr = (int64_t)((W + 61) / 62) * (((H + kBwrRows - 1) / kBwrRows + 3) / 4) * batch;
+    if (G < ((int64_t)1 << 31) && Gr < ((int64_t)1 << 31)) {
       const size_t lds = grey_bw2d_lds(tab);
       const bool e1 = ext[0] == 1 && ext[1] == 1;
+      // launch variant 0: the LDS-tiled kernel; 1: register streaming (flat
+      // 3 x 3 only)
+      auto run = [&](int v) -> int {
+        if (v == 1) {
+          if (!flat3) return TM_ERR_ARG;
+          if (dilation)
+            hipLaunchKernelGGL(grey_bwr_kernel<true>, dim3((unsigned)Gr), dim3(256), 0, st,
+                               input, grad_out, grad_in, H, W, mode, cval);
+          else
+            hipLaunchKernelGGL(grey_bwr_kernel<false>, dim3((unsigned)Gr), dim3(256), 0, st,
+                               input, grad_out, grad_in, H, W, mode, cval);
+          TM_LAUNCH_CHECK();
+          return TM_OK;
+        }
 #define TM_BW(MX, EE)                                                                           \
   hipLaunchKernelGGL((grey_bw2d_kernel<MX, EE>), dim3((unsigned)G), dim3(256), lds, st, input,     \
                      grad_out, grad_in, H, W, tab, mode, cval)
-      if (dilation) {
-        if (e1) TM_BW(true, 1);
-        else TM_BW(true, 0);
-      } else {
-        if (e1) TM_BW(false, 1);
-        else TM_BW(false, 0);
-      }
+        if (dilation) {
+          if (e1) TM_BW(true, 1);
+          else TM_BW(true, 0);
+        } else {
+          if (e1) TM_BW(false, 1);
+          else TM_BW(false, 0);
+        }
 #undef TM_BW
-      TM_LAUNCH_CHECK();
-      return TM_OK;
+        TM_LAUNCH_CHECK();
+        return TM_OK;
+      };
+      if (!flat3) return run(0);
+      auto p2 = [](int64_t v) { int64_t q = 1; while (q < v) q <<= 1; return q; };
+      const int64_t key[4] = {dilation ? 1 : 0, p2(H), p2(W), p2(batch)};
+      bool ran = false;
+      const int v = tune_pick("greybw", key, 4, 2, 1, run, st, &ran);
+      return ran ? TM_OK : run(v);
     }
   }
   for (int t = 0; t < tab.n; ++t) {
