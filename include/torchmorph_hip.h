@@ -223,6 +223,9 @@ int tm_grey_ex(const float* input, int64_t batch, int ndim, const int64_t* shape
                const int64_t* origin, int mode, float cval, int dilation, int epilogue,
                const float* a, const float* b, float* output, void* stream);
 
+/* Workspace bytes for tm_grey_backward's general path (0 is enough for 2-D
+ * items in modes other than wrap with elements of at most 33 x 33; otherwise
+ * TM_ERR_WORKSPACE is reported before anything is launched). */
 size_t tm_grey_backward_workspace_bytes(int64_t batch, int ndim, const int64_t* shape);
 
 /* No reference counterpart (the reference ops are not differentiable).

@@ -16,6 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtorchmorph_hip.so")
 
 TM_U8, TM_F32, TM_I32, TM_I64, TM_F64 = 0, 1, 2, 3, 4
+TM_OK, TM_ERR_ARG, TM_ERR_HIP, TM_ERR_WORKSPACE = 0, -1, -2, -3  # enum tm_status
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int

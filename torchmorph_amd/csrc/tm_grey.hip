@@ -2211,13 +2211,6 @@ extern "C" int tm_grey_backward(const float* input, int64_t batch, int ndim, con
   }
   TM_CHECK_ARG(n > 0, "tm_grey_backward: footprint must contain at least one active position");
   tab.n = n;
-  const size_t need = tm_grey_backward_workspace_bytes(batch, ndim, shape);
-  if (!workspace || workspace_bytes < need) {
-    set_error("tm_grey_backward: workspace of %zu bytes is smaller than the %zu required",
-              workspace_bytes, need);
-    return TM_ERR_WORKSPACE;
-  }
-  int16_t* arg = (int16_t*)workspace;
   for (int j = 0; j < kMaxNdim; ++j) tab.ext[j] = j < ndim ? ext[j] : 0;
   if (ndim == 2 && mode != 4 && ext[0] <= 16 && ext[1] <= 16) {
     const int H = (int)shape[0], W = (int)shape[1];
@@ -2271,6 +2264,15 @@ extern "C" int tm_grey_backward(const float* input, int64_t batch, int ndim, con
       return ran ? TM_OK : run(v);
     }
   }
+  // the general path (other ranks, wrap, larger elements) keeps the winning
+  // element of every output in the workspace; the 2-D kernels above need none
+  const size_t need = tm_grey_backward_workspace_bytes(batch, ndim, shape);
+  if (!workspace || workspace_bytes < need) {
+    set_error("tm_grey_backward: workspace of %zu bytes is smaller than the %zu required",
+              workspace_bytes, need);
+    return TM_ERR_WORKSPACE;
+  }
+  int16_t* arg = (int16_t*)workspace;
   for (int t = 0; t < tab.n; ++t) {
     int64_t f = 0;
     for (int j = 0; j < ndim; ++j) f += (int64_t)tab.o[t][j] * sh.stride[j];

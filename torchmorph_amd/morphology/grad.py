@@ -24,6 +24,8 @@ class _GreyMorph(torch.autograd.Function):
         _launch(x32, struct, fp, se_shape, origin, mode_int, cval, dilation, y)
         ctx.save_for_backward(x32)
         ctx.args = (struct, fp, se_shape, origin, mode_int, cval, dilation)
+        ctx.arrays = (_lib.i64_array(x32.shape[2:]), _lib.i64_array(se_shape),
+                      _lib.i64_array(origin))
         ctx.in_dtype = x.dtype
         return y
 
@@ -31,24 +33,31 @@ class _GreyMorph(torch.autograd.Function):
     def backward(ctx, gy):
         (x32,) = ctx.saved_tensors
         struct, fp, se_shape, origin, mode_int, cval, dilation = ctx.args
+        shp, se_arr, org_arr = ctx.arrays
         dx = torch.empty_like(x32)
         gy = gy.float().contiguous()
         L = _lib.lib()
         nd = x32.ndim - 2
         batch = x32.shape[0] * x32.shape[1]
-        shp = _lib.i64_array(x32.shape[2:])
         st = None if struct is None else struct.numpy()
         fpn = None if fp is None else fp.to(torch.uint8).numpy()
-        nbytes = L.tm_grey_backward_workspace_bytes(batch, nd, shp)
-        ws = _lib.workspace(nbytes, x32.device)
-        with torch.cuda.device(x32.device):
-            _lib.check(L.tm_grey_backward(
+
+        def call(ws, nbytes):
+            return L.tm_grey_backward(
                 _lib.ptr(x32), batch, nd, shp,
                 None if st is None else st.ctypes.data_as(_lib.ctypes.c_void_p),
                 None if fpn is None else fpn.ctypes.data_as(_lib.ctypes.c_void_p),
-                _lib.i64_array(se_shape), _lib.i64_array(origin), mode_int, float(cval),
-                int(dilation), _lib.ptr(gy), _lib.ptr(dx), _lib.ptr(ws), nbytes,
-                _lib.stream_of(x32.device)), "tm_grey_backward")
+                se_arr, org_arr, mode_int, float(cval), int(dilation), _lib.ptr(gy),
+                _lib.ptr(dx), _lib.ptr(ws), nbytes, _lib.stream_of(x32.device))
+
+        with torch.cuda.device(x32.device):
+            # 2-D items (modes other than wrap) usually run kernels that need
+            # no workspace; the library asks for one when they do not apply
+            rc = call(None, 0) if nd == 2 and mode_int != 4 else _lib.TM_ERR_WORKSPACE
+            if rc == _lib.TM_ERR_WORKSPACE:
+                nbytes = L.tm_grey_backward_workspace_bytes(batch, nd, shp)
+                rc = call(_lib.workspace(nbytes, x32.device), nbytes)
+            _lib.check(rc, "tm_grey_backward")
         return dx.to(ctx.in_dtype), None, None, None, None, None, None, None
 
 
