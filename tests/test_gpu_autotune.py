@@ -265,6 +265,57 @@ def test_grey_register_box_every_tile_matches_oracle(tm, oracle_mod, case, mode)
         autotune.clear()
 
 
+def _ball3():
+    r = np.sqrt(((np.indices((7, 7, 7)) - 3) ** 2).sum(0))
+    fp = r <= 3.0
+    return np.where(fp, np.floor(3.0 - r), 0.0).astype(np.float32), fp
+
+
+@pytest.mark.parametrize("shape", [(2, 1, 20, 30, 70), (1, 2, 13, 7, 130), (1, 1, 5, 9, 3),
+                                   (1, 1, 2, 1, 66)])
+@pytest.mark.parametrize("mode", ["reflect", "constant", "nearest", "mirror", "wrap"])
+def test_grey_ball3_every_variant_matches_oracle(tm, oracle_mod, shape, mode):
+    """The ball of radius 3 with heights floor(3 - r) (the 3-D morphology
+    benchmark's non-flat element): launch variants 4..7 (superlevel-ball
+    register kernel) and the element-table kernel give the oracle's bits for
+    dilation, erosion and the tophats' fused stores; NaN, infinities and
+    signed zeros in the input."""
+    from torchmorph_amd import autotune
+
+    rng = np.random.default_rng(sum(shape) + len(mode))
+    x = (np.floor(rng.random(shape) * 16) / 16 - 0.5).astype(np.float32)
+    flat = x.reshape(-1)
+    if flat.size > 40:
+        idx = rng.choice(flat.size, 20, replace=False)
+        flat[idx[:5]] = -0.0
+        flat[idx[5:10]] = np.inf
+        flat[idx[10:15]] = -np.inf
+        flat[idx[15:]] = np.nan
+    st, fp = _ball3()
+    kw = dict(structure=torch.from_numpy(st), footprint=torch.from_numpy(fp), mode=mode,
+              cval=0.25)
+    want_d = oracle_mod.grey(x, st, fp, (7, 7, 7), (0, 0, 0), mode=mode, cval=0.25, dilation=True)
+    want_e = oracle_mod.grey(x, st, fp, (7, 7, 7), (0, 0, 0), mode=mode, cval=0.25, dilation=False)
+    with np.errstate(invalid="ignore"):
+        want_wt = x - oracle_mod.grey(want_e, st, fp, (7, 7, 7), (0, 0, 0), mode=mode, cval=0.25,
+                                      dilation=True)
+        want_bt = oracle_mod.grey(want_d, st, fp, (7, 7, 7), (0, 0, 0), mode=mode, cval=0.25,
+                                  dilation=False) - x
+    key = " ".join(["grey3", "2", "7", "7", "7", *map(_p2s, shape[2:]),
+                    _p2s(shape[0] * shape[1])])
+    xt = torch.as_tensor(x, device="cuda")
+    try:
+        for v in (0, 4, 5, 6, 7):
+            autotune.clear()
+            autotune.load({key: v})
+            assert_bits_equal(tm.grey_dilation(xt, **kw).cpu(), want_d, f"dilation {v}")
+            assert_bits_equal(tm.grey_erosion(xt, **kw).cpu(), want_e, f"erosion {v}")
+            assert_bits_equal(tm.white_tophat(xt, **kw).cpu(), want_wt, f"white tophat {v}")
+            assert_bits_equal(tm.black_tophat(xt, **kw).cpu(), want_bt, f"black tophat {v}")
+    finally:
+        autotune.clear()
+
+
 def _edtnd_key(shape, idx):
     """Cache key of the N-D EDT on (B, C, *spatial)."""
     sp = shape[2:]

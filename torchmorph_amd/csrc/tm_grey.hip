@@ -438,6 +438,17 @@ __device__ __forceinline__ float rw_next_lane(float v, float last) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(last), __float_as_int(v),
                                                     0x130 /* wave_shl:1 */, 0xf, 0xf, false));
 }
+// Value of lane l - 1 (lane 0: `first`); the integer forms of both shifts.
+__device__ __forceinline__ float lane_prev(float v, float first) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(first), __float_as_int(v),
+                                                    0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+__device__ __forceinline__ int lane_prev_i(int v, int first) {
+  return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int lane_next_i(int v, int last) {
+  return __builtin_amdgcn_update_dpp(last, v, 0x130, 0xf, 0xf, false);
+}
 
 template <bool MAX, int KX, int KY, int KZ, int TY, int EPI>
 __global__ __launch_bounds__(256) void grey_boxr_kernel(const float* __restrict__ in,
@@ -605,6 +616,157 @@ __global__ __launch_bounds__(256) void grey_boxr_kernel(const float* __restrict_
   RwRows<KX, KY, TY> A, B;
   fetch(A, zb + sz);
   fetch(B, zb + sz + 1);
+  for (int p = 0; p < nplanes; p += 2) {
+    step(A, p);
+    if (p + 1 < nplanes) step(B, p + 1);
+  }
+}
+
+// The ball of radius 3 with heights floor(3 - r) (r <= 3, zero origin),
+// register streaming.  Grouping the elements by superlevel sets of their
+// height h (balls of radius 3, 2, 1 and 0 for h = 0..3), a dilation is
+//   max_h (h + max of x over the ball B_h),
+// exactly: x -> x + h rounds monotonically, so the terms the larger balls
+// add are never above the element's own, and max over a set of x + h is
+// (max over the set of x) + h bit for bit (no ties between -0 and +0 arise:
+// a sum x + h that is zero is +0).  Erosion likewise with min and -h (h = 0
+// adds -0.0f, the identity).  A ball is a union of z planes of 2-D disks, and
+// a disk a union of rows of 1-D x runs of half-width 0..3, so per input
+// plane a wave forms, for each of its rows, the x-run extrema S_0..S_3 (DPP
+// lane shifts; lanes 0..5 load the six columns past the strip), then per
+// output row the seven disks (radius^2 9, 8, 5, 4, 3, 1, 0) that the four
+// balls cut from the planes at |dz| = 0..3, adds the heights, and folds the
+// plane into the seven output planes z = p - 3 .. p + 3 still open.  A wave
+// owns a 64-wide x strip of kBallRows output rows and a z chunk; the four
+// waves of a workgroup take consecutive y tiles; no LDS, no barriers.
+constexpr int kBallRows = 4;
+
+template <bool MAX>
+__global__ __launch_bounds__(256) void grey_ballr_kernel(const float* __restrict__ in,
+                                                         float* __restrict__ out, int D, int H,
+                                                         int W, int mode, float cval,
+                                                         int zchunk, GreyEpi epi, int epil) {
+  constexpr int TY = kBallRows, NR = TY + 6;
+  const int ntx = (W + 63) >> 6;
+  const int nty = (H + TY - 1) / TY;
+  const int nyb = (nty + kRwWaves - 1) / kRwWaves;
+  const int nz = (D + zchunk - 1) / zchunk;
+  int tx, tyb, zci;
+  int64_t item;
+  grey_block(ntx, ntx * nyb, nz, &tx, &tyb, &zci, &item);
+  const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+  const int ty = tyb * kRwWaves + wave;
+  if (ty >= nty) return;  // whole wave; no barriers in this kernel
+  const int zb = zci * zchunk, ze = min(D, zb + zchunk);
+  const int x0 = tx * 64, y0 = ty * TY;
+  const int64_t n = (int64_t)D * H * W;
+  const float* vol = in + item * n;
+  float* ovol = out + item * n;
+  const GreyEpi ei{epi.a ? epi.a + item * n : nullptr, epi.b ? epi.b + item * n : nullptr};
+  const float ID = gid<MAX>();
+  const float h0 = MAX ? 0.0f : -0.0f;
+  const float hs = MAX ? 1.0f : -1.0f;  // heights 1..3 enter as h * hs
+
+  // border maps (byte offsets): this lane's column, the column past the strip
+  // it loads (lanes 0..2: x0 - 3 + l; lanes 3..5: x0 + 61 + l), the rows
+  bool xc, ec;
+  const uint32_t xo = (uint32_t)gmap(x0 + lane, W, mode, &xc) * 4u;
+  const int el = lane < 3 ? x0 - 3 + lane : x0 + 61 + min(lane, 5);
+  const uint32_t eo = (uint32_t)gmap(el, W, mode, &ec) * 4u;
+  uint32_t hcm = 0;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) hcm |= (__builtin_amdgcn_readlane((int)ec, j) ? 1u : 0u) << j;
+  uint32_t ro[NR];
+  uint32_t rcm = 0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    bool c;
+    ro[r] = (uint32_t)gmap(y0 - 3 + r, H, mode, &c) * (uint32_t)W * 4u;
+    rcm |= (c ? 1u : 0u) << r;
+  }
+  auto fetch = [&](RwRows<2, NR - TY + 1, TY>& R, int zi) {
+    bool zc;
+    const int64_t zm = gmap(zi, D, mode, &zc);
+    R.zc = zc;
+    const char* pl = (const char*)(vol + zm * H * W);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      R.v[r] = *(const float*)(pl + (ro[r] + xo));
+      R.e[r] = *(const float*)(pl + (ro[r] + eo));
+    }
+  };
+  auto ex = [&](const RwRows<2, NR - TY + 1, TY>& R, int r, int j, bool cr) {
+    const float h = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(R.e[r]), j));
+    return (cr || ((hcm >> j) & 1u)) ? cval : h;
+  };
+
+  // acc[k][t]: output plane z = p - 3 + k of output row t, open while the
+  // planes p - 3 + k - 3 .. p - 3 + k + 3 arrive
+  float acc[7][TY];
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+#pragma unroll
+    for (int t = 0; t < TY; ++t) acc[k][t] = ID;
+  const int nplanes = (ze - zb) + 6;
+  auto step = [&](RwRows<2, NR - TY + 1, TY>& R, int p) {
+    // x-run extrema of every row: S[w][r] over columns x - w .. x + w
+    float S0[NR], S1[NR], S2[NR], S3[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const bool cr = R.zc || ((rcm >> r) & 1u);
+      const float v = (cr || xc) ? cval : R.v[r];
+      const float l1 = lane_prev(v, ex(R, r, 2, cr));
+      const float l2 = lane_prev(l1, ex(R, r, 1, cr));
+      const float l3 = lane_prev(l2, ex(R, r, 0, cr));
+      const float r1 = rw_next_lane(v, ex(R, r, 3, cr));
+      const float r2 = rw_next_lane(r1, ex(R, r, 4, cr));
+      const float r3 = rw_next_lane(r2, ex(R, r, 5, cr));
+      S0[r] = v;
+      S1[r] = gop<MAX>(gop<MAX>(l1, v), r1);
+      S2[r] = gop<MAX>(gop<MAX>(l2, S1[r]), r2);
+      S3[r] = gop<MAX>(gop<MAX>(l3, S2[r]), r3);
+    }
+    // this buffer's registers are free: plane p + 2 (mapped, unused past the end)
+    fetch(R, zb - 3 + p + 2);
+    const int z = zb - 3 + p - 3;  // the output plane this plane completes
+#pragma unroll
+    for (int t = 0; t < TY; ++t) {
+      const int c = t + 3;  // row of output row t among the plane's rows
+      auto m3 = [&](const float* A, int i) { return gop<MAX>(gop<MAX>(A[c - i], A[c]), A[c + i]); };
+      auto m2 = [&](const float* A, int i) { return gop<MAX>(A[c - i], A[c + i]); };
+      const float d9 = gop<MAX>(gop<MAX>(gop<MAX>(S3[c], m2(S2, 1)), m2(S2, 2)), m2(S0, 3));
+      const float d8 = gop<MAX>(gop<MAX>(S2[c], m2(S2, 1)), m2(S2, 2));
+      const float d5 = gop<MAX>(gop<MAX>(S2[c], m2(S2, 1)), m2(S1, 2));
+      const float d4 = gop<MAX>(gop<MAX>(S2[c], m2(S1, 1)), m2(S0, 2));
+      const float d3 = m3(S1, 1);
+      const float d1 = gop<MAX>(S1[c], m2(S0, 1));
+      const float d0 = S0[c];
+      // the plane's part of each ball level at |dz| = 0..3, heights added
+      const float c0 = gop<MAX>(gop<MAX>(d9 + h0, d4 + hs), gop<MAX>(d1 + 2.0f * hs, d0 + 3.0f * hs));
+      const float c1 = gop<MAX>(gop<MAX>(d8 + h0, d3 + hs), d0 + 2.0f * hs);
+      const float c2 = gop<MAX>(d5 + h0, d0 + hs);
+      const float c3 = d0 + h0;
+      // open outputs z + k (k = 0..6) at dz = 3 - k
+      acc[0][t] = gop<MAX>(acc[0][t], c3);
+      acc[1][t] = gop<MAX>(acc[1][t], c2);
+      acc[2][t] = gop<MAX>(acc[2][t], c1);
+      acc[3][t] = gop<MAX>(acc[3][t], c0);
+      acc[4][t] = gop<MAX>(acc[4][t], c1);
+      acc[5][t] = gop<MAX>(acc[5][t], c2);
+      acc[6][t] = gop<MAX>(acc[6][t], c3);
+      const int y = y0 + t, x = x0 + lane;
+      if (z >= zb && y < H && x < W) {
+        const int64_t q = ((int64_t)z * H + y) * W + x;
+        ovol[q] = epil == 1 ? acc[0][t] - ei.a[q] : epil == 2 ? ei.a[q] - acc[0][t] : acc[0][t];
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[k][t] = acc[k + 1][t];
+      acc[6][t] = ID;
+    }
+  };
+  RwRows<2, NR - TY + 1, TY> A, B;
+  fetch(A, zb - 3);
+  fetch(B, zb - 2);
   for (int p = 0; p < nplanes; p += 2) {
     step(A, p);
     if (p + 1 < nplanes) step(B, p + 1);
@@ -1141,6 +1303,43 @@ static bool launch_boxr(bool mx, int epi, int64_t batch, int D, int H, int W, in
   return true;
 }
 
+// The ball of radius 3 with heights floor(3 - r) on 3-D items (grey_ballr_kernel):
+// 123 elements at every offset with r^2 <= 9, value h (dilation) or -h
+// (erosion, -0.0f for h = 0), h = 3, 2, 1, 0 for r^2 = 0, 1, 2..4, 5..9.
+static bool grey_is_ball3(int ndim, const GreyElems& E, bool is_max) {
+  if (ndim != 3 || E.n != 123) return false;
+  for (int t = 0; t < E.n; ++t) {
+    const int32_t* o = E.off + 3 * t;
+    const int r2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    if (r2 > 9) return false;
+    const int h = r2 == 0 ? 3 : r2 == 1 ? 2 : r2 <= 4 ? 1 : 0;
+    const float want = is_max ? (float)h : (h == 0 ? -0.0f : -(float)h);
+    uint32_t a, b;
+    memcpy(&a, &E.val[t], 4);
+    memcpy(&b, &want, 4);
+    if (a != b) return false;
+  }
+  return true;
+}
+
+static bool launch_ballr(bool mx, int epi, int64_t batch, int D, int H, int W, int mode,
+                         float cval, int zchunk, GreyEpi e, const float* in, float* out,
+                         hipStream_t st) {
+  if (epi < 0 || epi > 2) return false;
+  if ((int64_t)H * W > ((int64_t)1 << 30)) return false;  // 32-bit byte offsets in a plane
+  const int64_t ntx = (W + 63) / 64, nty = (H + kBallRows - 1) / kBallRows;
+  const int64_t nyb = (nty + kRwWaves - 1) / kRwWaves, nz = (D + zchunk - 1) / zchunk;
+  const int64_t G = ntx * nyb * nz * batch;
+  if (G >= ((int64_t)1 << 31)) return false;
+  if (mx)
+    hipLaunchKernelGGL(grey_ballr_kernel<true>, dim3((unsigned)G), dim3(256), 0, st, in, out, D,
+                       H, W, mode, cval, zchunk, e, epi);
+  else
+    hipLaunchKernelGGL(grey_ballr_kernel<false>, dim3((unsigned)G), dim3(256), 0, st, in, out, D,
+                       H, W, mode, cval, zchunk, e, epi);
+  return true;
+}
+
 template <bool MX, int NR>
 static void launch_table3(int epi, unsigned G, size_t lds, hipStream_t st, const float* in,
                           float* out, int D, int H, int W, const GreyTable& tab, const Grey3& g3,
@@ -1248,7 +1447,18 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
     if (grey_table3_fits(E, g3) && plane32(H, W)) {
       const GreyTable tab = grey_table3_of(ndim, E);
       const size_t lds = table3_lds(kz, ky, kx);
+      const bool ball3 = grey_is_ball3(ndim, E, is_max);
+      // variants 4..7: the ball of radius 3 with heights floor(3 - r)
+      // (grey_ballr_kernel), z chunks of 8 << (v - 4) planes; not applicable
+      // to other elements
       auto run = [&](int v) -> int {
+        if (v >= 4) {
+          if (!ball3 || !launch_ballr(is_max, epi, batch, D, H, W, mode, cval,
+                                      std::min(D, 8 << (v - 4)), ge, input, output, st))
+            return TM_ERR_ARG;
+          TM_LAUNCH_CHECK();
+          return TM_OK;
+        }
         const int zchunk = zchunk_of(v);
         const int64_t G = grid_of(zchunk);
         if (G >= ((int64_t)1 << 31)) {
@@ -1272,8 +1482,10 @@ static int grey_run(const float* input, int64_t batch, int ndim, const int64_t* 
       };
       const int64_t key[8] = {2, kz, ky, kx, p2(D), p2(H), p2(W), p2(batch)};
       bool ran = false;
-      const int v = tune_pick("grey3", key, 8, 4, 1, run, st, &ran);
-      return ran ? TM_OK : run(v);
+      const int v = tune_pick("grey3", key, 8, 8, 1, run, st, &ran);
+      if (ran) return TM_OK;
+      const int rc = run(v);
+      return (rc == TM_ERR_ARG && v != 1) ? run(1) : rc;  // a cached variant that does not apply
     }
   }
   TM_CHECK_ARG(epi == 0,
@@ -1992,16 +2204,6 @@ __global__ __launch_bounds__(256) void grey_bw2d_kernel(const float* __restrict_
 // grey_preimages' candidates in its order.
 constexpr int kBwrRows = 8;
 
-__device__ __forceinline__ float lane_prev(float v, float first) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(first), __float_as_int(v),
-                                                    0x138 /* wave_shr:1 */, 0xf, 0xf, false));
-}
-__device__ __forceinline__ int lane_prev_i(int v, int first) {
-  return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xf, 0xf, false);
-}
-__device__ __forceinline__ int lane_next_i(int v, int last) {
-  return __builtin_amdgcn_update_dpp(last, v, 0x130, 0xf, 0xf, false);
-}
 
 template <bool MAX>
 __global__ __launch_bounds__(256) void grey_bwr_kernel(const float* __restrict__ in,
