@@ -437,8 +437,12 @@ __global__ __launch_bounds__(kRgThreads) void rg_raster_kernel(const TIn* __rest
 
 // One schedule-2 launch: blocks [lo, hi) of the block table, all of one
 // kind of pass at one level.
+// chain: the group of jobs whose pass chain the launch belongs to (schedule
+// 2 runs each chain on its own stream): 0 EDT of 3-D items, 1 EDT of other
+// items, 2 taxicab, 3 chessboard and raster (chamfer with indices, on the
+// caller's stream).
 struct RgLaunch {
-  int level, kind, lo, hi;
+  int level, kind, lo, hi, chain;
 };
 
 struct RgPlan {
@@ -594,14 +598,22 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
   // schedule 2: per level, per kind, blocks of one line per lane
   p.blks.clear();
   p.launches.clear();
-  for (int lev = 0; lev < p.nlevels; ++lev) {
-    for (int g = 0; g < 9; ++g) {
-      // g: 0 EDT0, 1 EDT envelope, 2 CDT0, 3 taxicab, 4 chessboard (every
-      //    axis >= 1, voxel blocks, cheb_window), 5 raster; along the
-      //    contiguous last axis, row-staged: 6 EDT 2-D, 7 EDT 3-D, 8 taxicab
-      RgLaunch ln{lev, 0, (int)p.blks.size(), 0};
+  auto chain_of = [&](const RgJob& j) {
+    if (j.raster) return 3;
+    if (j.op == TM_RG_EDT) return j.sh.ndim == 3 ? 0 : 1;
+    return j.op == TM_RG_CDT_TAXICAB ? 2 : 3;
+  };
+  for (int lev = 0; lev < p.nlevels; ++lev)
+  for (int chain = 0; chain < 4; ++chain) {
+    for (int g = 0; g < 10; ++g) {
+      // g: 0 EDT0, 1 EDT envelope, 2 CDT0 of taxicab jobs, 3 taxicab, 4
+      //    chessboard (every axis >= 1, voxel blocks, cheb_window), 5 raster;
+      //    along the contiguous last axis, row-staged: 6 EDT 2-D, 7 EDT 3-D,
+      //    8 taxicab; 9 CDT0 of chessboard jobs
+      RgLaunch ln{lev, 0, (int)p.blks.size(), 0, 0};
       for (int32_t ji : order) {
         const RgJob& j = p.jobs[(size_t)ji];
+        if (chain_of(j) != chain) continue;
         if (j.raster) {
           if (g != 5 || lev >= 2) continue;
           p.blks.push_back(RgBlk{ji, 1, 0});
@@ -616,7 +628,7 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
           else if (last && rows_final(j)) grp = j.sh.ndim == 2 ? 6 : 7;
           else grp = 1;
         } else if (lev == 0) {
-          grp = 2;
+          grp = j.op == TM_RG_CDT_TAXICAB ? 2 : 9;
         } else if (j.op == TM_RG_CDT_TAXICAB) {
           grp = last ? 8 : 3;
         } else {
@@ -643,10 +655,12 @@ static int build_plan(const tm_ragged_job* in, int64_t njobs, bool want_dist, bo
                                   p.jobs[(size_t)y.job].sh.size[lev];
                          });
       }
-      static const int kinds[9] = {RG_EDT0,       RG_EDT_STACK,  RG_CDT0,
-                                   RG_CDT_TAXICAB, RG_CDT_CHEB_WINDOW, RG_CDT_RASTER,
-                                   RG_EDT_ROWS2,  RG_EDT_ROWS3,  RG_CDT_TAXICAB_ROWS};
+      static const int kinds[10] = {RG_EDT0,       RG_EDT_STACK,  RG_CDT0,
+                                    RG_CDT_TAXICAB, RG_CDT_CHEB_WINDOW, RG_CDT_RASTER,
+                                    RG_EDT_ROWS2,  RG_EDT_ROWS3,  RG_CDT_TAXICAB_ROWS,
+                                    RG_CDT0};
       ln.kind = kinds[g];
+      ln.chain = chain;
       if (ln.hi > ln.lo) p.launches.push_back(ln);
     }
   }
@@ -691,6 +705,36 @@ static int grid_limit(int dtype_slot, const void* kern) {
   return d.cus * d.occ[dtype_slot];
 }
 
+// Side streams of schedule 2, one set per host thread and device: the EDT,
+// taxicab and chessboard pass chains are independent (their jobs share no
+// memory), so each runs on its own stream and the long last blocks of one
+// chain's launches overlap the other chains' work.  The chains fork from
+// the caller's stream and join it again through events, which also holds
+// while the caller's stream is being captured into a graph.  Streams at the
+// default priority; never destroyed (a captured graph may refer to them).
+struct RgSideStreams {
+  bool ok = false, tried = false;
+  hipStream_t s[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+};
+constexpr int kRgMaxDevices = 16;
+static thread_local RgSideStreams g_rg_side[kRgMaxDevices];
+
+static RgSideStreams* rg_side_streams() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kRgMaxDevices) return nullptr;
+  RgSideStreams& r = g_rg_side[dev];
+  if (!r.tried) {
+    r.tried = true;
+    bool ok = hipEventCreateWithFlags(&r.fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < 3 && ok; ++i)
+      ok = hipStreamCreateWithFlags(&r.s[i], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&r.join[i], hipEventDisableTiming) == hipSuccess;
+    r.ok = ok;
+  }
+  return r.ok ? &r : nullptr;
+}
+
 template <typename TIn>
 static int rg_launch(const TIn* in, const RgPlan& p, RgArgs a, int schedule, int dslot,
                      hipStream_t st, int variant = 0) {
@@ -699,10 +743,24 @@ static int rg_launch(const TIn* in, const RgPlan& p, RgArgs a, int schedule, int
   TM_LAUNCH_CHECK();
   if (schedule == 2) {
     const RgBlk* blks = (const RgBlk*)(a.work - (p.work_off - p.blks_off));
+    RgSideStreams* side = rg_side_streams();
+    bool used[3] = {false, false, false};  // chain 3 runs on the caller's stream
+    for (const RgLaunch& ln : p.launches)
+      if (ln.chain < 3) used[ln.chain] = true;
+    if (side) {
+      if (hipEventRecord(side->fork, st) != hipSuccess) side = nullptr;
+      for (int i = 0; i < 3 && side; ++i)
+        if (used[i] && hipStreamWaitEvent(side->s[i], side->fork, 0) != hipSuccess) {
+          set_error("tm_ragged_dt: stream fork failed");
+          return TM_ERR_HIP;
+        }
+    }
+    hipStream_t const st0 = st;
     for (const RgLaunch& ln : p.launches) {
       const RgBlk* b = blks + ln.lo;
       const dim3 g((unsigned)(ln.hi - ln.lo));
       const int d = ln.level;
+      hipStream_t st = (side && ln.chain < 3) ? side->s[ln.chain] : st0;
       switch (ln.kind) {
         case RG_EDT0:
           hipLaunchKernelGGL((rg_level_kernel<TIn, RG_EDT0>), g, dim3(kRgThreads), 0, st, in, a,
@@ -755,6 +813,12 @@ static int rg_launch(const TIn* in, const RgPlan& p, RgArgs a, int schedule, int
       }
       TM_LAUNCH_CHECK();
     }
+    for (int i = 0; i < 3 && side; ++i)
+      if (used[i] && (hipEventRecord(side->join[i], side->s[i]) != hipSuccess ||
+                      hipStreamWaitEvent(st0, side->join[i], 0) != hipSuccess)) {
+        set_error("tm_ragged_dt: stream join failed");
+        return TM_ERR_HIP;
+      }
     return TM_OK;
   }
   const int cap = grid_limit(dslot, (const void*)rg_kernel<TIn>);
