@@ -10,8 +10,9 @@
 // Kernel arithmetic is in base 2: with L = log2(e),
 //   (g_j - C_ij) L/eps + log2 b_j = h_j + xt_i . y_j - c_i,
 //   h_j = L/eps (g_j - |y_j|^2) + log2 b_j,  xt_i = 2L/eps x_i,  c_i = L/eps |x_i|^2,
-// so each term is three FMAs and one v_exp_f32; clouds are centred by the
-// caller, which keeps the expansion accurate.  The sum for row i is taken
+// so each exponent is a dot product of (y_j, h_j) with (xt_i, 1), a matrix
+// product over blocks of rows and columns, less s_i, then one v_exp_f32; clouds
+// are centred by the caller, which keeps the expansion accurate.  The sum for row i is taken
 // relative to s_i = c_i - L/eps f_i(old); the resulting sum S_i is exactly
 // the row-marginal ratio r_i / a_i of the current plan, so the L1 marginal
 // error sum_i a_i |S_i - 1| of the state after the previous g-step comes for
@@ -26,11 +27,12 @@
 // launches skip it.  The host replays a captured hipGraph of K iterations and
 // reads the number of active pairs between replays.
 //
-// Work distribution: a device work-list of (pair, first row) tiles of 64, 16
-// or 4 rows per half-step covers ragged pairs in one launch; a workgroup of
-// 256 lanes stages the pair's column points and h_j in LDS and splits each
-// row's sum over 256 / rows lanes.  The list is rebuilt for the active pairs
-// as pairs converge.
+// Work distribution: a device work-list of (pair, first row) tiles of 128, 64
+// or 32 rows per half-step covers ragged pairs in one launch; a workgroup of
+// four wavefronts stages the pair's column points and h_j in LDS and forms
+// the exponents of 32 x 32 blocks on the matrix cores (three FMAs and a
+// subtraction per term move off the vector ALU, which keeps the v_exp_f32
+// and the add).  The list is rebuilt for the active pairs as pairs converge.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -57,6 +59,14 @@ struct SkPair {
 constexpr int kCtrStride = 4;
 
 struct SkTile {
+  int32_t pair;
+  int32_t r0;
+};
+
+// A half-step work-list entry: a tile of rows with its pair's fields, so a
+// workgroup reads one entry instead of a tile and then its pair.
+struct SkTask {
+  SkPair pr;
   int32_t pair;
   int32_t r0;
 };
@@ -97,195 +107,163 @@ struct SkCheck {
 // ROWS_ARE_X selects which offsets of SkPair index rows; CHECK (g-updates
 // of the iteration loop) runs the fused convergence check first.
 //
-// A tile is RPT rows (64, 16 or 4) with 256 / RPT lanes per row.  The
-// arithmetic does not depend on RPT, so the schedule can change tile shapes
-// between iterations (more, smaller tiles for a few straggling pairs) with
-// identical results: a row's sum runs over 64 interleaved column groups
-// (group g holds the columns j = g mod 64, summed in column order; columns
-// are padded to a multiple of 64 with terms that are exactly 0), and the 64
-// group sums combine by one fixed pairwise tree (g + 32 into g, then + 16,
-// ..., + 1), in registers where the partners share a lane and in LDS where
-// they do not.  The row-marginal error is written per group of 4 rows
-// (summed in row order) and a pair's error is summed over its groups by a
-// fixed tree as well.
-// The pair's L1 row-marginal error over its ceil(n / 4) error groups, called
-// by lanes 0..63: lane l sums groups l, l + 64, ... in order, then a fixed
-// tree over the lanes; the result is valid in lane 0.
-__device__ __forceinline__ float sk_pair_error(const float* __restrict__ fpart, const SkPair& pr) {
-  const int ng = (pr.n + 3) >> 2;
-  float e = 0.f;
-  for (int t = threadIdx.x; t < ng; t += 64) e += fpart[pr.gbase + t];
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) e += __shfl_down(e, off, 64);
-  return e;
+// A tile is RPT rows (128, 64 or 32) of one pair, computed by four wavefronts
+// on matrix cores: a wavefront forms the exponents of 32 rows x 32 columns
+// at a time with v_mfma_f32_32x32x2f32 (see SkMf) and keeps only the
+// exponentials and the sums on the vector ALU.  The arithmetic does not
+// depend on RPT, so the schedule can change tile shapes between iterations
+// (more, smaller tiles for a few straggling pairs) with identical results.
+// Columns are padded to a multiple of 128 with terms that are exactly 0 and
+// cut into 32-column blocks; block b belongs to phase b mod 4.  A row's sum
+// runs over 128 interleaved column groups (group g holds the columns
+// j = g mod 128, summed in column order); the 32 groups of a phase combine by
+// one fixed tree (the 16 groups a lane holds pairwise, r + 8 into r, then
+// + 4, + 2, + 1; then the two lane halves), and the four phase sums as
+// (P0 + P1) + (P2 + P3).  A wavefront covers 32 rows and RPT / 32 phases:
+// all four in 128-row tiles, two in 64-row tiles (two wavefronts per 32
+// rows), one in 32-row tiles (four wavefronts per 32 rows); phase sums of
+// different wavefronts meet in LDS.  The row-marginal error is written per
+// group of 4 rows (summed in row order) and a pair's error is summed over its
+// groups by a fixed tree as well.
+
+typedef float sk_f32x16 __attribute__((ext_vector_type(16)));
+
+// Exponents as a matrix product.  Column j has the components
+//   c = (y_0 .. y_{D-1}, h_j, 0 ..)
+// and row i the components
+//   r = (xt_0 .. xt_{D-1}, 1, 0 ..),
+// so c . r = h_j + xt_i . y_j over K = 2 NK >= D + 1 entries.  One 32 x 32
+// block is NK chained v_mfma_f32_32x32x2f32 whose accumulator starts at
+// -shift_i (a lane holds one row, so that is one value per lane): MFMA m
+// takes the component pair (2m, 2m + 1): lane l supplies component
+// 2m + l / 32 of column l mod 32 (the A operand) and of row l mod 32 (the B
+// operand), and receives the 16 results of row l mod 32 (columns
+// 8 (k / 4) + 4 (l / 32) + k mod 4 of the block, k = 0..15).  Column
+// components 0 .. D are staged in LDS (half h of the wavefront reads
+// components h and 2 + h as one float2, and 4 + h as a float when D = 4);
+// the rest are 0.  Matrix products and vector instructions do not overlap
+// on this part, so the two MFMAs of D = 3 (K = 4) cost about as much as the
+// v_exp_f32 and the add of the 16 terms a lane receives.
+template <int D>
+struct SkMf {
+  static constexpr int NK = (D + 2) / 2;
+  static constexpr int BYTES = D == 4 ? 24 : 16;  // LDS bytes per column
+};
+
+// The component pair of MFMA m for lane half hf, from the staged float2 p2
+// and the extra float p1 (D = 4).
+template <int D>
+__device__ __forceinline__ float sk_col_comp(int m, float2 p2, float p1) {
+  if (m == 0) return p2.x;
+  if (m == 1) return p2.y;
+  return p1;  // component 4 + hf
 }
 
 // One half-step tile: rows tile.r0 .. tile.r0 + RPT - 1 of the pair (every
 // thread of the workgroup calls it; barriers inside).
-template <int D, bool ROWS_ARE_X, int RPT>
-__device__ __forceinline__ void sk_tile(const SkTile tile, const SkPair& pr,
-                                        const float* __restrict__ rows,
+template <int D, bool ROWS_ARE_X, bool CHECK, int RPT>
+__device__ __forceinline__ void sk_tile(const SkTask& tk, const float* __restrict__ rows,
                                         const float* __restrict__ cols,
                                         const float* __restrict__ lwc,
                                         const float* __restrict__ pc,
                                         const float* __restrict__ pin, float* __restrict__ pout,
-                                        const float* __restrict__ wr, float* __restrict__ errpart,
-                                        int maxc, const float* __restrict__ centre) {
-  static_assert(RPT == 64 || RPT == 16 || RPT == 4, "tile rows");
-  constexpr int NQ = 256 / RPT;  // lanes per row
-  constexpr int GPL = 64 / NQ;   // column groups per lane
-  // column j in LDS as one float4: (y_0, .., y_{D-1}, h_j) for D <= 3, so a
-  // term is one LDS read, three FMAs (h_j as the addend), one subtraction,
-  // one v_exp_f32 and one add; D = 4 keeps h_j in its own array
-  constexpr bool PACK = D <= 3;
-  extern __shared__ float4 dyn4[];
-  const int maxcp = (maxc + 63) & ~63;
-  float4* s4 = dyn4;                    // [maxcp] points (+ h_j when PACK)
-  float* shx = (float*)(dyn4 + maxcp);  // [maxcp] h_j when !PACK
-  __shared__ float ssum[NQ][RPT];
+                                        const float* __restrict__ wr, const int* active,
+                                        float* __restrict__ errpart, int maxc,
+                                        const float* __restrict__ centre, const SkCheck& ck) {
+  static_assert(RPT == 128 || RPT == 64 || RPT == 32, "tile rows");
+  constexpr int NK = SkMf<D>::NK;
+  constexpr int RB = RPT / 32;  // 32-row blocks of the tile
+  constexpr int WPR = 4 / RB;   // wavefronts per row block
+  constexpr int NPH = RB;       // phases per wavefront
+  extern __shared__ float2 dyn2[];
+  const int maxcp = (maxc + 127) & ~127;
+  float2* sp2 = dyn2;                        // [2][maxcp]: components (h, 2 + h)
+  float* sp1 = (float*)(dyn2 + 2 * maxcp);   // [2][maxcp]: component 4 + h (D = 4)
+  __shared__ float sq[RB][WPR][32];
   __shared__ float serr4[RPT];
   __shared__ int sbad;
+  const SkPair& pr = tk.pr;
   const int64_t roff = ROWS_ARE_X ? pr.xoff : pr.yoff;
   const int64_t coff = ROWS_ARE_X ? pr.yoff : pr.xoff;
   const int nr = ROWS_ARE_X ? pr.n : pr.m;
   const int nc = ROWS_ARE_X ? pr.m : pr.n;
-  const int ncp = (nc + 63) & ~63;
+  const int ncp = (nc + 127) & ~127;
   const float k1 = kL2E / pr.eps;  // L / eps
+  // Everything below the task entry is issued before anything is waited
+  // for: the active flag, the pair's error partials (CHECK), this lane's row
+  // and the column staging; a converged pair's workgroup returns after the
+  // staging barrier.
+  const int act = active[tk.pair];
+  constexpr int kEp = kMaxCols / 4 / 64;  // error groups per lane, at most
+  // CHECK: error groups lane, lane + 64, ... of the pair, loaded by every
+  // wavefront (the first one uses them) unconditionally at clamped indices
+  // (group 0 exists for every pair) and masked where they are summed: no
+  // load waits behind a branch
+  const int ng = (pr.n + 3) >> 2;
+  float ep[kEp];
+  if (CHECK) {
+#pragma unroll
+    for (int u = 0; u < kEp; ++u) {
+      const int t = (int)(threadIdx.x & 63) + 64 * u;
+      ep[u] = ck.fpart[pr.gbase + (t < ng ? t : 0)];
+    }
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int hf = lane >> 5;
+  const int rb = wave / WPR, wq = wave % WPR;
+  const int i = tk.r0 + rb * 32 + (lane & 31);
+  const bool live = i < nr;
+  const bool block_live = tk.r0 + rb * 32 < nr;  // uniform per wavefront
+  float xr[D], fold = 0.f, wi = 0.f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) xr[d] = live ? rows[(roff + i) * D + d] : 0.f;
+  if (live) {
+    fold = pin[roff + i];
+    if (errpart) wi = wr[roff + i];
+  }
   float ctr[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) ctr[d] = centre[tile.pair * kCtrStride + d];
-  // stage the pair's columns; padding columns have h = -inf (terms exactly 0)
+  for (int d = 0; d < D; ++d) ctr[d] = centre[tk.pair * kCtrStride + d];
+  // stage the pair's columns; padding columns (and weights of 0) have
+  // h = -2^100, whose terms are exactly 0
   for (int j = threadIdx.x; j < ncp; j += blockDim.x) {
-    float y[4] = {0.f, 0.f, 0.f, 0.f};
-    float h = -INFINITY;
+    float c[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float h = -0x1p100f;
     if (j < nc) {
       float yy = 0.f;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        y[d] = cols[(coff + j) * D + d] - ctr[d];
-        yy = fmaf(y[d], y[d], yy);
+        c[d] = cols[(coff + j) * D + d] - ctr[d];
+        yy = fmaf(c[d], c[d], yy);
       }
       h = fmaf(k1, pc[coff + j] - yy, lwc[coff + j]);
+      h = h < -0x1p100f ? -0x1p100f : h;  // NaN stays NaN
     }
-    if (PACK) {
-      s4[j] = make_float4(y[0], y[1], y[2], h);
-    } else {
-      s4[j] = make_float4(y[0], y[1], y[2], y[3]);
-      shx[j] = h;
+    c[D] = h;
+    sp2[j] = make_float2(c[0], c[2]);
+    sp2[maxcp + j] = make_float2(c[1], c[3]);
+    if (D == 4) {
+      sp1[j] = c[4];
+      sp1[maxcp + j] = c[5];
     }
   }
   if (threadIdx.x == 0) sbad = 0;
   __syncthreads();
-  const int r = threadIdx.x % RPT;
-  const int q = threadIdx.x / RPT;  // this lane's first column group
-  const int i = tile.r0 + r;
-  const bool live = i < nr;
-  float xt[4] = {0.f, 0.f, 0.f, 0.f};
-  float ci = 0.f, s = 0.f, fold = 0.f;
-  if (live) {
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const float x = rows[(roff + i) * D + d] - ctr[d];
-      xt[d] = 2.f * k1 * x;
-      ci = fmaf(x, x, ci);
-    }
-    ci *= k1;
-    fold = pin[roff + i];
-    s = ci - k1 * fold;
-  }
-  // t_j = h_j + xt . y_j
-  auto term = [&](int j) -> float {
-    const float4 v = s4[j];
-    float t = PACK ? v.w : shx[j];
-    t = fmaf(xt[0], v.x, t);
-    if (D >= 2) t = fmaf(xt[1], v.y, t);
-    if (D >= 3) t = fmaf(xt[2], v.z, t);
-    if (D >= 4) t = fmaf(xt[3], v.w, t);
-    return t;
-  };
-  // canonical row sum of 2^(t_j - shift); the result is valid in the q == 0
-  // lanes.  Every thread of the workgroup calls it (barriers inside).
-  auto row_sum = [&](float shift) -> float {
-    float acc[GPL];
-#pragma unroll
-    for (int k = 0; k < GPL; ++k) acc[k] = 0.f;
-    if (live)
-      for (int j0 = 0; j0 < ncp; j0 += 64) {
-#pragma unroll
-        for (int k = 0; k < GPL; ++k) acc[k] += __builtin_amdgcn_exp2f(term(j0 + q + NQ * k) - shift);
-      }
-#pragma unroll
-    for (int dk = GPL / 2; dk >= 1; dk >>= 1)
-#pragma unroll
-      for (int k = 0; k < dk; ++k) acc[k] += acc[k + dk];
-    ssum[q][r] = acc[0];
-    __syncthreads();
-#pragma unroll
-    for (int dq = NQ / 2; dq >= 1; dq >>= 1) {
-      if (q < dq) ssum[q][r] += ssum[q + dq][r];
-      __syncthreads();
-    }
-    const float v = ssum[0][r];
-    __syncthreads();  // ssum is reused by the next call
-    return v;
-  };
-  float S = row_sum(s);
-  const bool bad = live && q == 0 && !(S >= 0x1p-60f && S <= 0x1p60f);
-  if (bad) sbad = 1;
-  __syncthreads();
-  if (sbad) {
-    // exact path for the whole workgroup: the row maximum (any order), then
-    // the canonical sum relative to it
-    float mx = -INFINITY;
-    if (live)
-      for (int j = q; j < nc; j += NQ) mx = fmaxf(mx, term(j));
-    ssum[q][r] = mx;
-    __syncthreads();
-    if (q == 0)
-      for (int k = 1; k < NQ; ++k) mx = fmaxf(mx, ssum[k][r]);
-    __syncthreads();
-    if (q == 0) ssum[0][r] = mx;
-    __syncthreads();
-    mx = ssum[0][r];
-    __syncthreads();
-    const float S2 = row_sum(mx);
-    // rebase to the stale shift: log2 S = mx - s + log2 S2
-    if (live && q == 0) {
-      const float l2 = mx - s + __builtin_amdgcn_logf(S2);
-      S = exp2f(l2);  // may be inf/0 for the error only
-      pout[roff + i] = fold - l2 / k1;
-    }
-  } else if (live && q == 0) {
-    pout[roff + i] = fold - __builtin_amdgcn_logf(S) / k1;
-  }
-  if (errpart) {
-    // row errors a_i |S_i - 1| summed per 4-row group in row order
-    if (q == 0) serr4[r] = live ? wr[roff + i] * fabsf(S - 1.f) : 0.f;
-    __syncthreads();
-    if (threadIdx.x < RPT / 4 && tile.r0 + 4 * (int)threadIdx.x < nr) {
-      const int b = 4 * threadIdx.x;
-      errpart[pr.gbase + ((tile.r0 + b) >> 2)] =
-          ((serr4[b] + serr4[b + 1]) + serr4[b + 2]) + serr4[b + 3];
-    }
-  }
-}
-
-template <int D, bool ROWS_ARE_X, bool CHECK, int RPT>
-__global__ __launch_bounds__(256) void sk_half_kernel(
-    const float* __restrict__ rows, const float* __restrict__ cols,
-    const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ pin,
-    float* __restrict__ pout, const float* __restrict__ wr, const SkPair* __restrict__ pairs,
-    const SkTile* __restrict__ tiles, const int* active, float* __restrict__ errpart,
-    int maxc, const float* __restrict__ centre, SkCheck ck) {
-  const SkTile tile = tiles[blockIdx.x];
-  if (!active[tile.pair]) return;  // uniform per workgroup
-  const SkPair pr = pairs[tile.pair];
+  if (!act) return;  // uniform per workgroup
   if constexpr (CHECK) {
+    // the pair's L1 row-marginal error over its ceil(n / 4) error groups:
+    // lane l sums groups l, l + 64, ... in order, then a fixed tree over the
+    // lanes (every g-tile of the pair forms the same sum)
     __shared__ int sdone;
     __shared__ float serr;
     __shared__ int sk;
     if (threadIdx.x < 64) {
-      const float e = sk_pair_error(ck.fpart, pr);
+      float e = 0.f;
+#pragma unroll
+      for (int u = 0; u < kEp; ++u) e += (int)threadIdx.x + 64 * u < ng ? ep[u] : 0.f;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) e += __shfl_down(e, off, 64);
       if (threadIdx.x == 0) {
         const int k = *ck.iter_base + ck.k_in_chunk;
         serr = e;
@@ -295,25 +273,169 @@ __global__ __launch_bounds__(256) void sk_half_kernel(
     }
     __syncthreads();
     if (sdone) {
-      if (tile.r0 == 0) {
-        for (int i = threadIdx.x; i < pr.n; i += blockDim.x)
-          ck.fout[pr.xoff + i] = ck.fold[pr.xoff + i];
-        for (int j = threadIdx.x; j < pr.m; j += blockDim.x)
-          ck.gout[pr.yoff + j] = pin[pr.yoff + j];
+      // converged (or at max_iter): the first tile copies the state the
+      // error belongs to out and deactivates the pair
+      if (tk.r0 == 0) {
+        for (int r = threadIdx.x; r < pr.n; r += blockDim.x)
+          ck.fout[pr.xoff + r] = ck.fold[pr.xoff + r];
+        for (int c = threadIdx.x; c < pr.m; c += blockDim.x)
+          ck.gout[pr.yoff + c] = pin[pr.yoff + c];
         __syncthreads();
         if (threadIdx.x == 0) {
-          ck.iters[tile.pair] = sk - 1;
-          ck.err[tile.pair] = serr;
-          ck.active[tile.pair] = 0;
+          ck.iters[tk.pair] = sk - 1;
+          ck.err[tk.pair] = serr;
+          ck.active[tk.pair] = 0;
           atomicSub(ck.n_active, 1);
         }
       }
       return;
     }
-    if (tile.r0 == 0 && threadIdx.x == 0) ck.err[tile.pair] = serr;
+    if (tk.r0 == 0 && threadIdx.x == 0) ck.err[tk.pair] = serr;
   }
-  sk_tile<D, ROWS_ARE_X, RPT>(tile, pr, rows, cols, lwc, pc, pin, pout, wr, errpart, maxc,
-                              centre);
+  // row components; rows past the pair are all 0 (their results are unused)
+  float rc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float s = 0.f;
+  if (live) {
+    float ci = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float x = xr[d] - ctr[d];
+      rc[d] = 2.f * k1 * x;
+      ci = fmaf(x, x, ci);
+    }
+    ci *= k1;
+    s = ci - k1 * fold;
+    rc[D] = 1.f;
+  }
+  const float2* cp2 = sp2 + hf * maxcp + (lane & 31);
+  const float* cp1 = sp1 + hf * maxcp + (lane & 31);
+  // B operands of this lane (row components 2m + hf)
+  // (both values pinned to registers first: a select of two array entries
+  // would otherwise become a select of addresses into scratch)
+  float bo[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    float lo = rc[2 * m], hi = rc[2 * m + 1];
+    asm volatile("" : "+v"(lo), "+v"(hi));
+    bo[m] = hf ? hi : lo;
+  }
+  // the exponents t_j - shift of block b for this lane's row (c0 = -shift
+  // in all 16 entries)
+  auto block = [&](int b, const sk_f32x16& c0) -> sk_f32x16 {
+    const float2 p2 = cp2[b * 32];
+    const float p1 = D == 4 ? cp1[b * 32] : 0.f;
+    sk_f32x16 acc =
+        __builtin_amdgcn_mfma_f32_32x32x2f32(sk_col_comp<D>(0, p2, p1), bo[0], c0, 0, 0, 0);
+    if (NK >= 2)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sk_col_comp<D>(1, p2, p1), bo[1], acc, 0, 0, 0);
+    if (NK == 3)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(sk_col_comp<D>(2, p2, p1), bo[2], acc, 0, 0, 0);
+    return acc;
+  };
+  auto splat = [&](float v) {
+    sk_f32x16 c0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) c0[e] = v;
+    return c0;
+  };
+  // canonical row sum of 2^(t_j - shift), valid in every lane of the row
+  auto row_sum = [&](float shift) -> float {
+    const sk_f32x16 c0 = splat(live ? -shift : 0.f);
+    float acc[NPH][16];
+#pragma unroll
+    for (int k = 0; k < NPH; ++k)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[k][e] = 0.f;
+    if (block_live)
+      for (int c = 0; c < ncp; c += 128) {
+#pragma unroll
+        for (int k = 0; k < NPH; ++k) {
+          const sk_f32x16 t = block(c / 32 + wq * NPH + k, c0);
+#pragma unroll
+          for (int e = 0; e < 16; ++e) acc[k][e] += __builtin_amdgcn_exp2f(t[e]);
+        }
+      }
+    float ph[NPH];
+#pragma unroll
+    for (int k = 0; k < NPH; ++k) {
+#pragma unroll
+      for (int dk = 8; dk >= 1; dk >>= 1)
+#pragma unroll
+        for (int e = 0; e < dk; ++e) acc[k][e] += acc[k][e + dk];
+      ph[k] = acc[k][0] + __shfl_xor(acc[k][0], 32, 64);
+    }
+    float q = ph[0];
+    if (NPH == 2) q = ph[0] + ph[1];
+    if (NPH == 4) q = (ph[0] + ph[1]) + (ph[2] + ph[3]);
+    if (WPR == 1) return q;
+    if (hf == 0) sq[rb][wq][lane] = q;
+    __syncthreads();
+    const int l = lane & 31;
+    const float v = WPR == 2 ? sq[rb][0][l] + sq[rb][1][l]
+                             : (sq[rb][0][l] + sq[rb][1][l]) + (sq[rb][2][l] + sq[rb][3][l]);
+    __syncthreads();  // sq is reused by the next call
+    return v;
+  };
+  float S = row_sum(s);
+  const bool writer = live && hf == 0 && wq == 0;  // one lane per row
+  if (writer && !(S >= 0x1p-60f && S <= 0x1p60f)) sbad = 1;
+  __syncthreads();
+  if (sbad) {
+    // exact path for the whole workgroup: the row maximum (any order), then
+    // the canonical sum relative to it
+    const sk_f32x16 c0 = splat(0.f);
+    float mx = -INFINITY;
+    if (block_live)
+      for (int c = 0; c < ncp; c += 128) {
+#pragma unroll
+        for (int k = 0; k < NPH; ++k) {
+          const sk_f32x16 t = block(c / 32 + wq * NPH + k, c0);
+#pragma unroll
+          for (int e = 0; e < 16; ++e) mx = fmaxf(mx, t[e]);
+        }
+      }
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    if (WPR > 1) {
+      if (hf == 0) sq[rb][wq][lane] = mx;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < WPR; ++k) mx = fmaxf(mx, sq[rb][k][lane & 31]);
+      __syncthreads();
+    }
+    const float S2 = row_sum(mx);
+    // rebase to the stale shift: log2 S = mx - s + log2 S2
+    if (writer) {
+      const float l2 = mx - s + __builtin_amdgcn_logf(S2);
+      S = exp2f(l2);  // may be inf/0 for the error only
+      pout[roff + i] = fold - l2 / k1;
+    }
+  } else if (writer) {
+    pout[roff + i] = fold - __builtin_amdgcn_logf(S) / k1;
+  }
+  if (errpart) {
+    // row errors a_i |S_i - 1| summed per 4-row group in row order
+    if (hf == 0 && wq == 0) serr4[rb * 32 + lane] = live ? wi * fabsf(S - 1.f) : 0.f;
+    __syncthreads();
+    if (threadIdx.x < RPT / 4 && tk.r0 + 4 * (int)threadIdx.x < nr) {
+      const int b = 4 * threadIdx.x;
+      errpart[pr.gbase + ((tk.r0 + b) >> 2)] =
+          ((serr4[b] + serr4[b + 1]) + serr4[b + 2]) + serr4[b + 3];
+    }
+  }
+}
+
+// At least four wavefronts per SIMD: within that register budget the
+// matrix products write their results straight to VGPRs (no AGPR copies).
+template <int D, bool ROWS_ARE_X, bool CHECK, int RPT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void sk_half_kernel(
+    const float* __restrict__ rows, const float* __restrict__ cols,
+    const float* __restrict__ lwc, const float* __restrict__ pc, const float* __restrict__ pin,
+    float* __restrict__ pout, const float* __restrict__ wr, const SkTask* __restrict__ tasks,
+    const int* active, float* __restrict__ errpart, int maxc, const float* __restrict__ centre,
+    SkCheck ck) {
+  const SkTask tk = tasks[blockIdx.x];
+  sk_tile<D, ROWS_ARE_X, CHECK, RPT>(tk, rows, cols, lwc, pc, pin, pout, wr, active, errpart,
+                                     maxc, centre, ck);
 }
 
 
@@ -560,7 +682,8 @@ static thread_local PinnedStage g_sk_tiles[2];
 
 extern "C" size_t tm_sinkhorn_points_workspace_bytes(int npairs, const int32_t* n,
                                                      const int32_t* m) {
-  // tiles sized for the smallest tile shape (4 rows), error groups of 4 rows
+  // tile tables sized for one tile per 4 rows (more than any tile shape
+  // needs), error groups of 4 rows
   int64_t sx = 0, sy = 0, gx = 0, gy = 0;
   for (int p = 0; p < npairs; ++p) {
     sx += n[p];
@@ -571,7 +694,7 @@ extern "C" size_t tm_sinkhorn_points_workspace_bytes(int npairs, const int32_t* 
   size_t b = 0;
   auto add = [&](size_t x) { b += (x + 255) & ~(size_t)255; };
   add(sizeof(tmh::SkPair) * npairs);
-  add(sizeof(tmh::SkTile) * (gx + gy));
+  add(sizeof(tmh::SkTask) * (gx + gy));
   add(sizeof(int32_t) * (npairs + 1));
   add(sizeof(float) * gx);        // errpart: one per 4-row group of x
   add(sizeof(int) * npairs);      // active
@@ -599,7 +722,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   TM_CHECK_ARG(max_iter >= 1 && chunk >= 1, "tm_sinkhorn_points: max_iter and chunk must be >= 1");
   hipStream_t st = (hipStream_t)stream;
   std::vector<SkPair> hp(npairs);
-  std::vector<SkTile> htile;
+  std::vector<SkTask> htile;
   std::vector<int32_t> hfirst(npairs + 1);
   int64_t sx = 0, sy = 0, gx = 0;
   int maxc = 1;
@@ -617,14 +740,41 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
     sy += m[p];
     gx += (n[p] + 3) / 4;
   }
+  // Tile shape of a work-list: the largest (128, 64 or 32 rows) that still
+  // gives at least kTailTiles tiles per launch, or else the smallest shape
+  // whose tiles per launch the GPU holds at once (256 CUs, each with up to 4
+  // workgroups within its 160 KB of LDS): the long tail of a batch launches
+  // only the tiles of its stragglers, spread over many small tiles (the
+  // results do not depend on the tile shape).
+  const size_t half_lds = (size_t)((maxc + 127) & ~127) * (dim == 4 ? 24 : 16);
+  constexpr int64_t kTailTiles = 1024;
+  const int64_t resident =
+      (int64_t)256 * std::max<int64_t>(1, std::min<int64_t>(4, (160 << 10) / (int64_t)half_lds));
+  auto choose_rpt = [&](const int* act) {
+    int64_t tf[3] = {0, 0, 0}, tg[3] = {0, 0, 0};
+    const int shapes[3] = {128, 64, 32};
+    for (int p = 0; p < npairs; ++p)
+      if (!act || act[p])
+        for (int k = 0; k < 3; ++k) {
+          tf[k] += (n[p] + shapes[k] - 1) / shapes[k];
+          tg[k] += (m[p] + shapes[k] - 1) / shapes[k];
+        }
+    const int64_t T128 = std::max(tf[0], tg[0]), T64 = std::max(tf[1], tg[1]),
+                  T32 = std::max(tf[2], tg[2]);
+    if (T128 >= kTailTiles) return 128;
+    if (T64 >= kTailTiles) return T64 <= resident ? 64 : 128;
+    if (T32 <= resident) return 32;
+    return T64 <= resident ? 64 : 128;
+  };
+  int rpt = choose_rpt(nullptr);
   for (int p = 0; p < npairs; ++p) {
     hfirst[p] = (int32_t)htile.size();
-    for (int r = 0; r < n[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+    for (int r = 0; r < n[p]; r += rpt) htile.push_back(SkTask{hp[p], p, r});
   }
   hfirst[npairs] = (int32_t)htile.size();
   const int64_t tx = (int64_t)htile.size();
   for (int p = 0; p < npairs; ++p)
-    for (int r = 0; r < m[p]; r += kRowsPerTile) htile.push_back(SkTile{p, r});
+    for (int r = 0; r < m[p]; r += rpt) htile.push_back(SkTask{hp[p], p, r});
   const int64_t ty = (int64_t)htile.size() - tx;
   const size_t need = tm_sinkhorn_points_workspace_bytes(npairs, n, m);
   if (!workspace || workspace_bytes < need) {
@@ -641,7 +791,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   SkPair* dpairs = (SkPair*)take(sizeof(SkPair) * npairs);
   int64_t gy = 0;
   for (int p = 0; p < npairs; ++p) gy += (m[p] + 3) / 4;
-  SkTile* dtiles = (SkTile*)take(sizeof(SkTile) * (gx + gy));
+  SkTask* dtiles = (SkTask*)take(sizeof(SkTask) * (gx + gy));
   int32_t* dfirst = (int32_t*)take(sizeof(int32_t) * (npairs + 1));
   float* errpart = (float*)take(sizeof(float) * gx);
   int* active = (int*)take(sizeof(int) * npairs);
@@ -657,7 +807,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   // the small tables travel with the stream (pageable copies are staged
   // before this call returns)
   TM_CHECK_HIP(hipMemcpyAsync(dpairs, hp.data(), sizeof(SkPair) * npairs, hipMemcpyHostToDevice, st));
-  TM_CHECK_HIP(hipMemcpyAsync(dtiles, htile.data(), sizeof(SkTile) * (tx + ty),
+  TM_CHECK_HIP(hipMemcpyAsync(dtiles, htile.data(), sizeof(SkTask) * (tx + ty),
                               hipMemcpyHostToDevice, st));
   TM_CHECK_HIP(hipMemcpyAsync(dfirst, hfirst.data(), sizeof(int32_t) * (npairs + 1),
                               hipMemcpyHostToDevice, st));
@@ -675,18 +825,15 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   // tile counts and tile rows of the current work-list (rebuilt as pairs
   // converge)
   int64_t ntx = tx, nty = ty;
-  int rpt = kRowsPerTile;
   SkCheck nock{};
-  const size_t half_lds = (size_t)((maxc + 63) & ~63) * 20;
 #define SK_HALF_R(D, RX, CH, R, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, NT, ERR, CK)         \
   hipLaunchKernelGGL((sk_half_kernel<D, RX, CH, R>), dim3((unsigned)(NT)), dim3(256), half_lds, \
-                     lst, ROWS, COLS, LWC, PC, PIN, POUT, WR, dpairs, TILES, active, ERR, maxc,  \
-                     centre, CK)
+                     lst, ROWS, COLS, LWC, PC, PIN, POUT, WR, TILES, active, ERR, maxc, centre, CK)
 #define SK_HALF(D, RX, CH, ...)                                                              \
   do {                                                                                       \
-    if (rpt == 64) SK_HALF_R(D, RX, CH, 64, __VA_ARGS__);                                    \
-    else if (rpt == 16) SK_HALF_R(D, RX, CH, 16, __VA_ARGS__);                               \
-    else SK_HALF_R(D, RX, CH, 4, __VA_ARGS__);                                               \
+    if (rpt == 128) SK_HALF_R(D, RX, CH, 128, __VA_ARGS__);                                  \
+    else if (rpt == 64) SK_HALF_R(D, RX, CH, 64, __VA_ARGS__);                               \
+    else SK_HALF_R(D, RX, CH, 32, __VA_ARGS__);                                              \
   } while (0)
   // g-update from fcur; with `check`, the fused convergence check of the
   // state (fold, g) of iteration k_in_chunk - 1 runs first
@@ -694,7 +841,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
     if (nty == 0) return TM_OK;
     SkCheck ck{errpart, iter_base, k_in_chunk, max_iter, tol, fold, f, g, iters, err, active,
                n_active};
-    const SkTile* gt = dtiles + ntx;
+    const SkTask* gt = dtiles + ntx;
     if (check) {
       switch (dim) {
         case 1: SK_HALF(1, false, true, y, x, la, fcur, gw, gw, b, gt, nty, nullptr, ck); break;
@@ -787,20 +934,15 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
     }
     return ok;
   };
-  if (use_graph) use_graph = build_graph();
+  // A (re)capture is deferred to the next chunk launch, which enqueues that
+  // chunk directly and then captures while the GPU runs it: the GPU does not
+  // idle through the capture and instantiation.
+  bool need_capture = use_graph;
   // Work-list compaction: once the active pairs fall below 3/4 of the pairs
   // in the list, the tile tables are rebuilt for the active pairs only, with
-  // the largest tile shape (64, 16 or 4 rows) that still gives at least
-  // kTailTiles tiles per launch, or else the smallest shape whose tiles per
-  // launch the GPU holds at once (256 CUs, each with up to 8 workgroups of
-  // half_lds bytes of column staging in its 160 KB of LDS): the long tail
-  // of a batch launches only the tiles of its stragglers, spread over many
-  // small tiles (the results do not depend on the tile shape).
-  // The new tables go up through pinned stages (two, alternating), ordered
-  // on the stream after the chunk in flight, which still uses the old ones.
-  constexpr int64_t kTailTiles = 1024;
-  const int64_t resident =
-      (int64_t)256 * std::max<int64_t>(1, std::min<int64_t>(8, (160 << 10) / (int64_t)half_lds));
+  // the tile shape of choose_rpt.  The new tables go up through pinned stages
+  // (two, alternating), ordered on the stream after the chunk in flight, which
+  // still uses the old ones.
   int listed = npairs;
   int nstage = 0;
   auto upload = [&](const void* tiles, size_t tb, const void* firsts, size_t fb) -> int {
@@ -817,40 +959,26 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
     return TM_OK;
   };
   auto compact = [&](const int* hact) -> int {
-    // tiles per launch at each shape: the larger of the f and g lists
-    int64_t tf[3] = {0, 0, 0}, tg[3] = {0, 0, 0};
-    const int shapes[3] = {64, 16, 4};
-    for (int p = 0; p < npairs; ++p)
-      if (hact[p])
-        for (int k = 0; k < 3; ++k) {
-          tf[k] += (n[p] + shapes[k] - 1) / shapes[k];
-          tg[k] += (m[p] + shapes[k] - 1) / shapes[k];
-        }
-    const int64_t T64 = std::max(tf[0], tg[0]), T16 = std::max(tf[1], tg[1]),
-                  T4 = std::max(tf[2], tg[2]);
-    if (T64 >= kTailTiles) rpt = 64;
-    else if (T16 >= kTailTiles) rpt = T16 <= resident ? 16 : 64;
-    else if (T4 <= resident) rpt = 4;
-    else rpt = T16 <= resident ? 16 : 64;
+    rpt = choose_rpt(hact);
     htile.clear();
     int kept = 0;
     for (int p = 0; p < npairs; ++p) {
       hfirst[p] = (int32_t)htile.size();
       if (!hact[p]) continue;
       ++kept;
-      for (int r = 0; r < n[p]; r += rpt) htile.push_back(SkTile{p, r});
+      for (int r = 0; r < n[p]; r += rpt) htile.push_back(SkTask{hp[p], p, r});
     }
     hfirst[npairs] = (int32_t)htile.size();
     ntx = (int64_t)htile.size();
     for (int p = 0; p < npairs; ++p)
       if (hact[p])
-        for (int r = 0; r < m[p]; r += rpt) htile.push_back(SkTile{p, r});
+        for (int r = 0; r < m[p]; r += rpt) htile.push_back(SkTask{hp[p], p, r});
     nty = (int64_t)htile.size() - ntx;
-    const int rc = upload(htile.data(), sizeof(SkTile) * htile.size(), hfirst.data(),
+    const int rc = upload(htile.data(), sizeof(SkTask) * htile.size(), hfirst.data(),
                           sizeof(int32_t) * (npairs + 1));
     if (rc != TM_OK) return rc;
     listed = kept;
-    if (use_graph) use_graph = build_graph();
+    need_capture = use_graph;
     return TM_OK;
   };
   // Pipelined replays: chunk c + 1 is enqueued before the host waits for
@@ -864,10 +992,14 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   int status = TM_OK;
   const int64_t max_chunks = ((int64_t)max_iter + K) / K + ((max_iter + K) % K != 0);
   auto launch_chunk = [&](int slot) -> int {
-    if (use_graph) {
+    if (use_graph && !need_capture) {
       if (hipGraphLaunch(exec, st) != hipSuccess) return TM_ERR_HIP;
-    } else if (enqueue_chunk() != TM_OK) {
-      return TM_ERR_HIP;
+    } else {
+      if (enqueue_chunk() != TM_OK) return TM_ERR_HIP;
+      if (need_capture) {
+        use_graph = build_graph();
+        need_capture = false;
+      }
     }
     return hipEventRecord(rb.ev[slot], st) == hipSuccess ? TM_OK : TM_ERR_HIP;
   };

@@ -321,7 +321,15 @@ def _as_ragged(clouds) -> tuple[Tensor, list]:
 
 def _weights(w, sizes, device) -> Tensor:
     if w is None:
-        return torch.cat([torch.full((s,), 1.0 / s, device=device) for s in sizes])
+        # uniform 1 / n_p per pair in one fill (equal sizes) or one
+        # repeat_interleave, not a fill per pair; 1 / n_p rounds to float32
+        # as a fill of it does
+        total = sum(sizes)
+        if len(set(sizes)) == 1:
+            return torch.full((total,), 1.0 / sizes[0], device=device)
+        vals = torch.tensor([1.0 / s for s in sizes], dtype=torch.float32).to(device)
+        reps = torch.tensor(sizes, dtype=torch.int64).to(device)
+        return torch.repeat_interleave(vals, reps, output_size=total)
     w = torch.cat(list(w)) if not isinstance(w, Tensor) else w.reshape(-1)
     return w.to(device=device, dtype=torch.float32)
 
