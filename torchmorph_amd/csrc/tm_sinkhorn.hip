@@ -903,7 +903,7 @@ extern "C" int tm_sinkhorn_points(const float* x, const float* y, const float* a
   const bool outer_capture = hipStreamIsCapturing(st, &cs) == hipSuccess &&
                              cs != hipStreamCaptureStatusNone;
   TM_CHECK_ARG(!outer_capture, "tm_sinkhorn_points synchronises on convergence and cannot be "
-                               "captured; capture tm_sinkhorn_points_iterate instead");
+                               "captured into a graph");
   // capture one chunk on a private stream (the caller's may be the null
   // stream, which cannot capture), replay it on the caller's stream; captured
   // again whenever the work-list shrinks

@@ -24,8 +24,14 @@ class _GreyMorph(torch.autograd.Function):
         _launch(x32, struct, fp, se_shape, origin, mode_int, cval, dilation, y)
         ctx.save_for_backward(x32)
         ctx.args = (struct, fp, se_shape, origin, mode_int, cval, dilation)
+        # host-side arguments of the backward call, formed once here
+        st = None if struct is None else struct.numpy()
+        fpn = None if fp is None else fp.to(torch.uint8).numpy()
         ctx.arrays = (_lib.i64_array(x32.shape[2:]), _lib.i64_array(se_shape),
-                      _lib.i64_array(origin))
+                      _lib.i64_array(origin),
+                      None if st is None else st.ctypes.data_as(_lib.ctypes.c_void_p),
+                      None if fpn is None else fpn.ctypes.data_as(_lib.ctypes.c_void_p),
+                      st, fpn)  # the last two keep the buffers alive
         ctx.in_dtype = x.dtype
         return y
 
@@ -33,22 +39,18 @@ class _GreyMorph(torch.autograd.Function):
     def backward(ctx, gy):
         (x32,) = ctx.saved_tensors
         struct, fp, se_shape, origin, mode_int, cval, dilation = ctx.args
-        shp, se_arr, org_arr = ctx.arrays
+        shp, se_arr, org_arr, st_p, fp_p = ctx.arrays[:5]
         dx = torch.empty_like(x32)
         gy = gy.float().contiguous()
         L = _lib.lib()
         nd = x32.ndim - 2
         batch = x32.shape[0] * x32.shape[1]
-        st = None if struct is None else struct.numpy()
-        fpn = None if fp is None else fp.to(torch.uint8).numpy()
 
         def call(ws, nbytes):
             return L.tm_grey_backward(
-                _lib.ptr(x32), batch, nd, shp,
-                None if st is None else st.ctypes.data_as(_lib.ctypes.c_void_p),
-                None if fpn is None else fpn.ctypes.data_as(_lib.ctypes.c_void_p),
-                se_arr, org_arr, mode_int, float(cval), int(dilation), _lib.ptr(gy),
-                _lib.ptr(dx), _lib.ptr(ws), nbytes, _lib.stream_of(x32.device))
+                _lib.ptr(x32), batch, nd, shp, st_p, fp_p, se_arr, org_arr, mode_int,
+                float(cval), int(dilation), _lib.ptr(gy), _lib.ptr(dx), _lib.ptr(ws), nbytes,
+                _lib.stream_of(x32.device))
 
         with torch.cuda.device(x32.device):
             # 2-D items (modes other than wrap) usually run kernels that need
