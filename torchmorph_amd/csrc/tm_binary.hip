@@ -374,39 +374,49 @@ __global__ __launch_bounds__(256) void bin_tile_kernel(const uint64_t* __restric
       es_s[k] = g.es[k];
     }
   }
-  // load (and write the pads of both buffers): RU words per lane per round,
-  // their loads issued together from addresses clamped into the item, the
-  // border and the pads applied afterwards
   const int64_t item_w = (int64_t)blockIdx.y * item_rows * g.ww;  // first word of the item
-  for (int i0 = 0; i0 < per; i0 += RU * 256) {
+  // the tile's words: RU per lane per round over the rows' real words only
+  // (not the pad entries), their loads issued together from addresses
+  // clamped into the item, the border and the last word's bits applied
+  // afterwards
+  const int nw = g.by * g.bz * g.ww;
+  for (int i0 = 0; i0 < nw; i0 += RU * 256) {
     uint64_t v[RU];
-    int wpv[RU];
+    int lw[RU], wv[RU];
     bool okv[RU];
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
       const int i = i0 + u * 256 + (int)threadIdx.x;
-      const int ic = i < per ? i : per - 1;
-      const int r = small_div(ic, g.inv_pitch), wp = ic - r * g.pitch;
+      const int ic = i < nw ? i : nw - 1;
+      const int r = small_div(ic, g.inv_ww), w = ic - r * g.ww;
       const int rz = small_div(r, g.inv_by);
       const int y = y0 + (r - rz * g.by), z = z0 + rz;
-      okv[u] = wp >= 1 && wp <= g.ww && y >= 0 && y < g.Y && z >= 0 && z < g.Z;
-      wpv[u] = wp;
+      okv[u] = y >= 0 && y < g.Y && z >= 0 && z < g.Z;
+      wv[u] = w;
+      lw[u] = r * g.pitch + w + 1;
       const int yc = y < 0 ? 0 : (y >= g.Y ? g.Y - 1 : y);
       const int zc = z < 0 ? 0 : (z >= g.Z ? g.Z - 1 : z);
-      const int wc = wp < 1 ? 0 : (wp > g.ww ? g.ww - 1 : wp - 1);
-      const int64_t wi = ((int64_t)zc * g.Y + yc) * g.ww + wc;
+      const int64_t wi = ((int64_t)zc * g.Y + yc) * g.ww + w;
       if (IN_B) v[u] = bin_pack64(in_b + (item_w + wi) * 64);
       else v[u] = src[wi];
     }
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
       const int i = i0 + u * 256 + (int)threadIdx.x;
-      if (i >= per) break;
+      if (i >= nw) break;
       uint64_t x = okv[u] ? v[u] : border;
-      if (okv[u] && wpv[u] == g.ww) x = (x & last_m) | (border & ~last_m);
-      A[i] = x;
-      if (wpv[u] == 0 || wpv[u] >= g.ww + 1) B[i] = border;
+      if (okv[u] && wv[u] == g.ww - 1) x = (x & last_m) | (border & ~last_m);
+      A[lw[u]] = x;
     }
+  }
+  // the pads of both buffers (word 0 and words ww + 1 .. pitch - 1 of every
+  // row) hold the border
+  const int npad = g.pitch - g.ww;
+  for (int i = threadIdx.x; i < g.by * g.bz * npad; i += blockDim.x) {
+    const int r = i / npad, k = i - r * npad;
+    const int at = r * g.pitch + (k == 0 ? 0 : g.ww + k);
+    A[at] = border;
+    B[at] = border;
   }
   __syncthreads();
   for (int s = 1; s <= steps; ++s) {
