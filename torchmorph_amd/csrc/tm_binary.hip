@@ -344,7 +344,8 @@ __global__ __launch_bounds__(256) void bin_tile_kernel(const uint64_t* __restric
                                                        BinTileGeom g, int dilation,
                                                        int border_value, int steps) {
   constexpr bool IN_B = (IO & 1) != 0, OUT_B = (IO & 2) != 0;
-  constexpr int RU = IN_B ? 4 : 8;  // words loaded per lane per round
+  constexpr int RU = IN_B ? 5 : 8;  // words loaded per lane per round (a 16 x 16-row tile
+                                    // of 2-word rows with 4-row halos: one round)
   extern __shared__ uint64_t bin_lds[];
   __shared__ int eo_s[NE > 0 ? 1 : kMaxBinTable];
   __shared__ int es_s[NE > 0 ? 1 : kMaxBinTable];
